@@ -1,0 +1,161 @@
+/*
+ * rsr_amd.h -- C ABI of the MI355X fitness-evaluation engine for RSR.
+ *
+ * Plain C types only (pointers, sizes, int status codes).  Every function returns
+ * RSR_OK (0) or a negative RSR_E_* code; rsr_last_error_message() gives the
+ * calling thread's detail string.  Output buffers are caller-owned.  Calls on one
+ * context are serialised internally (one HIP stream per context), so island
+ * threads may call concurrently, mirroring the reference's rayon callers
+ * (src/engine/search/engine.rs:52).
+ *
+ * Each entry point names the reference interface it replaces (daruandris/rsr).
+ */
+#ifndef RSR_AMD_H
+#define RSR_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSR_API __attribute__((visibility("default")))
+
+/* ---- status codes ---- */
+#define RSR_OK 0
+#define RSR_E_INVALID_ARG (-1)
+#define RSR_E_HIP (-2)          /* HIP runtime error */
+#define RSR_E_OOM (-3)          /* device allocation failed */
+#define RSR_E_BUDGET (-4)       /* HBM budget of the context exceeded */
+#define RSR_E_PROGRAM (-5)      /* one or more programs were rejected by lowering */
+#define RSR_E_NO_DEVICE (-6)
+#define RSR_E_UNSUPPORTED (-7)
+#define RSR_E_QUEUE_FULL (-8)   /* serving: request queue full (back-pressure) */
+#define RSR_E_TIMEOUT (-9)      /* serving: wait timed out */
+#define RSR_E_NOT_READY (-10)   /* serving: request still running */
+
+/* ---- per-program lowering status (rsr_programs_status) ---- */
+#define RSR_PROG_OK 0
+#define RSR_PROG_STACK (-1)    /* typed-stack underflow or depth > 32 (UB in the reference) */
+#define RSR_PROG_OPCODE (-2)   /* unknown opcode, or a Bool/Int load (no-op in the reference) */
+#define RSR_PROG_FEATURE (-3)  /* feature index outside the dataset */
+#define RSR_PROG_CONST (-4)    /* constant index out of range or constant type mismatch */
+#define RSR_PROG_ROOT (-5)     /* program does not reduce to exactly one Float value */
+#define RSR_PROG_REGS (-6)     /* needs more than the 32-register file */
+
+/* ---- wire formats ----
+ * Opcodes: the reference `Instruction` enum in declaration order
+ *   (src/engine/domain.rs:46-63, src/domains/basic/mod.rs:9-20,
+ *    src/domains/linalg/mod.rs:9-51, composed in src/lib.rs:44-48):
+ *   0 LoadVarF 1 LoadConstF 2 LoadVarB 3 LoadConstB 4 LoadVarI 5 LoadConstI
+ *   6 LoadVarV2 7 LoadConstV2 8 LoadVarV3 9 LoadConstV3 10 LoadVarM2 11 LoadConstM2
+ *   12 LoadVarM3 13 LoadConstM3
+ *   14 AddF 15 SubF 16 MulF 17 DivF 18 SinF 19 CosF 20 ExpF 21 SqrF 22 SqrtF 23 LnF
+ *   24 MakeVec2 25 MakeVec3 26 GetXV2 27 GetYV2 28 GetXV3 29 GetYV3 30 GetZV3
+ *   31 AddV2 32 SubV2 33 ScaleV2 34 DotV2 35 NormV2
+ *   36 AddV3 37 SubV3 38 ScaleV3 39 DotV3 40 NormV3 41 CrossV3
+ *   42 MakeMat2 43 AddM2 44 SubM2 45 ScaleM2 46 MulM2 47 MulM2V2 48 DetM2 49 TraceM2
+ *   50 TransposeM2 51 InverseM2
+ *   52 MakeMat3 53 AddM3 54 SubM3 55 ScaleM3 56 MulM3 57 MulM3V3 58 DetM3 59 TraceM3
+ *   60 TransposeM3 61 InverseM3
+ *   62 NegF (extension, not in the reference: exact sign flip)
+ * Operand (u16): feature component index for LoadVar*, constant index for LoadConst*.
+ * ValueType codes (src/engine/eval/types.rs:1-10):
+ *   0 Float 1 Int 2 Bool 3 Vec2 4 Vec3 5 Mat2 6 Mat3.
+ * Constant payloads are flat floats per constant: Float 1, Vec2 2, Vec3 3,
+ *   Mat2 4, Mat3 9 (column-major, src/engine/eval/scalar.rs:108-112), Int/Bool 1.
+ */
+
+typedef struct rsr_context rsr_context;
+typedef struct rsr_dataset rsr_dataset;
+typedef struct rsr_programs rsr_programs;
+
+RSR_API const char* rsr_error_string(int code);
+RSR_API const char* rsr_last_error_message(void);
+RSR_API const char* rsr_version(void);
+
+/* Context: one device, one HIP stream, an HBM budget for datasets and programs
+ * (0 = 90% of free device memory).  No reference counterpart (the reference is
+ * in-process CPU code). */
+RSR_API int rsr_context_create(int device_ordinal, uint64_t hbm_budget_bytes, rsr_context** out);
+RSR_API int rsr_context_destroy(rsr_context* ctx);
+RSR_API int rsr_context_memory(const rsr_context* ctx, uint64_t* used_bytes, uint64_t* budget_bytes);
+/* Profiling: when enabled, HIP events on the context's stream bracket every main
+ * evaluation kernel; _read returns the accumulated kernel milliseconds and launch
+ * count since the last enable (and keeps accumulating). */
+RSR_API int rsr_context_profile(rsr_context* ctx, int enable);
+RSR_API int rsr_context_profile_read(rsr_context* ctx, double* kernel_ms, int64_t* launches);
+/* Wait for all work queued on the context's stream. */
+RSR_API int rsr_context_synchronize(rsr_context* ctx);
+/* Evaluation tuning: points per lane (1, 2, 4) and programs per work item. */
+RSR_API int rsr_context_set_tuning(rsr_context* ctx, int points_per_lane, int programs_per_item);
+RSR_API int rsr_context_get_tuning(const rsr_context* ctx, int* points_per_lane, int* programs_per_item);
+
+/* Dataset::new (src/engine/data/dataset.rs:37-169): rows [num_samples][F]
+ * row-major where F = sum of the feature types' sizes; zero padding to a multiple
+ * of 8 (counted in MSE sums, divisor = num_samples); optional z-score. */
+RSR_API int rsr_dataset_create(rsr_context* ctx, const float* rows, const float* y, int64_t num_samples,
+                               const uint8_t* feature_types, int n_feature_types, int normalize,
+                               rsr_dataset** out);
+/* A dataset already in the reference's AoSoA layout (Dataset.feature_flat /
+ * target_batches, dataset.rs:17-34): feature_flat[(b*F + f)*8 + lane]. */
+RSR_API int rsr_dataset_create_aosoa(rsr_context* ctx, const float* feature_flat, const float* target_batches,
+                                     int num_features, int64_t num_batches, int64_t num_samples, rsr_dataset** out);
+/* Dataset::subset (dataset.rs:360-447): strided sample of target_samples points. */
+RSR_API int rsr_dataset_subset(rsr_context* ctx, const rsr_dataset* src, int64_t target_samples, rsr_dataset** out);
+RSR_API int rsr_dataset_info(const rsr_dataset* ds, int* num_features, int64_t* num_samples, int64_t* num_batches);
+/* z-score statistics (feature means[F], std devs[F], target mean, target std). */
+RSR_API int rsr_dataset_stats(const rsr_dataset* ds, float* stats_out);
+RSR_API int rsr_dataset_destroy(rsr_dataset* ds);
+
+/* A batch of programs (Program::from_nodes, src/engine/expr/program.rs:34-55):
+ * program i is ops/args[code_offsets[i] .. code_offsets[i+1]), its constants are
+ * const_types[const_offsets[i] .. const_offsets[i+1]) with payloads starting at
+ * const_values[value_offsets[i]]. */
+RSR_API int rsr_programs_create(rsr_context* ctx, int n_programs, const uint8_t* ops, const uint16_t* args,
+                                const int64_t* code_offsets, const uint8_t* const_types,
+                                const int64_t* const_offsets, const float* const_values,
+                                const int64_t* value_offsets, rsr_programs** out);
+RSR_API int rsr_programs_count(const rsr_programs* progs, int* n_programs, int64_t* n_values);
+/* Constants in the create-time layout (what the optimizers write back,
+ * lbfgs.rs:219-229, cmaes.rs:181-191). */
+RSR_API int rsr_programs_get_constants(const rsr_programs* progs, float* const_values_out);
+RSR_API int rsr_programs_set_constants(rsr_programs* progs, const float* const_values);
+/* Lowering status of every program against a dataset's feature count. */
+RSR_API int rsr_programs_status(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, int32_t* status_out);
+RSR_API int rsr_programs_destroy(rsr_programs* progs);
+
+/* compute_mse (src/engine/eval/evaluator.rs:153-174) for every program:
+ * non-finite -> FLT_MAX.  Rejected programs get NaN and the call returns
+ * RSR_E_PROGRAM after evaluating the others. */
+RSR_API int rsr_eval_mse(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, float* mse_out);
+
+/* eval_simd over every 8-point batch (evaluator.rs:20-150): per-point values of
+ * programs [first, first+count), out[(i-first) * num_batches*8 + point],
+ * padding lanes included. */
+RSR_API int rsr_eval_points(rsr_context* ctx, rsr_programs* progs, int first, int count, const rsr_dataset* ds,
+                            float* out);
+
+/* compute_mse_with_gradient (evaluator.rs:177-224): mse_out[n], grad_out[n][32]
+ * (gradient of the first min(param_count, 32) flat parameters, zeros after). */
+RSR_API int rsr_eval_mse_grad(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, float* mse_out,
+                              float* grad_out);
+
+/* eval_simd_dual (evaluator.rs:228-388): value and tangent of one program w.r.t.
+ * flat parameter `active_param`, per point. */
+RSR_API int rsr_eval_points_dual(rsr_context* ctx, rsr_programs* progs, int program, int active_param,
+                                 const rsr_dataset* ds, float* val_out, float* grad_out);
+
+/* Lowering self-check (test hook, no reference counterpart, no GPU): lower one
+ * program exactly as the evaluation entry points do and run the emitted micro-code
+ * on the host over n points (feat[f * n + i]).  Returns the RSR_PROG_* status. */
+RSR_API int rsr_lower_check(const uint8_t* ops, const uint16_t* args, int n_code, const uint8_t* const_types,
+                            const float* const_values, int n_consts, int num_features, int dual, int active_param,
+                            const float* feat, int64_t n, float* vals_out, float* tan_out, int32_t* n_uops,
+                            int32_t* max_reg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RSR_AMD_H */

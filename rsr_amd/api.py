@@ -1,0 +1,321 @@
+"""ctypes binding of librsr_amd.so and the host-side evaluator / fitness interface.
+
+The names follow the reference's evaluator and fitness surface
+(src/engine/eval/evaluator.rs: compute_mse, compute_mse_with_gradient, eval_simd,
+eval_simd_dual; src/engine/data/dataset.rs: Dataset::new / subset;
+src/engine/optimize/mod.rs: optimize_individual_constants), batched over many
+programs.  Everything here calls the HIP library; there is no CPU fallback: if the
+library is missing or no GPU is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .isa import Program, VT, pack_programs, feature_count
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librsr_amd.so")
+
+RSR_OK = 0
+RSR_E_PROGRAM = -5
+
+_lib = None
+
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+_f32p = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_u8p = C.POINTER(C.c_uint8)
+_u16p = C.POINTER(C.c_uint16)
+
+# name -> (restype, argtypes); the exported C ABI (include/rsr_amd.h)
+SIGNATURES = {
+    "rsr_error_string": (C.c_char_p, [C.c_int]),
+    "rsr_last_error_message": (C.c_char_p, []),
+    "rsr_version": (C.c_char_p, []),
+    "rsr_context_create": (C.c_int, [C.c_int, C.c_uint64, _PP]),
+    "rsr_context_destroy": (C.c_int, [_P]),
+    "rsr_context_memory": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "rsr_context_set_tuning": (C.c_int, [_P, C.c_int, C.c_int]),
+    "rsr_context_get_tuning": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "rsr_context_profile": (C.c_int, [_P, C.c_int]),
+    "rsr_context_profile_read": (C.c_int, [_P, C.POINTER(C.c_double), _i64p]),
+    "rsr_context_synchronize": (C.c_int, [_P]),
+    "rsr_dataset_create": (C.c_int, [_P, _f32p, _f32p, C.c_int64, _u8p, C.c_int, C.c_int, _PP]),
+    "rsr_dataset_create_aosoa": (C.c_int, [_P, _f32p, _f32p, C.c_int, C.c_int64, C.c_int64, _PP]),
+    "rsr_dataset_subset": (C.c_int, [_P, _P, C.c_int64, _PP]),
+    "rsr_dataset_info": (C.c_int, [_P, C.POINTER(C.c_int), _i64p, _i64p]),
+    "rsr_dataset_stats": (C.c_int, [_P, _f32p]),
+    "rsr_dataset_destroy": (C.c_int, [_P]),
+    "rsr_programs_create": (C.c_int, [_P, C.c_int, _u8p, _u16p, _i64p, _u8p, _i64p, _f32p, _i64p, _PP]),
+    "rsr_programs_count": (C.c_int, [_P, C.POINTER(C.c_int), _i64p]),
+    "rsr_programs_get_constants": (C.c_int, [_P, _f32p]),
+    "rsr_programs_set_constants": (C.c_int, [_P, _f32p]),
+    "rsr_programs_status": (C.c_int, [_P, _P, _P, _i32p]),
+    "rsr_programs_destroy": (C.c_int, [_P]),
+    "rsr_eval_mse": (C.c_int, [_P, _P, _P, _f32p]),
+    "rsr_eval_points": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _f32p]),
+    "rsr_eval_mse_grad": (C.c_int, [_P, _P, _P, _f32p, _f32p]),
+    "rsr_eval_points_dual": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _f32p, _f32p]),
+}
+
+
+class RsrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rsr error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load librsr_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built: run `python -m rsr_amd.build` (hipcc, gfx950)")
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(l, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _check(rc: int, allow=()):
+    if rc != RSR_OK and rc not in allow:
+        msg = lib().rsr_last_error_message().decode(errors="replace")
+        raise RsrError(rc, msg)
+    return rc
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+class Context:
+    """One GPU, one HIP stream, an HBM budget (0 = 90% of free memory)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        h = C.c_void_p()
+        _check(lib().rsr_context_create(device, hbm_budget_bytes, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().rsr_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def memory(self):
+        u, b = C.c_uint64(), C.c_uint64()
+        _check(lib().rsr_context_memory(self.h, C.byref(u), C.byref(b)))
+        return u.value, b.value
+
+    def set_tuning(self, points_per_lane: int, programs_per_item: int):
+        _check(lib().rsr_context_set_tuning(self.h, points_per_lane, programs_per_item))
+
+    def tuning(self):
+        a, b = C.c_int(), C.c_int()
+        _check(lib().rsr_context_get_tuning(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def profile(self, enable: bool):
+        _check(lib().rsr_context_profile(self.h, int(enable)))
+
+    def profile_read(self):
+        ms, n = C.c_double(), C.c_int64()
+        _check(lib().rsr_context_profile_read(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def synchronize(self):
+        _check(lib().rsr_context_synchronize(self.h))
+
+
+class Dataset:
+    """Device-resident dataset (Dataset::new, dataset.rs:37-169)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self.h = handle
+
+    @staticmethod
+    def new(ctx: Context, rows: np.ndarray, y: np.ndarray, feature_types: Sequence[int],
+            normalize: bool = False) -> "Dataset":
+        rows = np.ascontiguousarray(rows, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        n = y.shape[0]
+        ft = np.asarray([int(t) for t in feature_types], np.uint8)
+        F = feature_count(ft)
+        if rows.size != n * F:
+            raise ValueError(f"rows must be [{n}][{F}]")
+        h = C.c_void_p()
+        _check(lib().rsr_dataset_create(ctx.h, _ptr(rows, _f32p), _ptr(y, _f32p), n, _ptr(ft, _u8p), len(ft),
+                                        int(normalize), C.byref(h)))
+        return Dataset(ctx, h)
+
+    @staticmethod
+    def from_aosoa(ctx: Context, feature_flat: np.ndarray, target_batches: np.ndarray, num_features: int,
+                   num_samples: int) -> "Dataset":
+        ff = np.ascontiguousarray(feature_flat, np.float32)
+        tb = np.ascontiguousarray(target_batches, np.float32)
+        nb = tb.size // 8
+        h = C.c_void_p()
+        _check(lib().rsr_dataset_create_aosoa(ctx.h, _ptr(ff, _f32p), _ptr(tb, _f32p), num_features, nb,
+                                              num_samples, C.byref(h)))
+        return Dataset(ctx, h)
+
+    def subset(self, target_samples: int) -> "Dataset":
+        h = C.c_void_p()
+        _check(lib().rsr_dataset_subset(self.ctx.h, self.h, target_samples, C.byref(h)))
+        return Dataset(self.ctx, h)
+
+    def info(self):
+        f, s, b = C.c_int(), C.c_int64(), C.c_int64()
+        _check(lib().rsr_dataset_info(self.h, C.byref(f), C.byref(s), C.byref(b)))
+        return f.value, s.value, b.value
+
+    @property
+    def num_features(self):
+        return self.info()[0]
+
+    @property
+    def num_samples(self):
+        return self.info()[1]
+
+    @property
+    def num_batches(self):
+        return self.info()[2]
+
+    def stats(self):
+        F = self.num_features
+        out = np.zeros(2 * F + 2, np.float32)
+        _check(lib().rsr_dataset_stats(self.h, _ptr(out, _f32p)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().rsr_dataset_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Programs:
+    """A device-resident batch of programs."""
+
+    def __init__(self, ctx: Context, progs: Sequence[Program]):
+        self.ctx = ctx
+        self.n = len(progs)
+        ops, args, co, ct, cof, cv, vo = pack_programs(progs)
+        self._keep = (ops, args, co, ct, cof, cv, vo)
+        h = C.c_void_p()
+        _check(lib().rsr_programs_create(ctx.h, self.n, _ptr(ops, _u8p), _ptr(args, _u16p), _ptr(co, _i64p),
+                                         _ptr(ct, _u8p), _ptr(cof, _i64p), _ptr(cv, _f32p), _ptr(vo, _i64p),
+                                         C.byref(h)))
+        self.h = h
+        self.n_values = int(vo[-1])
+
+    def get_constants(self) -> np.ndarray:
+        out = np.zeros(max(self.n_values, 1), np.float32)
+        _check(lib().rsr_programs_get_constants(self.h, _ptr(out, _f32p)))
+        return out[: self.n_values]
+
+    def set_constants(self, values: np.ndarray):
+        v = np.ascontiguousarray(values, np.float32)
+        if v.size < self.n_values:
+            raise ValueError("too few constant values")
+        _check(lib().rsr_programs_set_constants(self.h, _ptr(v, _f32p)))
+
+    def status(self, ds: Dataset) -> np.ndarray:
+        out = np.zeros(max(self.n, 1), np.int32)
+        _check(lib().rsr_programs_status(self.ctx.h, self.h, ds.h, _ptr(out, _i32p)))
+        return out[: self.n]
+
+    def close(self):
+        if self.h:
+            lib().rsr_programs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def compute_mse(ctx: Context, progs: Programs, ds: Dataset, allow_rejected: bool = False) -> np.ndarray:
+    """Batched compute_mse (evaluator.rs:153-174)."""
+    out = np.zeros(max(progs.n, 1), np.float32)
+    _check(lib().rsr_eval_mse(ctx.h, progs.h, ds.h, _ptr(out, _f32p)), (RSR_E_PROGRAM,) if allow_rejected else ())
+    return out[: progs.n]
+
+
+def eval_points(ctx: Context, progs: Programs, ds: Dataset, first: int = 0, count: Optional[int] = None) -> np.ndarray:
+    """Per-point values (eval_simd over all batches): [count][num_batches*8]."""
+    count = progs.n - first if count is None else count
+    nb = ds.num_batches
+    out = np.zeros((max(count, 1), nb * 8), np.float32)
+    _check(lib().rsr_eval_points(ctx.h, progs.h, first, count, ds.h, _ptr(out, _f32p)))
+    return out[:count]
+
+
+def compute_mse_with_gradient(ctx: Context, progs: Programs, ds: Dataset, allow_rejected: bool = False):
+    """Batched compute_mse_with_gradient (evaluator.rs:177-224): (mse[n], grad[n][32])."""
+    mse = np.zeros(max(progs.n, 1), np.float32)
+    grad = np.zeros((max(progs.n, 1), 32), np.float32)
+    _check(lib().rsr_eval_mse_grad(ctx.h, progs.h, ds.h, _ptr(mse, _f32p), _ptr(grad, _f32p)),
+           (RSR_E_PROGRAM,) if allow_rejected else ())
+    return mse[: progs.n], grad[: progs.n]
+
+
+def eval_points_dual(ctx: Context, progs: Programs, ds: Dataset, program: int, active_param: int):
+    """eval_simd_dual over all batches: (values, tangents) per point."""
+    n = ds.num_batches * 8
+    v = np.zeros(n, np.float32)
+    g = np.zeros(n, np.float32)
+    _check(lib().rsr_eval_points_dual(ctx.h, progs.h, program, active_param, ds.h, _ptr(v, _f32p), _ptr(g, _f32p)))
+    return v, g
+
+
+def lower_check(prog: Program, rows_soa: np.ndarray, dual: bool = False, active_param: int = 0):
+    """Lower one program and run its micro-code on the host (lowering self-check, no GPU).
+
+    rows_soa: [F][n] feature-major.  Returns (status, values[n], tangents[n], n_uops, max_reg).
+    """
+    l = lib()
+    fn = l.rsr_lower_check
+    fn.restype = C.c_int
+    fn.argtypes = [_u8p, _u16p, C.c_int, _u8p, _f32p, C.c_int, C.c_int, C.c_int, C.c_int, _f32p, C.c_int64,
+                   _f32p, _f32p, _i32p, _i32p]
+    ops, args, co, ct, cof, cv, vo = pack_programs([prog])
+    ops = ops if ops.size else np.zeros(1, np.uint8)
+    args = args if args.size else np.zeros(1, np.uint16)
+    ct = ct if ct.size else np.zeros(1, np.uint8)
+    cv = cv if cv.size else np.zeros(1, np.float32)
+    feat = np.ascontiguousarray(rows_soa, np.float32)
+    F, n = (feat.shape[0], feat.shape[1]) if feat.ndim == 2 else (0, 0)
+    if feat.size == 0:
+        feat = np.zeros(1, np.float32)
+    vals = np.zeros(max(n, 1), np.float32)
+    tans = np.zeros(max(n, 1), np.float32)
+    nu, mr = C.c_int32(), C.c_int32()
+    st = fn(_ptr(ops, _u8p), _ptr(args, _u16p), len(prog.code), _ptr(ct, _u8p), _ptr(cv, _f32p),
+            len(prog.constants), F, int(dual), active_param, _ptr(feat, _f32p), n, _ptr(vals, _f32p),
+            _ptr(tans, _f32p), C.byref(nu), C.byref(mr))
+    return st, vals[:n], tans[:n], nu.value, mr.value

@@ -1,0 +1,85 @@
+// hostcheck.hip -- executes lowered micro-code on the host, one point at a time.
+//
+// A test hook for the lowering only: the CPU test-suite uses it to check the
+// postfix -> micro-code translation (register assignment, aliasing, op sequences)
+// against the oracle without a GPU.  No evaluation entry point calls it; they all
+// launch the HIP kernels.  The arithmetic helpers are the device ones (devmath.h is
+// __host__ __device__), so this also pins the device transcendental code on the CPU.
+#include <math.h>
+#include <string.h>
+#include <vector>
+#include "../../include/rsr_amd.h"
+#include "devmath.h"
+#include "lower.h"
+
+using namespace rsr;
+
+static void run_host(const std::vector<UInstr>& code, const float* params, float* R, float* G, int k,
+                     float* spill /* [slot][2] */) {
+    for (const UInstr& u : code) {
+        const uint32_t op = u.w0 & 255u, d = (u.w0 >> 8) & 31u, a = (u.w0 >> 16) & 31u, b = (u.w0 >> 24) & 31u;
+        const uint32_t c = u.w1 & 31u, e = (u.w1 >> 8) & 31u;
+        float v = 0.0f, g = 0.0f;
+        switch (op) {
+            case U_IMM: v = params[u.w1]; g = ((int)u.w1 == k) ? 1.0f : 0.0f; break;
+            case U_ADD: v = R[a] + R[b]; g = G[a] + G[b]; break;
+            case U_SUB: v = R[a] - R[b]; g = G[a] - G[b]; break;
+            case U_MUL: v = R[a] * R[b]; g = (R[a] * G[b]) + (G[a] * R[b]); break;
+            case U_DIV: v = R[a] / R[b]; g = ((G[a] * R[b]) - (R[a] * G[b])) / (R[b] * R[b]); break;
+            case U_SQR: v = R[a] * R[a]; g = (2.0f * R[a]) * G[a]; break;
+            case U_SQRT: v = sqrtf(R[a]); g = G[a] / (2.0f * v); break;
+            case U_SIN: v = dm_sin(R[a]); g = G[a] * dm_cos(R[a]); break;
+            case U_COS: v = dm_cos(R[a]); g = G[a] * (-dm_sin(R[a])); break;
+            case U_EXP: v = dm_exp(R[a]); g = G[a] * v; break;
+            case U_LN: v = dm_ln(R[a]); g = G[a] / R[a]; break;
+            case U_NEG: v = -R[a]; g = -G[a]; break;
+            case U_MULADD: v = (R[a] * R[b]) + (R[c] * R[e]);
+                g = ((R[a] * G[b]) + (G[a] * R[b])) + ((R[c] * G[e]) + (G[c] * R[e])); break;
+            case U_MULSUB: v = (R[a] * R[b]) - (R[c] * R[e]);
+                g = ((R[a] * G[b]) + (G[a] * R[b])) - ((R[c] * G[e]) + (G[c] * R[e])); break;
+            case U_ADDMUL: v = R[a] + (R[b] * R[c]); g = G[a] + ((R[b] * G[c]) + (G[b] * R[c])); break;
+            case U_SQRTACC: v = sqrtf(R[a] + (R[b] * R[b])); g = (G[a] + ((R[b] * G[b]) + (G[b] * R[b]))) / (2.0f * v); break;
+            case U_ADD3: v = (R[a] + R[b]) + R[c]; g = (G[a] + G[b]) + G[c]; break;
+            case U_RCPSAFE: v = 1.0f / ((fabsf(R[a]) < 1e-9f) ? 1.0f : R[a]); break;
+            case U_INVSEL: {
+                const float x = (u.w1 & (1u << 16)) ? -R[a] : R[a];
+                v = (fabsf(R[c]) < 1e-9f) ? ((u.w1 & (2u << 16)) ? 1.0f : 0.0f) : x * R[b];
+                break;
+            }
+            case U_SPILL: v = R[a]; g = G[a]; spill[2 * u.w1] = v; spill[2 * u.w1 + 1] = g; break;
+            case U_FILL: v = spill[2 * u.w1]; g = spill[2 * u.w1 + 1]; break;
+            default: break;
+        }
+        R[d] = v;
+        G[d] = g;
+    }
+}
+
+// Lower one program (reference wire format) and run it on n points given as
+// feature-major columns feat[f * n + i].  vals_out[i] = value; when dual != 0 the
+// tangent w.r.t. flat parameter `active_param` goes to tan_out[i].  Returns the
+// lowering status; n_uops / max_reg describe the emitted code.
+extern "C" RSR_API int rsr_lower_check(const uint8_t* ops, const uint16_t* args, int n_code, const uint8_t* ctypes,
+                                       const float* cvals, int n_consts, int num_features, int dual,
+                                       int active_param, const float* feat, int64_t n, float* vals_out,
+                                       float* tan_out, int32_t* n_uops, int32_t* max_reg) {
+    // n_uops / max_reg report the code size and register use; spill slots are folded
+    // into max_reg as 32 + slots when the program spills
+    RefProgram p{ops, args, n_code, ctypes, cvals, n_consts};
+    Lowered lw = lower_program(p, num_features, dual != 0);
+    if (n_uops) *n_uops = (int32_t)lw.code.size();
+    if (max_reg) *max_reg = (int32_t)(lw.spill_slots ? kRegs + lw.spill_slots : lw.max_reg);
+    if (lw.status != PROG_OK) return lw.status;
+    std::vector<float> params;
+    flatten_params(p, params);
+    params.push_back(0.0f);
+    float R[kRegs], G[kRegs], spill[2 * kMaxSpillSlots];
+    for (int64_t i = 0; i < n; ++i) {
+        for (int r = 0; r < kRegs; ++r) { R[r] = 0.0f; G[r] = 0.0f; }
+        for (int f = 0; f < num_features; ++f) R[f] = feat[(int64_t)f * n + i];
+        run_host(lw.code, params.data(), R, G, dual ? active_param : -1, spill);
+        if (vals_out) vals_out[i] = R[lw.out_reg];
+        if (tan_out) tan_out[i] = G[lw.out_reg];
+    }
+    return 0;
+}

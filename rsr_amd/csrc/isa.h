@@ -1,0 +1,110 @@
+// isa.h -- instruction sets shared by the host lowering and the HIP kernels.
+//
+// Two instruction sets live here:
+//   1. RefOp: the reference's postfix `Instruction` enum, numbered in
+//      declaration order (src/engine/domain.rs:46-63 load variants, then
+//      Basic (src/domains/basic/mod.rs:9-20) then Linalg
+//      (src/domains/linalg/mod.rs:9-51), composed in src/lib.rs:44-48).
+//      This is the wire format of the C ABI (opcode u8 + operand u16).
+//   2. UOp: the device micro-ISA.  Every micro-op writes ONE f32 register
+//      component and reads up to four register components, so vectors and
+//      matrices are carried component by component in a flat per-lane
+//      register file.  The host lowering maps the typed stacks onto that
+//      file statically (the stack pointer of every typed stack is known at
+//      every pc of a postfix program), so the kernel never keeps a stack
+//      pointer.
+#pragma once
+#include <stdint.h>
+
+namespace rsr {
+
+enum RefOp : uint8_t {
+    LoadVarF = 0, LoadConstF, LoadVarB, LoadConstB, LoadVarI, LoadConstI,
+    LoadVarV2, LoadConstV2, LoadVarV3, LoadConstV3, LoadVarM2, LoadConstM2,
+    LoadVarM3, LoadConstM3,
+    AddF = 14, SubF, MulF, DivF, SinF, CosF, ExpF, SqrF, SqrtF, LnF,
+    MakeVec2 = 24, MakeVec3, GetXV2, GetYV2, GetXV3, GetYV3, GetZV3,
+    AddV2, SubV2, ScaleV2, DotV2, NormV2,
+    AddV3, SubV3, ScaleV3, DotV3, NormV3, CrossV3,
+    MakeMat2, AddM2, SubM2, ScaleM2, MulM2, MulM2V2, DetM2, TraceM2, TransposeM2, InverseM2,
+    MakeMat3, AddM3, SubM3, ScaleM3, MulM3, MulM3V3, DetM3, TraceM3, TransposeM3, InverseM3,
+    // Not a reference opcode: exact sign flip, used by the scalar workload's `neg`.
+    NegF = 62,
+    RefOpCount = 63
+};
+
+// ValueType (src/engine/eval/types.rs:1-10)
+enum ValueType : uint8_t { VtFloat = 0, VtInt = 1, VtBool = 2, VtVec2 = 3, VtVec3 = 4, VtMat2 = 5, VtMat3 = 6 };
+
+static inline int vt_size(int vt) {
+    switch (vt) {
+        case VtFloat: return 1;
+        case VtVec2: return 2;
+        case VtVec3: return 3;
+        case VtMat2: return 4;
+        case VtMat3: return 9;
+        default: return 0;
+    }
+}
+
+// Device micro-ops.  Encoding (two dwords):
+//   w0 = op | d << 8 | a << 16 | b << 24
+//   w1 = c | e << 8 | flags << 16          (IMM: w1 = flat parameter index)
+// Semantics (every arithmetic step is one IEEE binary32 operation, left to
+// right, never contracted into an FMA):
+enum UOp : uint8_t {
+    U_IMM = 0,    // d = params[w1]
+    U_ADD,        // d = a + b
+    U_SUB,        // d = a - b
+    U_MUL,        // d = a * b
+    U_DIV,        // d = a / b
+    U_SQR,        // d = a * a
+    U_SQRT,       // d = sqrt(a)
+    U_SIN,        // d = sin(a)
+    U_COS,        // d = cos(a)
+    U_EXP,        // d = exp(a)
+    U_LN,         // d = ln(a)
+    U_NEG,        // d = -a
+    U_MULADD,     // d = (a * b) + (c * e)
+    U_MULSUB,     // d = (a * b) - (c * e)
+    U_ADDMUL,     // d = a + (b * c)
+    U_SQRTACC,    // d = sqrt(a + (b * b))
+    U_ADD3,       // d = (a + b) + c
+    U_RCPSAFE,    // d = 1 / ((|a| < 1e-9) ? 1 : a)
+    U_INVSEL,     // d = (|c| < 1e-9) ? K : (A * b), A = flags&1 ? -a : a, K = flags&2 ? 1 : 0
+    U_SPILL,      // LDS slot w1 <- a (d = a: the register is rewritten with its own value)
+    U_FILL,       // d <- LDS slot w1
+    U_COUNT
+};
+
+// LDS spill slots available to one program (per point of a lane, per channel).
+constexpr int kMaxSpillSlots = 48;
+
+// Register file geometry: 32 f32 components per point per lane.  Dataset
+// feature components occupy registers [0, F); temporaries use [F, 32).
+constexpr int kRegs = 32;
+
+// per-program lowering status (mirrors RSR_PROG_* in include/rsr_amd.h)
+enum ProgStatus : int32_t {
+    PROG_OK = 0,
+    PROG_STACK = -1,
+    PROG_OPCODE = -2,
+    PROG_FEATURE = -3,
+    PROG_CONST = -4,
+    PROG_ROOT = -5,
+    PROG_REGS = -6,
+};
+
+struct UInstr {
+    uint32_t w0, w1;
+};
+
+// Per-program descriptor consumed by the kernels.
+struct ProgDesc {
+    uint32_t code_off;   // first micro-op in the code array
+    uint32_t code_len;   // number of micro-ops
+    uint32_t param_off;  // first flat parameter in the params array
+    uint32_t out_reg;    // register holding the Float result
+};
+
+}  // namespace rsr

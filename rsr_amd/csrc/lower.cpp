@@ -1,0 +1,607 @@
+// lower.cpp -- postfix Instruction stream -> register-slot micro-code.
+//
+// The reference interprets a typed-stack machine (src/engine/eval/evaluator.rs:20-150,
+// VmState in src/engine/eval/state.rs:12-49).  A postfix program has no branches, so the
+// depth of every typed stack is a static function of the pc.  The lowering therefore
+// simulates the five typed stacks once on the host, with each stack entry holding the
+// register numbers of its components, and emits only the arithmetic:
+//   * LoadVar*, MakeVec*, MakeMat*, Get*, Transpose* (and Inverse* in the dual variant)
+//     emit nothing: they re-label registers (feature components are resident in
+//     registers [0, F) for the whole tile);
+//   * every other reference op expands into per-component micro-ops whose arithmetic is
+//     the reference formula, operation for operation (src/domains/basic/eval.rs,
+//     src/domains/linalg/eval.rs, and for the dual variant src/engine/eval/autodiff.rs);
+//   * destinations are assigned by a per-op linear scan that reuses a component register
+//     only after its last read inside the op, so multi-output ops (CrossV3, MulM3, ...)
+//     never read a clobbered input;
+//   * when the register file is full, the oldest value still on a stack (the one consumed
+//     last in postfix order) is spilled to a per-lane LDS slot and filled back when its
+//     consuming op pops it.
+// Validation replaces the reference's undefined behaviour with status codes: stack
+// underflow or depth > 32 (state.rs:14), out-of-range feature or constant indices,
+// constant type mismatch, Bool/Int loads (silent no-ops in the reference,
+// evaluator.rs:144), and programs that do not reduce to exactly one Float.
+#include "lower.h"
+#include <algorithm>
+#include <array>
+
+namespace rsr {
+namespace {
+
+struct Val {
+    uint8_t n = 0;
+    uint8_t reg[9] = {0};
+    uint8_t slot[9] = {0};   // LDS slot of a spilled component
+    uint16_t spilled = 0;    // bit k: component k lives in LDS slot[k]
+    uint32_t seq = 0;        // push order (older values are consumed later)
+};
+
+struct Src {
+    bool virt;
+    int id;  // physical register or virtual id
+};
+
+struct MI {
+    UOp op;
+    int ns;
+    Src s[4];
+    uint32_t w1extra;  // IMM parameter index or flags (<< 16)
+};
+
+constexpr int kMaxDepth = 32;
+constexpr int kInf = 1 << 30;
+
+class Lowerer {
+   public:
+    Lowerer(const RefProgram& p, int F, bool dual) : p_(p), F_(F), dual_(dual) {
+        for (int r = F; r < kRegs; ++r) free_[r] = true;
+        for (int s = 0; s < kMaxSpillSlots; ++s) slot_free_[s] = true;
+        // constant payload offsets and flat parameter starts
+        int off = 0, flat = 0;
+        coff_.resize(p.n_consts);
+        cflat_.resize(p.n_consts);
+        for (int i = 0; i < p.n_consts; ++i) {
+            coff_[i] = off;
+            cflat_[i] = flat;
+            int s = vt_size(p.const_types[i]);
+            off += s ? s : 1;
+            flat += s;
+        }
+        out_.n_params = (uint32_t)flat;
+    }
+
+    Lowered run() {
+        if (F_ < 0 || F_ > kRegs) return fail(PROG_FEATURE);
+        for (int pc = 0; pc < p_.n_code; ++pc) {
+            int st = step(p_.ops[pc], p_.args[pc]);
+            if (st == PROG_OK && !ok_) st = PROG_REGS;
+            if (st != PROG_OK) return fail(st);
+        }
+        if (stk_[0].size() != 1 || stk_[1].size() || stk_[2].size() || stk_[3].size() || stk_[4].size())
+            return fail(PROG_ROOT);
+        Val root = pop(VtFloat);  // fills the root if it was spilled
+        if (!ok_) return fail(PROG_REGS);
+        out_.out_reg = root.reg[0];
+        out_.status = PROG_OK;
+        return std::move(out_);
+    }
+
+   private:
+    const RefProgram& p_;
+    int F_;
+    bool dual_;
+    bool free_[kRegs] = {false};
+    bool slot_free_[kMaxSpillSlots];
+    bool ok_ = true;
+    uint32_t seq_ = 0;
+    std::vector<Val> stk_[5];  // F, V2, V3, M2, M3
+    std::vector<int> coff_, cflat_;
+    Lowered out_;
+
+    Lowered fail(int st) {
+        Lowered l;
+        l.status = st;
+        return l;
+    }
+
+    static int sidx(int vt) {
+        switch (vt) {
+            case VtFloat: return 0;
+            case VtVec2: return 1;
+            case VtVec3: return 2;
+            case VtMat2: return 3;
+            default: return 4;
+        }
+    }
+    bool can_pop(int vt, int n = 1) { return (int)stk_[sidx(vt)].size() >= n; }
+    // Pop a value and bring spilled components back into registers.  A failed fill
+    // leaves ok_ false (reported as PROG_REGS).
+    Val pop(int vt) {
+        Val v = stk_[sidx(vt)].back();
+        stk_[sidx(vt)].pop_back();
+        for (int k = 0; k < v.n; ++k)
+            if (v.spilled & (1u << k)) {
+                int r = alloc();
+                if (r < 0) { ok_ = false; return v; }
+                UInstr u;
+                u.w0 = (uint32_t)U_FILL | (uint32_t)r << 8;
+                u.w1 = v.slot[k];
+                out_.code.push_back(u);
+                slot_free_[v.slot[k]] = true;
+                v.reg[k] = (uint8_t)r;
+                v.spilled &= (uint16_t)~(1u << k);
+            }
+        return v;
+    }
+    bool push(int vt, Val v) {
+        if ((int)stk_[sidx(vt)].size() >= kMaxDepth) return false;
+        v.seq = ++seq_;
+        stk_[sidx(vt)].push_back(v);
+        return true;
+    }
+    // Spill one temp component of the oldest stacked value; false if none can be spilled.
+    bool spill_one() {
+        Val* victim = nullptr;
+        for (auto& st : stk_)
+            for (auto& v : st) {
+                bool has_temp = false;
+                for (int k = 0; k < v.n; ++k)
+                    if (!(v.spilled & (1u << k)) && is_temp(v.reg[k])) has_temp = true;
+                if (has_temp && (!victim || v.seq < victim->seq)) victim = &v;
+            }
+        if (!victim) return false;
+        int slot = -1;
+        for (int s = 0; s < kMaxSpillSlots; ++s)
+            if (slot_free_[s]) { slot = s; break; }
+        if (slot < 0) return false;
+        for (int k = 0; k < victim->n; ++k) {
+            if ((victim->spilled & (1u << k)) || !is_temp(victim->reg[k])) continue;
+            const int r = victim->reg[k];
+            UInstr u;
+            u.w0 = (uint32_t)U_SPILL | (uint32_t)r << 8 | (uint32_t)r << 16;
+            u.w1 = (uint32_t)slot;
+            out_.code.push_back(u);
+            slot_free_[slot] = false;
+            if ((uint32_t)slot + 1 > out_.spill_slots) out_.spill_slots = slot + 1;
+            victim->slot[k] = (uint8_t)slot;
+            victim->spilled |= (uint16_t)(1u << k);
+            free_[r] = true;
+            return true;  // one component per call
+        }
+        return false;
+    }
+    bool is_temp(int r) const { return r >= F_; }
+    void release(int r) {
+        if (is_temp(r)) free_[r] = true;
+    }
+    void release_val(const Val& v) {
+        for (int k = 0; k < v.n; ++k) release(v.reg[k]);
+    }
+    int alloc() {
+        for (;;) {
+            for (int r = F_; r < kRegs; ++r)
+                if (free_[r]) {
+                    free_[r] = false;
+                    if ((uint32_t)r + 1 > out_.max_reg) out_.max_reg = r + 1;
+                    return r;
+                }
+            if (!spill_one()) return -1;
+        }
+    }
+
+    // Emit a micro-op sequence.  `owned` lists the operand components consumed by the
+    // reference op; `outs` names the result components (virtual ids or physical
+    // pass-through registers).  Returns false when the register file is exhausted.
+    bool emit(const std::vector<MI>& seq, const std::vector<int>& owned, const std::vector<Src>& outs,
+              Val& result) {
+        const int n = (int)seq.size();
+        std::vector<int> vlast(n, -1), vphys(n, -1);
+        std::array<int, kRegs> plast;
+        plast.fill(-2);  // -2: not owned
+        for (int r : owned)
+            if (is_temp(r)) plast[r] = -1;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < seq[i].ns; ++k) {
+                const Src& s = seq[i].s[k];
+                if (s.virt) vlast[s.id] = i;
+                else if (plast[s.id] != -2) plast[s.id] = i;
+            }
+        for (const Src& o : outs) {
+            if (o.virt) vlast[o.id] = kInf;
+            else if (plast[o.id] != -2) plast[o.id] = kInf;
+        }
+        // owned registers never read and not passed through die immediately
+        for (int r = 0; r < kRegs; ++r)
+            if (plast[r] == -1) { free_[r] = true; plast[r] = -2; }
+        for (int i = 0; i < n; ++i) {
+            for (int r = 0; r < kRegs; ++r)
+                if (plast[r] == i) { free_[r] = true; plast[r] = -2; }
+            for (int v = 0; v < i; ++v)
+                if (vlast[v] == i && vphys[v] >= 0) free_[vphys[v]] = true;
+            int d = alloc();
+            if (d < 0) return false;
+            vphys[i] = d;
+            const MI& m = seq[i];
+            auto reg = [&](int k) -> uint32_t {
+                if (k >= m.ns) return 0;
+                const Src& s = m.s[k];
+                return (uint32_t)(s.virt ? vphys[s.id] : s.id);
+            };
+            UInstr u;
+            u.w0 = (uint32_t)m.op | (uint32_t)d << 8 | reg(0) << 16 | reg(1) << 24;
+            if (m.op == U_IMM) u.w1 = m.w1extra;
+            else u.w1 = reg(2) | reg(3) << 8 | m.w1extra;
+            out_.code.push_back(u);
+        }
+        // virtual values with no reads and no output role (none are generated) are freed
+        for (int v = 0; v < n; ++v)
+            if (vlast[v] == -1 && vphys[v] >= 0) free_[vphys[v]] = true;
+        result.n = (uint8_t)outs.size();
+        for (size_t k = 0; k < outs.size(); ++k)
+            result.reg[k] = (uint8_t)(outs[k].virt ? vphys[outs[k].id] : outs[k].id);
+        return true;
+    }
+
+    static Src P(int r) { return Src{false, r}; }
+    static Src V(int id) { return Src{true, id}; }
+    static MI mi(UOp op, std::initializer_list<Src> s, uint32_t extra = 0) {
+        MI m;
+        m.op = op;
+        m.ns = (int)s.size();
+        int k = 0;
+        for (const Src& x : s) m.s[k++] = x;
+        m.w1extra = extra;
+        return m;
+    }
+
+    static std::vector<int> regs_of(std::initializer_list<const Val*> vs) {
+        std::vector<int> r;
+        for (const Val* v : vs)
+            for (int k = 0; k < v->n; ++k) r.push_back(v->reg[k]);
+        return r;
+    }
+
+    int load_var(int vt, int idx) {
+        int n = vt_size(vt);
+        if (idx + n > F_) return PROG_FEATURE;
+        Val v;
+        v.n = (uint8_t)n;
+        for (int k = 0; k < n; ++k) v.reg[k] = (uint8_t)(idx + k);
+        return push(vt, v) ? PROG_OK : PROG_STACK;
+    }
+
+    int load_const(int vt, int c) {
+        if (c >= p_.n_consts) return PROG_CONST;
+        if (p_.const_types[c] != vt) return PROG_CONST;
+        int n = vt_size(vt);
+        if (vt != VtFloat) out_.has_vecmat_const = true;
+        else out_.n_float_consts++;
+        std::vector<MI> seq;
+        std::vector<Src> outs;
+        for (int k = 0; k < n; ++k) {
+            seq.push_back(mi(U_IMM, {}, (uint32_t)(cflat_[c] + k)));
+            outs.push_back(V(k));
+        }
+        Val r;
+        if (!emit(seq, {}, outs, r)) return PROG_REGS;
+        return push(vt, r) ? PROG_OK : PROG_STACK;
+    }
+
+    // componentwise binary op over n components: out_k = a_k op b_k
+    int cwise(int vt, UOp op) {
+        if (!can_pop(vt, 2)) return PROG_STACK;
+        Val b = pop(vt), a = pop(vt);
+        std::vector<MI> seq;
+        std::vector<Src> outs;
+        for (int k = 0; k < a.n; ++k) {
+            seq.push_back(mi(op, {P(a.reg[k]), P(b.reg[k])}));
+            outs.push_back(V(k));
+        }
+        Val r;
+        if (!emit(seq, regs_of({&a, &b}), outs, r)) return PROG_REGS;
+        return push(vt, r) ? PROG_OK : PROG_STACK;
+    }
+
+    // scale: (F s, T v) -> T, out_k = v_k * s (linalg/eval.rs:106-118, 163-175, 245-257, 384-406)
+    int scale(int vt) {
+        if (!can_pop(VtFloat) || !can_pop(vt)) return PROG_STACK;
+        Val s = pop(VtFloat), v = pop(vt);
+        std::vector<MI> seq;
+        std::vector<Src> outs;
+        for (int k = 0; k < v.n; ++k) {
+            seq.push_back(mi(U_MUL, {P(v.reg[k]), P(s.reg[0])}));
+            outs.push_back(V(k));
+        }
+        Val r;
+        if (!emit(seq, regs_of({&s, &v}), outs, r)) return PROG_REGS;
+        return push(vt, r) ? PROG_OK : PROG_STACK;
+    }
+
+    int unary_f(UOp op) {
+        if (!can_pop(VtFloat)) return PROG_STACK;
+        Val a = pop(VtFloat);
+        Val r;
+        if (!emit({mi(op, {P(a.reg[0])})}, regs_of({&a}), {V(0)}, r)) return PROG_REGS;
+        return push(VtFloat, r) ? PROG_OK : PROG_STACK;
+    }
+
+    int binary_f(UOp op) { return cwise(VtFloat, op); }
+
+    // generic "consume operands, produce one value" helper
+    int produce(int out_vt, const std::vector<MI>& seq, const std::vector<int>& owned,
+                const std::vector<Src>& outs) {
+        Val r;
+        if (!emit(seq, owned, outs, r)) return PROG_REGS;
+        return push(out_vt, r) ? PROG_OK : PROG_STACK;
+    }
+
+    int step(int op, int arg) {
+        switch (op) {
+            case LoadVarF: return load_var(VtFloat, arg);
+            case LoadVarV2: return load_var(VtVec2, arg);
+            case LoadVarV3: return load_var(VtVec3, arg);
+            case LoadVarM2: return load_var(VtMat2, arg);
+            case LoadVarM3: return load_var(VtMat3, arg);
+            case LoadConstF: return load_const(VtFloat, arg);
+            case LoadConstV2: return load_const(VtVec2, arg);
+            case LoadConstV3: return load_const(VtVec3, arg);
+            case LoadConstM2: return load_const(VtMat2, arg);
+            case LoadConstM3: return load_const(VtMat3, arg);
+            case LoadVarB: case LoadConstB: case LoadVarI: case LoadConstI: return PROG_OPCODE;
+
+            case AddF: return binary_f(U_ADD);
+            case SubF: return binary_f(U_SUB);
+            case MulF: return binary_f(U_MUL);
+            case DivF: return binary_f(U_DIV);
+            case SinF: return unary_f(U_SIN);
+            case CosF: return unary_f(U_COS);
+            case ExpF: return unary_f(U_EXP);
+            case SqrF: return unary_f(U_SQR);
+            case SqrtF: return unary_f(U_SQRT);
+            case LnF: return unary_f(U_LN);
+            case NegF: return unary_f(U_NEG);
+
+            case MakeVec2: case MakeVec3: {
+                int n = op == MakeVec2 ? 2 : 3;
+                if (!can_pop(VtFloat, n)) return PROG_STACK;
+                Val r;
+                r.n = (uint8_t)n;
+                for (int k = n - 1; k >= 0; --k) r.reg[k] = pop(VtFloat).reg[0];
+                return push(op == MakeVec2 ? VtVec2 : VtVec3, r) ? PROG_OK : PROG_STACK;
+            }
+            case GetXV2: case GetYV2: case GetXV3: case GetYV3: case GetZV3: {
+                int vt = (op == GetXV2 || op == GetYV2) ? VtVec2 : VtVec3;
+                int k = op == GetXV2 ? 0 : op == GetYV2 ? 1 : op - GetXV3;
+                if (!can_pop(vt)) return PROG_STACK;
+                Val v = pop(vt);
+                for (int j = 0; j < v.n; ++j)
+                    if (j != k) release(v.reg[j]);
+                Val r;
+                r.n = 1;
+                r.reg[0] = v.reg[k];
+                return push(VtFloat, r) ? PROG_OK : PROG_STACK;
+            }
+            case AddV2: return cwise(VtVec2, U_ADD);
+            case SubV2: return cwise(VtVec2, U_SUB);
+            case AddV3: return cwise(VtVec3, U_ADD);
+            case SubV3: return cwise(VtVec3, U_SUB);
+            case AddM2: return cwise(VtMat2, U_ADD);
+            case SubM2: return cwise(VtMat2, U_SUB);
+            case AddM3: return cwise(VtMat3, U_ADD);
+            case SubM3: return cwise(VtMat3, U_SUB);
+            case ScaleV2: return scale(VtVec2);
+            case ScaleV3: return scale(VtVec3);
+            case ScaleM2: return scale(VtMat2);
+            case ScaleM3: return scale(VtMat3);
+
+            case DotV2: {  // (a0*b0) + (a1*b1)
+                if (!can_pop(VtVec2, 2)) return PROG_STACK;
+                Val b = pop(VtVec2), a = pop(VtVec2);
+                return produce(VtFloat, {mi(U_MULADD, {P(a.reg[0]), P(b.reg[0]), P(a.reg[1]), P(b.reg[1])})},
+                               regs_of({&a, &b}), {V(0)});
+            }
+            case DotV3: {  // ((a0*b0) + (a1*b1)) + (a2*b2)
+                if (!can_pop(VtVec3, 2)) return PROG_STACK;
+                Val b = pop(VtVec3), a = pop(VtVec3);
+                return produce(VtFloat,
+                               {mi(U_MULADD, {P(a.reg[0]), P(b.reg[0]), P(a.reg[1]), P(b.reg[1])}),
+                                mi(U_ADDMUL, {V(0), P(a.reg[2]), P(b.reg[2])})},
+                               regs_of({&a, &b}), {V(1)});
+            }
+            case NormV2: {  // sqrt((v0*v0) + (v1*v1))
+                if (!can_pop(VtVec2)) return PROG_STACK;
+                Val v = pop(VtVec2);
+                return produce(VtFloat,
+                               {mi(U_MULADD, {P(v.reg[0]), P(v.reg[0]), P(v.reg[1]), P(v.reg[1])}),
+                                mi(U_SQRT, {V(0)})},
+                               regs_of({&v}), {V(1)});
+            }
+            case NormV3: {  // sqrt(((v0*v0) + (v1*v1)) + (v2*v2))
+                if (!can_pop(VtVec3)) return PROG_STACK;
+                Val v = pop(VtVec3);
+                return produce(VtFloat,
+                               {mi(U_MULADD, {P(v.reg[0]), P(v.reg[0]), P(v.reg[1]), P(v.reg[1])}),
+                                mi(U_SQRTACC, {V(0), P(v.reg[2])})},
+                               regs_of({&v}), {V(1)});
+            }
+            case CrossV3: {
+                if (!can_pop(VtVec3, 2)) return PROG_STACK;
+                Val b = pop(VtVec3), a = pop(VtVec3);
+                auto A = [&](int k) { return P(a.reg[k]); };
+                auto B = [&](int k) { return P(b.reg[k]); };
+                return produce(VtVec3,
+                               {mi(U_MULSUB, {A(1), B(2), A(2), B(1)}), mi(U_MULSUB, {A(2), B(0), A(0), B(2)}),
+                                mi(U_MULSUB, {A(0), B(1), A(1), B(0)})},
+                               regs_of({&a, &b}), {V(0), V(1), V(2)});
+            }
+            case MakeMat2: {  // columns c0, c1 -> [c0x, c0y, c1x, c1y]
+                if (!can_pop(VtVec2, 2)) return PROG_STACK;
+                Val c1 = pop(VtVec2), c0 = pop(VtVec2);
+                Val r;
+                r.n = 4;
+                r.reg[0] = c0.reg[0]; r.reg[1] = c0.reg[1]; r.reg[2] = c1.reg[0]; r.reg[3] = c1.reg[1];
+                return push(VtMat2, r) ? PROG_OK : PROG_STACK;
+            }
+            case MakeMat3: {
+                if (!can_pop(VtVec3, 3)) return PROG_STACK;
+                Val c2 = pop(VtVec3), c1 = pop(VtVec3), c0 = pop(VtVec3);
+                Val r;
+                r.n = 9;
+                for (int k = 0; k < 3; ++k) { r.reg[k] = c0.reg[k]; r.reg[3 + k] = c1.reg[k]; r.reg[6 + k] = c2.reg[k]; }
+                return push(VtMat3, r) ? PROG_OK : PROG_STACK;
+            }
+            case MulM2: {  // out[r+2c] = a[r]*b[2c] + a[r+2]*b[2c+1]
+                if (!can_pop(VtMat2, 2)) return PROG_STACK;
+                Val b = pop(VtMat2), a = pop(VtMat2);
+                std::vector<MI> seq;
+                for (int c = 0; c < 2; ++c)
+                    for (int r = 0; r < 2; ++r)
+                        seq.push_back(mi(U_MULADD, {P(a.reg[r]), P(b.reg[2 * c]), P(a.reg[r + 2]), P(b.reg[2 * c + 1])}));
+                return produce(VtMat2, seq, regs_of({&a, &b}), {V(0), V(1), V(2), V(3)});
+            }
+            case MulM2V2: {
+                if (!can_pop(VtMat2) || !can_pop(VtVec2)) return PROG_STACK;
+                Val m = pop(VtMat2), v = pop(VtVec2);
+                return produce(VtVec2,
+                               {mi(U_MULADD, {P(m.reg[0]), P(v.reg[0]), P(m.reg[2]), P(v.reg[1])}),
+                                mi(U_MULADD, {P(m.reg[1]), P(v.reg[0]), P(m.reg[3]), P(v.reg[1])})},
+                               regs_of({&m, &v}), {V(0), V(1)});
+            }
+            case DetM2: {
+                if (!can_pop(VtMat2)) return PROG_STACK;
+                Val m = pop(VtMat2);
+                return produce(VtFloat, {mi(U_MULSUB, {P(m.reg[0]), P(m.reg[3]), P(m.reg[1]), P(m.reg[2])})},
+                               regs_of({&m}), {V(0)});
+            }
+            case TraceM2: {
+                if (!can_pop(VtMat2)) return PROG_STACK;
+                Val m = pop(VtMat2);
+                return produce(VtFloat, {mi(U_ADD, {P(m.reg[0]), P(m.reg[3])})}, regs_of({&m}), {V(0)});
+            }
+            case TransposeM2: {
+                if (!can_pop(VtMat2)) return PROG_STACK;
+                Val m = pop(VtMat2);
+                std::swap(m.reg[1], m.reg[2]);
+                return push(VtMat2, m) ? PROG_OK : PROG_STACK;
+            }
+            case InverseM2: {
+                out_.differentiable = false;
+                if (!can_pop(VtMat2)) return PROG_STACK;
+                if (dual_) return PROG_OK;  // dual inverse: stack unchanged
+                Val m = pop(VtMat2);
+                auto M = [&](int k) { return P(m.reg[k]); };
+                // det, inv_d, then blended entries (linalg/eval.rs:316-329)
+                return produce(VtMat2,
+                               {mi(U_MULSUB, {M(0), M(3), M(1), M(2)}),          // v0 det
+                                mi(U_RCPSAFE, {V(0)}),                            // v1 inv_d
+                                mi(U_INVSEL, {M(3), V(1), V(0)}, 2u << 16),       // K=1
+                                mi(U_INVSEL, {M(1), V(1), V(0)}, 1u << 16),       // -m1, K=0
+                                mi(U_INVSEL, {M(2), V(1), V(0)}, 1u << 16),       // -m2, K=0
+                                mi(U_INVSEL, {M(0), V(1), V(0)}, 2u << 16)},      // K=1
+                               regs_of({&m}), {V(2), V(3), V(4), V(5)});
+            }
+            case MulM3: {  // out[r+3c] = (a[r]*b[3c] + a[r+3]*b[3c+1]) + a[r+6]*b[3c+2]
+                if (!can_pop(VtMat3, 2)) return PROG_STACK;
+                Val b = pop(VtMat3), a = pop(VtMat3);
+                std::vector<MI> seq;
+                std::vector<Src> outs;
+                for (int c = 0; c < 3; ++c)
+                    for (int r = 0; r < 3; ++r) {
+                        int t = (int)seq.size();
+                        seq.push_back(mi(U_MULADD, {P(a.reg[r]), P(b.reg[3 * c]), P(a.reg[r + 3]), P(b.reg[3 * c + 1])}));
+                        seq.push_back(mi(U_ADDMUL, {V(t), P(a.reg[r + 6]), P(b.reg[3 * c + 2])}));
+                        outs.push_back(V(t + 1));
+                    }
+                return produce(VtMat3, seq, regs_of({&a, &b}), outs);
+            }
+            case MulM3V3: {
+                if (!can_pop(VtMat3) || !can_pop(VtVec3)) return PROG_STACK;
+                Val m = pop(VtMat3), v = pop(VtVec3);
+                std::vector<MI> seq;
+                std::vector<Src> outs;
+                for (int r = 0; r < 3; ++r) {
+                    int t = (int)seq.size();
+                    seq.push_back(mi(U_MULADD, {P(m.reg[r]), P(v.reg[0]), P(m.reg[r + 3]), P(v.reg[1])}));
+                    seq.push_back(mi(U_ADDMUL, {V(t), P(m.reg[r + 6]), P(v.reg[2])}));
+                    outs.push_back(V(t + 1));
+                }
+                return produce(VtVec3, seq, regs_of({&m, &v}), outs);
+            }
+            case DetM3: {
+                if (!can_pop(VtMat3)) return PROG_STACK;
+                Val m = pop(VtMat3);
+                auto M = [&](int k) { return P(m.reg[k]); };
+                return produce(VtFloat,
+                               {mi(U_MULSUB, {M(4), M(8), M(5), M(7)}),  // t1
+                                mi(U_MULSUB, {M(1), M(8), M(2), M(7)}),  // t2
+                                mi(U_MULSUB, {M(1), M(5), M(2), M(4)}),  // t3
+                                mi(U_MULSUB, {M(0), V(0), M(3), V(1)}),  // m0*t1 - m3*t2
+                                mi(U_ADDMUL, {V(3), M(6), V(2)})},       // + m6*t3
+                               regs_of({&m}), {V(4)});
+            }
+            case TraceM3: {
+                if (!can_pop(VtMat3)) return PROG_STACK;
+                Val m = pop(VtMat3);
+                return produce(VtFloat, {mi(U_ADD3, {P(m.reg[0]), P(m.reg[4]), P(m.reg[8])})}, regs_of({&m}), {V(0)});
+            }
+            case TransposeM3: {
+                if (!can_pop(VtMat3)) return PROG_STACK;
+                Val m = pop(VtMat3);
+                Val t = m;
+                static const int perm[9] = {0, 3, 6, 1, 4, 7, 2, 5, 8};
+                for (int k = 0; k < 9; ++k) t.reg[k] = m.reg[perm[k]];
+                return push(VtMat3, t) ? PROG_OK : PROG_STACK;
+            }
+            case InverseM3: {
+                out_.differentiable = false;
+                if (!can_pop(VtMat3)) return PROG_STACK;
+                if (dual_) return PROG_OK;
+                Val m = pop(VtMat3);
+                auto M = [&](int k) { return P(m.reg[k]); };
+                // det (same sequence as DetM3), inv_d, adjugate * inv_d blended (linalg/eval.rs:476-506)
+                std::vector<MI> seq = {
+                    mi(U_MULSUB, {M(4), M(8), M(5), M(7)}),  // 0 t1 = adj0 numerator
+                    mi(U_MULSUB, {M(1), M(8), M(2), M(7)}),  // 1 t2 (adj1 = -t2)
+                    mi(U_MULSUB, {M(1), M(5), M(2), M(4)}),  // 2 t3 = adj2 numerator
+                    mi(U_MULSUB, {M(0), V(0), M(3), V(1)}),  // 3
+                    mi(U_ADDMUL, {V(3), M(6), V(2)}),        // 4 det
+                    mi(U_RCPSAFE, {V(4)}),                   // 5 inv_d
+                    mi(U_MULSUB, {M(3), M(8), M(5), M(6)}),  // 6 adj3 = -(.)
+                    mi(U_MULSUB, {M(0), M(8), M(2), M(6)}),  // 7 adj4
+                    mi(U_MULSUB, {M(0), M(5), M(2), M(3)}),  // 8 adj5 = -(.)
+                    mi(U_MULSUB, {M(3), M(7), M(4), M(6)}),  // 9 adj6
+                    mi(U_MULSUB, {M(0), M(7), M(1), M(6)}),  // 10 adj7 = -(.)
+                    mi(U_MULSUB, {M(0), M(4), M(1), M(3)}),  // 11 adj8
+                };
+                const int num[9] = {0, 1, 2, 6, 7, 8, 9, 10, 11};
+                const bool neg[9] = {false, true, false, true, false, true, false, true, false};
+                std::vector<Src> outs;
+                for (int k = 0; k < 9; ++k) {
+                    uint32_t fl = (neg[k] ? 1u : 0u) | ((k == 0 || k == 4 || k == 8) ? 2u : 0u);
+                    outs.push_back(V((int)seq.size()));
+                    seq.push_back(mi(U_INVSEL, {V(num[k]), V(5), V(4)}, fl << 16));
+                }
+                return produce(VtMat3, seq, regs_of({&m}), outs);
+            }
+            default: return PROG_OPCODE;
+        }
+    }
+};
+
+}  // namespace
+
+Lowered lower_program(const RefProgram& p, int num_features, bool dual) {
+    Lowerer l(p, num_features, dual);
+    return l.run();
+}
+
+void flatten_params(const RefProgram& p, std::vector<float>& out) {
+    int off = 0;
+    for (int i = 0; i < p.n_consts; ++i) {
+        int s = vt_size(p.const_types[i]);
+        for (int k = 0; k < s; ++k) out.push_back(p.const_values[off + k]);
+        off += s ? s : 1;
+    }
+}
+
+}  // namespace rsr

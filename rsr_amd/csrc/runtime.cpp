@@ -1,0 +1,778 @@
+// runtime.cpp -- C ABI implementation: contexts, device-resident datasets and program
+// batches, lowering cache, and the evaluation entry points.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/rsr_amd.h"
+#include "kernels.h"
+#include "lower.h"
+#include "runtime_internal.h"
+
+using namespace rsr;
+
+// ---------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+int rsr::set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int rsr::hip_error(hipError_t e, const char* what) {
+    return set_error(e == hipErrorOutOfMemory ? RSR_E_OOM : RSR_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                              \
+    do {                                                 \
+        hipError_t e_ = (expr);                          \
+        if (e_ != hipSuccess) return hip_error(e_, what); \
+    } while (0)
+
+extern "C" const char* rsr_error_string(int code) {
+    switch (code) {
+        case RSR_OK: return "ok";
+        case RSR_E_INVALID_ARG: return "invalid argument";
+        case RSR_E_HIP: return "HIP runtime error";
+        case RSR_E_OOM: return "device out of memory";
+        case RSR_E_BUDGET: return "HBM budget exceeded";
+        case RSR_E_PROGRAM: return "program rejected by lowering";
+        case RSR_E_NO_DEVICE: return "no HIP device";
+        case RSR_E_UNSUPPORTED: return "unsupported";
+        case RSR_E_QUEUE_FULL: return "request queue full";
+        case RSR_E_TIMEOUT: return "timed out";
+        case RSR_E_NOT_READY: return "not ready";
+        default: return "unknown error";
+    }
+}
+
+extern "C" const char* rsr_last_error_message(void) { return g_last_error.c_str(); }
+extern "C" const char* rsr_version(void) { return "rsr_amd 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------------------------
+// device buffers with budget accounting
+// ---------------------------------------------------------------------------------
+int DevBuf::alloc(rsr_context* c, size_t n) {
+    release();
+    if (n == 0) return RSR_OK;
+    if (c->used + n > c->budget)
+        return set_error(RSR_E_BUDGET, "allocation of %zu bytes exceeds the HBM budget (%llu of %llu used)", n,
+                         (unsigned long long)c->used, (unsigned long long)c->budget);
+    hipError_t e = hipMalloc(&ptr, n);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return hip_error(e, "hipMalloc");
+    }
+    ctx = c;
+    bytes = n;
+    c->used += n;
+    return RSR_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) {
+        (void)hipFree(ptr);
+        if (ctx) ctx->used -= bytes;
+    }
+    ptr = nullptr;
+    bytes = 0;
+    ctx = nullptr;
+}
+
+int DevBuf::upload(rsr_context* c, const void* src, size_t n) {
+    int rc = alloc(c, n);
+    if (rc) return rc;
+    if (n) HIP_TRY(hipMemcpyAsync(ptr, src, n, hipMemcpyHostToDevice, c->stream), "upload");
+    return RSR_OK;
+}
+
+int rsr::ensure_scratch(rsr_context* c, DevBuf& b, size_t n) {
+    if (b.bytes >= n) return RSR_OK;
+    return b.alloc(c, n + n / 4);
+}
+
+// ---------------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------------
+extern "C" int rsr_context_create(int device_ordinal, uint64_t hbm_budget_bytes, rsr_context** out) {
+    if (!out) return set_error(RSR_E_INVALID_ARG, "out is null");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(RSR_E_NO_DEVICE, "no HIP device");
+    if (device_ordinal < 0 || device_ordinal >= ndev)
+        return set_error(RSR_E_INVALID_ARG, "device %d out of range (%d devices)", device_ordinal, ndev);
+    HIP_TRY(hipSetDevice(device_ordinal), "hipSetDevice");
+    auto c = std::make_unique<rsr_context>();
+    c->device = device_ordinal;
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(free_b * 0.9);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device_ordinal) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    *out = c.release();
+    return RSR_OK;
+}
+
+// A fresh pair of events for the next main-kernel launch when profiling is on.
+const KernelTimer* rsr_context::next_timer() {
+    if (!profiling) return nullptr;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return nullptr;
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return nullptr;
+    }
+    prof_events.emplace_back(a, b);
+    timer_slot.begin = a;
+    timer_slot.end = b;
+    return &timer_slot;
+}
+
+static void drain_profile(rsr_context* ctx) {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& ev : ctx->prof_events) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) {
+            ctx->prof_ms += ms;
+            ctx->prof_launches++;
+        }
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    ctx->prof_events.clear();
+}
+
+extern "C" int rsr_context_profile(rsr_context* ctx, int enable) {
+    if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    drain_profile(ctx);
+    ctx->profiling = enable != 0;
+    ctx->prof_ms = 0.0;
+    ctx->prof_launches = 0;
+    return RSR_OK;
+}
+
+extern "C" int rsr_context_profile_read(rsr_context* ctx, double* kernel_ms, int64_t* launches) {
+    if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    drain_profile(ctx);
+    if (kernel_ms) *kernel_ms = ctx->prof_ms;
+    if (launches) *launches = ctx->prof_launches;
+    return RSR_OK;
+}
+
+extern "C" int rsr_context_synchronize(rsr_context* ctx) {
+    if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return RSR_OK;
+}
+
+extern "C" int rsr_context_destroy(rsr_context* ctx) {
+    if (!ctx) return RSR_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    drain_profile(ctx);
+    ctx->scratch_act.release();
+    ctx->scratch_partials.release();
+    ctx->scratch_out.release();
+    ctx->scratch_out2.release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RSR_OK;
+}
+
+extern "C" int rsr_context_memory(const rsr_context* ctx, uint64_t* used_bytes, uint64_t* budget_bytes) {
+    if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
+    if (used_bytes) *used_bytes = ctx->used;
+    if (budget_bytes) *budget_bytes = ctx->budget;
+    return RSR_OK;
+}
+
+extern "C" int rsr_context_set_tuning(rsr_context* ctx, int ppl, int group) {
+    if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
+    if (ppl != 1 && ppl != 2 && ppl != 4) return set_error(RSR_E_INVALID_ARG, "points_per_lane must be 1, 2 or 4");
+    if (group < 1 || group > 4096) return set_error(RSR_E_INVALID_ARG, "programs_per_item out of range");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->tuning.ppl = ppl;
+    ctx->tuning.group = group;
+    return RSR_OK;
+}
+
+extern "C" int rsr_context_get_tuning(const rsr_context* ctx, int* ppl, int* group) {
+    if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
+    if (ppl) *ppl = ctx->tuning.ppl;
+    if (group) *group = ctx->tuning.group;
+    return RSR_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// datasets
+// ---------------------------------------------------------------------------------
+static int dataset_upload(rsr_context* ctx, std::unique_ptr<rsr_dataset>& d) {
+    d->ctx = ctx;
+    const int64_t npad = std::max<int64_t>(256, (d->count8 + 255) / 256 * 256);
+    d->npad = npad;
+    // host SoA padded to npad with zeros (tile padding is masked in the kernels)
+    std::vector<float> f((size_t)std::max(d->num_features, 1) * npad, 0.0f), t((size_t)npad, 0.0f);
+    for (int k = 0; k < d->num_features; ++k)
+        memcpy(&f[(size_t)k * npad], &d->soa[(size_t)k * d->count8], (size_t)d->count8 * sizeof(float));
+    memcpy(t.data(), d->target.data(), (size_t)d->count8 * sizeof(float));
+    int rc = d->dfeat.upload(ctx, f.data(), f.size() * sizeof(float));
+    if (rc) return rc;
+    rc = d->dtarget.upload(ctx, t.data(), t.size() * sizeof(float));
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "dataset upload");
+    return RSR_OK;
+}
+
+DevDataset rsr::dev_view(const rsr_dataset* d) {
+    DevDataset v;
+    v.feat = (const float*)d->dfeat.ptr;
+    v.target = (const float*)d->dtarget.ptr;
+    v.num_features = d->num_features;
+    v.num_samples = d->num_samples;
+    v.count8 = d->count8;
+    v.npad = d->npad;
+    return v;
+}
+
+extern "C" int rsr_dataset_create(rsr_context* ctx, const float* rows, const float* y, int64_t num_samples,
+                                  const uint8_t* feature_types, int n_feature_types, int normalize,
+                                  rsr_dataset** out) {
+    if (!ctx || !out || (!rows && num_samples) || (!y && num_samples) || num_samples < 0 || n_feature_types < 0)
+        return set_error(RSR_E_INVALID_ARG, "invalid dataset arguments");
+    int F = 0;
+    for (int i = 0; i < n_feature_types; ++i) {
+        int s = vt_size(feature_types[i]);
+        F += s ? s : 1;  // dataset.rs:47-54: Int/Bool count one column
+    }
+    if (F > kRegs) return set_error(RSR_E_UNSUPPORTED, "%d feature components exceed the 32-register file", F);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    auto d = std::make_unique<rsr_dataset>();
+    d->num_features = F;
+    d->num_samples = num_samples;
+    d->count8 = (num_samples + 7) / 8 * 8;
+    d->feature_types.assign(feature_types, feature_types + n_feature_types);
+    d->means.assign(F, 0.0f);
+    d->stds.assign(F, 1.0f);
+    const float ns = (float)num_samples;
+    if (normalize && num_samples > 0) {  // dataset.rs:63-94, f32 left-to-right sums
+        for (int f = 0; f < F; ++f) {
+            float sum = 0.0f;
+            for (int64_t i = 0; i < num_samples; ++i) sum += rows[i * F + f];
+            const float m = sum / ns;
+            float var = 0.0f;
+            for (int64_t i = 0; i < num_samples; ++i) {
+                const float dd = rows[i * F + f] - m;
+                var += dd * dd;
+            }
+            var = var / ns;
+            d->means[f] = m;
+            d->stds[f] = var < 1e-9f ? 1.0f : sqrtf(var);
+        }
+        float ts = 0.0f;
+        for (int64_t i = 0; i < num_samples; ++i) ts += y[i];
+        d->tmean = ts / ns;
+        float tv = 0.0f;
+        for (int64_t i = 0; i < num_samples; ++i) {
+            const float dd = y[i] - d->tmean;
+            tv += dd * dd;
+        }
+        tv = tv / ns;
+        d->tstd = tv < 1e-9f ? 1.0f : sqrtf(tv);
+    }
+    d->soa.assign((size_t)F * d->count8, 0.0f);
+    d->target.assign((size_t)d->count8, 0.0f);
+    for (int f = 0; f < F; ++f)
+        for (int64_t i = 0; i < num_samples; ++i)
+            d->soa[(size_t)f * d->count8 + i] = (rows[i * F + f] - d->means[f]) / d->stds[f];
+    for (int64_t i = 0; i < num_samples; ++i) d->target[i] = (y[i] - d->tmean) / d->tstd;
+    int rc = dataset_upload(ctx, d);
+    if (rc) return rc;
+    *out = d.release();
+    return RSR_OK;
+}
+
+extern "C" int rsr_dataset_create_aosoa(rsr_context* ctx, const float* feature_flat, const float* target_batches,
+                                        int num_features, int64_t num_batches, int64_t num_samples,
+                                        rsr_dataset** out) {
+    if (!ctx || !out || num_features < 0 || num_batches < 0 || num_samples < 0 || num_samples > num_batches * 8 ||
+        (num_batches && (!target_batches || (num_features && !feature_flat))))
+        return set_error(RSR_E_INVALID_ARG, "invalid AoSoA dataset arguments");
+    if (num_features > kRegs) return set_error(RSR_E_UNSUPPORTED, "too many feature components");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    auto d = std::make_unique<rsr_dataset>();
+    const int F = num_features;
+    d->num_features = F;
+    d->num_samples = num_samples;
+    d->count8 = num_batches * 8;
+    d->means.assign(F, 0.0f);
+    d->stds.assign(F, 1.0f);
+    d->soa.assign((size_t)F * d->count8, 0.0f);
+    d->target.assign((size_t)d->count8, 0.0f);
+    for (int64_t b = 0; b < num_batches; ++b)
+        for (int l = 0; l < 8; ++l) {
+            for (int f = 0; f < F; ++f) d->soa[(size_t)f * d->count8 + b * 8 + l] = feature_flat[(b * F + f) * 8 + l];
+            d->target[b * 8 + l] = target_batches[b * 8 + l];
+        }
+    int rc = dataset_upload(ctx, d);
+    if (rc) return rc;
+    *out = d.release();
+    return RSR_OK;
+}
+
+extern "C" int rsr_dataset_subset(rsr_context* ctx, const rsr_dataset* src, int64_t target_samples,
+                                  rsr_dataset** out) {
+    if (!ctx || !src || !out || target_samples < 0) return set_error(RSR_E_INVALID_ARG, "invalid subset arguments");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    auto d = std::make_unique<rsr_dataset>();
+    const int F = src->num_features;
+    const int64_t samples = std::min(target_samples, src->num_samples);
+    d->num_features = F;
+    d->feature_types = src->feature_types;
+    d->means = src->means;
+    d->stds = src->stds;
+    d->tmean = src->tmean;
+    d->tstd = src->tstd;
+    if (samples == src->num_samples) {  // clone (dataset.rs:362-364)
+        d->num_samples = src->num_samples;
+        d->count8 = src->count8;
+        d->soa = src->soa;
+        d->target = src->target;
+    } else {
+        d->num_samples = samples;
+        d->count8 = (samples + 7) / 8 * 8;
+        const double step = ((double)src->num_samples - 1.0) / std::max((double)samples - 1.0, 1.0);
+        d->soa.assign((size_t)F * d->count8, 0.0f);
+        d->target.assign((size_t)d->count8, 0.0f);
+        for (int64_t i = 0; i < d->count8; ++i) {
+            const int64_t orig = (int64_t)llround((double)i * step);  // f64 round half away from zero
+            if (orig >= src->num_samples) continue;
+            for (int f = 0; f < F; ++f) d->soa[(size_t)f * d->count8 + i] = src->soa[(size_t)f * src->count8 + orig];
+            d->target[i] = src->target[orig];
+        }
+    }
+    int rc = dataset_upload(ctx, d);
+    if (rc) return rc;
+    *out = d.release();
+    return RSR_OK;
+}
+
+extern "C" int rsr_dataset_info(const rsr_dataset* ds, int* num_features, int64_t* num_samples,
+                                int64_t* num_batches) {
+    if (!ds) return set_error(RSR_E_INVALID_ARG, "ds is null");
+    if (num_features) *num_features = ds->num_features;
+    if (num_samples) *num_samples = ds->num_samples;
+    if (num_batches) *num_batches = ds->count8 / 8;
+    return RSR_OK;
+}
+
+extern "C" int rsr_dataset_stats(const rsr_dataset* ds, float* stats_out) {
+    if (!ds || !stats_out) return set_error(RSR_E_INVALID_ARG, "null argument");
+    const int F = ds->num_features;
+    for (int f = 0; f < F; ++f) {
+        stats_out[f] = ds->means[f];
+        stats_out[F + f] = ds->stds[f];
+    }
+    stats_out[2 * F] = ds->tmean;
+    stats_out[2 * F + 1] = ds->tstd;
+    return RSR_OK;
+}
+
+extern "C" int rsr_dataset_destroy(rsr_dataset* ds) {
+    if (!ds) return RSR_OK;
+    rsr_context* ctx = ds->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete ds;
+    return RSR_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// program batches
+// ---------------------------------------------------------------------------------
+RefProgram rsr::ref_program(const rsr_programs* p, int i) {
+    RefProgram r;
+    r.ops = p->ops.data() + p->code_off[i];
+    r.args = p->args.data() + p->code_off[i];
+    r.n_code = (int)(p->code_off[i + 1] - p->code_off[i]);
+    r.const_types = p->ctypes.data() + p->const_off[i];
+    r.n_consts = (int)(p->const_off[i + 1] - p->const_off[i]);
+    r.const_values = p->values.data() + p->value_off[i];
+    return r;
+}
+
+static void build_params(rsr_programs* p) {
+    p->params.clear();
+    p->param_off.assign(p->n + 1, 0);
+    for (int i = 0; i < p->n; ++i) {
+        p->param_off[i] = (uint32_t)p->params.size();
+        flatten_params(ref_program(p, i), p->params);
+    }
+    p->param_off[p->n] = (uint32_t)p->params.size();
+    p->params.push_back(0.0f);  // never empty
+}
+
+extern "C" int rsr_programs_create(rsr_context* ctx, int n, const uint8_t* ops, const uint16_t* args,
+                                   const int64_t* code_offsets, const uint8_t* const_types,
+                                   const int64_t* const_offsets, const float* const_values,
+                                   const int64_t* value_offsets, rsr_programs** out) {
+    if (!ctx || !out || n < 0 || (n > 0 && (!code_offsets || !const_offsets || !value_offsets)))
+        return set_error(RSR_E_INVALID_ARG, "invalid program batch arguments");
+    auto p = std::make_unique<rsr_programs>();
+    p->ctx = ctx;
+    p->n = n;
+    const int64_t ncode = n ? code_offsets[n] - code_offsets[0] : 0;
+    const int64_t ncons = n ? const_offsets[n] - const_offsets[0] : 0;
+    if (ncode < 0 || ncons < 0 || (ncode && (!ops || !args)) || (ncons && !const_types))
+        return set_error(RSR_E_INVALID_ARG, "invalid offsets");
+    p->code_off.resize(n + 1);
+    p->const_off.resize(n + 1);
+    p->value_off.resize(n + 1);
+    for (int i = 0; i <= n; ++i) {
+        p->code_off[i] = code_offsets[i] - code_offsets[0];
+        p->const_off[i] = const_offsets[i] - const_offsets[0];
+    }
+    for (int i = 0; i < n; ++i) {
+        if (p->code_off[i + 1] < p->code_off[i] || p->const_off[i + 1] < p->const_off[i])
+            return set_error(RSR_E_INVALID_ARG, "offsets of program %d are decreasing", i);
+    }
+    p->ops.assign(ops, ops + ncode);
+    p->args.assign(args, args + ncode);
+    p->ctypes.assign(const_types, const_types + ncons);
+    // payload layout: one contiguous run per program, value_offsets relative to the first
+    int64_t nvals = 0;
+    for (int i = 0; i < n; ++i) {
+        p->value_off[i] = nvals;
+        for (int64_t c = p->const_off[i]; c < p->const_off[i + 1]; ++c) {
+            int s = vt_size(p->ctypes[c]);
+            nvals += s ? s : 1;
+        }
+    }
+    p->value_off[n] = nvals;
+    if (nvals && !const_values) return set_error(RSR_E_INVALID_ARG, "const_values is null");
+    p->values.resize(nvals + 1);
+    for (int i = 0; i < n; ++i) {
+        const int64_t len = p->value_off[i + 1] - p->value_off[i];
+        if (len) memcpy(&p->values[p->value_off[i]], const_values + (value_offsets[i] - value_offsets[0]),
+                        (size_t)len * sizeof(float));
+    }
+    build_params(p.get());
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+        int rc = p->dparams.upload(ctx, p->params.data(), p->params.size() * sizeof(float));
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "params upload");
+    }
+    *out = p.release();
+    return RSR_OK;
+}
+
+extern "C" int rsr_programs_count(const rsr_programs* p, int* n, int64_t* n_values) {
+    if (!p) return set_error(RSR_E_INVALID_ARG, "progs is null");
+    if (n) *n = p->n;
+    if (n_values) *n_values = p->value_off.empty() ? 0 : p->value_off[p->n];
+    return RSR_OK;
+}
+
+extern "C" int rsr_programs_get_constants(const rsr_programs* p, float* out) {
+    if (!p || !out) return set_error(RSR_E_INVALID_ARG, "null argument");
+    memcpy(out, p->values.data(), (size_t)p->value_off[p->n] * sizeof(float));
+    return RSR_OK;
+}
+
+int rsr::upload_params(rsr_programs* p) {
+    rsr_context* ctx = p->ctx;
+    HIP_TRY(hipMemcpyAsync(p->dparams.ptr, p->params.data(), p->params.size() * sizeof(float), hipMemcpyHostToDevice,
+                           ctx->stream),
+            "params upload");
+    return RSR_OK;
+}
+
+extern "C" int rsr_programs_set_constants(rsr_programs* p, const float* values) {
+    if (!p || !values) return set_error(RSR_E_INVALID_ARG, "null argument");
+    memcpy(p->values.data(), values, (size_t)p->value_off[p->n] * sizeof(float));
+    build_params(p);
+    std::lock_guard<std::mutex> g(p->ctx->mu);
+    HIP_TRY(hipSetDevice(p->ctx->device), "hipSetDevice");
+    int rc = upload_params(p);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream), "params upload");
+    return RSR_OK;
+}
+
+// Lower (or fetch the cached lowering of) a batch for a feature count.
+int rsr::get_lowering(rsr_programs* p, int F, bool dual, Lowering** out) {
+    const int key = F * 2 + (dual ? 1 : 0);
+    auto it = p->lowerings.find(key);
+    if (it != p->lowerings.end()) {
+        *out = it->second.get();
+        return RSR_OK;
+    }
+    rsr_context* ctx = p->ctx;
+    auto L = std::make_unique<Lowering>();
+    L->status.resize(p->n);
+    L->nparams.resize(p->n);
+    L->differentiable.resize(p->n);
+    std::vector<UInstr> code;
+    std::vector<ProgDesc> descs(p->n);
+    std::vector<uint8_t> active(p->n, 0);
+    std::vector<uint32_t> cost(p->n, 0);
+    for (int i = 0; i < p->n; ++i) {
+        Lowered lw = lower_program(ref_program(p, i), F, dual);
+        L->status[i] = lw.status;
+        L->nparams[i] = (uint8_t)std::min<uint32_t>(lw.n_params, 32);
+        L->differentiable[i] = lw.differentiable ? 1 : 0;
+        descs[i].code_off = (uint32_t)code.size();
+        descs[i].param_off = p->param_off[i];
+        if (lw.status == PROG_OK) {
+            descs[i].code_len = (uint32_t)lw.code.size();
+            descs[i].out_reg = lw.out_reg;
+            code.insert(code.end(), lw.code.begin(), lw.code.end());
+            active[i] = 1;
+            cost[i] = (uint32_t)lw.code.size();
+            L->total_uops += lw.code.size();
+            L->max_params = std::max<int>(L->max_params, L->nparams[i]);
+            L->spill_slots = std::max<int>(L->spill_slots, (int)lw.spill_slots);
+        } else {
+            descs[i].code_len = 0;
+            descs[i].out_reg = 0;
+            L->n_invalid++;
+        }
+    }
+    code.push_back(UInstr{0u, 0u});  // prefetch padding
+    // scheduling order: most expensive first, so groups are balanced and the tail is short
+    L->order.resize(p->n);
+    std::iota(L->order.begin(), L->order.end(), 0);
+    std::stable_sort(L->order.begin(), L->order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    L->active_host = active;
+    int rc;
+    if ((rc = L->dcode.upload(ctx, code.data(), code.size() * sizeof(UInstr)))) return rc;
+    if ((rc = L->ddesc.upload(ctx, descs.data(), std::max<size_t>(1, descs.size()) * sizeof(ProgDesc)))) return rc;
+    if ((rc = L->dorder.upload(ctx, L->order.data(), std::max<size_t>(1, L->order.size()) * sizeof(int32_t))))
+        return rc;
+    if ((rc = L->dactive.upload(ctx, active.data(), std::max<size_t>(1, active.size())))) return rc;
+    if ((rc = L->dnparams.upload(ctx, L->nparams.data(), std::max<size_t>(1, L->nparams.size())))) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "lowering upload");
+    *out = L.get();
+    p->lowerings[key] = std::move(L);
+    return RSR_OK;
+}
+
+DevPrograms rsr::dev_view(const rsr_programs* p, const Lowering* L) {
+    DevPrograms v;
+    v.code = (const UInstr*)L->dcode.ptr;
+    v.progs = (const ProgDesc*)L->ddesc.ptr;
+    v.params = (const float*)p->dparams.ptr;
+    v.order = (const int32_t*)L->dorder.ptr;
+    v.n_progs = p->n;
+    v.spill_slots = L->spill_slots;
+    return v;
+}
+
+extern "C" int rsr_programs_status(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int32_t* status_out) {
+    if (!ctx || !p || !ds || !status_out) return set_error(RSR_E_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    Lowering* L;
+    int rc = get_lowering(p, ds->num_features, false, &L);
+    if (rc) return rc;
+    memcpy(status_out, L->status.data(), (size_t)p->n * sizeof(int32_t));
+    return RSR_OK;
+}
+
+extern "C" int rsr_programs_destroy(rsr_programs* p) {
+    if (!p) return RSR_OK;
+    rsr_context* ctx = p->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete p;
+    return RSR_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// evaluation
+// ---------------------------------------------------------------------------------
+int rsr::eval_mse_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* active_override,
+                         float* d_mse, Lowering** Lout) {
+    Lowering* L;
+    int rc = get_lowering(p, ds->num_features, false, &L);
+    if (rc) return rc;
+    if (Lout) *Lout = L;
+    const DevDataset D = dev_view(ds);
+    const EvalTuning t = ctx->tuning;
+    const size_t nparts = (size_t)eval_tiles(D, t.ppl) * std::max(p->n, 1);
+    if ((rc = ensure_scratch(ctx, ctx->scratch_partials, nparts * sizeof(float)))) return rc;
+    const uint8_t* act = active_override ? active_override : (L->n_invalid ? (const uint8_t*)L->dactive.ptr : nullptr);
+    HIP_TRY(launch_eval_mse(dev_view(p, L), D, act, t, (float*)ctx->scratch_partials.ptr, d_mse, ctx->stream,
+                            ctx->next_timer()),
+            "eval_mse launch");
+    return RSR_OK;
+}
+
+extern "C" int rsr_eval_mse(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, float* mse_out) {
+    if (!ctx || !p || !ds || (!mse_out && p->n)) return set_error(RSR_E_INVALID_ARG, "null argument");
+    if (p->n == 0) return RSR_OK;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    int rc = ensure_scratch(ctx, ctx->scratch_out, (size_t)p->n * sizeof(float));
+    if (rc) return rc;
+    Lowering* L;
+    if ((rc = eval_mse_device(ctx, p, ds, nullptr, (float*)ctx->scratch_out.ptr, &L))) return rc;
+    HIP_TRY(hipMemcpyAsync(mse_out, ctx->scratch_out.ptr, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream),
+            "mse download");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "eval_mse");
+    if (L->n_invalid) {
+        for (int i = 0; i < p->n; ++i)
+            if (L->status[i] != PROG_OK) mse_out[i] = NAN;
+        return set_error(RSR_E_PROGRAM, "%d program(s) rejected by lowering", L->n_invalid);
+    }
+    return RSR_OK;
+}
+
+extern "C" int rsr_eval_points(rsr_context* ctx, rsr_programs* p, int first, int count, const rsr_dataset* ds,
+                               float* out) {
+    if (!ctx || !p || !ds || !out || first < 0 || count < 0 || first + count > p->n)
+        return set_error(RSR_E_INVALID_ARG, "invalid eval_points arguments");
+    if (count == 0) return RSR_OK;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    Lowering* L;
+    int rc = get_lowering(p, ds->num_features, false, &L);
+    if (rc) return rc;
+    for (int i = first; i < first + count; ++i)
+        if (L->status[i] != PROG_OK) return set_error(RSR_E_PROGRAM, "program %d rejected (%d)", i, L->status[i]);
+    const size_t n = (size_t)count * ds->count8;
+    if ((rc = ensure_scratch(ctx, ctx->scratch_out, n * sizeof(float)))) return rc;
+    HIP_TRY(launch_eval_points(dev_view(p, L), dev_view(ds), first, count, (float*)ctx->scratch_out.ptr, ctx->stream),
+            "eval_points launch");
+    HIP_TRY(hipMemcpyAsync(out, ctx->scratch_out.ptr, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+            "points download");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "eval_points");
+    return RSR_OK;
+}
+
+// MSE + gradient into device buffers d_mse[n], d_grad[n*32].
+int rsr::eval_grad_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* host_active,
+                          float* d_mse, float* d_grad, Lowering** Ldual_out, Lowering** Lprim_out) {
+    Lowering *Lp, *Ld;
+    int rc = get_lowering(p, ds->num_features, false, &Lp);
+    if (rc) return rc;
+    if ((rc = get_lowering(p, ds->num_features, true, &Ld))) return rc;
+    if (Ldual_out) *Ldual_out = Ld;
+    if (Lprim_out) *Lprim_out = Lp;
+    const DevDataset D = dev_view(ds);
+    // zero-parameter programs: compute_mse on the primal lowering (evaluator.rs:187-189)
+    bool any_zero = false, any_pos = false;
+    std::vector<uint8_t> act0(p->n, 0), act1(p->n, 0);
+    for (int i = 0; i < p->n; ++i) {
+        const bool on = host_active == nullptr || host_active[i];
+        if (Ld->nparams[i] == 0) {
+            if (on && Lp->status[i] == PROG_OK) { act0[i] = 1; any_zero = true; }
+        } else if (on && Ld->status[i] == PROG_OK) {
+            act1[i] = 1;
+            any_pos = true;
+        }
+    }
+    if ((rc = ensure_scratch(ctx, ctx->scratch_act, 2 * (size_t)std::max(p->n, 1)))) return rc;
+    uint8_t* dact0 = (uint8_t*)ctx->scratch_act.ptr;
+    uint8_t* dact1 = dact0 + std::max(p->n, 1);
+    HIP_TRY(hipMemcpyAsync(dact0, act0.data(), (size_t)p->n, hipMemcpyHostToDevice, ctx->stream), "act upload");
+    HIP_TRY(hipMemcpyAsync(dact1, act1.data(), (size_t)p->n, hipMemcpyHostToDevice, ctx->stream), "act upload");
+    HIP_TRY(hipMemsetAsync(d_grad, 0, (size_t)p->n * 32 * sizeof(float), ctx->stream), "grad clear");
+    if (any_zero) {
+        const size_t nparts = (size_t)eval_tiles(D, ctx->tuning.ppl) * p->n;
+        if ((rc = ensure_scratch(ctx, ctx->scratch_partials, nparts * sizeof(float)))) return rc;
+        HIP_TRY(launch_eval_mse(dev_view(p, Lp), D, dact0, ctx->tuning, (float*)ctx->scratch_partials.ptr, d_mse,
+                                ctx->stream),
+                "eval_mse launch");
+    }
+    if (any_pos) {
+        const size_t nparts = (size_t)eval_tiles(D, 1) * p->n * 33;
+        if ((rc = ensure_scratch(ctx, ctx->scratch_partials, nparts * sizeof(float)))) return rc;
+        HIP_TRY(launch_eval_grad(dev_view(p, Ld), D, (const uint8_t*)Ld->dnparams.ptr, dact1, Ld->max_params,
+                                 (float*)ctx->scratch_partials.ptr, d_mse, d_grad, ctx->stream, ctx->next_timer()),
+                "eval_grad launch");
+    }
+    return RSR_OK;
+}
+
+extern "C" int rsr_eval_mse_grad(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, float* mse_out,
+                                 float* grad_out) {
+    if (!ctx || !p || !ds || (p->n && (!mse_out || !grad_out))) return set_error(RSR_E_INVALID_ARG, "null argument");
+    if (p->n == 0) return RSR_OK;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    int rc = ensure_scratch(ctx, ctx->scratch_out, (size_t)p->n * sizeof(float));
+    if (rc) return rc;
+    if ((rc = ensure_scratch(ctx, ctx->scratch_out2, (size_t)p->n * 32 * sizeof(float)))) return rc;
+    Lowering *Ld, *Lp;
+    if ((rc = eval_grad_device(ctx, p, ds, nullptr, (float*)ctx->scratch_out.ptr, (float*)ctx->scratch_out2.ptr, &Ld,
+                               &Lp)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(mse_out, ctx->scratch_out.ptr, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream),
+            "mse download");
+    HIP_TRY(hipMemcpyAsync(grad_out, ctx->scratch_out2.ptr, (size_t)p->n * 32 * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream),
+            "grad download");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "eval_mse_grad");
+    int bad = 0;
+    for (int i = 0; i < p->n; ++i) {
+        const bool ok = Ld->nparams[i] ? Ld->status[i] == PROG_OK : Lp->status[i] == PROG_OK;
+        if (!ok) {
+            mse_out[i] = NAN;
+            bad++;
+        }
+    }
+    if (bad) return set_error(RSR_E_PROGRAM, "%d program(s) rejected by lowering", bad);
+    return RSR_OK;
+}
+
+extern "C" int rsr_eval_points_dual(rsr_context* ctx, rsr_programs* p, int prog, int active_param,
+                                    const rsr_dataset* ds, float* val_out, float* grad_out) {
+    if (!ctx || !p || !ds || !val_out || !grad_out || prog < 0 || prog >= p->n || active_param < 0)
+        return set_error(RSR_E_INVALID_ARG, "invalid eval_points_dual arguments");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    Lowering* L;
+    int rc = get_lowering(p, ds->num_features, true, &L);
+    if (rc) return rc;
+    if (L->status[prog] != PROG_OK) return set_error(RSR_E_PROGRAM, "program %d rejected (%d)", prog, L->status[prog]);
+    const size_t n = (size_t)ds->count8;
+    if ((rc = ensure_scratch(ctx, ctx->scratch_out, 2 * n * sizeof(float)))) return rc;
+    float* dv = (float*)ctx->scratch_out.ptr;
+    HIP_TRY(launch_eval_points_dual(dev_view(p, L), dev_view(ds), prog, active_param, dv, dv + n, ctx->stream),
+            "eval_points_dual launch");
+    HIP_TRY(hipMemcpyAsync(val_out, dv, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "download");
+    HIP_TRY(hipMemcpyAsync(grad_out, dv + n, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "download");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "eval_points_dual");
+    return RSR_OK;
+}

@@ -1,0 +1,109 @@
+// runtime_internal.h -- host-side objects behind the C ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "kernels.h"
+#include "lower.h"
+
+struct rsr_context;
+
+namespace rsr {
+
+// Device allocation charged against the context's HBM budget.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    rsr_context* ctx = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    int alloc(rsr_context* c, size_t n);
+    int upload(rsr_context* c, const void* src, size_t n);
+    void release();
+};
+
+// One lowering of a program batch (per feature count, primal or dual).
+struct Lowering {
+    std::vector<int32_t> status;
+    std::vector<uint8_t> nparams;         // min(param_count, 32)
+    std::vector<uint8_t> differentiable;  // no Inverse ops
+    std::vector<int32_t> order;           // cost-sorted scheduling order
+    std::vector<uint8_t> active_host;
+    int n_invalid = 0;
+    int max_params = 0;
+    int spill_slots = 0;
+    uint64_t total_uops = 0;
+    DevBuf dcode, ddesc, dorder, dactive, dnparams;
+};
+
+int set_error(int code, const char* fmt, ...);
+int hip_error(hipError_t e, const char* what);
+
+}  // namespace rsr
+
+struct rsr_context {
+    int device = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    std::recursive_mutex rmu;
+    std::mutex mu;
+    rsr::EvalTuning tuning;
+    rsr::DevBuf scratch_partials, scratch_out, scratch_out2, scratch_act;
+    // profiling: events around every main evaluation kernel, accumulated on read
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
+    double prof_ms = 0.0;
+    int64_t prof_launches = 0;
+    const rsr::KernelTimer* next_timer();
+    rsr::KernelTimer timer_slot;
+};
+
+struct rsr_dataset {
+    rsr_context* ctx = nullptr;
+    int num_features = 0;
+    int64_t num_samples = 0;
+    int64_t count8 = 0;
+    int64_t npad = 0;
+    std::vector<uint8_t> feature_types;
+    std::vector<float> soa;     // host copy [F][count8]
+    std::vector<float> target;  // [count8]
+    std::vector<float> means, stds;
+    float tmean = 0.0f, tstd = 1.0f;
+    rsr::DevBuf dfeat, dtarget;
+};
+
+struct rsr_programs {
+    rsr_context* ctx = nullptr;
+    int n = 0;
+    std::vector<uint8_t> ops;
+    std::vector<uint16_t> args;
+    std::vector<uint8_t> ctypes;
+    std::vector<float> values;  // constant payloads (create-time layout, compacted)
+    std::vector<int64_t> code_off, const_off, value_off;
+    std::vector<float> params;  // flat parameters of all programs
+    std::vector<uint32_t> param_off;
+    rsr::DevBuf dparams;
+    std::map<int, std::unique_ptr<rsr::Lowering>> lowerings;
+};
+
+namespace rsr {
+int ensure_scratch(rsr_context* c, DevBuf& b, size_t n);
+DevDataset dev_view(const rsr_dataset* d);
+DevPrograms dev_view(const rsr_programs* p, const Lowering* L);
+RefProgram ref_program(const rsr_programs* p, int i);
+int get_lowering(rsr_programs* p, int F, bool dual, Lowering** out);
+int upload_params(rsr_programs* p);
+int eval_mse_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* active_override,
+                    float* d_mse, Lowering** Lout);
+int eval_grad_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* host_active,
+                     float* d_mse, float* d_grad, Lowering** Ldual_out, Lowering** Lprim_out);
+}  // namespace rsr

