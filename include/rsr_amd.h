@@ -88,6 +88,15 @@ RSR_API int rsr_context_profile(rsr_context* ctx, int enable);
 RSR_API int rsr_context_profile_read(rsr_context* ctx, double* kernel_ms, int64_t* launches);
 /* Wait for all work queued on the context's stream. */
 RSR_API int rsr_context_synchronize(rsr_context* ctx);
+/* Summation of the per-point squared errors and gradient terms:
+ *   RSR_SUM_KAHAN (default): per-tile pairwise sums combined with Kahan-compensated
+ *     FP32 (fast; within 1e-5 relative of the reference's f32 sums);
+ *   RSR_SUM_REFERENCE: the reference's order -- 8 lane accumulators fed point by point
+ *     (evaluator.rs:159-170) and the f32x8 horizontal add -- bit-identical to the CPU
+ *     evaluator; one wave per program. */
+#define RSR_SUM_KAHAN 0
+#define RSR_SUM_REFERENCE 1
+RSR_API int rsr_context_set_summation(rsr_context* ctx, int mode);
 /* Evaluation tuning: points per lane (1, 2, 4) and programs per work item. */
 RSR_API int rsr_context_set_tuning(rsr_context* ctx, int points_per_lane, int programs_per_item);
 RSR_API int rsr_context_get_tuning(const rsr_context* ctx, int* points_per_lane, int* programs_per_item);
@@ -146,6 +155,29 @@ RSR_API int rsr_eval_mse_grad(rsr_context* ctx, rsr_programs* progs, const rsr_d
  * flat parameter `active_param`, per point. */
 RSR_API int rsr_eval_points_dual(rsr_context* ctx, rsr_programs* progs, int program, int active_param,
                                  const rsr_dataset* ds, float* val_out, float* grad_out);
+
+/* ---- constant fitting (src/engine/optimize/) ----
+ * Fitted constants are written back into the program batch (device and host copies;
+ * read them with rsr_programs_get_constants), like the reference's write-back into
+ * Individual.constants (lbfgs.rs:219-229, cmaes.rs:181-191).  fitness_out[i] is the
+ * optimizer's final MSE, NaN for programs not fitted (mask[i] == 0, rejected, or no
+ * parameters).  mask may be NULL (fit every program). */
+#define RSR_LBFGS_REFERENCE 0  /* run_lbfgs (lbfgs.rs:9-165): f32, M = history, its quirks */
+#define RSR_LBFGS_FP64 1       /* FP64 two-loop dots, curvature pairs and Armijo test; f32 parameters */
+RSR_API int rsr_fit_lbfgs(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, int max_iterations,
+                          int history, int mode, const uint8_t* mask, float* fitness_out);
+/* run_cma_es (cmaes.rs:19-192), diagonal, lambda candidates / mu parents per
+ * individual (reference: 16 / 8), doubled dimension as the reference.  Normals come
+ * from a Philox4x32-10 stream keyed by `seed` (see rsr_cmaes_normal), or, when
+ * `normals` is non-NULL, from normals[((it * n_fit + k) * lambda + c) * 32 + j] with k
+ * the fitted individuals in program order.  iters_out (optional): iterations run. */
+RSR_API int rsr_fit_cmaes(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, int max_iterations,
+                          int lambda, int mu, uint64_t seed, const float* normals, const uint8_t* mask,
+                          float* fitness_out, int32_t* iters_out);
+/* The standard normal the CMA-ES sampler draws for (seed, program, iteration,
+ * candidate, dimension): Philox4x32-10 -> two 24-bit uniforms -> Box-Muller as
+ * cmaes.rs:13-17. */
+RSR_API float rsr_cmaes_normal(uint64_t seed, uint32_t program, uint32_t iteration, uint32_t candidate, uint32_t dim);
 
 /* Lowering self-check (test hook, no reference counterpart, no GPU): lower one
  * program exactly as the evaluation entry points do and run the emitted micro-code

@@ -49,6 +49,7 @@ SIGS = {
     "orc_assign_rank_and_crowding": (C.c_int, [C.c_int, _f32p, _u64p, _u32p, _f32p]),
     "orc_tournament": (C.c_int, [C.c_int, _i32p, _u32p, _f32p]),
     "orc_fitness": (C.c_float, [C.c_float, C.c_int, C.c_float]),
+    "orc_cmaes_constants": (None, [C.c_int, _f32p, _f32p, _f32p]),
 }
 
 
@@ -196,34 +197,25 @@ def run_lbfgs(prog, ds: ODataset, iters: int = 15, m: int = 6, fp64: bool = Fals
 
 
 def run_cmaes(prog, ds: ODataset, iters: int, lam: int, mu: int, normals: np.ndarray):
-    """cmaes.rs:19-192 with an injected normal stream normals[it][lam][32]."""
-    weights = np.array([max(np.log(np.float32(mu + 0.5)) - np.log(np.float32(i + 1)), 0.0) for i in range(mu)],
-                       np.float32)
-    weights = cmaes_weights(mu)[0]
-    mu_eff = cmaes_weights(mu)[1]
+    """cmaes.rs:19-192 with an injected normal stream normals[it][lam][32].
+    Returns (fitness, fitted constant payloads, iterations run)."""
+    weights, mu_eff, decay = cmaes_weights(mu)
     op = OProgram(prog)
     fit = C.c_float()
     its = C.c_int32()
     nrm = np.ascontiguousarray(normals, np.float32)
     rc = lib().orc_run_cmaes(*op.args_(), *ds.dargs(), iters, lam, _p(weights, _f32p), mu, C.c_float(mu_eff),
-                             C.c_float(np.float32(np.exp(np.float32(-0.01)))), _p(nrm, _f32p), C.byref(fit),
-                             C.byref(its))
+                             C.c_float(decay), _p(nrm, _f32p), C.byref(fit), C.byref(its))
     return (fit.value if rc == 0 else None), op.cvals.copy(), its.value
 
 
 def cmaes_weights(mu: int):
-    """Recombination weights and mu_eff exactly as cmaes.rs:85-98 (f32 arithmetic)."""
+    """Recombination weights, mu_eff and the sigma decay exactly as cmaes.rs:85-98, 178 (libm f32)."""
     w = np.zeros(mu, np.float32)
-    s = np.float32(0.0)
-    lm = np.float32(np.log(np.float32(mu) + np.float32(0.5)))
-    for i in range(mu):
-        w[i] = max(np.float32(lm - np.float32(np.log(np.float32(i + 1)))), np.float32(0.0))
-        s = np.float32(s + w[i])
-    me = np.float32(0.0)
-    for i in range(mu):
-        w[i] = np.float32(w[i] / s)
-        me = np.float32(me + w[i] * w[i])
-    return w, float(np.float32(np.float32(1.0) / me))
+    me = C.c_float()
+    dec = C.c_float()
+    lib().orc_cmaes_constants(mu, _p(w, _f32p), C.byref(me), C.byref(dec))
+    return w, me.value, dec.value
 
 
 def run_nelder_mead(prog, ds: ODataset, iters: int):

@@ -1060,7 +1060,7 @@ ORC_API int orc_run_lbfgs_f64(const uint8_t *ops, const uint16_t *args, int n_co
     for (int it = 0; it < max_iterations; ++it) {
         double gn = 0.0;
         for (int j = 0; j < n; ++j) gn += (double)cur_grad[j] * (double)cur_grad[j];
-        if (!(sqrt(gn) >= 1e-5)) { if (sqrt(gn) < 1e-5) break; }
+        if (sqrt(gn) < 1e-5) break;
         for (int j = 0; j < n; ++j) q[j] = cur_grad[j];
         int ci = head;
         for (int h = 0; h < hist; ++h) {
@@ -1089,7 +1089,7 @@ ORC_API int orc_run_lbfgs_f64(const uint8_t *ops, const uint16_t *args, int n_co
         float p[MAXP] = {0};
         double ddg = 0.0;
         for (int j = 0; j < n; ++j) { p[j] = (float)(-q[j]); ddg += (double)p[j] * (double)cur_grad[j]; }
-        if (!(ddg <= -1e-8)) {
+        if (ddg > -1e-8) {
             for (int j = 0; j < n; ++j) p[j] = -cur_grad[j];
             ddg = -gn;
             hist = 0;
@@ -1215,6 +1215,20 @@ ORC_API int orc_run_cmaes(const uint8_t *ops, const uint16_t *args, int n_code,
     *fitness_out = best_mse;
     *iters_run = it;
     return 0;
+}
+
+/* CMA-ES strategy constants as cmaes.rs:85-98 with libm logf / expf (Rust f32::ln / exp). */
+ORC_API void orc_cmaes_constants(int mu, float *weights, float *mu_eff, float *sigma_decay) {
+    float sum = 0.0f, lm = logf((float)mu + 0.5f);
+    for (int i = 0; i < mu; ++i) {
+        float v = lm - logf((float)(i + 1));
+        weights[i] = v > 0.0f ? v : 0.0f;
+        sum += weights[i];
+    }
+    float me = 0.0f;
+    for (int i = 0; i < mu; ++i) { weights[i] = weights[i] / sum; me += weights[i] * weights[i]; }
+    *mu_eff = 1.0f / me;
+    *sigma_decay = expf(-0.01f);
 }
 
 /* run_nelder_mead (nelder_mead.rs:9-165), deterministic. Only Float constants are optimised. */

@@ -46,6 +46,7 @@ SIGNATURES = {
     "rsr_context_profile": (C.c_int, [_P, C.c_int]),
     "rsr_context_profile_read": (C.c_int, [_P, C.POINTER(C.c_double), _i64p]),
     "rsr_context_synchronize": (C.c_int, [_P]),
+    "rsr_context_set_summation": (C.c_int, [_P, C.c_int]),
     "rsr_dataset_create": (C.c_int, [_P, _f32p, _f32p, C.c_int64, _u8p, C.c_int, C.c_int, _PP]),
     "rsr_dataset_create_aosoa": (C.c_int, [_P, _f32p, _f32p, C.c_int, C.c_int64, C.c_int64, _PP]),
     "rsr_dataset_subset": (C.c_int, [_P, _P, C.c_int64, _PP]),
@@ -138,6 +139,10 @@ class Context:
         ms, n = C.c_double(), C.c_int64()
         _check(lib().rsr_context_profile_read(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def set_summation(self, reference_order: bool):
+        """False: pairwise + Kahan (default); True: the reference's 8-lane order (bit-identical)."""
+        _check(lib().rsr_context_set_summation(self.h, 1 if reference_order else 0))
 
     def synchronize(self):
         _check(lib().rsr_context_synchronize(self.h))
@@ -319,3 +324,58 @@ def lower_check(prog: Program, rows_soa: np.ndarray, dual: bool = False, active_
             len(prog.constants), F, int(dual), active_param, _ptr(feat, _f32p), n, _ptr(vals, _f32p),
             _ptr(tans, _f32p), C.byref(nu), C.byref(mr))
     return st, vals[:n], tans[:n], nu.value, mr.value
+
+
+# ---------------------------------------------------------------------------------
+# constant fitting (src/engine/optimize/)
+# ---------------------------------------------------------------------------------
+LBFGS_REFERENCE = 0
+LBFGS_FP64 = 1
+
+
+def _bind_fit():
+    l = lib()
+    l.rsr_fit_lbfgs.restype = C.c_int
+    l.rsr_fit_lbfgs.argtypes = [_P, _P, _P, C.c_int, C.c_int, C.c_int, _u8p, _f32p]
+    l.rsr_fit_cmaes.restype = C.c_int
+    l.rsr_fit_cmaes.argtypes = [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_uint64, _f32p, _u8p, _f32p, _i32p]
+    l.rsr_cmaes_normal.restype = C.c_float
+    l.rsr_cmaes_normal.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+    return l
+
+
+def _mask(progs: Programs, mask):
+    if mask is None:
+        return None
+    m = np.ascontiguousarray(np.asarray(mask, np.uint8))
+    return m
+
+
+def fit_lbfgs(ctx: Context, progs: Programs, ds: Dataset, iterations: int = 15, history: int = 6,
+              mode: int = LBFGS_REFERENCE, mask=None) -> np.ndarray:
+    """Batched run_lbfgs (lbfgs.rs:9-165); constants are written back into `progs`."""
+    l = _bind_fit()
+    fit = np.zeros(max(progs.n, 1), np.float32)
+    m = _mask(progs, mask)
+    _check(l.rsr_fit_lbfgs(ctx.h, progs.h, ds.h, iterations, history, mode,
+                           _ptr(m, _u8p) if m is not None else None, _ptr(fit, _f32p)))
+    return fit[: progs.n]
+
+
+def fit_cmaes(ctx: Context, progs: Programs, ds: Dataset, iterations: int, lam: int = 16, mu: int = 8,
+              seed: int = 0, normals: Optional[np.ndarray] = None, mask=None):
+    """Batched run_cma_es (cmaes.rs:19-192): returns (fitness[n], iterations_run[n])."""
+    l = _bind_fit()
+    fit = np.zeros(max(progs.n, 1), np.float32)
+    its = np.zeros(max(progs.n, 1), np.int32)
+    m = _mask(progs, mask)
+    nv = None if normals is None else np.ascontiguousarray(normals, np.float32)
+    _check(l.rsr_fit_cmaes(ctx.h, progs.h, ds.h, iterations, lam, mu, seed,
+                           _ptr(nv, _f32p) if nv is not None else None,
+                           _ptr(m, _u8p) if m is not None else None, _ptr(fit, _f32p), _ptr(its, _i32p)))
+    return fit[: progs.n], its[: progs.n]
+
+
+def cmaes_normal(seed: int, program: int, iteration: int, candidate: int, dim: int) -> float:
+    """The normal the CMA-ES sampler draws (host restatement of the device function)."""
+    return _bind_fit().rsr_cmaes_normal(seed, program, iteration, candidate, dim)

@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librsr_amd.so")
-SOURCES = ["kernels.hip", "lower.cpp", "runtime.cpp", "hostcheck.hip", "optim.hip", "serve.cpp"]
+SOURCES = ["kernels.hip", "lower.cpp", "runtime.cpp", "hostcheck.hip", "optim.hip", "fit.cpp", "serve.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [

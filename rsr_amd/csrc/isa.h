@@ -78,7 +78,7 @@ enum UOp : uint8_t {
 };
 
 // LDS spill slots available to one program (per point of a lane, per channel).
-constexpr int kMaxSpillSlots = 48;
+constexpr int kMaxSpillSlots = 16;
 
 // Register file geometry: 32 f32 components per point per lane.  Dataset
 // feature components occupy registers [0, F); temporaries use [F, 32).

@@ -45,6 +45,19 @@ hipError_t launch_eval_mse(const DevPrograms& P, const DevDataset& D, const uint
                            float* partials, float* mse_out, hipStream_t s, const KernelTimer* timer = nullptr);
 int64_t eval_tiles(const DevDataset& D, int ppl);
 
+// Dynamic LDS bytes a launch needs for the batch's spill slots.
+size_t eval_mse_lds_bytes(const DevPrograms& P, int ppl);
+size_t eval_grad_lds_bytes(const DevPrograms& P);
+
+// Reference-order summation variants: per-point terms are folded into 8 serial lane
+// accumulators in point order and added horizontally like f32x8::reduce_add, so the
+// results are bit-identical to the CPU evaluator.  One wave per program.
+hipError_t launch_eval_mse_exact(const DevPrograms& P, const DevDataset& D, const uint8_t* active, float* mse_out,
+                                 hipStream_t s, const KernelTimer* timer = nullptr);
+hipError_t launch_eval_grad_exact(const DevPrograms& P, const DevDataset& D, const uint8_t* nparams_eff,
+                                  const uint8_t* active, int max_params, float* mse_out, float* grad_out,
+                                  hipStream_t s, const KernelTimer* timer = nullptr);
+
 // Per-point values of programs [first, first+count) (eval_simd over every batch);
 // out[(i - first) * count8 + point].
 hipError_t launch_eval_points(const DevPrograms& P, const DevDataset& D, int first, int count, float* out,

@@ -102,6 +102,10 @@ __global__ void k_finalize_mse(const float* __restrict__ partials, int64_t n_til
     mse_out[prog] = __builtin_isfinite(mse) ? mse : FLT_MAX;
 }
 
+size_t eval_mse_lds_bytes(const DevPrograms& P, int ppl) {
+    return (size_t)kWaves * P.spill_slots * ppl * 64 * sizeof(float);
+}
+
 int64_t eval_tiles(const DevDataset& D, int ppl) {
     const int64_t tp = 64 * (int64_t)ppl;
     return (D.count8 + tp - 1) / tp;
@@ -109,7 +113,7 @@ int64_t eval_tiles(const DevDataset& D, int ppl) {
 
 hipError_t launch_eval_mse(const DevPrograms& P, const DevDataset& D, const uint8_t* active, const EvalTuning& t,
                            float* partials, float* mse_out, hipStream_t s, const KernelTimer* timer) {
-    const size_t lds_bytes = (size_t)kWaves * P.spill_slots * t.ppl * 64 * sizeof(float);
+    const size_t lds_bytes = eval_mse_lds_bytes(P, t.ppl);
     if (P.n_progs <= 0) return hipSuccess;
     const int64_t n_tiles = eval_tiles(D, t.ppl);
     const int group = t.group > 0 ? t.group : 8;
@@ -136,6 +140,80 @@ hipError_t launch_eval_mse(const DevPrograms& P, const DevDataset& D, const uint
     hipLaunchKernelGGL(k_finalize_mse, dim3((P.n_progs + tb - 1) / tb), dim3(tb), 0, s, partials, n_tiles, P.n_progs,
                        P.order, active, (float)D.num_samples, mse_out);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Reference-order summation (bit-identical to the CPU evaluator's f32 sums)
+// ---------------------------------------------------------------------------------
+// The reference accumulates diff^2 in 8 lanes, lane l receiving points l, l+8, l+16, ...
+// in order (evaluator.rs:159-170), then adds the 8 lanes horizontally.  Here one wave
+// owns one program and walks the tiles in order; inside a 64-point tile the eight
+// 8-point batches are folded into the 8 accumulators (held by lanes 0..7) one batch
+// after the other with shuffles, which reproduces the serial chain exactly.
+__device__ __forceinline__ float fold_batches(float acc, float term, int lane) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float v = __shfl(term, (lane & 7) + 8 * k, 64);
+        acc = acc + v;
+    }
+    return acc;
+}
+
+// f32x8::reduce_add order: (lo4 + hi4), then ((0 + 2) + (1 + 3)); result in every lane.
+__device__ __forceinline__ float reduce_lanes8(float acc) {
+    const float a0 = __shfl(acc, 0, 64), a1 = __shfl(acc, 1, 64), a2 = __shfl(acc, 2, 64), a3 = __shfl(acc, 3, 64);
+    const float a4 = __shfl(acc, 4, 64), a5 = __shfl(acc, 5, 64), a6 = __shfl(acc, 6, 64), a7 = __shfl(acc, 7, 64);
+    const float s0 = a0 + a4, s1 = a1 + a5, s2 = a2 + a6, s3 = a3 + a7;
+    return (s0 + s2) + (s1 + s3);
+}
+
+__global__ __launch_bounds__(256) void k_eval_mse_exact(const UInstr* __restrict__ code,
+                                                        const ProgDesc* __restrict__ progs,
+                                                        const float* __restrict__ params,
+                                                        const int32_t* __restrict__ order,
+                                                        const uint8_t* __restrict__ active, int n_progs,
+                                                        const float* __restrict__ feat,
+                                                        const float* __restrict__ target, int F, int64_t npad,
+                                                        int64_t count8, int64_t n_tiles, float n_samples,
+                                                        float* __restrict__ mse_out, int spill_slots) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    const int sp = (int)(uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6));
+    if (sp >= n_progs) return;
+    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * 64;
+    const int prog = (int)uni((uint32_t)order[sp]);
+    if (active != nullptr && !active[prog]) return;
+    const ProgDesc pd = progs[prog];
+    const uint32_t o = uni(pd.out_reg) & 31u;
+    float acc = 0.0f;
+    for (int64_t tile = 0; tile < n_tiles; ++tile) {
+        const int64_t base = tile * 64;
+        Files<1> R;
+        R.a = (v32f)(0.0f);
+        load_features<1>(R, feat, F, npad, base, lane);
+        run_program<1>(R, code + uni(pd.code_off), uni(pd.code_len), params + uni(pd.param_off), lds, lane);
+        const int64_t pt = base + lane;
+        const v32f rf = R.a;
+        const float diff = rf[o] - target[pt];
+        const float sq = pt < count8 ? diff * diff : 0.0f;
+        acc = fold_batches(acc, sq, lane);
+    }
+    const float mse = reduce_lanes8(acc) / n_samples;
+    if (lane == 0) mse_out[prog] = __builtin_isfinite(mse) ? mse : FLT_MAX;
+}
+
+hipError_t launch_eval_mse_exact(const DevPrograms& P, const DevDataset& D, const uint8_t* active, float* mse_out,
+                                 hipStream_t s, const KernelTimer* timer) {
+    if (P.n_progs <= 0) return hipSuccess;
+    const int64_t n_tiles = eval_tiles(D, 1);
+    const size_t lds_bytes = eval_mse_lds_bytes(P, 1);
+    if (timer) (void)hipEventRecord(timer->begin, s);
+    hipLaunchKernelGGL(k_eval_mse_exact, dim3((unsigned)((P.n_progs + kWaves - 1) / kWaves)), dim3(64 * kWaves),
+                       lds_bytes, s, P.code, P.progs, P.params, P.order, active, P.n_progs, D.feat, D.target,
+                       D.num_features, D.npad, D.count8, n_tiles, (float)D.num_samples, mse_out, P.spill_slots);
+    hipError_t e = hipGetLastError();
+    if (timer) (void)hipEventRecord(timer->end, s);
+    return e;
 }
 
 // ---------------------------------------------------------------------------------
@@ -179,6 +257,10 @@ hipError_t launch_eval_points(const DevPrograms& P, const DevDataset& D, int fir
 // Dual (forward-mode) evaluation: MSE and gradient
 // ---------------------------------------------------------------------------------
 static constexpr int kTan = 4;  // tangent channels per pass
+
+size_t eval_grad_lds_bytes(const DevPrograms& P) {
+    return (size_t)kWaves * P.spill_slots * (1 + kTan) * 64 * sizeof(float);
+}
 
 __global__ __launch_bounds__(256) void k_eval_grad(const UInstr* __restrict__ code, const ProgDesc* __restrict__ progs,
                                                    const float* __restrict__ params, const int32_t* __restrict__ order,
@@ -286,7 +368,7 @@ hipError_t launch_eval_grad(const DevPrograms& P, const DevDataset& D, const uin
     const int64_t waves = n_tiles * P.n_progs * n_pass;
     const int64_t blocks = (waves + kWaves - 1) / kWaves;
     if (blocks > INT32_MAX) return hipErrorInvalidValue;
-    const size_t lds_bytes = (size_t)kWaves * P.spill_slots * (1 + kTan) * 64 * sizeof(float);
+    const size_t lds_bytes = eval_grad_lds_bytes(P);
     if (timer) (void)hipEventRecord(timer->begin, s);
     hipLaunchKernelGGL(k_eval_grad, dim3((unsigned)blocks), dim3(64 * kWaves), lds_bytes, s, P.code, P.progs, P.params,
                        P.order, nparams_eff, active, P.n_progs, n_pass, D.feat, D.target, D.num_features, D.npad,
@@ -299,6 +381,80 @@ hipError_t launch_eval_grad(const DevPrograms& P, const DevDataset& D, const uin
     hipLaunchKernelGGL(k_finalize_grad, dim3((unsigned)((threads + tb - 1) / tb)), dim3(tb), 0, s, partials, n_tiles,
                        P.n_progs, P.order, nparams_eff, active, (float)D.num_samples, mse_out, grad_out);
     return hipGetLastError();
+}
+
+// Reference-order gradient sums: one wave per (program, pass) walks the tiles in order.
+__global__ __launch_bounds__(256) void k_eval_grad_exact(const UInstr* __restrict__ code,
+                                                         const ProgDesc* __restrict__ progs,
+                                                         const float* __restrict__ params,
+                                                         const int32_t* __restrict__ order,
+                                                         const uint8_t* __restrict__ nparams,
+                                                         const uint8_t* __restrict__ active, int n_progs, int n_pass,
+                                                         const float* __restrict__ feat,
+                                                         const float* __restrict__ target, int F, int64_t npad,
+                                                         int64_t count8, int64_t n_tiles, float n_samples,
+                                                         float* __restrict__ mse_out, float* __restrict__ grad_out,
+                                                         int spill_slots) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+    const int sp = (int)(wave / n_pass);
+    const int pass = (int)(wave - (int64_t)sp * n_pass);
+    if (sp >= n_progs) return;
+    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * (1 + kTan) * 64;
+    const int prog = (int)uni((uint32_t)order[sp]);
+    const int np = (int)uni(nparams[prog]);
+    if ((active != nullptr && !active[prog]) || np == 0 || pass * kTan >= np) return;
+    const ProgDesc pd = progs[prog];
+    const uint32_t o = uni(pd.out_reg) & 31u;
+    const uint32_t k0 = (uint32_t)(pass * kTan);
+    float acc = 0.0f, gacc[kTan] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int64_t tile = 0; tile < n_tiles; ++tile) {
+        const int64_t base = tile * 64;
+        Files<1> R1;
+        R1.a = (v32f)(0.0f);
+        load_features<1>(R1, feat, F, npad, base, lane);
+        DualFiles<kTan> G;
+        G.v = R1.a;
+        G.a = G.b = G.c = G.d = (v32f)(0.0f);
+        run_dual<kTan>(G, code + uni(pd.code_off), uni(pd.code_len), params + uni(pd.param_off), k0, lds, lane);
+        const int64_t pt = base + lane;
+        const bool valid = pt < count8;
+        const v32f vf = G.v;
+        const float diff = vf[o] - target[pt];
+        acc = fold_batches(acc, valid ? diff * diff : 0.0f, lane);
+        const float two_diff = 2.0f * diff;
+        const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
+#pragma unroll
+        for (int j = 0; j < kTan; ++j) gacc[j] = fold_batches(gacc[j], valid ? two_diff * gf[j][o] : 0.0f, lane);
+    }
+    const float mse = reduce_lanes8(acc) / n_samples;
+    const bool fin = __builtin_isfinite(mse);
+    if (pass == 0 && lane == 0) mse_out[prog] = fin ? mse : FLT_MAX;
+#pragma unroll
+    for (int j = 0; j < kTan; ++j) {
+        const int k = (int)k0 + j;
+        const float g = reduce_lanes8(gacc[j]) / n_samples;
+        if (lane == 0 && k < 32) grad_out[(int64_t)prog * 32 + k] = (fin && k < np) ? g : 0.0f;
+    }
+}
+
+hipError_t launch_eval_grad_exact(const DevPrograms& P, const DevDataset& D, const uint8_t* nparams_eff,
+                                  const uint8_t* active, int max_params, float* mse_out, float* grad_out,
+                                  hipStream_t s, const KernelTimer* timer) {
+    if (P.n_progs <= 0 || max_params <= 0) return hipSuccess;
+    const int64_t n_tiles = eval_tiles(D, 1);
+    const int n_pass = (max_params + kTan - 1) / kTan;
+    const int64_t waves = (int64_t)P.n_progs * n_pass;
+    const size_t lds_bytes = eval_grad_lds_bytes(P);
+    if (timer) (void)hipEventRecord(timer->begin, s);
+    hipLaunchKernelGGL(k_eval_grad_exact, dim3((unsigned)((waves + kWaves - 1) / kWaves)), dim3(64 * kWaves),
+                       lds_bytes, s, P.code, P.progs, P.params, P.order, nparams_eff, active, P.n_progs, n_pass,
+                       D.feat, D.target, D.num_features, D.npad, D.count8, n_tiles, (float)D.num_samples, mse_out,
+                       grad_out, P.spill_slots);
+    hipError_t e = hipGetLastError();
+    if (timer) (void)hipEventRecord(timer->end, s);
+    return e;
 }
 
 __global__ __launch_bounds__(64) void k_eval_points_dual(const UInstr* __restrict__ code, const ProgDesc* __restrict__ progs,

@@ -1,0 +1,57 @@
+// optim.h -- device state and launch interface of the batched constant fitters.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rsr {
+
+constexpr int kMaxP = 32;        // flat parameters an optimizer touches (lbfgs.rs:6, cmaes.rs:8)
+constexpr int kMaxLambda = 64;   // CMA-ES population per individual
+
+enum : int { LB_DONE = 1, LB_SEARCH = 2, LB_MOVED = 4 };
+enum : int { CMA_DONE = 1 };
+
+// L-BFGS state of n individuals, field-major ([j * n + i]); S, Y: [slot][j][i] of T.
+struct LbfgsDev {
+    int n = 0, M = 6, fp64 = 0;
+    const int* prog;        // program index of individual i
+    const int* nparam;      // min(param_count, 32)
+    const uint32_t* param_off;
+    float *x, *g, *p, *step, *cur_mse, *next_mse;
+    void *S, *Y, *rho, *alpha, *ddg;  // float or double
+    int *hist, *head, *ls, *flags;
+};
+
+hipError_t lbfgs_begin(const LbfgsDev& st, const float* params, const float* d_mse, const float* d_grad,
+                       hipStream_t s);
+hipError_t lbfgs_direction(const LbfgsDev& st, float* params, uint8_t* active_ls, int* counter, hipStream_t s);
+hipError_t lbfgs_ls_check(const LbfgsDev& st, float* params, const float* d_mse, uint8_t* active_ls,
+                          uint8_t* active_grad, int* counter, hipStream_t s);
+hipError_t lbfgs_update(const LbfgsDev& st, const float* d_grad, uint8_t* active_grad, hipStream_t s);
+hipError_t lbfgs_finish(const LbfgsDev& st, float* params, float* fitness, hipStream_t s);
+
+// Diagonal CMA-ES state of n individuals with lambda candidates each.
+struct CmaDev {
+    int n = 0, lambda = 16, mu = 8;
+    float mu_eff = 0.0f, sigma_decay = 1.0f;
+    uint64_t seed = 0;
+    const float* weights;      // [mu]
+    const float* normals;      // optional injected normals [it][n][lambda][32]
+    const int* prog;
+    const int* nparam;         // min(param_count, 32)
+    const int* ndim;           // doubled dimension (cmaes.rs:31-75)
+    const uint32_t* param_off; // individual's own parameter block
+    const int64_t* cand_off;   // candidate ic's parameter block
+    const float *c_c, *c_cov;
+    float *mean, *cdiag, *pc, *best, *sigma, *best_mse, *best_fit;
+    float *steps, *pop, *l1;   // [ic][32], [ic][32], [ic]
+    uint8_t* cand_active;      // [ic]
+    int *flags, *iters;
+};
+
+hipError_t cmaes_sample(const CmaDev& st, float* cand_params, int it, hipStream_t s);
+hipError_t cmaes_update(const CmaDev& st, const float* cand_mse, int* counter, hipStream_t s);
+hipError_t cmaes_finish(const CmaDev& st, float* params, float* fitness, int* iters_out, hipStream_t s);
+float cma_normal_host(uint64_t seed, uint32_t ind, uint32_t it, uint32_t cand, uint32_t j);
+
+}  // namespace rsr

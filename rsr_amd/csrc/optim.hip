@@ -1,0 +1,407 @@
+// optim.hip -- batched constant fitting on the device: L-BFGS and diagonal CMA-ES.
+//
+// Every individual's optimizer state lives in device arrays laid out field-major
+// (value j of individual i at [j * n + i]), one thread per individual.  The expensive
+// part -- the MSE / gradient evaluations of all individuals -- runs in the evaluation
+// kernels with an active mask, so one launch evaluates every individual still
+// working on that step.  Trial points are written straight into the device parameter
+// array the evaluation kernels read.
+#include <float.h>
+#include <math.h>
+#include "devmath.h"
+#include "optim.h"
+
+namespace rsr {
+
+// ------------------------------------------------------------------------------------
+// L-BFGS (src/engine/optimize/lbfgs.rs:9-165)
+//   T = float : reference parity (f32 everywhere, second loop starts at slot 0 when the
+//               history is not full -- lbfgs.rs:80, including after a reset with head != 0)
+//   T = double: the two-loop dots, curvature pairs, gamma, directional derivative and the
+//               Armijo test in FP64, second loop from the oldest live pair
+// ------------------------------------------------------------------------------------
+template <typename T>
+__global__ void k_lbfgs_begin(LbfgsDev st, const float* __restrict__ params, const float* __restrict__ d_mse,
+                              const float* __restrict__ d_grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    const int prog = st.prog[i];
+    const int np = st.nparam[i];
+    const float* pp = params + st.param_off[i];
+    for (int j = 0; j < kMaxP; ++j) {
+        st.x[j * st.n + i] = j < np ? pp[j] : 0.0f;
+        st.g[j * st.n + i] = d_grad[(int64_t)prog * 32 + j];
+    }
+    st.cur_mse[i] = d_mse[prog];
+    st.hist[i] = 0;
+    st.head[i] = 0;
+    st.flags[i] = np == 0 ? LB_DONE : 0;
+}
+
+template <typename T>
+__device__ __forceinline__ T& at(T* a, int slot, int j, int n, int i) {
+    return a[((int64_t)slot * kMaxP + j) * n + i];
+}
+
+template <typename T>
+__global__ void k_lbfgs_direction(LbfgsDev st, float* __restrict__ params, uint8_t* __restrict__ active_ls,
+                                  int* __restrict__ n_working) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    const int prog = st.prog[i];
+    active_ls[prog] = 0;
+    if (st.flags[i] & LB_DONE) return;
+    const int n = st.nparam[i];
+    const int N = st.n;
+    const int M = st.M;
+    T* S = (T*)st.S;
+    T* Y = (T*)st.Y;
+    T* rho = (T*)st.rho;
+    T* alpha = (T*)st.alpha;
+    T q[kMaxP];
+    T gn = (T)0;
+    for (int j = 0; j < n; ++j) {
+        const T gj = (T)st.g[j * N + i];
+        gn += gj * gj;
+    }
+    if (sqrt(gn) < (T)1e-5) {
+        st.flags[i] |= LB_DONE;
+        return;
+    }
+    for (int j = 0; j < n; ++j) q[j] = (T)st.g[j * N + i];
+    int hist = st.hist[i], head = st.head[i];
+    int ci = head;
+    for (int h = 0; h < hist; ++h) {
+        ci = ci == 0 ? M - 1 : ci - 1;
+        T d = (T)0;
+        for (int j = 0; j < n; ++j) d += at(S, ci, j, N, i) * q[j];
+        const T a = rho[ci * N + i] * d;
+        alpha[ci * N + i] = a;
+        for (int j = 0; j < n; ++j) q[j] -= a * at(Y, ci, j, N, i);
+    }
+    T gamma = (T)1;
+    if (hist > 0) {
+        const int li = head == 0 ? M - 1 : head - 1;
+        T sy = (T)0, yy = (T)0;
+        for (int j = 0; j < n; ++j) {
+            sy += at(S, li, j, N, i) * at(Y, li, j, N, i);
+            yy += at(Y, li, j, N, i) * at(Y, li, j, N, i);
+        }
+        if (yy > (T)1e-10) gamma = sy / yy;
+    }
+    for (int j = 0; j < n; ++j) q[j] *= gamma;
+    if constexpr (sizeof(T) == 4) ci = hist < M ? 0 : head;  // reference (lbfgs.rs:80)
+    else ci = (head - hist + M) % M;                         // oldest live pair
+    for (int h = 0; h < hist; ++h) {
+        T d = (T)0;
+        for (int j = 0; j < n; ++j) d += at(Y, ci, j, N, i) * q[j];
+        const T beta = rho[ci * N + i] * d;
+        const T am = alpha[ci * N + i] - beta;
+        for (int j = 0; j < n; ++j) q[j] += at(S, ci, j, N, i) * am;
+        ci = (ci + 1) % M;
+    }
+    T ddg = (T)0;
+    for (int j = 0; j < n; ++j) {
+        const float pj = (float)(-q[j]);
+        st.p[j * N + i] = pj;
+        ddg += (T)pj * (T)st.g[j * N + i];
+    }
+    if (ddg > (T)-1e-8) {  // steepest-descent reset (lbfgs.rs:100-106)
+        for (int j = 0; j < n; ++j) st.p[j * N + i] = -st.g[j * N + i];
+        ddg = -gn;
+        hist = 0;
+    }
+    st.hist[i] = hist;
+    ((T*)st.ddg)[i] = ddg;
+    st.step[i] = 1.0f;
+    st.ls[i] = 0;
+    st.flags[i] |= LB_SEARCH;
+    float* pp = params + st.param_off[i];
+    for (int j = 0; j < n; ++j) pp[j] = st.x[j * N + i] + 1.0f * st.p[j * N + i];
+    active_ls[prog] = 1;
+    atomicAdd(n_working, 1);
+}
+
+template <typename T>
+__global__ void k_lbfgs_ls_check(LbfgsDev st, float* __restrict__ params, const float* __restrict__ d_mse,
+                                 uint8_t* __restrict__ active_ls, uint8_t* __restrict__ active_grad,
+                                 int* __restrict__ n_searching) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    if (!(st.flags[i] & LB_SEARCH)) return;
+    const int prog = st.prog[i];
+    const int n = st.nparam[i];
+    const int N = st.n;
+    const float m = d_mse[prog];
+    const float step = st.step[i];
+    bool accept;
+    if constexpr (sizeof(T) == 4) {
+        const float c1 = 1e-4f;
+        accept = m <= st.cur_mse[i] + c1 * step * ((float*)st.ddg)[i];
+    } else {
+        accept = (double)m <= (double)st.cur_mse[i] + 1e-4 * (double)step * ((double*)st.ddg)[i];
+    }
+    if (accept || st.ls[i] > 10) {
+        st.next_mse[i] = m;
+        st.flags[i] &= ~LB_SEARCH;
+        st.flags[i] |= LB_MOVED;
+        active_ls[prog] = 0;
+        active_grad[prog] = 1;
+        return;
+    }
+    const float ns = step * 0.5f;
+    st.step[i] = ns;
+    st.ls[i] += 1;
+    float* pp = params + st.param_off[i];
+    for (int j = 0; j < n; ++j) pp[j] = st.x[j * N + i] + ns * st.p[j * N + i];
+    atomicAdd(n_searching, 1);
+}
+
+template <typename T>
+__global__ void k_lbfgs_update(LbfgsDev st, const float* __restrict__ d_grad, uint8_t* __restrict__ active_grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    const int prog = st.prog[i];
+    active_grad[prog] = 0;
+    if (!(st.flags[i] & LB_MOVED)) return;
+    st.flags[i] &= ~LB_MOVED;
+    const int n = st.nparam[i];
+    const int N = st.n;
+    const int M = st.M;
+    T* S = (T*)st.S;
+    T* Y = (T*)st.Y;
+    T sn[kMaxP], yn[kMaxP];
+    T sy = (T)0;
+    const float step = st.step[i];
+    for (int j = 0; j < n; ++j) {
+        const float x = st.x[j * N + i];
+        const float nx = x + step * st.p[j * N + i];
+        const float g = st.g[j * N + i];
+        const float ng = d_grad[(int64_t)prog * 32 + j];
+        if constexpr (sizeof(T) == 4) {
+            sn[j] = nx - x;
+            yn[j] = ng - g;
+        } else {
+            sn[j] = (T)nx - (T)x;
+            yn[j] = (T)ng - (T)g;
+        }
+        sy += sn[j] * yn[j];
+        st.x[j * N + i] = nx;
+    }
+    for (int j = 0; j < kMaxP; ++j) st.g[j * N + i] = d_grad[(int64_t)prog * 32 + j];
+    if (sy > (T)1e-10) {
+        const int head = st.head[i];
+        for (int j = 0; j < n; ++j) {
+            at(S, head, j, N, i) = sn[j];
+            at(Y, head, j, N, i) = yn[j];
+        }
+        ((T*)st.rho)[head * N + i] = (T)1 / sy;
+        st.head[i] = (head + 1) % M;
+        if (st.hist[i] < M) st.hist[i] += 1;
+    }
+    st.cur_mse[i] = st.next_mse[i];
+}
+
+__global__ void k_lbfgs_finish(LbfgsDev st, float* __restrict__ params, float* __restrict__ fitness) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    const int n = st.nparam[i];
+    float* pp = params + st.param_off[i];
+    for (int j = 0; j < n; ++j) pp[j] = st.x[j * st.n + i];
+    fitness[i] = n > 0 ? st.cur_mse[i] : __builtin_nanf("");
+}
+
+#define RSR_BLK(n) dim3((unsigned)(((n) + 127) / 128)), dim3(128)
+
+hipError_t lbfgs_begin(const LbfgsDev& st, const float* params, const float* d_mse, const float* d_grad,
+                       hipStream_t s) {
+    if (st.fp64) hipLaunchKernelGGL(k_lbfgs_begin<double>, RSR_BLK(st.n), 0, s, st, params, d_mse, d_grad);
+    else hipLaunchKernelGGL(k_lbfgs_begin<float>, RSR_BLK(st.n), 0, s, st, params, d_mse, d_grad);
+    return hipGetLastError();
+}
+
+hipError_t lbfgs_direction(const LbfgsDev& st, float* params, uint8_t* active_ls, int* counter, hipStream_t s) {
+    if (st.fp64) hipLaunchKernelGGL(k_lbfgs_direction<double>, RSR_BLK(st.n), 0, s, st, params, active_ls, counter);
+    else hipLaunchKernelGGL(k_lbfgs_direction<float>, RSR_BLK(st.n), 0, s, st, params, active_ls, counter);
+    return hipGetLastError();
+}
+
+hipError_t lbfgs_ls_check(const LbfgsDev& st, float* params, const float* d_mse, uint8_t* active_ls,
+                          uint8_t* active_grad, int* counter, hipStream_t s) {
+    if (st.fp64)
+        hipLaunchKernelGGL(k_lbfgs_ls_check<double>, RSR_BLK(st.n), 0, s, st, params, d_mse, active_ls, active_grad,
+                           counter);
+    else
+        hipLaunchKernelGGL(k_lbfgs_ls_check<float>, RSR_BLK(st.n), 0, s, st, params, d_mse, active_ls, active_grad,
+                           counter);
+    return hipGetLastError();
+}
+
+hipError_t lbfgs_update(const LbfgsDev& st, const float* d_grad, uint8_t* active_grad, hipStream_t s) {
+    if (st.fp64) hipLaunchKernelGGL(k_lbfgs_update<double>, RSR_BLK(st.n), 0, s, st, d_grad, active_grad);
+    else hipLaunchKernelGGL(k_lbfgs_update<float>, RSR_BLK(st.n), 0, s, st, d_grad, active_grad);
+    return hipGetLastError();
+}
+
+hipError_t lbfgs_finish(const LbfgsDev& st, float* params, float* fitness, hipStream_t s) {
+    hipLaunchKernelGGL(k_lbfgs_finish, RSR_BLK(st.n), 0, s, st, params, fitness);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// Diagonal CMA-ES (src/engine/optimize/cmaes.rs:19-192)
+// ------------------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as
+// 1, 2, 3", SC'11): counter-based, so candidate (i, it, c, j) draws the same numbers
+// regardless of launch geometry.
+__host__ __device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Standard normal for (individual, iteration, candidate, dimension): two uniforms in
+// [0, 1) from the top 24 bits, then Box-Muller exactly as cmaes.rs:13-17.
+__host__ __device__ __forceinline__ float cma_normal(uint64_t seed, uint32_t ind, uint32_t it, uint32_t cand,
+                                                     uint32_t j) {
+    uint32_t c[4] = {cand * 32u + j, it, ind, 0x43414D45u};
+    philox4x32(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    float u1 = (float)(c[0] >> 8) * (1.0f / 16777216.0f);
+    const float u2 = (float)(c[1] >> 8) * (1.0f / 16777216.0f);
+    u1 = u1 > 1e-8f ? u1 : 1e-8f;  // .max(1e-8)
+    return __builtin_sqrtf(-2.0f * dm_ln(u1)) * dm_cos(2.0f * 3.14159274101257324f * u2);
+}
+
+__global__ void k_cmaes_sample(CmaDev st, float* __restrict__ cand_params, int it) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / st.lambda, c = t - i * st.lambda;
+    if (i >= st.n) return;
+    const int N = st.n;
+    const int64_t ic = (int64_t)i * st.lambda + c;
+    const bool done = (st.flags[i] & CMA_DONE) != 0;
+    st.cand_active[ic] = done ? 0 : 1;
+    if (done) return;
+    const int nd = st.ndim[i];
+    float l1 = 0.0f;
+    float* pp = cand_params + st.cand_off[ic];
+    const float sigma = st.sigma[i];
+    for (int j = 0; j < nd; ++j) {
+        const float z = st.normals ? st.normals[(((int64_t)it * N + i) * st.lambda + c) * kMaxP + j]
+                                   : cma_normal(st.seed, (uint32_t)st.prog[i], (uint32_t)it, (uint32_t)c, (uint32_t)j);
+        const float step = z * __builtin_sqrtf(st.cdiag[j * N + i]);
+        const float val = st.mean[j * N + i] + sigma * step;
+        st.steps[ic * kMaxP + j] = step;
+        st.pop[ic * kMaxP + j] = val;
+        l1 += __builtin_fabsf(val);
+        if (j < st.nparam[i]) pp[j] = val;  // unflatten_params of the 32-slot buffer
+    }
+    st.l1[ic] = l1;
+}
+
+__global__ void k_cmaes_update(CmaDev st, const float* __restrict__ cand_mse, int* __restrict__ n_working) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    if (st.flags[i] & CMA_DONE) return;
+    const int N = st.n, L = st.lambda, mu = st.mu;
+    const int nd = st.ndim[i];
+    const int64_t base = (int64_t)i * L;
+    float fit[kMaxLambda];
+    int idx[kMaxLambda];
+    for (int c = 0; c < L; ++c) {
+        fit[c] = cand_mse[base + c] + 0.01f * st.l1[base + c];
+        idx[c] = c;
+    }
+    // ascending fitness; ties by candidate index (deterministic)
+    for (int a = 1; a < L; ++a) {
+        const int k = idx[a];
+        int b = a - 1;
+        while (b >= 0 && (fit[idx[b]] > fit[k] || (fit[idx[b]] == fit[k] && idx[b] > k))) {
+            idx[b + 1] = idx[b];
+            --b;
+        }
+        idx[b + 1] = k;
+    }
+    const int bi = idx[0];
+    if (fit[bi] < st.best_fit[i]) {
+        st.best_fit[i] = fit[bi];
+        st.best_mse[i] = cand_mse[base + bi];
+        for (int j = 0; j < nd; ++j) st.best[j * N + i] = st.pop[(base + bi) * kMaxP + j];
+    }
+    if (st.best_mse[i] < 1e-8f) {
+        st.flags[i] |= CMA_DONE;
+        return;
+    }
+    float sm[kMaxP], old[kMaxP];
+    for (int j = 0; j < nd; ++j) {
+        sm[j] = 0.0f;
+        old[j] = st.mean[j * N + i];
+    }
+    for (int r = 0; r < mu; ++r) {
+        const int k = idx[r];
+        const float w = st.weights[r];
+        for (int j = 0; j < nd; ++j) {
+            sm[j] += w * st.steps[(base + k) * kMaxP + j];
+            st.mean[j * N + i] += w * (st.pop[(base + k) * kMaxP + j] - old[j]);
+        }
+    }
+    const float cc = st.c_c[i], ccov = st.c_cov[i], me = st.mu_eff;
+    const float sq = __builtin_sqrtf(cc * (2.0f - cc) * me);
+    for (int j = 0; j < nd; ++j) {
+        const float pc = (1.0f - cc) * st.pc[j * N + i] + sq * sm[j];
+        st.pc[j * N + i] = pc;
+        float cu = 0.0f;
+        for (int r = 0; r < mu; ++r) {
+            const float sv = st.steps[(base + idx[r]) * kMaxP + j];
+            cu += st.weights[r] * (sv * sv);
+        }
+        float cd = (1.0f - ccov) * st.cdiag[j * N + i] + (ccov / me) * (pc * pc) + ccov * (1.0f - 1.0f / me) * cu;
+        cd = cd < 1e-6f ? 1e-6f : (cd > 1e6f ? 1e6f : cd);
+        st.cdiag[j * N + i] = cd;
+    }
+    st.sigma[i] *= st.sigma_decay;
+    st.iters[i] += 1;
+    atomicAdd(n_working, 1);
+}
+
+__global__ void k_cmaes_finish(CmaDev st, float* __restrict__ params, float* __restrict__ fitness,
+                               int* __restrict__ iters_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    const int np = st.nparam[i];
+    float* pp = params + st.param_off[i];
+    for (int j = 0; j < np; ++j) pp[j] = st.best[j * st.n + i];
+    fitness[i] = st.ndim[i] > 0 ? st.best_mse[i] : __builtin_nanf("");
+    if (iters_out) iters_out[i] = st.iters[i] + ((st.flags[i] & CMA_DONE) ? 1 : 0);
+}
+
+hipError_t cmaes_sample(const CmaDev& st, float* cand_params, int it, hipStream_t s) {
+    const int64_t t = (int64_t)st.n * st.lambda;
+    hipLaunchKernelGGL(k_cmaes_sample, dim3((unsigned)((t + 127) / 128)), dim3(128), 0, s, st, cand_params, it);
+    return hipGetLastError();
+}
+
+hipError_t cmaes_update(const CmaDev& st, const float* cand_mse, int* counter, hipStream_t s) {
+    hipLaunchKernelGGL(k_cmaes_update, RSR_BLK(st.n), 0, s, st, cand_mse, counter);
+    return hipGetLastError();
+}
+
+hipError_t cmaes_finish(const CmaDev& st, float* params, float* fitness, int* iters_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_cmaes_finish, RSR_BLK(st.n), 0, s, st, params, fitness, iters_out);
+    return hipGetLastError();
+}
+
+float cma_normal_host(uint64_t seed, uint32_t ind, uint32_t it, uint32_t cand, uint32_t j) {
+    return cma_normal(seed, ind, it, cand, j);
+}
+
+}  // namespace rsr

@@ -125,7 +125,10 @@ extern "C" int rsr_context_create(int device_ordinal, uint64_t hbm_budget_bytes,
     HIP_TRY(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
     c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(free_b * 0.9);
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device_ordinal) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device_ordinal) == hipSuccess) {
+        c->num_cus = prop.multiProcessorCount;
+        c->max_lds = prop.sharedMemPerBlock;
+    }
     *out = c.release();
     return RSR_OK;
 }
@@ -215,6 +218,14 @@ extern "C" int rsr_context_set_tuning(rsr_context* ctx, int ppl, int group) {
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->tuning.ppl = ppl;
     ctx->tuning.group = group;
+    return RSR_OK;
+}
+
+extern "C" int rsr_context_set_summation(rsr_context* ctx, int mode) {
+    if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
+    if (mode != RSR_SUM_KAHAN && mode != RSR_SUM_REFERENCE) return set_error(RSR_E_INVALID_ARG, "bad summation mode");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->summation = mode;
     return RSR_OK;
 }
 
@@ -626,11 +637,23 @@ int rsr::eval_mse_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* d
     if (rc) return rc;
     if (Lout) *Lout = L;
     const DevDataset D = dev_view(ds);
-    const EvalTuning t = ctx->tuning;
+    EvalTuning t = ctx->tuning;
+    const DevPrograms DP = dev_view(p, L);
+    const uint8_t* act0 = active_override ? active_override : (L->n_invalid ? (const uint8_t*)L->dactive.ptr : nullptr);
+    if (ctx->summation == RSR_SUM_REFERENCE) {
+        if (eval_mse_lds_bytes(DP, 1) > ctx->max_lds)
+            return set_error(RSR_E_UNSUPPORTED, "spill area exceeds the workgroup LDS limit");
+        HIP_TRY(launch_eval_mse_exact(DP, D, act0, d_mse, ctx->stream, ctx->next_timer()), "eval_mse_exact launch");
+        return RSR_OK;
+    }
+    while (t.ppl > 1 && eval_mse_lds_bytes(DP, t.ppl) > ctx->max_lds) t.ppl /= 2;  // spill area must fit
+    if (eval_mse_lds_bytes(DP, t.ppl) > ctx->max_lds)
+        return set_error(RSR_E_UNSUPPORTED, "spill area of %zu bytes exceeds the workgroup LDS limit",
+                         eval_mse_lds_bytes(DP, t.ppl));
     const size_t nparts = (size_t)eval_tiles(D, t.ppl) * std::max(p->n, 1);
     if ((rc = ensure_scratch(ctx, ctx->scratch_partials, nparts * sizeof(float)))) return rc;
     const uint8_t* act = active_override ? active_override : (L->n_invalid ? (const uint8_t*)L->dactive.ptr : nullptr);
-    HIP_TRY(launch_eval_mse(dev_view(p, L), D, act, t, (float*)ctx->scratch_partials.ptr, d_mse, ctx->stream,
+    HIP_TRY(launch_eval_mse(DP, D, act, t, (float*)ctx->scratch_partials.ptr, d_mse, ctx->stream,
                             ctx->next_timer()),
             "eval_mse launch");
     return RSR_OK;
@@ -691,7 +714,9 @@ int rsr::eval_grad_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* 
     const DevDataset D = dev_view(ds);
     // zero-parameter programs: compute_mse on the primal lowering (evaluator.rs:187-189)
     bool any_zero = false, any_pos = false;
-    std::vector<uint8_t> act0(p->n, 0), act1(p->n, 0);
+    ctx->host_act.assign(2 * (size_t)std::max(p->n, 1), 0);
+    uint8_t* act0 = ctx->host_act.data();
+    uint8_t* act1 = act0 + std::max(p->n, 1);
     for (int i = 0; i < p->n; ++i) {
         const bool on = host_active == nullptr || host_active[i];
         if (Ld->nparams[i] == 0) {
@@ -704,17 +729,30 @@ int rsr::eval_grad_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* 
     if ((rc = ensure_scratch(ctx, ctx->scratch_act, 2 * (size_t)std::max(p->n, 1)))) return rc;
     uint8_t* dact0 = (uint8_t*)ctx->scratch_act.ptr;
     uint8_t* dact1 = dact0 + std::max(p->n, 1);
-    HIP_TRY(hipMemcpyAsync(dact0, act0.data(), (size_t)p->n, hipMemcpyHostToDevice, ctx->stream), "act upload");
-    HIP_TRY(hipMemcpyAsync(dact1, act1.data(), (size_t)p->n, hipMemcpyHostToDevice, ctx->stream), "act upload");
+    HIP_TRY(hipMemcpyAsync(dact0, act0, 2 * (size_t)std::max(p->n, 1), hipMemcpyHostToDevice, ctx->stream),
+            "act upload");
     HIP_TRY(hipMemsetAsync(d_grad, 0, (size_t)p->n * 32 * sizeof(float), ctx->stream), "grad clear");
-    if (any_zero) {
-        const size_t nparts = (size_t)eval_tiles(D, ctx->tuning.ppl) * p->n;
+    if (any_zero && ctx->summation == RSR_SUM_REFERENCE) {
+        HIP_TRY(launch_eval_mse_exact(dev_view(p, Lp), D, dact0, d_mse, ctx->stream), "eval_mse_exact launch");
+    } else if (any_zero) {
+        EvalTuning t = ctx->tuning;
+        const DevPrograms DP = dev_view(p, Lp);
+        while (t.ppl > 1 && eval_mse_lds_bytes(DP, t.ppl) > ctx->max_lds) t.ppl /= 2;
+        if (eval_mse_lds_bytes(DP, t.ppl) > ctx->max_lds)
+            return set_error(RSR_E_UNSUPPORTED, "spill area exceeds the workgroup LDS limit");
+        const size_t nparts = (size_t)eval_tiles(D, t.ppl) * p->n;
         if ((rc = ensure_scratch(ctx, ctx->scratch_partials, nparts * sizeof(float)))) return rc;
-        HIP_TRY(launch_eval_mse(dev_view(p, Lp), D, dact0, ctx->tuning, (float*)ctx->scratch_partials.ptr, d_mse,
-                                ctx->stream),
+        HIP_TRY(launch_eval_mse(DP, D, dact0, t, (float*)ctx->scratch_partials.ptr, d_mse, ctx->stream),
                 "eval_mse launch");
     }
-    if (any_pos) {
+    if (any_pos && eval_grad_lds_bytes(dev_view(p, Ld)) > ctx->max_lds)
+        return set_error(RSR_E_UNSUPPORTED, "dual spill area of %zu bytes exceeds the workgroup LDS limit",
+                         eval_grad_lds_bytes(dev_view(p, Ld)));
+    if (any_pos && ctx->summation == RSR_SUM_REFERENCE) {
+        HIP_TRY(launch_eval_grad_exact(dev_view(p, Ld), D, (const uint8_t*)Ld->dnparams.ptr, dact1, Ld->max_params,
+                                       d_mse, d_grad, ctx->stream, ctx->next_timer()),
+                "eval_grad_exact launch");
+    } else if (any_pos) {
         const size_t nparts = (size_t)eval_tiles(D, 1) * p->n * 33;
         if ((rc = ensure_scratch(ctx, ctx->scratch_partials, nparts * sizeof(float)))) return rc;
         HIP_TRY(launch_eval_grad(dev_view(p, Ld), D, (const uint8_t*)Ld->dnparams.ptr, dact1, Ld->max_params,

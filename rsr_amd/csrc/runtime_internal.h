@@ -51,12 +51,15 @@ int hip_error(hipError_t e, const char* what);
 struct rsr_context {
     int device = 0;
     int num_cus = 256;
+    size_t max_lds = 65536;  // dynamic LDS bytes per workgroup
+    std::vector<uint8_t> host_act;  // staging for active masks (lives until the stream syncs)
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
     uint64_t used = 0;
     std::recursive_mutex rmu;
     std::mutex mu;
     rsr::EvalTuning tuning;
+    int summation = 0;  // 0: pairwise + Kahan (fast), 1: reference order (bit-identical to the CPU evaluator)
     rsr::DevBuf scratch_partials, scratch_out, scratch_out2, scratch_act;
     // profiling: events around every main evaluation kernel, accumulated on read
     bool profiling = false;
@@ -106,4 +109,8 @@ int eval_mse_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, co
                     float* d_mse, Lowering** Lout);
 int eval_grad_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* host_active,
                      float* d_mse, float* d_grad, Lowering** Ldual_out, Lowering** Lprim_out);
+int fit_lbfgs_impl(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int max_iterations, int history, int mode,
+                   const uint8_t* mask, float* fitness_out);
+int fit_cmaes_impl(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int max_iterations, int lambda, int mu,
+                   uint64_t seed, const float* normals, const uint8_t* mask, float* fitness_out, int32_t* iters_out);
 }  // namespace rsr

@@ -261,3 +261,25 @@ def test_full_size_c1_properties(gpu_ctx):
     P2 = api.Programs(gpu_ctx, [progs[i] for i in perm])
     mse2 = api.compute_mse(gpu_ctx, P2, ds)
     assert np.array_equal(mse2, mse[perm])
+
+
+@pytest.mark.parametrize("gen,data,npts", [(W.gen_c0, W.data_c0, 403), (W.gen_c1, W.data_c1, 1001),
+                                            (W.gen_c2, W.data_c2, 500), (W.gen_c3, W.data_c3, 777)])
+def test_reference_order_summation_bit_identical(gpu_ctx, gen, data, npts):
+    X, y, t = data(npts, seed=41)
+    progs = gen(96, seed=42)
+    ds, ods = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, progs)
+    gpu_ctx.set_summation(True)
+    try:
+        mse = api.compute_mse(gpu_ctx, P, ds)
+        m2, g2 = api.compute_mse_with_gradient(gpu_ctx, P, ds)
+    finally:
+        gpu_ctx.set_summation(False)
+    for i, p in enumerate(progs):
+        ref = O.compute_mse(p, ods)
+        assert mse[i] == ref, (i, mse[i], ref)
+        rm, rg = O.compute_mse_with_gradient(p, ods)
+        assert m2[i] == rm, (i, m2[i], rm)
+        ok, nbad = same_bits(g2[i], rg)
+        assert ok, (i, g2[i][:4], rg[:4])

@@ -324,7 +324,8 @@ def test_cmaes_with_injected_normals_improves():
 
 
 def test_cmaes_weights_match_formula():
-    w, mu_eff = O.cmaes_weights(8)
+    w, mu_eff, decay = O.cmaes_weights(8)
+    assert decay == pytest.approx(math.exp(-0.01), rel=1e-7)
     raw = np.array([math.log(8.5) - math.log(i + 1) for i in range(8)])
     assert np.allclose(w, raw / raw.sum(), rtol=1e-6)
     assert mu_eff == pytest.approx(1.0 / np.sum((raw / raw.sum()) ** 2), rel=1e-5)
