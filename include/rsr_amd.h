@@ -90,13 +90,21 @@ RSR_API int rsr_context_profile_read(rsr_context* ctx, double* kernel_ms, int64_
 RSR_API int rsr_context_synchronize(rsr_context* ctx);
 /* Summation of the per-point squared errors and gradient terms:
  *   RSR_SUM_KAHAN (default): per-tile pairwise sums combined with Kahan-compensated
- *     FP32 (fast; within 1e-5 relative of the reference's f32 sums);
+ *     FP32: close to the exact sum (about 1e-6 relative to FP64), so it differs from the
+ *     reference's plain f32 sums by their own rounding error, which grows with the
+ *     number of points (about 1e-5 relative at 10^5 points);
  *   RSR_SUM_REFERENCE: the reference's order -- 8 lane accumulators fed point by point
  *     (evaluator.rs:159-170) and the f32x8 horizontal add -- bit-identical to the CPU
- *     evaluator; one wave per program. */
+ *     evaluator.  MSE keeps the per-point terms in a device arena and folds them in
+ *     order; gradients use one wave per (program, tangent pass). */
 #define RSR_SUM_KAHAN 0
 #define RSR_SUM_REFERENCE 1
 RSR_API int rsr_context_set_summation(rsr_context* ctx, int mode);
+/* Upper bound (bytes, >= 4096) of the device arena that holds the per-tile Kahan partials
+ * (or, in reference summation, the per-point terms) of one launch; larger evaluations are
+ * split into tile ranges with the summation state carried between them, which gives the
+ * same results.  Default 4 GiB. */
+RSR_API int rsr_context_set_partials_limit(rsr_context* ctx, uint64_t bytes);
 /* Evaluation tuning: points per lane (1, 2, 4) and programs per work item. */
 RSR_API int rsr_context_set_tuning(rsr_context* ctx, int points_per_lane, int programs_per_item);
 RSR_API int rsr_context_get_tuning(const rsr_context* ctx, int* points_per_lane, int* programs_per_item);
@@ -178,6 +186,64 @@ RSR_API int rsr_fit_cmaes(rsr_context* ctx, rsr_programs* progs, const rsr_datas
  * candidate, dimension): Philox4x32-10 -> two 24-bit uniforms -> Box-Muller as
  * cmaes.rs:13-17. */
 RSR_API float rsr_cmaes_normal(uint64_t seed, uint32_t program, uint32_t iteration, uint32_t candidate, uint32_t dim);
+
+/* ---------------------------------------------------------------------------------
+ * Serving (the "async submit/poll" of the boundary; the reference's callers are rayon
+ * workers calling the evaluator in-process, engine.rs:52, island.rs:150-238).
+ *
+ * A server owns one device thread.  Every admitted request is a state machine over
+ * evaluation steps (an evaluation, a gradient, one L-BFGS line-search round, one CMA-ES
+ * generation, ...).  Each loop iteration (a tick) runs the next step of every admitted
+ * request together in merged launches, then each request's optimizer update, so requests
+ * that arrive while others are on the GPU join at the next tick (continuous batching).
+ * Requests are admitted in arrival order while fewer than max_inflight are in flight and
+ * their device state fits budget_bytes (0: the context's budget); two requests on the
+ * same program batch run one after the other.  A request's results are the same alone
+ * and in any mix.
+ * ------------------------------------------------------------------------------- */
+typedef struct rsr_server rsr_server;
+#define RSR_REQ_EVAL 0   /* compute_mse of the masked programs: out = mse[n] */
+#define RSR_REQ_GRAD 1   /* compute_mse_with_gradient: out = mse[n], out2 = grad[n][32] */
+#define RSR_REQ_LBFGS 2  /* rsr_fit_lbfgs: out = fitness[n]; constants updated in place */
+#define RSR_REQ_CMAES 3  /* rsr_fit_cmaes: out = fitness[n], iters_out[n] (optional) */
+typedef struct rsr_request {
+    int kind;                    /* RSR_REQ_* */
+    rsr_programs* programs;      /* not used by any other call until the request completes */
+    const rsr_dataset* dataset;  /* read only; requests may share it */
+    const uint8_t* mask;         /* optional [n]; copied at submit */
+    int iterations;              /* L-BFGS, CMA-ES */
+    int history;                 /* L-BFGS history (reference: 6) */
+    int mode;                    /* L-BFGS: RSR_LBFGS_* */
+    int lambda, mu;              /* CMA-ES */
+    uint64_t seed;               /* CMA-ES normal stream */
+    float* out;                  /* caller-owned outputs, valid until completion */
+    float* out2;
+    int32_t* iters_out;
+} rsr_request;
+typedef struct rsr_server_stats {
+    uint64_t submitted, completed, failed;
+    uint64_t queued, inflight, max_inflight_seen;
+    uint64_t ticks;               /* loop iterations that ran at least one request */
+    uint64_t node_evals;          /* program nodes x points evaluated by completed requests */
+    uint64_t inflight_bytes;      /* device bytes held by in-flight requests */
+    double busy_seconds;          /* wall time with at least one request in flight */
+} rsr_server_stats;
+RSR_API int rsr_server_create(rsr_context* ctx, int max_inflight, uint64_t budget_bytes, int max_queue,
+                              rsr_server** out);
+/* Queue a request; returns RSR_E_QUEUE_FULL when max_queue requests are waiting. */
+RSR_API int rsr_server_submit(rsr_server* s, const rsr_request* req, uint64_t* ticket);
+/* RSR_OK with *status = the request's result code once it completed, else RSR_E_NOT_READY. */
+RSR_API int rsr_server_poll(rsr_server* s, uint64_t ticket, int* status);
+/* Block until the request completes (timeout_seconds < 0: no limit); RSR_E_TIMEOUT otherwise. */
+RSR_API int rsr_server_wait(rsr_server* s, uint64_t ticket, double timeout_seconds, int* status);
+/* Seconds from submit to admission and from admission to completion, and the ticks it took. */
+RSR_API int rsr_server_timing(rsr_server* s, uint64_t ticket, double* queued_seconds, double* run_seconds,
+                              int32_t* ticks);
+/* Forget a completed request (its timing is no longer available). */
+RSR_API int rsr_server_release(rsr_server* s, uint64_t ticket);
+RSR_API int rsr_server_get_stats(rsr_server* s, rsr_server_stats* out);
+/* Stop after every queued and in-flight request has completed. */
+RSR_API int rsr_server_destroy(rsr_server* s);
 
 /* Lowering self-check (test hook, no reference counterpart, no GPU): lower one
  * program exactly as the evaluation entry points do and run the emitted micro-code

@@ -47,6 +47,7 @@ SIGNATURES = {
     "rsr_context_profile_read": (C.c_int, [_P, C.POINTER(C.c_double), _i64p]),
     "rsr_context_synchronize": (C.c_int, [_P]),
     "rsr_context_set_summation": (C.c_int, [_P, C.c_int]),
+    "rsr_context_set_partials_limit": (C.c_int, [_P, C.c_uint64]),
     "rsr_dataset_create": (C.c_int, [_P, _f32p, _f32p, C.c_int64, _u8p, C.c_int, C.c_int, _PP]),
     "rsr_dataset_create_aosoa": (C.c_int, [_P, _f32p, _f32p, C.c_int, C.c_int64, C.c_int64, _PP]),
     "rsr_dataset_subset": (C.c_int, [_P, _P, C.c_int64, _PP]),
@@ -143,6 +144,10 @@ class Context:
     def set_summation(self, reference_order: bool):
         """False: pairwise + Kahan (default); True: the reference's 8-lane order (bit-identical)."""
         _check(lib().rsr_context_set_summation(self.h, 1 if reference_order else 0))
+
+    def set_partials_limit(self, nbytes: int):
+        """Bound the Kahan-partials arena of one launch (larger evaluations split into tile ranges)."""
+        _check(lib().rsr_context_set_partials_limit(self.h, int(nbytes)))
 
     def synchronize(self):
         _check(lib().rsr_context_synchronize(self.h))
@@ -379,3 +384,154 @@ def fit_cmaes(ctx: Context, progs: Programs, ds: Dataset, iterations: int, lam: 
 def cmaes_normal(seed: int, program: int, iteration: int, candidate: int, dim: int) -> float:
     """The normal the CMA-ES sampler draws (host restatement of the device function)."""
     return _bind_fit().rsr_cmaes_normal(seed, program, iteration, candidate, dim)
+
+
+# ---------------------------------------------------------------------------------
+# serving: submit / poll / wait (rsr_server_*)
+# ---------------------------------------------------------------------------------
+REQ_EVAL, REQ_GRAD, REQ_LBFGS, REQ_CMAES = 0, 1, 2, 3
+RSR_E_NOT_READY = -10
+RSR_E_TIMEOUT = -9
+
+
+class _Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("programs", C.c_void_p), ("dataset", C.c_void_p), ("mask", C.c_void_p),
+                ("iterations", C.c_int), ("history", C.c_int), ("mode", C.c_int), ("lam", C.c_int),
+                ("mu", C.c_int), ("seed", C.c_uint64), ("out", C.c_void_p), ("out2", C.c_void_p),
+                ("iters_out", C.c_void_p)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("queued", C.c_uint64), ("inflight", C.c_uint64), ("max_inflight_seen", C.c_uint64),
+                ("ticks", C.c_uint64), ("node_evals", C.c_uint64), ("inflight_bytes", C.c_uint64),
+                ("busy_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _bind_server():
+    l = lib()
+    if getattr(l, "_server_bound", False):
+        return l
+    l.rsr_server_create.restype = C.c_int
+    l.rsr_server_create.argtypes = [_P, C.c_int, C.c_uint64, C.c_int, _PP]
+    l.rsr_server_submit.restype = C.c_int
+    l.rsr_server_submit.argtypes = [_P, C.POINTER(_Request), C.POINTER(C.c_uint64)]
+    l.rsr_server_poll.restype = C.c_int
+    l.rsr_server_poll.argtypes = [_P, C.c_uint64, C.POINTER(C.c_int)]
+    l.rsr_server_wait.restype = C.c_int
+    l.rsr_server_wait.argtypes = [_P, C.c_uint64, C.c_double, C.POINTER(C.c_int)]
+    l.rsr_server_timing.restype = C.c_int
+    l.rsr_server_timing.argtypes = [_P, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double), _i32p]
+    l.rsr_server_release.restype = C.c_int
+    l.rsr_server_release.argtypes = [_P, C.c_uint64]
+    l.rsr_server_get_stats.restype = C.c_int
+    l.rsr_server_get_stats.argtypes = [_P, C.POINTER(ServerStats)]
+    l.rsr_server_destroy.restype = C.c_int
+    l.rsr_server_destroy.argtypes = [_P]
+    l._server_bound = True
+    return l
+
+
+class Ticket:
+    """A submitted request: its id, its output arrays and the objects it keeps alive."""
+
+    def __init__(self, server: "Server", tid: int, kind: int, n: int, keep, out, out2=None, iters=None):
+        self.server, self.id, self.kind, self.n = server, tid, kind, n
+        self._keep = keep
+        self.out, self.out2, self.iters = out, out2, iters
+
+    def done(self) -> bool:
+        return self.server.poll(self)
+
+    def wait(self, timeout: float = -1.0):
+        """Block until complete; returns the outputs (raises RsrError if the request failed)."""
+        self.server.wait(self, timeout)
+        if self.kind == REQ_EVAL:
+            return self.out[: self.n]
+        if self.kind == REQ_GRAD:
+            return self.out[: self.n], self.out2[: self.n]
+        if self.kind == REQ_CMAES:
+            return self.out[: self.n], self.iters[: self.n]
+        return self.out[: self.n]
+
+    def timing(self):
+        """(seconds queued, seconds running, ticks)."""
+        q, r, t = C.c_double(), C.c_double(), C.c_int32()
+        _check(_bind_server().rsr_server_timing(self.server.h, self.id, C.byref(q), C.byref(r), C.byref(t)))
+        return q.value, r.value, t.value
+
+
+class Server:
+    """The request loop (rsr_server_*): continuous batching of evaluation and constant-fit
+    requests over device-resident datasets and populations."""
+
+    def __init__(self, ctx: Context, max_inflight: int = 64, budget_bytes: int = 0, max_queue: int = 4096):
+        l = _bind_server()
+        h = C.c_void_p()
+        _check(l.rsr_server_create(ctx.h, max_inflight, budget_bytes, max_queue, C.byref(h)))
+        self.h = h
+        self.ctx = ctx
+
+    def _submit(self, kind, progs: Programs, ds: Dataset, mask=None, iterations=0, history=6,
+                mode=0, lam=16, mu=8, seed=0):
+        n = progs.n
+        out = np.zeros(max(n, 1), np.float32)
+        out2 = np.zeros((max(n, 1), 32), np.float32) if kind == REQ_GRAD else None
+        iters = np.zeros(max(n, 1), np.int32) if kind == REQ_CMAES else None
+        m = _mask(progs, mask)
+        r = _Request(kind, progs.h.value if isinstance(progs.h, C.c_void_p) else progs.h,
+                     ds.h.value if isinstance(ds.h, C.c_void_p) else ds.h,
+                     m.ctypes.data if m is not None else None, iterations, history, mode, lam, mu, seed,
+                     out.ctypes.data, out2.ctypes.data if out2 is not None else None,
+                     iters.ctypes.data if iters is not None else None)
+        tid = C.c_uint64()
+        _check(_bind_server().rsr_server_submit(self.h, C.byref(r), C.byref(tid)))
+        return Ticket(self, tid.value, kind, n, (progs, ds, m), out, out2, iters)
+
+    def submit_eval(self, progs: Programs, ds: Dataset, mask=None) -> Ticket:
+        return self._submit(REQ_EVAL, progs, ds, mask)
+
+    def submit_grad(self, progs: Programs, ds: Dataset, mask=None) -> Ticket:
+        return self._submit(REQ_GRAD, progs, ds, mask)
+
+    def submit_lbfgs(self, progs: Programs, ds: Dataset, iterations: int = 15, history: int = 6,
+                     mode: int = LBFGS_REFERENCE, mask=None) -> Ticket:
+        return self._submit(REQ_LBFGS, progs, ds, mask, iterations=iterations, history=history, mode=mode)
+
+    def submit_cmaes(self, progs: Programs, ds: Dataset, iterations: int, lam: int = 16, mu: int = 8,
+                     seed: int = 0, mask=None) -> Ticket:
+        return self._submit(REQ_CMAES, progs, ds, mask, iterations=iterations, lam=lam, mu=mu, seed=seed)
+
+    def poll(self, t: Ticket) -> bool:
+        st = C.c_int()
+        rc = _bind_server().rsr_server_poll(self.h, t.id, C.byref(st))
+        if rc == RSR_E_NOT_READY:
+            return False
+        _check(rc)
+        _check(st.value)
+        return True
+
+    def wait(self, t: Ticket, timeout: float = -1.0):
+        st = C.c_int()
+        rc = _bind_server().rsr_server_wait(self.h, t.id, timeout, C.byref(st))
+        _check(rc)
+        _check(st.value)
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _check(_bind_server().rsr_server_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            _bind_server().rsr_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

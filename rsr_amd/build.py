@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librsr_amd.so")
-SOURCES = ["kernels.hip", "lower.cpp", "runtime.cpp", "hostcheck.hip", "optim.hip", "fit.cpp", "serve.cpp"]
+SOURCES = ["kernels.hip", "lower.cpp", "runtime.cpp", "hostcheck.hip", "optim.hip", "jobs.cpp", "fit.cpp", "serve.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [
@@ -35,7 +35,9 @@ def build(verbose: bool = False) -> str:
             cmd[1:1] = ["-x", "hip"]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True, stderr=None if verbose else subprocess.PIPE)
+        r = subprocess.run(cmd, stderr=None if verbose else subprocess.PIPE, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("compile of %s failed:\n%s" % (src, r.stderr or ""))
         objs.append(obj)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]

@@ -1,0 +1,787 @@
+// jobs.cpp -- evaluation, gradient, L-BFGS and CMA-ES requests as resumable jobs, and
+// the tick that merges the evaluation steps of many jobs into shared launches.
+#include "jobs.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+
+#include "../../include/rsr_amd.h"
+
+#define HIP_TRY(expr, what)                               \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return hip_error(e_, what); \
+    } while (0)
+
+namespace rsr {
+
+int JobBuffers::raw(size_t bytes, void** out) {
+    const size_t n = std::max<size_t>(bytes, 16);
+    bufs.emplace_back(new DevBuf());
+    int rc = bufs.back()->alloc(ctx, n);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(bufs.back()->ptr, 0, n, ctx->stream), "job buffer clear");
+    *out = bufs.back()->ptr;
+    return RSR_OK;
+}
+
+int JobBuffers::upload(void* dst, const void* src, size_t bytes) {
+    keep.emplace_back((const char*)src, (const char*)src + bytes);
+    HIP_TRY(hipMemcpyAsync(dst, keep.back().data(), bytes, hipMemcpyHostToDevice, ctx->stream), "job upload");
+    return RSR_OK;
+}
+
+size_t JobBuffers::bytes() const {
+    size_t s = 0;
+    for (const auto& b : bufs) s += b->bytes;
+    return s;
+}
+
+namespace {
+
+uint64_t prog_nodes(const rsr_programs* p, int i) { return (uint64_t)(p->code_off[i + 1] - p->code_off[i]); }
+
+// Copy the device parameters back into the host constants (unflatten, program.rs:147-192).
+int download_params(rsr_programs* p) {
+    rsr_context* ctx = p->ctx;
+    HIP_TRY(hipMemcpyAsync(p->params.data(), p->dparams.ptr, p->params.size() * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream),
+            "params download");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "params download");
+    for (int i = 0; i < p->n; ++i) {
+        int64_t v = p->value_off[i];
+        uint32_t f = p->param_off[i];
+        for (int64_t c = p->const_off[i]; c < p->const_off[i + 1]; ++c) {
+            const int s = vt_size(p->ctypes[c]);
+            for (int k = 0; k < s; ++k) p->values[v + k] = p->params[f + k];
+            v += s ? s : 1;
+            f += s;
+        }
+    }
+    return RSR_OK;
+}
+
+EvalStep mse_step(const rsr_programs* p, const Lowering* L, const rsr_dataset* ds, const uint8_t* active,
+                  float* d_mse) {
+    EvalStep st;
+    st.P = dev_view(p, L);
+    st.D = dev_view(ds);
+    st.active = active;
+    st.mse_out = d_mse;
+    return st;
+}
+
+EvalStep grad_step(const rsr_programs* p, const Lowering* Ld, const rsr_dataset* ds, const uint8_t* active,
+                   float* d_mse, float* d_grad) {
+    EvalStep st = mse_step(p, Ld, ds, active, d_mse);
+    st.grad_out = d_grad;
+    st.nparams = (const uint8_t*)Ld->dnparams.ptr;
+    st.max_params = Ld->max_params;
+    return st;
+}
+
+// ---------------------------------------------------------------------------------
+// compute_mse (evaluator.rs:153-174)
+// ---------------------------------------------------------------------------------
+class EvalJob final : public Job {
+   public:
+    explicit EvalJob(rsr_context* c) : ctx(c), jb(c) {}
+    rsr_context* ctx;
+    rsr_programs* p = nullptr;
+    const rsr_dataset* ds = nullptr;
+    Lowering* L = nullptr;
+    JobBuffers jb;
+    float* d_mse = nullptr;
+    uint8_t* d_act = nullptr;
+    float* out = nullptr;
+    std::vector<uint8_t> on;
+    int n_bad = 0;
+    uint64_t nodes = 0;
+
+    int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>&) override {
+        mse.push_back(mse_step(p, L, ds, d_act, d_mse));
+        node_evals += nodes * (uint64_t)ds->num_samples;
+        return RSR_OK;
+    }
+    int advance(const int*) override {
+        done = true;
+        return RSR_OK;
+    }
+    bool needs_counters() const override { return false; }
+    int finish() override {
+        HIP_TRY(hipMemcpyAsync(out, d_mse, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+                "mse download");
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "mse download");
+        for (int i = 0; i < p->n; ++i)
+            if (!on[i]) out[i] = NAN;
+        if (n_bad) return set_error(RSR_E_PROGRAM, "%d program(s) rejected by lowering", n_bad);
+        return RSR_OK;
+    }
+    size_t device_bytes() const override { return jb.bytes(); }
+};
+
+// ---------------------------------------------------------------------------------
+// compute_mse_with_gradient (evaluator.rs:177-224)
+// ---------------------------------------------------------------------------------
+class GradJob final : public Job {
+   public:
+    explicit GradJob(rsr_context* c) : ctx(c), jb(c) {}
+    rsr_context* ctx;
+    rsr_programs* p = nullptr;
+    const rsr_dataset* ds = nullptr;
+    Lowering *Lp = nullptr, *Ld = nullptr;
+    JobBuffers jb;
+    float *d_mse = nullptr, *d_grad = nullptr;
+    uint8_t *d_act0 = nullptr, *d_act1 = nullptr;
+    bool any_zero = false, any_pos = false;
+    float *mse_out = nullptr, *grad_out = nullptr;
+    std::vector<uint8_t> on;
+    int n_bad = 0;
+    uint64_t nodes = 0;
+
+    int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>& grad) override {
+        // zero-parameter programs are scored by compute_mse on the primal lowering (evaluator.rs:187-189)
+        if (any_zero) mse.push_back(mse_step(p, Lp, ds, d_act0, d_mse));
+        if (any_pos) grad.push_back(grad_step(p, Ld, ds, d_act1, d_mse, d_grad));
+        node_evals += nodes * (uint64_t)ds->num_samples;
+        return RSR_OK;
+    }
+    int advance(const int*) override {
+        done = true;
+        return RSR_OK;
+    }
+    bool needs_counters() const override { return false; }
+    int finish() override {
+        HIP_TRY(hipMemcpyAsync(mse_out, d_mse, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+                "mse download");
+        HIP_TRY(hipMemcpyAsync(grad_out, d_grad, (size_t)p->n * 32 * sizeof(float), hipMemcpyDeviceToHost,
+                               ctx->stream),
+                "grad download");
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "grad download");
+        for (int i = 0; i < p->n; ++i)
+            if (!on[i]) mse_out[i] = NAN;
+        if (n_bad) return set_error(RSR_E_PROGRAM, "%d program(s) rejected by lowering", n_bad);
+        return RSR_OK;
+    }
+    size_t device_bytes() const override { return jb.bytes(); }
+};
+
+// ---------------------------------------------------------------------------------
+// run_lbfgs (src/engine/optimize/lbfgs.rs:9-165), one state machine for the batch:
+//   INIT   : gradient of every individual; begin; first direction (counter = working)
+//   SEARCH : MSE of the searching individuals; Armijo check (counter = still searching);
+//            at most 12 rounds per iteration
+//   GRAD   : gradient of the moved individuals; curvature update; next direction
+// ---------------------------------------------------------------------------------
+class LbfgsJob final : public Job {
+   public:
+    enum Stage { INIT, SEARCH, GRAD };
+    explicit LbfgsJob(rsr_context* c) : ctx(c), jb(c) {}
+    rsr_context* ctx;
+    rsr_programs* p = nullptr;
+    const rsr_dataset* ds = nullptr;
+    Lowering *Lp = nullptr, *Ld = nullptr;
+    JobBuffers jb;
+    LbfgsDev st;
+    std::vector<int> prog;
+    uint8_t *act_fit = nullptr, *act_ls = nullptr, *act_grad = nullptr;
+    float *d_mse = nullptr, *d_grad = nullptr, *d_fit = nullptr;
+    float* fitness_out = nullptr;
+    int iters = 0, it = 0, ls_round = 0, working = 0;
+    Stage stage = INIT;
+    uint64_t mean_nodes = 0;
+
+    int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>& grad) override {
+        const uint64_t n_eval = stage == INIT ? prog.size() : (uint64_t)working;
+        node_evals += n_eval * mean_nodes * (uint64_t)ds->num_samples;
+        if (stage == SEARCH)
+            mse.push_back(mse_step(p, Lp, ds, act_ls, d_mse));
+        else
+            grad.push_back(grad_step(p, Ld, ds, stage == INIT ? act_fit : act_grad, d_mse, d_grad));
+        return RSR_OK;
+    }
+    int post() override {
+        float* params = (float*)p->dparams.ptr;
+        switch (stage) {
+            case INIT:
+                HIP_TRY(lbfgs_begin(st, params, d_mse, d_grad, ctx->stream), "lbfgs_begin");
+                if (iters > 0) HIP_TRY(lbfgs_direction(st, params, act_ls, d_counters, ctx->stream), "lbfgs_direction");
+                break;
+            case SEARCH:
+                HIP_TRY(lbfgs_ls_check(st, params, d_mse, act_ls, act_grad, d_counters, ctx->stream), "ls_check");
+                break;
+            case GRAD:
+                HIP_TRY(lbfgs_update(st, d_grad, act_grad, ctx->stream), "lbfgs_update");
+                if (it + 1 < iters)
+                    HIP_TRY(lbfgs_direction(st, params, act_ls, d_counters, ctx->stream), "lbfgs_direction");
+                break;
+        }
+        return RSR_OK;
+    }
+    int advance(const int* c) override {
+        switch (stage) {
+            case INIT:
+                working = c[0];
+                if (iters == 0 || working == 0) {
+                    done = true;
+                } else {
+                    stage = SEARCH;
+                    ls_round = 0;
+                }
+                break;
+            case SEARCH:
+                ++ls_round;
+                if (c[0] == 0 || ls_round == 12) stage = GRAD;  // Armijo backtracking, at most 12 evaluations
+                break;
+            case GRAD:
+                ++it;
+                if (it >= iters || c[0] == 0) {
+                    done = true;
+                } else {
+                    working = c[0];
+                    stage = SEARCH;
+                    ls_round = 0;
+                }
+                break;
+        }
+        return RSR_OK;
+    }
+    int finish() override {
+        if (prog.empty()) return RSR_OK;
+        HIP_TRY(lbfgs_finish(st, (float*)p->dparams.ptr, d_fit, ctx->stream), "lbfgs_finish");
+        std::vector<float> fit(prog.size());
+        HIP_TRY(hipMemcpyAsync(fit.data(), d_fit, fit.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+                "fitness");
+        int rc = download_params(p);  // synchronises the stream
+        if (rc) return rc;
+        for (size_t k = 0; k < prog.size(); ++k) fitness_out[prog[k]] = fit[k];
+        return RSR_OK;
+    }
+    size_t device_bytes() const override { return jb.bytes(); }
+};
+
+// ---------------------------------------------------------------------------------
+// run_cma_es (src/engine/optimize/cmaes.rs:19-192), one state machine for the batch:
+//   INIT : MSE of every parent; best = doubled mean
+//   RUN  : sample lambda candidates per individual; MSE of every live candidate (one
+//          candidate batch: each program repeated lambda times with its own parameter
+//          block); selection and update (counter = individuals still working)
+// ---------------------------------------------------------------------------------
+class CmaesJob final : public Job {
+   public:
+    enum Stage { INIT, RUN };
+    explicit CmaesJob(rsr_context* c) : ctx(c), jb(c) {}
+    rsr_context* ctx;
+    rsr_programs* p = nullptr;
+    const rsr_dataset* ds = nullptr;
+    Lowering* Lp = nullptr;
+    JobBuffers jb;
+    CmaDev st;
+    std::vector<int> prog;
+    uint8_t* act_fit = nullptr;
+    float *d_mse0 = nullptr, *d_cmse = nullptr, *d_cparams = nullptr, *d_fit = nullptr;
+    int* d_iters = nullptr;
+    DevPrograms CP{};
+    float* fitness_out = nullptr;
+    int32_t* iters_out = nullptr;
+    int iters = 0, it = 0, working = 0;
+    Stage stage = INIT;
+    uint64_t mean_nodes = 0;
+
+    int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>&) override {
+        if (stage == INIT) {
+            mse.push_back(mse_step(p, Lp, ds, act_fit, d_mse0));
+            node_evals += prog.size() * mean_nodes * (uint64_t)ds->num_samples;
+            return RSR_OK;
+        }
+        HIP_TRY(cmaes_sample(st, d_cparams, it, ctx->stream), "cmaes_sample");
+        EvalStep s;
+        s.P = CP;
+        s.D = dev_view(ds);
+        s.active = st.cand_active;
+        s.mse_out = d_cmse;
+        mse.push_back(s);
+        node_evals += (uint64_t)working * st.lambda * mean_nodes * (uint64_t)ds->num_samples;
+        return RSR_OK;
+    }
+    int post() override {
+        if (stage == INIT)
+            HIP_TRY(cmaes_begin(st, d_mse0, ctx->stream), "cmaes_begin");
+        else
+            HIP_TRY(cmaes_update(st, d_cmse, d_counters, ctx->stream), "cmaes_update");
+        return RSR_OK;
+    }
+    int advance(const int* c) override {
+        if (stage == INIT) {
+            working = (int)prog.size();
+            stage = RUN;
+            if (iters == 0) done = true;
+            return RSR_OK;
+        }
+        ++it;
+        working = c[0];
+        if (it >= iters || working == 0) done = true;  // every individual has converged
+        return RSR_OK;
+    }
+    int finish() override {
+        if (prog.empty()) return RSR_OK;
+        HIP_TRY(cmaes_finish(st, (float*)p->dparams.ptr, d_fit, d_iters, ctx->stream), "cmaes_finish");
+        std::vector<float> fit(prog.size());
+        std::vector<int> its(prog.size());
+        HIP_TRY(hipMemcpyAsync(fit.data(), d_fit, fit.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+                "fit");
+        HIP_TRY(hipMemcpyAsync(its.data(), d_iters, its.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+                "iters");
+        int rc = download_params(p);  // synchronises the stream
+        if (rc) return rc;
+        for (size_t k = 0; k < prog.size(); ++k) {
+            fitness_out[prog[k]] = fit[k];
+            if (iters_out) iters_out[prog[k]] = its[k];
+        }
+        return RSR_OK;
+    }
+    size_t device_bytes() const override { return jb.bytes(); }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// job construction
+// ---------------------------------------------------------------------------------
+int make_eval_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* mask, float* mse_out,
+                  std::unique_ptr<Job>* out) {
+    auto j = std::make_unique<EvalJob>(ctx);
+    j->p = p;
+    j->ds = ds;
+    j->out = mse_out;
+    int rc = get_lowering(p, ds->num_features, false, &j->L);
+    if (rc) return rc;
+    j->on.assign(p->n, 0);
+    bool all = true;
+    for (int i = 0; i < p->n; ++i) {
+        const bool want = mask == nullptr || mask[i] != 0;
+        const bool ok = j->L->status[i] == PROG_OK;
+        if (want && !ok) j->n_bad++;
+        j->on[i] = want && ok;
+        if (!j->on[i]) all = false;
+        if (j->on[i]) j->nodes += prog_nodes(p, i);
+    }
+    if ((rc = j->jb.get((size_t)p->n, &j->d_mse))) return rc;
+    if (!all && (rc = j->jb.put(j->on, &j->d_act))) return rc;
+    *out = std::move(j);
+    return RSR_OK;
+}
+
+int make_grad_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* mask, float* mse_out,
+                  float* grad_out, std::unique_ptr<Job>* out) {
+    auto j = std::make_unique<GradJob>(ctx);
+    j->p = p;
+    j->ds = ds;
+    j->mse_out = mse_out;
+    j->grad_out = grad_out;
+    int rc = get_lowering(p, ds->num_features, false, &j->Lp);
+    if (rc) return rc;
+    if ((rc = get_lowering(p, ds->num_features, true, &j->Ld))) return rc;
+    const Lowering* Lp = j->Lp;
+    const Lowering* Ld = j->Ld;
+    std::vector<uint8_t> act0(p->n, 0), act1(p->n, 0);
+    j->on.assign(p->n, 0);
+    for (int i = 0; i < p->n; ++i) {
+        const bool want = mask == nullptr || mask[i] != 0;
+        if (!want) continue;
+        const bool zero = Ld->nparams[i] == 0;
+        const bool ok = zero ? Lp->status[i] == PROG_OK : Ld->status[i] == PROG_OK;
+        if (!ok) {
+            j->n_bad++;
+            continue;
+        }
+        j->on[i] = 1;
+        j->nodes += prog_nodes(p, i);
+        if (zero) {
+            act0[i] = 1;
+            j->any_zero = true;
+        } else {
+            act1[i] = 1;
+            j->any_pos = true;
+        }
+    }
+    if ((rc = j->jb.get((size_t)p->n, &j->d_mse)) || (rc = j->jb.get((size_t)p->n * 32, &j->d_grad)) ||
+        (rc = j->jb.put(act0, &j->d_act0)) || (rc = j->jb.put(act1, &j->d_act1)))
+        return rc;
+    *out = std::move(j);
+    return RSR_OK;
+}
+
+int make_lbfgs_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, int history, int mode,
+                   const uint8_t* mask, float* fitness_out, std::unique_ptr<Job>* out) {
+    if (history < 1 || history > 8 || iterations < 0)
+        return set_error(RSR_E_INVALID_ARG, "history must be 1..8, iterations >= 0");
+    if (mode != RSR_LBFGS_REFERENCE && mode != RSR_LBFGS_FP64) return set_error(RSR_E_INVALID_ARG, "bad mode");
+    auto j = std::make_unique<LbfgsJob>(ctx);
+    j->p = p;
+    j->ds = ds;
+    j->iters = iterations;
+    j->fitness_out = fitness_out;
+    int rc = get_lowering(p, ds->num_features, false, &j->Lp);
+    if (rc) return rc;
+    if ((rc = get_lowering(p, ds->num_features, true, &j->Ld))) return rc;
+    std::vector<int> nparam;
+    std::vector<uint32_t> poff;
+    uint64_t nodes = 0;
+    for (int i = 0; i < p->n; ++i) {
+        fitness_out[i] = NAN;
+        if (mask && !mask[i]) continue;
+        if (j->Ld->status[i] != PROG_OK || j->Lp->status[i] != PROG_OK) continue;
+        if (j->Ld->nparams[i] == 0) continue;  // lbfgs.rs:18-21: nothing to fit, fitness untouched
+        j->prog.push_back(i);
+        nparam.push_back(j->Ld->nparams[i]);
+        poff.push_back(p->param_off[i]);
+        nodes += prog_nodes(p, i);
+    }
+    const int n = (int)j->prog.size();
+    j->mean_nodes = n ? nodes / n : 0;
+    if (n == 0) {
+        j->done = true;
+        *out = std::move(j);
+        return RSR_OK;
+    }
+    JobBuffers& sc = j->jb;
+    LbfgsDev& st = j->st;
+    st.n = n;
+    st.M = history;
+    st.fp64 = mode == RSR_LBFGS_FP64 ? 1 : 0;
+    const size_t tb = st.fp64 ? sizeof(double) : sizeof(float);
+    int *d_prog, *d_np;
+    uint32_t* d_poff;
+    if ((rc = sc.put(j->prog, &d_prog)) || (rc = sc.put(nparam, &d_np)) || (rc = sc.put(poff, &d_poff))) return rc;
+    st.prog = d_prog;
+    st.nparam = d_np;
+    st.param_off = d_poff;
+    if ((rc = sc.get((size_t)kMaxP * n, &st.x)) || (rc = sc.get((size_t)kMaxP * n, &st.g)) ||
+        (rc = sc.get((size_t)kMaxP * n, &st.p)) || (rc = sc.get((size_t)n, &st.step)) ||
+        (rc = sc.get((size_t)n, &st.cur_mse)) || (rc = sc.get((size_t)n, &st.next_mse)))
+        return rc;
+    if ((rc = sc.raw((size_t)8 * kMaxP * n * tb, &st.S)) || (rc = sc.raw((size_t)8 * kMaxP * n * tb, &st.Y)) ||
+        (rc = sc.raw((size_t)8 * n * tb, &st.rho)) || (rc = sc.raw((size_t)8 * n * tb, &st.alpha)) ||
+        (rc = sc.raw((size_t)n * tb, &st.ddg)))
+        return rc;
+    if ((rc = sc.get((size_t)n, &st.hist)) || (rc = sc.get((size_t)n, &st.head)) || (rc = sc.get((size_t)n, &st.ls)) ||
+        (rc = sc.get((size_t)n, &st.flags)))
+        return rc;
+    std::vector<uint8_t> fitmask(p->n, 0);
+    for (int i : j->prog) fitmask[i] = 1;
+    if ((rc = sc.put(fitmask, &j->act_fit)) || (rc = sc.get((size_t)p->n, &j->act_ls)) ||
+        (rc = sc.get((size_t)p->n, &j->act_grad)) || (rc = sc.get((size_t)p->n, &j->d_mse)) ||
+        (rc = sc.get((size_t)p->n * 32, &j->d_grad)) || (rc = sc.get((size_t)n, &j->d_fit)))
+        return rc;
+    *out = std::move(j);
+    return RSR_OK;
+}
+
+int make_cmaes_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, int lambda, int mu,
+                   uint64_t seed, const float* normals, const uint8_t* mask, float* fitness_out, int32_t* iters_out,
+                   std::unique_ptr<Job>* out) {
+    if (lambda < 2 || lambda > kMaxLambda || mu < 1 || mu > lambda || iterations < 0)
+        return set_error(RSR_E_INVALID_ARG, "lambda must be 2..%d and 1 <= mu <= lambda", kMaxLambda);
+    auto j = std::make_unique<CmaesJob>(ctx);
+    j->p = p;
+    j->ds = ds;
+    j->iters = iterations;
+    j->fitness_out = fitness_out;
+    j->iters_out = iters_out;
+    int rc = get_lowering(p, ds->num_features, false, &j->Lp);
+    if (rc) return rc;
+    const Lowering* Lp = j->Lp;
+    std::vector<int> nparam, ndim;
+    std::vector<uint32_t> poff;
+    uint64_t nodes = 0;
+    for (int i = 0; i < p->n; ++i) {
+        fitness_out[i] = NAN;
+        if (iters_out) iters_out[i] = 0;
+        if (mask && !mask[i]) continue;
+        if (Lp->status[i] != PROG_OK) continue;
+        const int pc = (int)(p->param_off[i + 1] - p->param_off[i]);
+        if (pc == 0) continue;  // cmaes.rs:25-28
+        j->prog.push_back(i);
+        nparam.push_back(std::min(pc, kMaxP));
+        ndim.push_back(std::min(std::min(pc, kMaxP) + pc, kMaxP));
+        poff.push_back(p->param_off[i]);
+        nodes += prog_nodes(p, i);
+    }
+    const std::vector<int>& prog = j->prog;
+    const int n = (int)prog.size();
+    j->mean_nodes = n ? nodes / n : 0;
+    if (n == 0) {
+        j->done = true;
+        *out = std::move(j);
+        return RSR_OK;
+    }
+    // recombination weights and strategy constants in f32 as cmaes.rs:85-102
+    std::vector<float> w(mu);
+    float sumw = 0.0f;
+    const float lmu = logf((float)mu + 0.5f);
+    for (int i = 0; i < mu; ++i) {
+        const float v = lmu - logf((float)(i + 1));
+        w[i] = v > 0.0f ? v : 0.0f;
+        sumw += w[i];
+    }
+    float me = 0.0f;
+    for (int i = 0; i < mu; ++i) {
+        w[i] = w[i] / sumw;
+        me += w[i] * w[i];
+    }
+    me = 1.0f / me;
+    std::vector<float> cc(n), ccov(n), mean((size_t)kMaxP * n, 0.0f), cdiag((size_t)kMaxP * n, 1.0f),
+        best((size_t)kMaxP * n, 0.0f), sigma(n, 0.5f);
+    std::vector<int64_t> cand_off((size_t)n * lambda);
+    std::vector<float> cand_params;
+    for (int k = 0; k < n; ++k) {
+        const int i = prog[k];
+        const float nf = (float)ndim[k];
+        cc[k] = (4.0f + me / nf) / (nf + 4.0f + 2.0f * me / nf);
+        const float t1 = nf + 1.414f, t2 = nf + 2.0f;
+        ccov[k] = (1.0f / me) * (2.0f / (t1 * t1)) + (1.0f - 1.0f / me) * ((2.0f * me - 1.0f) / (t2 * t2 + me));
+        const float* pp = &p->params[p->param_off[i]];
+        const int pc = (int)(p->param_off[i + 1] - p->param_off[i]);
+        for (int jj = 0; jj < nparam[k]; ++jj) mean[(size_t)jj * n + k] = pp[jj];
+        for (int jj = nparam[k]; jj < ndim[k]; ++jj) mean[(size_t)jj * n + k] = pp[jj - nparam[k]];
+        for (int jj = 0; jj < ndim[k]; ++jj) best[(size_t)jj * n + k] = mean[(size_t)jj * n + k];
+        for (int c = 0; c < lambda; ++c) {
+            cand_off[(size_t)k * lambda + c] = (int64_t)cand_params.size();
+            cand_params.insert(cand_params.end(), pp, pp + pc);
+        }
+    }
+    cand_params.push_back(0.0f);
+    JobBuffers& sc = j->jb;
+    CmaDev& st = j->st;
+    std::vector<uint8_t> fitmask(p->n, 0);
+    for (int i : prog) fitmask[i] = 1;
+    if ((rc = sc.put(fitmask, &j->act_fit)) || (rc = sc.get((size_t)p->n, &j->d_mse0))) return rc;
+    st.n = n;
+    st.lambda = lambda;
+    st.mu = mu;
+    st.mu_eff = me;
+    st.sigma_decay = expf(-0.01f);  // (-0.01_f32).exp() (cmaes.rs:178)
+    st.seed = seed;
+    float *d_w, *d_normals = nullptr, *d_cc, *d_ccov;
+    int *d_prog, *d_np, *d_nd;
+    uint32_t* d_poff;
+    int64_t* d_coff;
+    if ((rc = sc.put(w, &d_w)) || (rc = sc.put(prog, &d_prog)) || (rc = sc.put(nparam, &d_np)) ||
+        (rc = sc.put(ndim, &d_nd)) || (rc = sc.put(poff, &d_poff)) || (rc = sc.put(cand_off, &d_coff)) ||
+        (rc = sc.put(cc, &d_cc)) || (rc = sc.put(ccov, &d_ccov)))
+        return rc;
+    if (normals && iterations > 0) {
+        const size_t nn = (size_t)iterations * n * lambda * kMaxP;
+        if ((rc = sc.get(nn, &d_normals)) || (rc = sc.upload(d_normals, normals, nn * sizeof(float)))) return rc;
+    }
+    st.weights = d_w;
+    st.normals = d_normals;
+    st.prog = d_prog;
+    st.nparam = d_np;
+    st.ndim = d_nd;
+    st.param_off = d_poff;
+    st.cand_off = d_coff;
+    st.c_c = d_cc;
+    st.c_cov = d_ccov;
+    if ((rc = sc.put(mean, &st.mean)) || (rc = sc.put(cdiag, &st.cdiag)) || (rc = sc.get((size_t)kMaxP * n, &st.pc)) ||
+        (rc = sc.put(best, &st.best)) || (rc = sc.put(sigma, &st.sigma)) || (rc = sc.get((size_t)n, &st.best_mse)) ||
+        (rc = sc.get((size_t)n, &st.best_fit)) || (rc = sc.get((size_t)n * lambda * kMaxP, &st.steps)) ||
+        (rc = sc.get((size_t)n * lambda * kMaxP, &st.pop)) || (rc = sc.get((size_t)n * lambda, &st.l1)) ||
+        (rc = sc.get((size_t)n * lambda, &st.cand_active)) || (rc = sc.get((size_t)n, &st.flags)) ||
+        (rc = sc.get((size_t)n, &st.iters)))
+        return rc;
+    // candidate batch: every individual's program repeated lambda times, own parameter blocks,
+    // scheduled in the parent batch's cost order
+    if ((rc = sc.put(cand_params, &j->d_cparams))) return rc;
+    std::vector<ProgDesc> hdesc((size_t)n * lambda);
+    for (int k = 0; k < n; ++k)
+        for (int c = 0; c < lambda; ++c) {
+            ProgDesc d = Lp->desc[prog[k]];
+            d.param_off = (uint32_t)cand_off[(size_t)k * lambda + c];
+            hdesc[(size_t)k * lambda + c] = d;
+        }
+    std::vector<int> rank(p->n);
+    for (int r = 0; r < p->n; ++r) rank[Lp->order[r]] = r;
+    std::vector<int32_t> corder((size_t)n * lambda);
+    std::iota(corder.begin(), corder.end(), 0);
+    std::stable_sort(corder.begin(), corder.end(),
+                     [&](int a, int b) { return rank[prog[a / lambda]] < rank[prog[b / lambda]]; });
+    ProgDesc* d_desc;
+    int32_t* d_order;
+    if ((rc = sc.put(hdesc, &d_desc)) || (rc = sc.put(corder, &d_order)) ||
+        (rc = sc.get((size_t)n * lambda, &j->d_cmse)) || (rc = sc.get((size_t)n, &j->d_fit)) ||
+        (rc = sc.get((size_t)n, &j->d_iters)))
+        return rc;
+    j->CP.code = (const UInstr*)Lp->dcode.ptr;
+    j->CP.progs = d_desc;
+    j->CP.params = j->d_cparams;
+    j->CP.order = d_order;
+    j->CP.n_progs = n * lambda;
+    j->CP.spill_slots = Lp->spill_slots;
+    *out = std::move(j);
+    return RSR_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// ticks
+// ---------------------------------------------------------------------------------
+namespace {
+
+int ensure_counters(rsr_context* ctx, size_t n_ints) {
+    if (ctx->host_counters_cap < n_ints) {
+        if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
+        ctx->host_counters = nullptr;
+        ctx->host_counters_cap = 0;
+        const size_t cap = std::max<size_t>(n_ints, 64);
+        HIP_TRY(hipHostMalloc((void**)&ctx->host_counters, cap * sizeof(int), hipHostMallocDefault), "counters");
+        ctx->host_counters_cap = cap;
+    }
+    return ensure_scratch(ctx, ctx->dev_counters, n_ints * sizeof(int));
+}
+
+// Run steps of one kind (MSE or gradient) at one points-per-lane.  The per-tile Kahan
+// partials (or, for reference-order MSE, the per-point terms) live in a shared arena of at
+// most ctx->arena_bytes; when the steps need more, every step is split into the same
+// number of tile ranges, evaluated round by round with the summation state carried
+// between rounds (the sums are the same as unsplit).
+int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int ppl) {
+    const bool ref = ctx->summation == RSR_SUM_REFERENCE;
+    EvalTuning t = ctx->tuning;
+    t.ppl = ppl;
+    if (ref && grad) {  // one wave per (program, tangent pass) walks the tiles in order
+        HIP_TRY(launch_grad_steps(steps.data(), (int)steps.size(), true, ctx->stream, ctx->next_timer()),
+                "reference-order gradient launch");
+        return RSR_OK;
+    }
+    const bool terms = ref;  // reference-order MSE: per-point terms, folded in order
+    auto part_floats = [&](const EvalStep& st) {
+        return grad ? grad_partials_floats(st) : mse_partials_floats(st, ppl, terms);
+    };
+    auto state_floats = [&](const EvalStep& st) { return grad ? grad_state_floats(st) : mse_state_floats(st, terms); };
+    size_t total = 0;
+    std::vector<int64_t> tiles(steps.size());
+    for (size_t i = 0; i < steps.size(); ++i) {
+        tiles[i] = eval_tiles(steps[i].D, ppl);
+        total += part_floats(steps[i]);
+    }
+    total *= sizeof(float);
+    const size_t cap = std::max<size_t>(ctx->arena_bytes, 4096);
+    const int64_t rounds = (int64_t)((total + cap - 1) / cap);
+    int rc;
+    float* state = nullptr;
+    if (rounds > 1) {
+        size_t sf = 0;
+        for (const auto& st : steps) sf += state_floats(st);
+        DevBuf& sb = grad ? ctx->scratch_gstate : ctx->scratch_state;
+        if ((rc = ensure_scratch(ctx, sb, sf * sizeof(float)))) return rc;
+        state = (float*)sb.ptr;
+    }
+    for (int64_t r = 0; r < rounds; ++r) {
+        std::vector<EvalStep> round;
+        size_t need = 0;
+        float* st_next = state;
+        for (size_t i = 0; i < steps.size(); ++i) {
+            EvalStep s = steps[i];
+            const int64_t chunk = (tiles[i] + rounds - 1) / rounds;
+            s.tile_begin = r * chunk;
+            s.tile_count = std::min<int64_t>(chunk, tiles[i] - s.tile_begin);
+            s.first = r == 0;
+            s.last = s.tile_begin + s.tile_count >= tiles[i];
+            s.kstate = st_next;
+            if (state) st_next += state_floats(s);
+            if (s.tile_count <= 0) continue;
+            need += part_floats(s);
+            round.push_back(s);
+        }
+        if (round.empty()) continue;
+        DevBuf& arena_buf = grad ? ctx->scratch_gpartials : ctx->scratch_partials;
+        if ((rc = ensure_scratch(ctx, arena_buf, std::max<size_t>(need, 1) * sizeof(float)))) return rc;
+        float* arena = (float*)arena_buf.ptr;
+        for (auto& s : round) {
+            s.partials = arena;
+            arena += part_floats(s);
+        }
+        HIP_TRY(grad ? launch_grad_steps(round.data(), (int)round.size(), false, ctx->stream, ctx->next_timer())
+                     : launch_mse_steps(round.data(), (int)round.size(), t, terms, ctx->stream, ctx->next_timer()),
+                "evaluation launch");
+    }
+    return RSR_OK;
+}
+
+// MSE steps: each step at the tuning's points per lane, lowered until its own spill area
+// fits the workgroup LDS (per step, so a request's result does not depend on the others).
+int run_mse(rsr_context* ctx, std::vector<EvalStep>& steps) {
+    std::vector<EvalStep> by_ppl[3];
+    for (const auto& s : steps) {
+        int ppl = ctx->tuning.ppl;
+        while (ppl > 1 && eval_mse_lds_bytes(s.P.spill_slots, ppl) > ctx->max_lds) ppl /= 2;
+        if (eval_mse_lds_bytes(s.P.spill_slots, ppl) > ctx->max_lds)
+            return set_error(RSR_E_UNSUPPORTED, "spill area of %zu bytes exceeds the workgroup LDS limit",
+                             eval_mse_lds_bytes(s.P.spill_slots, ppl));
+        by_ppl[ppl == 4 ? 2 : ppl - 1].push_back(s);
+    }
+    int rc;
+    for (int k = 2; k >= 0; --k)
+        if (!by_ppl[k].empty() && (rc = launch_rounds(ctx, by_ppl[k], false, k == 2 ? 4 : k + 1))) return rc;
+    return RSR_OK;
+}
+
+int run_grad(rsr_context* ctx, std::vector<EvalStep>& steps) {
+    for (const auto& s : steps)
+        if (eval_grad_lds_bytes(s.P.spill_slots) > ctx->max_lds)
+            return set_error(RSR_E_UNSUPPORTED, "dual spill area of %zu bytes exceeds the workgroup LDS limit",
+                             eval_grad_lds_bytes(s.P.spill_slots));
+    return launch_rounds(ctx, steps, true, 1);
+}
+
+}  // namespace
+
+int run_tick(rsr_context* ctx, const std::vector<Job*>& jobs) {
+    const size_t nj = jobs.size();
+    if (nj == 0) return RSR_OK;
+    bool counters = false;
+    for (Job* j : jobs) counters = counters || j->needs_counters();
+    int rc;
+    int* dc = nullptr;
+    if (counters) {
+        if ((rc = ensure_counters(ctx, 2 * nj))) return rc;
+        dc = (int*)ctx->dev_counters.ptr;
+        HIP_TRY(hipMemsetAsync(dc, 0, 2 * nj * sizeof(int), ctx->stream), "counter clear");
+    }
+    std::vector<EvalStep> mse, grad;
+    for (size_t k = 0; k < nj; ++k) {
+        jobs[k]->d_counters = dc ? dc + 2 * k : nullptr;
+        if ((rc = jobs[k]->plan(mse, grad))) return rc;
+    }
+    if (!mse.empty() && (rc = run_mse(ctx, mse))) return rc;
+    if (!grad.empty() && (rc = run_grad(ctx, grad))) return rc;
+    for (Job* j : jobs)
+        if ((rc = j->post())) return rc;
+    // without counters the next stream synchronisation is the job's own finish()
+    int* hc = nullptr;
+    if (counters) {
+        hc = ctx->host_counters;
+        HIP_TRY(hipMemcpyAsync(hc, dc, 2 * nj * sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "counter download");
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "tick");
+    }
+    for (size_t k = 0; k < nj; ++k) {
+        jobs[k]->ticks++;
+        if ((rc = jobs[k]->advance(hc ? hc + 2 * k : nullptr))) return rc;
+    }
+    return RSR_OK;
+}
+
+int run_job(rsr_context* ctx, Job& job) {
+    std::vector<Job*> one{&job};
+    while (!job.done) {
+        int rc = run_tick(ctx, one);
+        if (rc) return rc;
+    }
+    return job.finish();
+}
+
+}  // namespace rsr

@@ -1,0 +1,89 @@
+// jobs.h -- resumable requests (evaluate, gradient, L-BFGS, CMA-ES) and the tick that
+// advances many of them together.
+//
+// A job is a state machine over evaluation steps.  One tick:
+//   1. every job enqueues its pre-step kernels and describes its evaluation step(s);
+//   2. all MSE steps of all jobs run as merged launches (segment tables), then all
+//      gradient steps;
+//   3. every job enqueues its post-step kernels (optimizer updates), which write two
+//      counters per job;
+//   4. the counters come back in one copy and one stream synchronisation;
+//   5. every job advances its state from its counters.
+// The direct C-ABI calls run one job until it is done; the server runs the ticks of
+// every admitted request together (continuous batching at the step level).
+#pragma once
+#include <stdint.h>
+
+#include <memory>
+#include <vector>
+
+#include "../../include/rsr_amd.h"
+#include "kernels.h"
+#include "optim.h"
+#include "runtime_internal.h"
+
+namespace rsr {
+
+// Device buffers owned by a job (released with it; charged to the context budget).
+// Buffers come zero-filled; uploads are asynchronous from host copies the job keeps.
+struct JobBuffers {
+    rsr_context* ctx;
+    std::vector<std::unique_ptr<DevBuf>> bufs;
+    std::vector<std::vector<char>> keep;
+    explicit JobBuffers(rsr_context* c) : ctx(c) {}
+    int raw(size_t bytes, void** out);
+    int upload(void* dst, const void* src, size_t bytes);
+    template <typename T>
+    int get(size_t count, T** out) {
+        return raw(count * sizeof(T), (void**)out);
+    }
+    template <typename T>
+    int put(const std::vector<T>& v, T** out) {
+        int rc = get<T>(v.size(), out);
+        if (rc || v.empty()) return rc;
+        return upload(*out, v.data(), v.size() * sizeof(T));
+    }
+    size_t bytes() const;
+};
+
+class Job {
+   public:
+    virtual ~Job() = default;
+    // Enqueue pre-step kernels and append this tick's evaluation steps.
+    virtual int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>& grad) = 0;
+    // Enqueue post-step kernels (after the merged evaluation launches).
+    virtual int post() { return RSR_OK; }
+    // Consume this tick's counters (host copies) and move to the next state.
+    virtual int advance(const int* counters) = 0;
+    // Copy results to the caller's buffers (stream is synchronised afterwards).
+    virtual int finish() = 0;
+    // Device bytes the job holds (admission accounting).
+    virtual size_t device_bytes() const = 0;
+    // Whether advance() reads the tick's counters (single-step jobs do not).
+    virtual bool needs_counters() const { return true; }
+    bool done = false;
+    int* d_counters = nullptr;  // two device ints owned by the tick driver
+    uint64_t node_evals = 0;    // node-evals launched so far (statistics; fits estimate from active counts)
+    int ticks = 0;
+};
+
+// compute_mse of every program (or the masked ones); mse_out host array [n].
+int make_eval_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* mask, float* mse_out,
+                  std::unique_ptr<Job>* out);
+// compute_mse_with_gradient; mse_out [n], grad_out [n][32].
+int make_grad_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* mask, float* mse_out,
+                  float* grad_out, std::unique_ptr<Job>* out);
+// run_lbfgs over the masked programs; fitness_out [n] (NaN if not fitted).
+int make_lbfgs_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, int history, int mode,
+                   const uint8_t* mask, float* fitness_out, std::unique_ptr<Job>* out);
+// run_cma_es over the masked programs; fitness_out [n], iters_out [n] optional.
+int make_cmaes_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, int lambda, int mu,
+                   uint64_t seed, const float* normals, const uint8_t* mask, float* fitness_out, int32_t* iters_out,
+                   std::unique_ptr<Job>* out);
+
+// Advance every job by one tick (caller holds ctx->mu).
+int run_tick(rsr_context* ctx, const std::vector<Job*>& jobs);
+// Run one job to completion and finish it (caller holds ctx->mu).
+int run_job(rsr_context* ctx, Job& job);
+
+}  // namespace rsr

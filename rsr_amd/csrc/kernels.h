@@ -39,37 +39,81 @@ struct KernelTimer {
     hipEvent_t begin = nullptr, end = nullptr;
 };
 
-// MSE of every program (compute_mse, evaluator.rs:153-174).  `active` may be null.
-// `partials` must hold n_tiles(ds, ppl) * n_progs floats.
-hipError_t launch_eval_mse(const DevPrograms& P, const DevDataset& D, const uint8_t* active, const EvalTuning& t,
-                           float* partials, float* mse_out, hipStream_t s, const KernelTimer* timer = nullptr);
+// One request's evaluation step (host description).  For MSE steps, grad_out /
+// nparams / max_params are unused; for gradient steps nparams[i] = min(param_count, 32).
+//
+// Kahan summation keeps one partial per (tile, program[, slot]); a step may be split into
+// tile ranges (tile_begin, tile_count) evaluated by successive launches, with the Kahan
+// state (sum, compensation) carried between them in kstate, so the result does not
+// depend on the split.
+struct EvalStep {
+    DevPrograms P;
+    DevDataset D;
+    const uint8_t* active = nullptr;  // per program, null = all
+    const int* live = nullptr;        // skip the whole step when *live == 0 (device flag)
+    float* partials = nullptr;        // Kahan summation: step_partials_floats
+    float* mse_out = nullptr;         // [n_progs]
+    float* grad_out = nullptr;        // [n_progs][32]
+    const uint8_t* nparams = nullptr;
+    int max_params = 0;
+    int64_t tile_begin = 0, tile_count = -1;  // -1: every tile from tile_begin
+    float* kstate = nullptr;                  // step_state_floats, needed unless first && last
+    bool first = true, last = true;
+};
+
+// Kernel-argument segment table (kernarg segment, read with scalar loads).
+struct SegDesc {
+    const UInstr* code;
+    const ProgDesc* progs;
+    const float* params;
+    const int32_t* order;
+    const uint8_t* active;
+    const int* live;
+    const float* feat;
+    const float* target;
+    float* partials;
+    float* mse_out;
+    float* grad_out;
+    const uint8_t* nparams;
+    float* kstate;
+    int64_t npad, count8, tile0, n_tiles, first_wave, first_thread;
+    int n_progs, group, n_groups, F, n_pass, slots, first, last, terms, tile_pts;
+    float n_samples;
+};
+constexpr int kMaxSegs = 16;
+struct SegTable {
+    int n;
+    SegDesc s[kMaxSegs];
+};
+
 int64_t eval_tiles(const DevDataset& D, int ppl);
+size_t eval_mse_lds_bytes(int spill_slots, int ppl);
+size_t eval_grad_lds_bytes(int spill_slots);
+// Tiles of the step's range at the given points per lane.
+int64_t step_tiles(const EvalStep& st, int ppl);
+// Partial slots per (tile, program): 1 for MSE, 1 + max_params for gradients.
+int grad_slots(const EvalStep& st);
+// MSE partials: one per (tile, program), or with terms one per (point, program).
+size_t mse_partials_floats(const EvalStep& st, int ppl, bool terms);
+size_t grad_partials_floats(const EvalStep& st);
+size_t mse_state_floats(const EvalStep& st, bool terms);
+size_t grad_state_floats(const EvalStep& st);
 
-// Dynamic LDS bytes a launch needs for the batch's spill slots.
-size_t eval_mse_lds_bytes(const DevPrograms& P, int ppl);
-size_t eval_grad_lds_bytes(const DevPrograms& P);
-
-// Reference-order summation variants: per-point terms are folded into 8 serial lane
-// accumulators in point order and added horizontally like f32x8::reduce_add, so the
-// results are bit-identical to the CPU evaluator.  One wave per program.
-hipError_t launch_eval_mse_exact(const DevPrograms& P, const DevDataset& D, const uint8_t* active, float* mse_out,
-                                 hipStream_t s, const KernelTimer* timer = nullptr);
-hipError_t launch_eval_grad_exact(const DevPrograms& P, const DevDataset& D, const uint8_t* nparams_eff,
-                                  const uint8_t* active, int max_params, float* mse_out, float* grad_out,
-                                  hipStream_t s, const KernelTimer* timer = nullptr);
+// MSE of every active program of every step (compute_mse, evaluator.rs:153-174), one
+// merged launch per kMaxSegs steps.  terms = false: per-tile partials summed with
+// Kahan-compensated FP32 by the finalize kernel.  terms = true: per-point terms folded in
+// the reference's order (bit-identical to the CPU evaluator).
+hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune, bool terms, hipStream_t s,
+                            const KernelTimer* timer = nullptr);
+// MSE + gradient (compute_mse_with_gradient, evaluator.rs:177-224) of every active
+// program with parameters, from the dual lowering.
+hipError_t launch_grad_steps(const EvalStep* steps, int n, bool reference_order, hipStream_t s,
+                             const KernelTimer* timer = nullptr);
 
 // Per-point values of programs [first, first+count) (eval_simd over every batch);
 // out[(i - first) * count8 + point].
 hipError_t launch_eval_points(const DevPrograms& P, const DevDataset& D, int first, int count, float* out,
                               hipStream_t s);
-
-// MSE and gradient (compute_mse_with_gradient, evaluator.rs:177-224) from the dual
-// lowering.  nparams_eff[i] = min(param_count, 32); programs with zero parameters are
-// flagged inactive here and handled by the primal path.  partials must hold
-// n_tiles(ds, 1) * n_progs * 33 floats.
-hipError_t launch_eval_grad(const DevPrograms& P, const DevDataset& D, const uint8_t* nparams_eff,
-                            const uint8_t* active, int max_params, float* partials, float* mse_out,
-                            float* grad_out, hipStream_t s, const KernelTimer* timer = nullptr);
 
 // Per-point value and tangent for one program and one active parameter (eval_simd_dual).
 hipError_t launch_eval_points_dual(const DevPrograms& P, const DevDataset& D, int prog, int active_param,

@@ -1,14 +1,23 @@
 // kernels.hip -- fitness-evaluation kernels for gfx950.
 //
-// Work decomposition (evaluation): a work item is (point tile, program group).  The
-// four waves of a 256-thread workgroup take four consecutive groups of the same tile,
-// so the tile's feature columns are fetched from HBM once and served to the other
-// waves from L2/L1.  Each wave loads its 64 * PPL points' features straight into the
-// low registers of its register file, where they stay resident while the wave
-// interprets every program of its group: a program never re-reads the dataset.  Per
-// (program, tile) the wave writes one partial sum of squared errors; a finalize kernel
-// adds the partials of each program in a fixed order with Kahan-compensated FP32, so
-// results are bitwise reproducible run to run.
+// Every evaluation launch takes a segment table (kernel arguments, up to kMaxSegs
+// segments).  A segment is one request's evaluation step: a program batch, a dataset,
+// an active mask and output buffers.  One launch therefore evaluates the steps of many
+// requests with different datasets, feature counts and population sizes together --
+// the continuous-batching path of the server -- and a plain call is a one-segment
+// table.
+//
+// Work decomposition (Kahan summation): a work item is (point tile, program group) of
+// one segment.  The four waves of a 256-thread workgroup take four consecutive groups
+// of the same tile, so the tile's feature columns are fetched from HBM once and served
+// to the other waves from L2/L1.  Each wave loads its 64 * PPL points' features straight
+// into the low registers of its register file, where they stay resident while the wave
+// interprets every program of its group.  Per (program, tile) the wave writes one
+// partial sum; the finalize kernel adds the partials of each program in tile order with
+// Kahan-compensated FP32, so results are bitwise reproducible run to run.
+//
+// Reference-order summation: one wave per (program[, pass]) walks the tiles in order
+// and folds each 8-point batch into 8 lane accumulators (see fold_batches).
 #include <float.h>
 #include "interp.h"
 #include "kernels.h"
@@ -16,6 +25,7 @@
 namespace rsr {
 
 static constexpr int kWaves = 4;  // waves per workgroup
+static constexpr int kTan = 4;    // tangent channels per dual pass
 
 // Load feature columns [0, F) of the lane's points into the register files.
 template <int PPL>
@@ -33,130 +43,16 @@ __device__ __forceinline__ void load_features(Files<PPL>& R, const float* __rest
     }
 }
 
-template <int PPL>
-__global__ __launch_bounds__(256) void k_eval_mse(const UInstr* __restrict__ code, const ProgDesc* __restrict__ progs,
-                                                  const float* __restrict__ params, const int32_t* __restrict__ order,
-                                                  const uint8_t* __restrict__ active, int n_progs, int group,
-                                                  int n_groups, const float* __restrict__ feat,
-                                                  const float* __restrict__ target, int F, int64_t npad,
-                                                  int64_t count8, int64_t n_tiles, float* __restrict__ partials,
-                                                  int spill_slots) {
-    extern __shared__ float smem[];
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
-    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * PPL * 64;
-    const int64_t tile = wave / n_groups;
-    const int g = (int)(wave - tile * n_groups);
-    if (tile >= n_tiles) return;
-    const int64_t base = tile * (64 * PPL);
-
-    Files<PPL> R;
-    sfor<PPL>([&](auto P) { fget<P(), PPL>(R) = (v32f)(0.0f); });
-    load_features<PPL>(R, feat, F, npad, base, lane);
-    float y[PPL];
-    bool valid[PPL];
-#pragma unroll
-    for (int p = 0; p < PPL; ++p) {
-        const int64_t pt = base + p * 64 + lane;
-        y[p] = target[pt];
-        valid[p] = pt < count8;
-    }
-
-    const int first = g * group;
-    const int last = min(first + group, n_progs);
-    for (int sp = first; sp < last; ++sp) {
-        const int prog = (int)uni((uint32_t)order[sp]);
-        float s = 0.0f;
-        if (active == nullptr || active[prog]) {
-            const ProgDesc pd = progs[prog];
-            run_program<PPL>(R, code + uni(pd.code_off), uni(pd.code_len), params + uni(pd.param_off), lds, lane);
-            const uint32_t o = uni(pd.out_reg) & 31u;
-            sfor<PPL>([&](auto P) {
-                const v32f r = fget<P(), PPL>(R);
-                const float diff = r[o] - y[P()];
-                const float sq = diff * diff;
-                s = s + (valid[P()] ? sq : 0.0f);
-            });
-            s = wave_sum(s);
-        }
-        if (lane == 0) partials[tile * n_progs + sp] = s;
-    }
+// Segment owning global work index w (first_wave / first_thread are prefix sums).
+__device__ __forceinline__ int find_seg_wave(const SegTable& T, int64_t w) {
+    int s = 0;
+    while (s + 1 < T.n && w >= T.s[s + 1].first_wave) ++s;
+    return s;
 }
-
-// One thread per scheduling position: Kahan-compensated sum of the tile partials.
-__global__ void k_finalize_mse(const float* __restrict__ partials, int64_t n_tiles, int n_progs,
-                               const int32_t* __restrict__ order, const uint8_t* __restrict__ active, float n_samples,
-                               float* __restrict__ mse_out) {
-    const int sp = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sp >= n_progs) return;
-    const int prog = order[sp];
-    if (active != nullptr && !active[prog]) return;
-    float s = 0.0f, c = 0.0f;
-    for (int64_t t = 0; t < n_tiles; ++t) {
-        const float yv = partials[t * n_progs + sp] - c;
-        const float tv = s + yv;
-        c = (tv - s) - yv;
-        s = tv;
-    }
-    const float mse = s / n_samples;
-    mse_out[prog] = __builtin_isfinite(mse) ? mse : FLT_MAX;
-}
-
-size_t eval_mse_lds_bytes(const DevPrograms& P, int ppl) {
-    return (size_t)kWaves * P.spill_slots * ppl * 64 * sizeof(float);
-}
-
-int64_t eval_tiles(const DevDataset& D, int ppl) {
-    const int64_t tp = 64 * (int64_t)ppl;
-    return (D.count8 + tp - 1) / tp;
-}
-
-hipError_t launch_eval_mse(const DevPrograms& P, const DevDataset& D, const uint8_t* active, const EvalTuning& t,
-                           float* partials, float* mse_out, hipStream_t s, const KernelTimer* timer) {
-    const size_t lds_bytes = eval_mse_lds_bytes(P, t.ppl);
-    if (P.n_progs <= 0) return hipSuccess;
-    const int64_t n_tiles = eval_tiles(D, t.ppl);
-    const int group = t.group > 0 ? t.group : 8;
-    const int n_groups = (P.n_progs + group - 1) / group;
-    const int64_t waves = n_tiles * n_groups;
-    const int64_t blocks = (waves + kWaves - 1) / kWaves;
-    if (blocks > INT32_MAX) return hipErrorInvalidValue;
-    dim3 grid((unsigned)blocks), block(64 * kWaves);
-#define RSR_LAUNCH(PPLV)                                                                                       \
-    hipLaunchKernelGGL(k_eval_mse<PPLV>, grid, block, lds_bytes, s, P.code, P.progs, P.params, P.order, active, \
-                       P.n_progs, group, n_groups, D.feat, D.target, D.num_features, D.npad, D.count8, n_tiles, \
-                       partials, P.spill_slots)
-    if (timer) (void)hipEventRecord(timer->begin, s);
-    switch (t.ppl) {
-        case 1: RSR_LAUNCH(1); break;
-        case 4: RSR_LAUNCH(4); break;
-        default: RSR_LAUNCH(2); break;
-    }
-#undef RSR_LAUNCH
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (timer) (void)hipEventRecord(timer->end, s);
-    const int tb = 256;
-    hipLaunchKernelGGL(k_finalize_mse, dim3((P.n_progs + tb - 1) / tb), dim3(tb), 0, s, partials, n_tiles, P.n_progs,
-                       P.order, active, (float)D.num_samples, mse_out);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------------
-// Reference-order summation (bit-identical to the CPU evaluator's f32 sums)
-// ---------------------------------------------------------------------------------
-// The reference accumulates diff^2 in 8 lanes, lane l receiving points l, l+8, l+16, ...
-// in order (evaluator.rs:159-170), then adds the 8 lanes horizontally.  Here one wave
-// owns one program and walks the tiles in order; inside a 64-point tile the eight
-// 8-point batches are folded into the 8 accumulators (held by lanes 0..7) one batch
-// after the other with shuffles, which reproduces the serial chain exactly.
-__device__ __forceinline__ float fold_batches(float acc, float term, int lane) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const float v = __shfl(term, (lane & 7) + 8 * k, 64);
-        acc = acc + v;
-    }
-    return acc;
+__device__ __forceinline__ int find_seg_thread(const SegTable& T, int64_t t) {
+    int s = 0;
+    while (s + 1 < T.n && t >= T.s[s + 1].first_thread) ++s;
+    return s;
 }
 
 // f32x8::reduce_add order: (lo4 + hi4), then ((0 + 2) + (1 + 3)); result in every lane.
@@ -167,53 +63,442 @@ __device__ __forceinline__ float reduce_lanes8(float acc) {
     return (s0 + s2) + (s1 + s3);
 }
 
-__global__ __launch_bounds__(256) void k_eval_mse_exact(const UInstr* __restrict__ code,
-                                                        const ProgDesc* __restrict__ progs,
-                                                        const float* __restrict__ params,
-                                                        const int32_t* __restrict__ order,
-                                                        const uint8_t* __restrict__ active, int n_progs,
-                                                        const float* __restrict__ feat,
-                                                        const float* __restrict__ target, int F, int64_t npad,
-                                                        int64_t count8, int64_t n_tiles, float n_samples,
-                                                        float* __restrict__ mse_out, int spill_slots) {
-    extern __shared__ float smem[];
-    const int lane = threadIdx.x & 63;
-    const int sp = (int)(uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6));
-    if (sp >= n_progs) return;
-    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * 64;
-    const int prog = (int)uni((uint32_t)order[sp]);
-    if (active != nullptr && !active[prog]) return;
-    const ProgDesc pd = progs[prog];
-    const uint32_t o = uni(pd.out_reg) & 31u;
-    float acc = 0.0f;
-    for (int64_t tile = 0; tile < n_tiles; ++tile) {
-        const int64_t base = tile * 64;
-        Files<1> R;
-        R.a = (v32f)(0.0f);
-        load_features<1>(R, feat, F, npad, base, lane);
-        run_program<1>(R, code + uni(pd.code_off), uni(pd.code_len), params + uni(pd.param_off), lds, lane);
-        const int64_t pt = base + lane;
-        const v32f rf = R.a;
-        const float diff = rf[o] - target[pt];
-        const float sq = pt < count8 ? diff * diff : 0.0f;
-        acc = fold_batches(acc, sq, lane);
+// The reference accumulates per-point terms in 8 lanes, lane l receiving points l, l+8,
+// l+16, ... in order (evaluator.rs:159-170).  Inside a 64-point tile the eight 8-point
+// batches are folded into the 8 accumulators (lanes 0..7) one batch after the other,
+// which reproduces that serial chain exactly when the tiles are visited in order.
+__device__ __forceinline__ float fold_batches(float acc, float term, int lane) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float v = __shfl(term, (lane & 7) + 8 * k, 64);
+        acc = acc + v;
     }
-    const float mse = reduce_lanes8(acc) / n_samples;
-    if (lane == 0) mse_out[prog] = __builtin_isfinite(mse) ? mse : FLT_MAX;
+    return acc;
 }
 
-hipError_t launch_eval_mse_exact(const DevPrograms& P, const DevDataset& D, const uint8_t* active, float* mse_out,
-                                 hipStream_t s, const KernelTimer* timer) {
-    if (P.n_progs <= 0) return hipSuccess;
-    const int64_t n_tiles = eval_tiles(D, 1);
-    const size_t lds_bytes = eval_mse_lds_bytes(P, 1);
-    if (timer) (void)hipEventRecord(timer->begin, s);
-    hipLaunchKernelGGL(k_eval_mse_exact, dim3((unsigned)((P.n_progs + kWaves - 1) / kWaves)), dim3(64 * kWaves),
-                       lds_bytes, s, P.code, P.progs, P.params, P.order, active, P.n_progs, D.feat, D.target,
-                       D.num_features, D.npad, D.count8, n_tiles, (float)D.num_samples, mse_out, P.spill_slots);
-    hipError_t e = hipGetLastError();
-    if (timer) (void)hipEventRecord(timer->end, s);
-    return e;
+// ---------------------------------------------------------------------------------
+// MSE, Kahan summation
+// ---------------------------------------------------------------------------------
+template <int PPL>
+__global__ __launch_bounds__(256) void k_eval_mse(const SegTable T, int spill_slots) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+    const int si = find_seg_wave(T, wave);
+    const SegDesc& S = T.s[si];
+    const int64_t w = wave - S.first_wave;
+    const int64_t lt = w / S.n_groups;  // tile within the launch's range
+    if (lt >= S.n_tiles) return;
+    if (S.live != nullptr && *S.live == 0) return;
+    const int g = (int)(w - lt * S.n_groups);
+    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * PPL * 64;
+    const int64_t base = (S.tile0 + lt) * (64 * PPL);
+
+    Files<PPL> R;
+    sfor<PPL>([&](auto P) { fget<P(), PPL>(R) = (v32f)(0.0f); });
+    load_features<PPL>(R, S.feat, S.F, S.npad, base, lane);
+    float y[PPL];
+    bool valid[PPL];
+#pragma unroll
+    for (int p = 0; p < PPL; ++p) {
+        const int64_t pt = base + p * 64 + lane;
+        y[p] = S.target[pt];
+        valid[p] = pt < S.count8;
+    }
+    const int first = g * S.group;
+    const int last = min(first + S.group, S.n_progs);
+    for (int sp = first; sp < last; ++sp) {
+        const int prog = (int)uni((uint32_t)S.order[sp]);
+        float s = 0.0f;
+        if (S.active == nullptr || S.active[prog]) {
+            const ProgDesc pd = S.progs[prog];
+            run_program<PPL>(R, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), lds, lane);
+            const uint32_t o = uni(pd.out_reg) & 31u;
+            if (S.terms) {
+                // reference order: keep every per-point term; the fold kernel adds them
+                float* row = S.partials + (int64_t)sp * S.n_tiles * (64 * PPL) + lt * (64 * PPL);
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    const float diff = r[o] - y[P()];
+                    row[P() * 64 + lane] = diff * diff;
+                });
+                continue;
+            }
+            sfor<PPL>([&](auto P) {
+                const v32f r = fget<P(), PPL>(R);
+                const float diff = r[o] - y[P()];
+                const float sq = diff * diff;
+                s = s + (valid[P()] ? sq : 0.0f);
+            });
+            s = wave_sum(s);
+        }
+        if (!S.terms && lane == 0) S.partials[lt * S.n_progs + sp] = s;
+    }
+}
+
+// One thread per (segment, scheduling position): Kahan sum of the tile partials.
+__global__ void k_finalize_mse(const SegTable T) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int si = find_seg_thread(T, t);
+    const SegDesc& S = T.s[si];
+    const int64_t sp = t - S.first_thread;
+    if (sp >= S.n_progs) return;
+    if (S.live != nullptr && *S.live == 0) return;
+    const int prog = S.order[sp];
+    if (S.active != nullptr && !S.active[prog]) return;
+    float s = 0.0f, c = 0.0f;
+    if (!S.first) {
+        s = S.kstate[2 * sp];
+        c = S.kstate[2 * sp + 1];
+    }
+    for (int64_t lt = 0; lt < S.n_tiles; ++lt) {
+        const float yv = S.partials[lt * S.n_progs + sp] - c;
+        const float tv = s + yv;
+        c = (tv - s) - yv;
+        s = tv;
+    }
+    if (!S.last) {
+        S.kstate[2 * sp] = s;
+        S.kstate[2 * sp + 1] = c;
+        return;
+    }
+    const float mse = s / S.n_samples;
+    S.mse_out[prog] = __builtin_isfinite(mse) ? mse : FLT_MAX;
+}
+
+// ---------------------------------------------------------------------------------
+// MSE, reference-order summation from per-point terms
+// ---------------------------------------------------------------------------------
+// Eight threads per (segment, scheduling position): thread l is the reference's lane
+// accumulator l and adds the terms of points l, l+8, l+16, ... of the launch's range in
+// order (evaluator.rs:159-170), starting from the carried accumulator of earlier ranges.
+// The last range reduces the eight accumulators in f32x8::reduce_add order.
+__global__ void k_fold_terms(const SegTable T) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int si = find_seg_thread(T, t);
+    const SegDesc& S = T.s[si];
+    const int64_t idx = t - S.first_thread;
+    const int64_t sp = idx >> 3;
+    const int l = (int)(idx & 7);
+    if (sp >= S.n_progs) return;
+    if (S.live != nullptr && *S.live == 0) return;
+    const int prog = S.order[sp];
+    if (S.active != nullptr && !S.active[prog]) return;
+    const int64_t p0 = S.tile0 * S.tile_pts;
+    const int64_t p1 = min(p0 + S.n_tiles * S.tile_pts, S.count8);
+    const float* row = S.partials + sp * S.n_tiles * S.tile_pts - p0;
+    float acc = S.first ? 0.0f : S.kstate[sp * 8 + l];
+    for (int64_t q = p0 + l; q < p1; q += 8) acc = acc + row[q];
+    if (!S.last) {
+        S.kstate[sp * 8 + l] = acc;
+        return;
+    }
+    // the eight threads of a program are consecutive and aligned to 8 (width-8 shuffles)
+    const float a0 = __shfl(acc, 0, 8), a1 = __shfl(acc, 1, 8), a2 = __shfl(acc, 2, 8), a3 = __shfl(acc, 3, 8);
+    const float a4 = __shfl(acc, 4, 8), a5 = __shfl(acc, 5, 8), a6 = __shfl(acc, 6, 8), a7 = __shfl(acc, 7, 8);
+    const float s0 = a0 + a4, s1 = a1 + a5, s2 = a2 + a6, s3 = a3 + a7;
+    const float mse = ((s0 + s2) + (s1 + s3)) / S.n_samples;
+    if (l == 0) S.mse_out[prog] = __builtin_isfinite(mse) ? mse : FLT_MAX;
+}
+
+// ---------------------------------------------------------------------------------
+// Gradients (dual), Kahan summation: work item = (tile, program, pass)
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_slots) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+    const int si = find_seg_wave(T, wave);
+    const SegDesc& S = T.s[si];
+    const int64_t w = wave - S.first_wave;
+    const int64_t per_tile = (int64_t)S.n_progs * S.n_pass;
+    const int64_t lt = w / per_tile;  // tile within the launch's range
+    if (lt >= S.n_tiles) return;
+    if (S.live != nullptr && *S.live == 0) return;
+    const int64_t rem = w - lt * per_tile;
+    const int sp = (int)(rem / S.n_pass);
+    const int pass = (int)(rem - (int64_t)sp * S.n_pass);
+    const int prog = (int)uni((uint32_t)S.order[sp]);
+    const int np = (int)uni(S.nparams[prog]);
+    if ((S.active != nullptr && !S.active[prog]) || np == 0 || pass * kTan >= np) return;
+    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * (1 + kTan) * 64;
+    const int64_t base = (S.tile0 + lt) * 64;
+    const int64_t pt = base + lane;
+
+    Files<1> R1;
+    R1.a = (v32f)(0.0f);
+    load_features<1>(R1, S.feat, S.F, S.npad, base, lane);
+    DualFiles<kTan> G;
+    G.v = R1.a;
+    G.a = G.b = G.c = G.d = (v32f)(0.0f);
+    const ProgDesc pd = S.progs[prog];
+    const uint32_t k0 = (uint32_t)(pass * kTan);
+    run_dual<kTan>(G, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), k0, lds, lane);
+    const uint32_t o = uni(pd.out_reg) & 31u;
+    const bool valid = pt < S.count8;
+    const v32f vfile = G.v;
+    const float diff = vfile[o] - S.target[pt];
+    float* dst = S.partials + (lt * S.n_progs + sp) * S.slots;
+    if (pass == 0) {
+        const float sq = diff * diff;
+        const float s = wave_sum(valid ? sq : 0.0f);
+        if (lane == 0) dst[0] = s;
+    }
+    const float two_diff = 2.0f * diff;
+    const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
+#pragma unroll
+    for (int j = 0; j < kTan; ++j) {
+        const int k = (int)k0 + j;
+        if (k < np) {
+            const float gterm = two_diff * gf[j][o];
+            const float s = wave_sum(valid ? gterm : 0.0f);
+            if (lane == 0) dst[1 + k] = s;
+        }
+    }
+}
+
+// One thread per (segment, scheduling position, slot); slot 0 = squared error, 1 + k =
+// gradient k.  Every thread also carries the slot-0 sum, because the MSE decides whether
+// the gradient is reported (evaluator.rs:214-216); it is recomputed from the same partials
+// in the same order, so all threads of a program agree.  kstate: 4 floats per (sp, slot).
+__global__ void k_finalize_grad(const SegTable T) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int si = find_seg_thread(T, t);
+    const SegDesc& S = T.s[si];
+    const int64_t idx = t - S.first_thread;
+    const int sp = (int)(idx / S.slots);
+    const int slot = (int)(idx - (int64_t)sp * S.slots);
+    if (sp >= S.n_progs) return;
+    if (S.live != nullptr && *S.live == 0) return;
+    const int prog = S.order[sp];
+    if (S.active != nullptr && !S.active[prog]) return;
+    const int np = S.nparams[prog];
+    if (np == 0) return;
+    float s = 0.0f, c = 0.0f, m = 0.0f, mc = 0.0f;
+    float* ks = S.kstate + ((int64_t)sp * S.slots + slot) * 4;
+    if (!S.first) {
+        s = ks[0];
+        c = ks[1];
+        m = ks[2];
+        mc = ks[3];
+    }
+    const bool mine = slot == 0 || slot - 1 < np;
+    for (int64_t lt = 0; lt < S.n_tiles; ++lt) {
+        const float* row = S.partials + (lt * S.n_progs + sp) * S.slots;
+        if (mine) {
+            const float yv = row[slot] - c;
+            const float tv = s + yv;
+            c = (tv - s) - yv;
+            s = tv;
+        }
+        const float ym = row[0] - mc;
+        const float tm = m + ym;
+        mc = (tm - m) - ym;
+        m = tm;
+    }
+    if (!S.last) {
+        ks[0] = s;
+        ks[1] = c;
+        ks[2] = m;
+        ks[3] = mc;
+        return;
+    }
+    const float mse = m / S.n_samples;
+    const bool fin = __builtin_isfinite(mse);
+    if (slot == 0) {
+        S.mse_out[prog] = fin ? mse : FLT_MAX;
+    } else {
+        const int k = slot - 1;
+        S.grad_out[(int64_t)prog * 32 + k] = (fin && k < np) ? s / S.n_samples : 0.0f;
+    }
+}
+
+// Reference-order gradient sums: one wave per (program, pass) walks the tiles in order.
+__global__ __launch_bounds__(256) void k_eval_grad_exact(const SegTable T, int spill_slots) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+    const int si = find_seg_wave(T, wave);
+    const SegDesc& S = T.s[si];
+    const int64_t w = wave - S.first_wave;
+    const int sp = (int)(w / S.n_pass);
+    const int pass = (int)(w - (int64_t)sp * S.n_pass);
+    if (sp >= S.n_progs) return;
+    if (S.live != nullptr && *S.live == 0) return;
+    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * (1 + kTan) * 64;
+    const int prog = (int)uni((uint32_t)S.order[sp]);
+    const int np = (int)uni(S.nparams[prog]);
+    if ((S.active != nullptr && !S.active[prog]) || np == 0 || pass * kTan >= np) return;
+    const ProgDesc pd = S.progs[prog];
+    const uint32_t o = uni(pd.out_reg) & 31u;
+    const uint32_t k0 = (uint32_t)(pass * kTan);
+    float acc = 0.0f, gacc[kTan] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int64_t tile = 0; tile < S.n_tiles; ++tile) {
+        const int64_t base = tile * 64;
+        Files<1> R1;
+        R1.a = (v32f)(0.0f);
+        load_features<1>(R1, S.feat, S.F, S.npad, base, lane);
+        DualFiles<kTan> G;
+        G.v = R1.a;
+        G.a = G.b = G.c = G.d = (v32f)(0.0f);
+        run_dual<kTan>(G, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), k0, lds, lane);
+        const int64_t pt = base + lane;
+        const bool valid = pt < S.count8;
+        const v32f vf = G.v;
+        const float diff = vf[o] - S.target[pt];
+        acc = fold_batches(acc, valid ? diff * diff : 0.0f, lane);
+        const float two_diff = 2.0f * diff;
+        const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
+#pragma unroll
+        for (int j = 0; j < kTan; ++j) gacc[j] = fold_batches(gacc[j], valid ? two_diff * gf[j][o] : 0.0f, lane);
+    }
+    const float mse = reduce_lanes8(acc) / S.n_samples;
+    const bool fin = __builtin_isfinite(mse);
+    if (pass == 0 && lane == 0) S.mse_out[prog] = fin ? mse : FLT_MAX;
+#pragma unroll
+    for (int j = 0; j < kTan; ++j) {
+        const int k = (int)k0 + j;
+        const float g = reduce_lanes8(gacc[j]) / S.n_samples;
+        if (lane == 0 && k < 32) S.grad_out[(int64_t)prog * 32 + k] = (fin && k < np) ? g : 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// host side: segment tables and launches
+// ---------------------------------------------------------------------------------
+int64_t eval_tiles(const DevDataset& D, int ppl) {
+    const int64_t tp = 64 * (int64_t)ppl;
+    return (D.count8 + tp - 1) / tp;
+}
+
+size_t eval_mse_lds_bytes(int spill_slots, int ppl) { return (size_t)kWaves * spill_slots * ppl * 64 * sizeof(float); }
+size_t eval_grad_lds_bytes(int spill_slots) { return (size_t)kWaves * spill_slots * (1 + kTan) * 64 * sizeof(float); }
+
+int64_t step_tiles(const EvalStep& st, int ppl) {
+    return st.tile_count >= 0 ? st.tile_count : eval_tiles(st.D, ppl) - st.tile_begin;
+}
+int grad_slots(const EvalStep& st) { return 1 + (st.max_params > 0 ? st.max_params : 0); }
+size_t mse_partials_floats(const EvalStep& st, int ppl, bool terms) {
+    return (size_t)step_tiles(st, ppl) * st.P.n_progs * (terms ? 64 * ppl : 1);
+}
+size_t grad_partials_floats(const EvalStep& st) {
+    return (size_t)step_tiles(st, 1) * st.P.n_progs * grad_slots(st);
+}
+size_t mse_state_floats(const EvalStep& st, bool terms) { return (size_t)(terms ? 8 : 2) * st.P.n_progs; }
+size_t grad_state_floats(const EvalStep& st) { return (size_t)4 * st.P.n_progs * grad_slots(st); }
+
+static SegDesc make_seg(const EvalStep& st, int ppl, int group) {
+    SegDesc d{};
+    d.tile_pts = 64 * ppl;
+    d.code = st.P.code;
+    d.progs = st.P.progs;
+    d.params = st.P.params;
+    d.order = st.P.order;
+    d.active = st.active;
+    d.live = st.live;
+    d.feat = st.D.feat;
+    d.target = st.D.target;
+    d.partials = st.partials;
+    d.mse_out = st.mse_out;
+    d.grad_out = st.grad_out;
+    d.nparams = st.nparams;
+    d.kstate = st.kstate;
+    d.npad = st.D.npad;
+    d.count8 = st.D.count8;
+    d.tile0 = st.tile_begin;
+    d.n_tiles = step_tiles(st, ppl);
+    d.slots = grad_slots(st);
+    d.first = st.first ? 1 : 0;
+    d.last = st.last ? 1 : 0;
+    d.n_progs = st.P.n_progs;
+    d.group = group;
+    d.n_groups = (st.P.n_progs + group - 1) / group;
+    d.F = st.D.num_features;
+    d.n_pass = st.max_params > 0 ? (st.max_params + kTan - 1) / kTan : 1;
+    d.n_samples = (float)st.D.num_samples;
+    return d;
+}
+
+hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune, bool terms, hipStream_t s,
+                            const KernelTimer* timer) {
+    for (int base = 0; base < n; base += kMaxSegs) {
+        const int cnt = min(kMaxSegs, n - base);
+        SegTable T{};
+        int64_t waves = 0, threads = 0;
+        int spill = 0;
+        for (int i = 0; i < cnt; ++i) {
+            const EvalStep& st = steps[base + i];
+            if (st.P.n_progs <= 0) continue;
+            SegDesc d = make_seg(st, tune.ppl, tune.group > 0 ? tune.group : 8);
+            if (d.n_tiles <= 0) continue;
+            d.terms = terms ? 1 : 0;
+            d.first_wave = waves;
+            d.first_thread = threads;
+            waves += d.n_tiles * d.n_groups;
+            threads += (int64_t)d.n_progs * (terms ? 8 : 1);
+            spill = max(spill, st.P.spill_slots);
+            T.s[T.n++] = d;
+        }
+        if (waves == 0) continue;
+        const int64_t blocks = (waves + kWaves - 1) / kWaves;
+        if (blocks > INT32_MAX) return hipErrorInvalidValue;
+        if (timer && base == 0) (void)hipEventRecord(timer->begin, s);
+        const size_t lds = eval_mse_lds_bytes(spill, tune.ppl);
+        switch (tune.ppl) {
+            case 1: hipLaunchKernelGGL(k_eval_mse<1>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill); break;
+            case 4: hipLaunchKernelGGL(k_eval_mse<4>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill); break;
+            default: hipLaunchKernelGGL(k_eval_mse<2>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill); break;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if (timer && base + cnt >= n) (void)hipEventRecord(timer->end, s);
+        const int tb = 256;
+        if (terms)
+            hipLaunchKernelGGL(k_fold_terms, dim3((unsigned)((threads + tb - 1) / tb)), dim3(tb), 0, s, T);
+        else
+            hipLaunchKernelGGL(k_finalize_mse, dim3((unsigned)((threads + tb - 1) / tb)), dim3(tb), 0, s, T);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_grad_steps(const EvalStep* steps, int n, bool reference_order, hipStream_t s,
+                             const KernelTimer* timer) {
+    for (int base = 0; base < n; base += kMaxSegs) {
+        const int cnt = min(kMaxSegs, n - base);
+        SegTable T{};
+        int64_t waves = 0, threads = 0;
+        int spill = 0;
+        for (int i = 0; i < cnt; ++i) {
+            const EvalStep& st = steps[base + i];
+            if (st.max_params <= 0 || st.P.n_progs <= 0) continue;
+            SegDesc d = make_seg(st, 1, 1);
+            d.first_wave = waves;
+            d.first_thread = threads;
+            waves += reference_order ? (int64_t)d.n_progs * d.n_pass : d.n_tiles * d.n_progs * d.n_pass;
+            threads += (int64_t)d.n_progs * d.slots;
+            spill = max(spill, st.P.spill_slots);
+            T.s[T.n++] = d;
+        }
+        if (waves == 0) continue;
+        const int64_t blocks = (waves + kWaves - 1) / kWaves;
+        if (blocks > INT32_MAX) return hipErrorInvalidValue;
+        const size_t lds = eval_grad_lds_bytes(spill);
+        if (timer && base == 0) (void)hipEventRecord(timer->begin, s);
+        if (reference_order)
+            hipLaunchKernelGGL(k_eval_grad_exact, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
+        else
+            hipLaunchKernelGGL(k_eval_grad, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if (timer && base + cnt >= n) (void)hipEventRecord(timer->end, s);
+        if (!reference_order) {
+            const int tb = 256;
+            hipLaunchKernelGGL(k_finalize_grad, dim3((unsigned)((threads + tb - 1) / tb)), dim3(tb), 0, s, T);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------------
@@ -247,225 +532,19 @@ hipError_t launch_eval_points(const DevPrograms& P, const DevDataset& D, int fir
     if (count <= 0) return hipSuccess;
     const int64_t n_tiles = eval_tiles(D, 1);
     const int64_t blocks = (n_tiles * count + kWaves - 1) / kWaves;
-    const size_t lds_bytes = (size_t)kWaves * P.spill_slots * 64 * sizeof(float);
+    const size_t lds_bytes = eval_mse_lds_bytes(P.spill_slots, 1);
     hipLaunchKernelGGL(k_eval_points, dim3((unsigned)blocks), dim3(64 * kWaves), lds_bytes, s, P.code, P.progs,
                        P.params, first, count, D.feat, D.num_features, D.npad, D.count8, n_tiles, out, P.spill_slots);
     return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------------
-// Dual (forward-mode) evaluation: MSE and gradient
-// ---------------------------------------------------------------------------------
-static constexpr int kTan = 4;  // tangent channels per pass
-
-size_t eval_grad_lds_bytes(const DevPrograms& P) {
-    return (size_t)kWaves * P.spill_slots * (1 + kTan) * 64 * sizeof(float);
-}
-
-__global__ __launch_bounds__(256) void k_eval_grad(const UInstr* __restrict__ code, const ProgDesc* __restrict__ progs,
-                                                   const float* __restrict__ params, const int32_t* __restrict__ order,
-                                                   const uint8_t* __restrict__ nparams, const uint8_t* __restrict__ active,
-                                                   int n_progs, int n_pass, const float* __restrict__ feat,
-                                                   const float* __restrict__ target, int F, int64_t npad,
-                                                   int64_t count8, int64_t n_tiles, float* __restrict__ partials,
-                                                   int spill_slots) {
-    extern __shared__ float smem[];
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
-    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * (1 + kTan) * 64;
-    // work item = (tile, program, pass); consecutive waves share the tile
-    const int64_t per_tile = (int64_t)n_progs * n_pass;
-    const int64_t tile = wave / per_tile;
-    const int64_t rem = wave - tile * per_tile;
-    const int sp = (int)(rem / n_pass);
-    const int pass = (int)(rem - (int64_t)sp * n_pass);
-    if (tile >= n_tiles) return;
-    const int prog = (int)uni((uint32_t)order[sp]);
-    const int np = (int)uni(nparams[prog]);
-    if ((active != nullptr && !active[prog]) || np == 0 || pass * kTan >= np) return;
-    const int64_t base = tile * 64;
-    const int64_t pt = base + lane;
-
-    Files<1> R1;
-    R1.a = (v32f)(0.0f);
-    load_features<1>(R1, feat, F, npad, base, lane);
-    DualFiles<kTan> G;
-    G.v = R1.a;
-    G.a = G.b = G.c = G.d = (v32f)(0.0f);
-    const ProgDesc pd = progs[prog];
-    const uint32_t k0 = (uint32_t)(pass * kTan);
-    run_dual<kTan>(G, code + uni(pd.code_off), uni(pd.code_len), params + uni(pd.param_off), k0, lds, lane);
-    const uint32_t o = uni(pd.out_reg) & 31u;
-    const bool valid = pt < count8;
-    const v32f vfile = G.v;
-    const float diff = vfile[o] - target[pt];
-    float* dst = partials + (tile * n_progs + sp) * 33;
-    if (pass == 0) {
-        const float sq = diff * diff;
-        const float s = wave_sum(valid ? sq : 0.0f);
-        if (lane == 0) dst[0] = s;
-    }
-    const float two_diff = 2.0f * diff;
-    const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
-#pragma unroll
-    for (int j = 0; j < kTan; ++j) {
-        const int k = (int)k0 + j;
-        if (k < np) {
-            const float gterm = two_diff * gf[j][o];
-            const float s = wave_sum(valid ? gterm : 0.0f);
-            if (lane == 0) dst[1 + k] = s;
-        }
-    }
-}
-
-// One thread per (scheduling position, slot); slot 0 = squared error, 1 + k = gradient k.
-__global__ void k_finalize_grad(const float* __restrict__ partials, int64_t n_tiles, int n_progs,
-                                const int32_t* __restrict__ order, const uint8_t* __restrict__ nparams,
-                                const uint8_t* __restrict__ active, float n_samples, float* __restrict__ mse_out,
-                                float* __restrict__ grad_out) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int sp = (int)(idx / 33);
-    const int slot = (int)(idx - (int64_t)sp * 33);
-    if (sp >= n_progs) return;
-    const int prog = order[sp];
-    if (active != nullptr && !active[prog]) return;
-    const int np = nparams[prog];
-    if (np == 0) return;
-    float s = 0.0f, c = 0.0f;
-    if (slot == 0 || slot - 1 < np) {
-        for (int64_t t = 0; t < n_tiles; ++t) {
-            const float yv = partials[(t * n_progs + sp) * 33 + slot] - c;
-            const float tv = s + yv;
-            c = (tv - s) - yv;
-            s = tv;
-        }
-    }
-    // the MSE decides whether the gradient is reported (evaluator.rs:214-216); every
-    // thread of the program recomputes it from the same partials in the same order
-    float m = 0.0f, mc = 0.0f;
-    for (int64_t t = 0; t < n_tiles; ++t) {
-        const float yv = partials[(t * n_progs + sp) * 33] - mc;
-        const float tv = m + yv;
-        mc = (tv - m) - yv;
-        m = tv;
-    }
-    const float mse = m / n_samples;
-    const bool fin = __builtin_isfinite(mse);
-    if (slot == 0) {
-        mse_out[prog] = fin ? mse : FLT_MAX;
-    } else {
-        const int k = slot - 1;
-        if (k < 32) grad_out[(int64_t)prog * 32 + k] = (fin && k < np) ? s / n_samples : 0.0f;
-    }
-}
-
-hipError_t launch_eval_grad(const DevPrograms& P, const DevDataset& D, const uint8_t* nparams_eff,
-                            const uint8_t* active, int max_params, float* partials, float* mse_out,
-                            float* grad_out, hipStream_t s, const KernelTimer* timer) {
-    if (P.n_progs <= 0 || max_params <= 0) return hipSuccess;
-    const int64_t n_tiles = eval_tiles(D, 1);
-    const int n_pass = (max_params + kTan - 1) / kTan;
-    const int64_t waves = n_tiles * P.n_progs * n_pass;
-    const int64_t blocks = (waves + kWaves - 1) / kWaves;
-    if (blocks > INT32_MAX) return hipErrorInvalidValue;
-    const size_t lds_bytes = eval_grad_lds_bytes(P);
-    if (timer) (void)hipEventRecord(timer->begin, s);
-    hipLaunchKernelGGL(k_eval_grad, dim3((unsigned)blocks), dim3(64 * kWaves), lds_bytes, s, P.code, P.progs, P.params,
-                       P.order, nparams_eff, active, P.n_progs, n_pass, D.feat, D.target, D.num_features, D.npad,
-                       D.count8, n_tiles, partials, P.spill_slots);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (timer) (void)hipEventRecord(timer->end, s);
-    const int64_t threads = (int64_t)P.n_progs * 33;
-    const int tb = 256;
-    hipLaunchKernelGGL(k_finalize_grad, dim3((unsigned)((threads + tb - 1) / tb)), dim3(tb), 0, s, partials, n_tiles,
-                       P.n_progs, P.order, nparams_eff, active, (float)D.num_samples, mse_out, grad_out);
-    return hipGetLastError();
-}
-
-// Reference-order gradient sums: one wave per (program, pass) walks the tiles in order.
-__global__ __launch_bounds__(256) void k_eval_grad_exact(const UInstr* __restrict__ code,
-                                                         const ProgDesc* __restrict__ progs,
-                                                         const float* __restrict__ params,
-                                                         const int32_t* __restrict__ order,
-                                                         const uint8_t* __restrict__ nparams,
-                                                         const uint8_t* __restrict__ active, int n_progs, int n_pass,
-                                                         const float* __restrict__ feat,
-                                                         const float* __restrict__ target, int F, int64_t npad,
-                                                         int64_t count8, int64_t n_tiles, float n_samples,
-                                                         float* __restrict__ mse_out, float* __restrict__ grad_out,
-                                                         int spill_slots) {
-    extern __shared__ float smem[];
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
-    const int sp = (int)(wave / n_pass);
-    const int pass = (int)(wave - (int64_t)sp * n_pass);
-    if (sp >= n_progs) return;
-    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * (1 + kTan) * 64;
-    const int prog = (int)uni((uint32_t)order[sp]);
-    const int np = (int)uni(nparams[prog]);
-    if ((active != nullptr && !active[prog]) || np == 0 || pass * kTan >= np) return;
-    const ProgDesc pd = progs[prog];
-    const uint32_t o = uni(pd.out_reg) & 31u;
-    const uint32_t k0 = (uint32_t)(pass * kTan);
-    float acc = 0.0f, gacc[kTan] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int64_t tile = 0; tile < n_tiles; ++tile) {
-        const int64_t base = tile * 64;
-        Files<1> R1;
-        R1.a = (v32f)(0.0f);
-        load_features<1>(R1, feat, F, npad, base, lane);
-        DualFiles<kTan> G;
-        G.v = R1.a;
-        G.a = G.b = G.c = G.d = (v32f)(0.0f);
-        run_dual<kTan>(G, code + uni(pd.code_off), uni(pd.code_len), params + uni(pd.param_off), k0, lds, lane);
-        const int64_t pt = base + lane;
-        const bool valid = pt < count8;
-        const v32f vf = G.v;
-        const float diff = vf[o] - target[pt];
-        acc = fold_batches(acc, valid ? diff * diff : 0.0f, lane);
-        const float two_diff = 2.0f * diff;
-        const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
-#pragma unroll
-        for (int j = 0; j < kTan; ++j) gacc[j] = fold_batches(gacc[j], valid ? two_diff * gf[j][o] : 0.0f, lane);
-    }
-    const float mse = reduce_lanes8(acc) / n_samples;
-    const bool fin = __builtin_isfinite(mse);
-    if (pass == 0 && lane == 0) mse_out[prog] = fin ? mse : FLT_MAX;
-#pragma unroll
-    for (int j = 0; j < kTan; ++j) {
-        const int k = (int)k0 + j;
-        const float g = reduce_lanes8(gacc[j]) / n_samples;
-        if (lane == 0 && k < 32) grad_out[(int64_t)prog * 32 + k] = (fin && k < np) ? g : 0.0f;
-    }
-}
-
-hipError_t launch_eval_grad_exact(const DevPrograms& P, const DevDataset& D, const uint8_t* nparams_eff,
-                                  const uint8_t* active, int max_params, float* mse_out, float* grad_out,
-                                  hipStream_t s, const KernelTimer* timer) {
-    if (P.n_progs <= 0 || max_params <= 0) return hipSuccess;
-    const int64_t n_tiles = eval_tiles(D, 1);
-    const int n_pass = (max_params + kTan - 1) / kTan;
-    const int64_t waves = (int64_t)P.n_progs * n_pass;
-    const size_t lds_bytes = eval_grad_lds_bytes(P);
-    if (timer) (void)hipEventRecord(timer->begin, s);
-    hipLaunchKernelGGL(k_eval_grad_exact, dim3((unsigned)((waves + kWaves - 1) / kWaves)), dim3(64 * kWaves),
-                       lds_bytes, s, P.code, P.progs, P.params, P.order, nparams_eff, active, P.n_progs, n_pass,
-                       D.feat, D.target, D.num_features, D.npad, D.count8, n_tiles, (float)D.num_samples, mse_out,
-                       grad_out, P.spill_slots);
-    hipError_t e = hipGetLastError();
-    if (timer) (void)hipEventRecord(timer->end, s);
-    return e;
 }
 
 __global__ __launch_bounds__(64) void k_eval_points_dual(const UInstr* __restrict__ code, const ProgDesc* __restrict__ progs,
                                                          const float* __restrict__ params, int prog, int active_param,
                                                          const float* __restrict__ feat, int F, int64_t npad,
                                                          int64_t count8, float* __restrict__ val_out,
-                                                         float* __restrict__ grad_out, int spill_slots) {
+                                                         float* __restrict__ grad_out) {
     extern __shared__ float smem[];
     const int lane = threadIdx.x & 63;
-    float* lds = smem;
-    (void)spill_slots;
     const int64_t base = (int64_t)blockIdx.x * 64;
     Files<1> R1;
     R1.a = (v32f)(0.0f);
@@ -475,7 +554,7 @@ __global__ __launch_bounds__(64) void k_eval_points_dual(const UInstr* __restric
     G.a = (v32f)(0.0f);
     const ProgDesc pd = progs[prog];
     run_dual<1>(G, code + uni(pd.code_off), uni(pd.code_len), params + uni(pd.param_off), (uint32_t)active_param,
-                lds, lane);
+                smem, lane);
     const uint32_t o = uni(pd.out_reg) & 31u;
     const int64_t pt = base + lane;
     const v32f vf = G.v, gfl = G.a;
@@ -490,7 +569,7 @@ hipError_t launch_eval_points_dual(const DevPrograms& P, const DevDataset& D, in
     const int64_t n_tiles = eval_tiles(D, 1);
     const size_t lds_bytes = (size_t)P.spill_slots * 2 * 64 * sizeof(float);
     hipLaunchKernelGGL(k_eval_points_dual, dim3((unsigned)n_tiles), dim3(64), lds_bytes, s, P.code, P.progs, P.params,
-                       prog, active_param, D.feat, D.num_features, D.npad, D.count8, val_out, grad_out, P.spill_slots);
+                       prog, active_param, D.feat, D.num_features, D.npad, D.count8, val_out, grad_out);
     return hipGetLastError();
 }
 

@@ -49,6 +49,7 @@ struct CmaDev {
     int *flags, *iters;
 };
 
+hipError_t cmaes_begin(const CmaDev& st, const float* d_mse, hipStream_t s);
 hipError_t cmaes_sample(const CmaDev& st, float* cand_params, int it, hipStream_t s);
 hipError_t cmaes_update(const CmaDev& st, const float* cand_mse, int* counter, hipStream_t s);
 hipError_t cmaes_finish(const CmaDev& st, float* params, float* fitness, int* iters_out, hipStream_t s);

@@ -282,6 +282,19 @@ __host__ __device__ __forceinline__ float cma_normal(uint64_t seed, uint32_t ind
     return __builtin_sqrtf(-2.0f * dm_ln(u1)) * dm_cos(2.0f * 3.14159274101257324f * u2);
 }
 
+// Starting point of every individual (cmaes.rs:104-110): best = the doubled mean, scored
+// by the parent's MSE plus 0.01 * its L1 norm (summed in dimension order).
+__global__ void k_cmaes_begin(CmaDev st, const float* __restrict__ d_mse) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    const int N = st.n;
+    float l1 = 0.0f;
+    for (int j = 0; j < st.ndim[i]; ++j) l1 += __builtin_fabsf(st.mean[j * N + i]);
+    const float m = d_mse[st.prog[i]];
+    st.best_mse[i] = m;
+    st.best_fit[i] = m + 0.01f * l1;
+}
+
 __global__ void k_cmaes_sample(CmaDev st, float* __restrict__ cand_params, int it) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = t / st.lambda, c = t - i * st.lambda;
@@ -382,6 +395,11 @@ __global__ void k_cmaes_finish(CmaDev st, float* __restrict__ params, float* __r
     for (int j = 0; j < np; ++j) pp[j] = st.best[j * st.n + i];
     fitness[i] = st.ndim[i] > 0 ? st.best_mse[i] : __builtin_nanf("");
     if (iters_out) iters_out[i] = st.iters[i] + ((st.flags[i] & CMA_DONE) ? 1 : 0);
+}
+
+hipError_t cmaes_begin(const CmaDev& st, const float* d_mse, hipStream_t s) {
+    hipLaunchKernelGGL(k_cmaes_begin, dim3((unsigned)((st.n + 127) / 128)), dim3(128), 0, s, st, d_mse);
+    return hipGetLastError();
 }
 
 hipError_t cmaes_sample(const CmaDev& st, float* cand_params, int it, hipStream_t s) {

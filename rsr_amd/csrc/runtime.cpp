@@ -17,6 +17,7 @@
 #include "../../include/rsr_amd.h"
 #include "kernels.h"
 #include "lower.h"
+#include "jobs.h"
 #include "runtime_internal.h"
 
 using namespace rsr;
@@ -69,27 +70,69 @@ extern "C" const char* rsr_version(void) { return "rsr_amd 0.1 (gfx950)"; }
 // ---------------------------------------------------------------------------------
 // device buffers with budget accounting
 // ---------------------------------------------------------------------------------
+// Block sizes: powers of two from 256 B up to 1 MiB, then multiples of 2 MiB, so that
+// released blocks are reused by later requests of similar size.
+static size_t block_size(size_t n) {
+    if (n <= (1u << 20)) {
+        size_t b = 256;
+        while (b < n) b <<= 1;
+        return b;
+    }
+    const size_t g = (size_t)2 << 20;
+    return (n + g - 1) / g * g;
+}
+
+// Give cached blocks back to the device (all work on the stream is finished first).
+void rsr::trim_pool(rsr_context* c) {
+    if (c->pool.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->pool) (void)hipFree(kv.second);
+    c->used -= c->pool_bytes;
+    c->pool.clear();
+    c->pool_bytes = 0;
+}
+
+// Every buffer of a context is used only by work on the context's one stream, so a
+// released block can be handed to the next request at once: stream order puts its new
+// use after every earlier use.
 int DevBuf::alloc(rsr_context* c, size_t n) {
     release();
     if (n == 0) return RSR_OK;
-    if (c->used + n > c->budget)
-        return set_error(RSR_E_BUDGET, "allocation of %zu bytes exceeds the HBM budget (%llu of %llu used)", n,
+    const size_t sz = block_size(n);
+    auto it = c->pool.find(sz);
+    if (it != c->pool.end()) {
+        ptr = it->second;
+        c->pool.erase(it);
+        c->pool_bytes -= sz;
+        ctx = c;
+        bytes = sz;
+        return RSR_OK;
+    }
+    if (c->used + sz > c->budget) trim_pool(c);
+    if (c->used + sz > c->budget)
+        return set_error(RSR_E_BUDGET, "allocation of %zu bytes exceeds the HBM budget (%llu of %llu used)", sz,
                          (unsigned long long)c->used, (unsigned long long)c->budget);
-    hipError_t e = hipMalloc(&ptr, n);
+    hipError_t e = hipMalloc(&ptr, sz);
+    if (e != hipSuccess) {
+        trim_pool(c);
+        e = hipMalloc(&ptr, sz);
+    }
     if (e != hipSuccess) {
         ptr = nullptr;
         return hip_error(e, "hipMalloc");
     }
     ctx = c;
-    bytes = n;
-    c->used += n;
+    bytes = sz;
+    c->used += sz;
     return RSR_OK;
 }
 
 void DevBuf::release() {
-    if (ptr) {
+    if (ptr && ctx) {
+        ctx->pool.emplace(bytes, ptr);
+        ctx->pool_bytes += bytes;
+    } else if (ptr) {
         (void)hipFree(ptr);
-        if (ctx) ctx->used -= bytes;
     }
     ptr = nullptr;
     bytes = 0;
@@ -197,8 +240,14 @@ extern "C" int rsr_context_destroy(rsr_context* ctx) {
     drain_profile(ctx);
     ctx->scratch_act.release();
     ctx->scratch_partials.release();
+    ctx->scratch_gpartials.release();
+    ctx->scratch_state.release();
+    ctx->scratch_gstate.release();
     ctx->scratch_out.release();
     ctx->scratch_out2.release();
+    ctx->dev_counters.release();
+    rsr::trim_pool(ctx);
+    if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RSR_OK;
@@ -226,6 +275,14 @@ extern "C" int rsr_context_set_summation(rsr_context* ctx, int mode) {
     if (mode != RSR_SUM_KAHAN && mode != RSR_SUM_REFERENCE) return set_error(RSR_E_INVALID_ARG, "bad summation mode");
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->summation = mode;
+    return RSR_OK;
+}
+
+extern "C" int rsr_context_set_partials_limit(rsr_context* ctx, uint64_t bytes) {
+    if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
+    if (bytes < 4096) return set_error(RSR_E_INVALID_ARG, "partials limit below 4096 bytes");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->arena_bytes = (size_t)bytes;
     return RSR_OK;
 }
 
@@ -582,6 +639,7 @@ int rsr::get_lowering(rsr_programs* p, int F, bool dual, Lowering** out) {
     std::iota(L->order.begin(), L->order.end(), 0);
     std::stable_sort(L->order.begin(), L->order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
     L->active_host = active;
+    L->desc = descs;
     int rc;
     if ((rc = L->dcode.upload(ctx, code.data(), code.size() * sizeof(UInstr)))) return rc;
     if ((rc = L->ddesc.upload(ctx, descs.data(), std::max<size_t>(1, descs.size()) * sizeof(ProgDesc)))) return rc;
@@ -630,54 +688,15 @@ extern "C" int rsr_programs_destroy(rsr_programs* p) {
 // ---------------------------------------------------------------------------------
 // evaluation
 // ---------------------------------------------------------------------------------
-int rsr::eval_mse_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* active_override,
-                         float* d_mse, Lowering** Lout) {
-    Lowering* L;
-    int rc = get_lowering(p, ds->num_features, false, &L);
-    if (rc) return rc;
-    if (Lout) *Lout = L;
-    const DevDataset D = dev_view(ds);
-    EvalTuning t = ctx->tuning;
-    const DevPrograms DP = dev_view(p, L);
-    const uint8_t* act0 = active_override ? active_override : (L->n_invalid ? (const uint8_t*)L->dactive.ptr : nullptr);
-    if (ctx->summation == RSR_SUM_REFERENCE) {
-        if (eval_mse_lds_bytes(DP, 1) > ctx->max_lds)
-            return set_error(RSR_E_UNSUPPORTED, "spill area exceeds the workgroup LDS limit");
-        HIP_TRY(launch_eval_mse_exact(DP, D, act0, d_mse, ctx->stream, ctx->next_timer()), "eval_mse_exact launch");
-        return RSR_OK;
-    }
-    while (t.ppl > 1 && eval_mse_lds_bytes(DP, t.ppl) > ctx->max_lds) t.ppl /= 2;  // spill area must fit
-    if (eval_mse_lds_bytes(DP, t.ppl) > ctx->max_lds)
-        return set_error(RSR_E_UNSUPPORTED, "spill area of %zu bytes exceeds the workgroup LDS limit",
-                         eval_mse_lds_bytes(DP, t.ppl));
-    const size_t nparts = (size_t)eval_tiles(D, t.ppl) * std::max(p->n, 1);
-    if ((rc = ensure_scratch(ctx, ctx->scratch_partials, nparts * sizeof(float)))) return rc;
-    const uint8_t* act = active_override ? active_override : (L->n_invalid ? (const uint8_t*)L->dactive.ptr : nullptr);
-    HIP_TRY(launch_eval_mse(DP, D, act, t, (float*)ctx->scratch_partials.ptr, d_mse, ctx->stream,
-                            ctx->next_timer()),
-            "eval_mse launch");
-    return RSR_OK;
-}
-
 extern "C" int rsr_eval_mse(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, float* mse_out) {
     if (!ctx || !p || !ds || (!mse_out && p->n)) return set_error(RSR_E_INVALID_ARG, "null argument");
     if (p->n == 0) return RSR_OK;
     std::lock_guard<std::mutex> g(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
-    int rc = ensure_scratch(ctx, ctx->scratch_out, (size_t)p->n * sizeof(float));
+    std::unique_ptr<Job> job;
+    int rc = make_eval_job(ctx, p, ds, nullptr, mse_out, &job);
     if (rc) return rc;
-    Lowering* L;
-    if ((rc = eval_mse_device(ctx, p, ds, nullptr, (float*)ctx->scratch_out.ptr, &L))) return rc;
-    HIP_TRY(hipMemcpyAsync(mse_out, ctx->scratch_out.ptr, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost,
-                           ctx->stream),
-            "mse download");
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "eval_mse");
-    if (L->n_invalid) {
-        for (int i = 0; i < p->n; ++i)
-            if (L->status[i] != PROG_OK) mse_out[i] = NAN;
-        return set_error(RSR_E_PROGRAM, "%d program(s) rejected by lowering", L->n_invalid);
-    }
-    return RSR_OK;
+    return run_job(ctx, *job);
 }
 
 extern "C" int rsr_eval_points(rsr_context* ctx, rsr_programs* p, int first, int count, const rsr_dataset* ds,
@@ -702,96 +721,16 @@ extern "C" int rsr_eval_points(rsr_context* ctx, rsr_programs* p, int first, int
     return RSR_OK;
 }
 
-// MSE + gradient into device buffers d_mse[n], d_grad[n*32].
-int rsr::eval_grad_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* host_active,
-                          float* d_mse, float* d_grad, Lowering** Ldual_out, Lowering** Lprim_out) {
-    Lowering *Lp, *Ld;
-    int rc = get_lowering(p, ds->num_features, false, &Lp);
-    if (rc) return rc;
-    if ((rc = get_lowering(p, ds->num_features, true, &Ld))) return rc;
-    if (Ldual_out) *Ldual_out = Ld;
-    if (Lprim_out) *Lprim_out = Lp;
-    const DevDataset D = dev_view(ds);
-    // zero-parameter programs: compute_mse on the primal lowering (evaluator.rs:187-189)
-    bool any_zero = false, any_pos = false;
-    ctx->host_act.assign(2 * (size_t)std::max(p->n, 1), 0);
-    uint8_t* act0 = ctx->host_act.data();
-    uint8_t* act1 = act0 + std::max(p->n, 1);
-    for (int i = 0; i < p->n; ++i) {
-        const bool on = host_active == nullptr || host_active[i];
-        if (Ld->nparams[i] == 0) {
-            if (on && Lp->status[i] == PROG_OK) { act0[i] = 1; any_zero = true; }
-        } else if (on && Ld->status[i] == PROG_OK) {
-            act1[i] = 1;
-            any_pos = true;
-        }
-    }
-    if ((rc = ensure_scratch(ctx, ctx->scratch_act, 2 * (size_t)std::max(p->n, 1)))) return rc;
-    uint8_t* dact0 = (uint8_t*)ctx->scratch_act.ptr;
-    uint8_t* dact1 = dact0 + std::max(p->n, 1);
-    HIP_TRY(hipMemcpyAsync(dact0, act0, 2 * (size_t)std::max(p->n, 1), hipMemcpyHostToDevice, ctx->stream),
-            "act upload");
-    HIP_TRY(hipMemsetAsync(d_grad, 0, (size_t)p->n * 32 * sizeof(float), ctx->stream), "grad clear");
-    if (any_zero && ctx->summation == RSR_SUM_REFERENCE) {
-        HIP_TRY(launch_eval_mse_exact(dev_view(p, Lp), D, dact0, d_mse, ctx->stream), "eval_mse_exact launch");
-    } else if (any_zero) {
-        EvalTuning t = ctx->tuning;
-        const DevPrograms DP = dev_view(p, Lp);
-        while (t.ppl > 1 && eval_mse_lds_bytes(DP, t.ppl) > ctx->max_lds) t.ppl /= 2;
-        if (eval_mse_lds_bytes(DP, t.ppl) > ctx->max_lds)
-            return set_error(RSR_E_UNSUPPORTED, "spill area exceeds the workgroup LDS limit");
-        const size_t nparts = (size_t)eval_tiles(D, t.ppl) * p->n;
-        if ((rc = ensure_scratch(ctx, ctx->scratch_partials, nparts * sizeof(float)))) return rc;
-        HIP_TRY(launch_eval_mse(DP, D, dact0, t, (float*)ctx->scratch_partials.ptr, d_mse, ctx->stream),
-                "eval_mse launch");
-    }
-    if (any_pos && eval_grad_lds_bytes(dev_view(p, Ld)) > ctx->max_lds)
-        return set_error(RSR_E_UNSUPPORTED, "dual spill area of %zu bytes exceeds the workgroup LDS limit",
-                         eval_grad_lds_bytes(dev_view(p, Ld)));
-    if (any_pos && ctx->summation == RSR_SUM_REFERENCE) {
-        HIP_TRY(launch_eval_grad_exact(dev_view(p, Ld), D, (const uint8_t*)Ld->dnparams.ptr, dact1, Ld->max_params,
-                                       d_mse, d_grad, ctx->stream, ctx->next_timer()),
-                "eval_grad_exact launch");
-    } else if (any_pos) {
-        const size_t nparts = (size_t)eval_tiles(D, 1) * p->n * 33;
-        if ((rc = ensure_scratch(ctx, ctx->scratch_partials, nparts * sizeof(float)))) return rc;
-        HIP_TRY(launch_eval_grad(dev_view(p, Ld), D, (const uint8_t*)Ld->dnparams.ptr, dact1, Ld->max_params,
-                                 (float*)ctx->scratch_partials.ptr, d_mse, d_grad, ctx->stream, ctx->next_timer()),
-                "eval_grad launch");
-    }
-    return RSR_OK;
-}
-
 extern "C" int rsr_eval_mse_grad(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, float* mse_out,
                                  float* grad_out) {
     if (!ctx || !p || !ds || (p->n && (!mse_out || !grad_out))) return set_error(RSR_E_INVALID_ARG, "null argument");
     if (p->n == 0) return RSR_OK;
     std::lock_guard<std::mutex> g(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
-    int rc = ensure_scratch(ctx, ctx->scratch_out, (size_t)p->n * sizeof(float));
+    std::unique_ptr<Job> job;
+    int rc = make_grad_job(ctx, p, ds, nullptr, mse_out, grad_out, &job);
     if (rc) return rc;
-    if ((rc = ensure_scratch(ctx, ctx->scratch_out2, (size_t)p->n * 32 * sizeof(float)))) return rc;
-    Lowering *Ld, *Lp;
-    if ((rc = eval_grad_device(ctx, p, ds, nullptr, (float*)ctx->scratch_out.ptr, (float*)ctx->scratch_out2.ptr, &Ld,
-                               &Lp)))
-        return rc;
-    HIP_TRY(hipMemcpyAsync(mse_out, ctx->scratch_out.ptr, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost,
-                           ctx->stream),
-            "mse download");
-    HIP_TRY(hipMemcpyAsync(grad_out, ctx->scratch_out2.ptr, (size_t)p->n * 32 * sizeof(float), hipMemcpyDeviceToHost,
-                           ctx->stream),
-            "grad download");
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "eval_mse_grad");
-    int bad = 0;
-    for (int i = 0; i < p->n; ++i) {
-        const bool ok = Ld->nparams[i] ? Ld->status[i] == PROG_OK : Lp->status[i] == PROG_OK;
-        if (!ok) {
-            mse_out[i] = NAN;
-            bad++;
-        }
-    }
-    if (bad) return set_error(RSR_E_PROGRAM, "%d program(s) rejected by lowering", bad);
-    return RSR_OK;
+    return run_job(ctx, *job);
 }
 
 extern "C" int rsr_eval_points_dual(rsr_context* ctx, rsr_programs* p, int prog, int active_param,

@@ -36,6 +36,7 @@ struct Lowering {
     std::vector<uint8_t> differentiable;  // no Inverse ops
     std::vector<int32_t> order;           // cost-sorted scheduling order
     std::vector<uint8_t> active_host;
+    std::vector<ProgDesc> desc;           // host copy of the program descriptors
     int n_invalid = 0;
     int max_params = 0;
     int spill_slots = 0;
@@ -55,12 +56,19 @@ struct rsr_context {
     std::vector<uint8_t> host_act;  // staging for active masks (lives until the stream syncs)
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
-    uint64_t used = 0;
+    uint64_t used = 0;  // device bytes held, including cached free blocks
+    std::multimap<size_t, void*> pool;  // released blocks by size
+    uint64_t pool_bytes = 0;
     std::recursive_mutex rmu;
     std::mutex mu;
     rsr::EvalTuning tuning;
     int summation = 0;  // 0: pairwise + Kahan (fast), 1: reference order (bit-identical to the CPU evaluator)
-    rsr::DevBuf scratch_partials, scratch_out, scratch_out2, scratch_act;
+    rsr::DevBuf scratch_partials, scratch_gpartials, scratch_state, scratch_gstate, scratch_out, scratch_out2, scratch_act;
+    size_t arena_bytes = (size_t)4 << 30;  // summation partials / terms per launch round
+    // per-job tick counters: device array and its pinned host mirror
+    rsr::DevBuf dev_counters;
+    int* host_counters = nullptr;
+    size_t host_counters_cap = 0;
     // profiling: events around every main evaluation kernel, accumulated on read
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
@@ -99,18 +107,11 @@ struct rsr_programs {
 };
 
 namespace rsr {
+void trim_pool(rsr_context* c);
 int ensure_scratch(rsr_context* c, DevBuf& b, size_t n);
 DevDataset dev_view(const rsr_dataset* d);
 DevPrograms dev_view(const rsr_programs* p, const Lowering* L);
 RefProgram ref_program(const rsr_programs* p, int i);
 int get_lowering(rsr_programs* p, int F, bool dual, Lowering** out);
 int upload_params(rsr_programs* p);
-int eval_mse_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* active_override,
-                    float* d_mse, Lowering** Lout);
-int eval_grad_device(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, const uint8_t* host_active,
-                     float* d_mse, float* d_grad, Lowering** Ldual_out, Lowering** Lprim_out);
-int fit_lbfgs_impl(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int max_iterations, int history, int mode,
-                   const uint8_t* mask, float* fitness_out);
-int fit_cmaes_impl(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int max_iterations, int lambda, int mu,
-                   uint64_t seed, const float* normals, const uint8_t* mask, float* fitness_out, int32_t* iters_out);
 }  // namespace rsr

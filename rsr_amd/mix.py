@@ -1,0 +1,166 @@
+"""The serving job mix: independent regression jobs arriving at random, each running
+per-generation evaluation + constant-fit requests against its device-resident dataset
+and population (the island loop of src/engine/search/island.rs:215-261 reduced to its
+evaluator traffic: score the population, then fit the constants of the promising
+individuals).
+
+Jobs: Poisson arrivals (mean `rate` jobs/s); dataset sizes log-uniform in
+[400, 1,000,000] points; feature types and targets of the four workloads (scalar x;
+E, v, B Vec3; M Mat3 + v, u Vec3; A, B Mat2); populations uniform in [1,000, 16,384]
+programs drawn from per-type program pools.  A generation is one MSE request over the
+whole population followed by one fit request over its best `fit_frac` (L-BFGS, 15
+iterations, for differentiable programs; CMA-ES for programs with matrix inverses).
+"""
+from __future__ import annotations
+
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import api
+from . import workloads as W
+
+DATA = [W.data_c0, W.data_c1, W.data_c2, W.data_c3]
+GEN = [W.gen_c0, W.gen_c1, W.gen_c2, W.gen_c3]
+TYPE_NAMES = ["scalar", "3xVec3", "Mat3+2xVec3", "2xMat2"]
+
+
+@dataclass
+class MixJob:
+    index: int
+    kind: int
+    arrival: float
+    n_points: int
+    programs: list
+    ds: object = None
+    latencies: list = field(default_factory=list)
+    node_evals: int = 0
+    error: str = ""
+
+
+def make_pools(size: int, seed: int = 4100, c1_pool=None):
+    pools = []
+    for t, g in enumerate(GEN):
+        if t == 1 and c1_pool is not None:
+            pools.append(list(c1_pool))
+        else:
+            pools.append(g(size, seed=seed + t))
+    return pools
+
+
+def make_jobs(ctx, pools, n_jobs=64, rate=4.0, seed=4000, pop_range=(1000, 16384), n_range=(400, 1_000_000)):
+    """Jobs with device-resident datasets and populations (lowered ahead of the run)."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    arrivals -= arrivals[0]
+    kinds = rng.integers(0, 4, n_jobs)
+    npts = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), n_jobs)).astype(np.int64)
+    pops = rng.integers(pop_range[0], pop_range[1] + 1, n_jobs)
+    jobs = []
+    for j in range(n_jobs):
+        k = int(kinds[j])
+        X, y, t = DATA[k](int(npts[j]), seed=5000 + j)
+        ds = api.Dataset.new(ctx, X, y, t)
+        pool = pools[k]
+        idx = rng.integers(0, len(pool), int(pops[j]))
+        progs = [pool[i] for i in idx]
+        jobs.append(MixJob(j, k, float(arrivals[j]), int(npts[j]), progs, ds))
+    return jobs
+
+
+def _fit_mask(mse: np.ndarray, frac: float, min_fit: int):
+    k = max(min_fit, int(len(mse) * frac))
+    k = min(k, len(mse))
+    order = np.argsort(np.where(np.isfinite(mse), mse, np.inf), kind="stable")
+    m = np.zeros(len(mse), np.uint8)
+    m[order[:k]] = 1
+    return m
+
+
+def _nodes(progs, n):
+    return int(sum(len(p.code) for p in progs)) * int(n)
+
+
+def run_served(ctx, jobs, generations=3, fit_frac=0.01, min_fit=8, max_inflight=64, cmaes_iters=6):
+    """Every job is a client thread submitting to one server as it arrives."""
+    srv = api.Server(ctx, max_inflight=max_inflight)
+    for jb in jobs:
+        jb.latencies, jb.error = [], ""
+    t_start = time.perf_counter()
+
+    def client(jb: MixJob):
+        try:
+            P = api.Programs(ctx, jb.programs)
+            P.status(jb.ds)  # lowered ahead of the first generation
+            delay = jb.arrival - (time.perf_counter() - t_start)
+            if delay > 0:
+                time.sleep(delay)
+            for g in range(generations):
+                t0 = time.perf_counter()
+                mse = np.array(srv.submit_eval(P, jb.ds).wait())
+                m = _fit_mask(mse, fit_frac, min_fit)
+                if jb.kind == 3:
+                    srv.submit_cmaes(P, jb.ds, iterations=cmaes_iters, lam=16, mu=8, seed=1000 * jb.index + g,
+                                     mask=m).wait()
+                else:
+                    srv.submit_lbfgs(P, jb.ds, iterations=15, history=6, mask=m).wait()
+                jb.latencies.append(time.perf_counter() - t0)
+            P.close()
+        except Exception as e:  # reported in the summary
+            jb.error = repr(e)
+
+    threads = [threading.Thread(target=client, args=(jb,)) for jb in jobs]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    wall = time.perf_counter() - t_start
+    st = srv.stats()
+    srv.close()
+    return summarize(jobs, wall, st, generations)
+
+
+def run_sequential(ctx, jobs, generations=3, fit_frac=0.01, min_fit=8, cmaes_iters=6):
+    """The same requests one at a time through the direct entry points (no server)."""
+    for jb in jobs:
+        jb.latencies, jb.error = [], ""
+    t_start = time.perf_counter()
+    for jb in jobs:
+        P = api.Programs(ctx, jb.programs)
+        P.status(jb.ds)
+        for g in range(generations):
+            t0 = time.perf_counter()
+            mse = api.compute_mse(ctx, P, jb.ds, allow_rejected=True)
+            m = _fit_mask(mse, fit_frac, min_fit)
+            if jb.kind == 3:
+                api.fit_cmaes(ctx, P, jb.ds, iterations=cmaes_iters, lam=16, mu=8, seed=1000 * jb.index + g, mask=m)
+            else:
+                api.fit_lbfgs(ctx, P, jb.ds, iterations=15, history=6, mask=m)
+            jb.latencies.append(time.perf_counter() - t0)
+        P.close()
+    wall = time.perf_counter() - t_start
+    return summarize(jobs, wall, None, generations)
+
+
+def summarize(jobs, wall, stats, generations):
+    lat = np.array([x for jb in jobs for x in jb.latencies], np.float64)
+    eval_nodes = sum(_nodes(jb.programs, jb.n_points) for jb in jobs if jb.latencies) * generations
+    out = {
+        "jobs": len(jobs),
+        "generations": generations,
+        "completed_generations": int(len(lat)),
+        "errors": [jb.error for jb in jobs if jb.error][:3],
+        "wall_s": wall,
+        "p50_gen_latency_ms": float(np.percentile(lat, 50) * 1e3) if len(lat) else None,
+        "p99_gen_latency_ms": float(np.percentile(lat, 99) * 1e3) if len(lat) else None,
+        "eval_node_evals": int(eval_nodes),
+        "eval_node_evals_per_s": eval_nodes / wall if wall > 0 else None,
+    }
+    if stats is not None:
+        out["server"] = {k: stats[k] for k in ("ticks", "max_inflight_seen", "busy_seconds", "node_evals",
+                                                "completed", "failed")}
+        out["node_evals_per_s"] = stats["node_evals"] / wall if wall > 0 else None
+        out["node_evals_per_busy_s"] = stats["node_evals"] / stats["busy_seconds"] if stats["busy_seconds"] else None
+    return out

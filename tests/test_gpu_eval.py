@@ -263,19 +263,22 @@ def test_full_size_c1_properties(gpu_ctx):
     assert np.array_equal(mse2, mse[perm])
 
 
+@pytest.mark.parametrize("ppl", [1, 2, 4])
 @pytest.mark.parametrize("gen,data,npts", [(W.gen_c0, W.data_c0, 403), (W.gen_c1, W.data_c1, 1001),
                                             (W.gen_c2, W.data_c2, 500), (W.gen_c3, W.data_c3, 777)])
-def test_reference_order_summation_bit_identical(gpu_ctx, gen, data, npts):
+def test_reference_order_summation_bit_identical(gpu_ctx, gen, data, npts, ppl):
     X, y, t = data(npts, seed=41)
     progs = gen(96, seed=42)
     ds, ods = make(gpu_ctx, X, y, t)
     P = api.Programs(gpu_ctx, progs)
     gpu_ctx.set_summation(True)
+    gpu_ctx.set_tuning(ppl, 8)
     try:
         mse = api.compute_mse(gpu_ctx, P, ds)
         m2, g2 = api.compute_mse_with_gradient(gpu_ctx, P, ds)
     finally:
         gpu_ctx.set_summation(False)
+        gpu_ctx.set_tuning(2, 8)
     for i, p in enumerate(progs):
         ref = O.compute_mse(p, ods)
         assert mse[i] == ref, (i, mse[i], ref)
@@ -283,3 +286,42 @@ def test_reference_order_summation_bit_identical(gpu_ctx, gen, data, npts):
         assert m2[i] == rm, (i, m2[i], rm)
         ok, nbad = same_bits(g2[i], rg)
         assert ok, (i, g2[i][:4], rg[:4])
+
+
+def test_reference_order_full_size_c1(gpu_ctx):
+    """Reference summation at 100,000 points: bit-identical MSE to the CPU f32 oracle."""
+    X, y, t = W.data_c1(100_000, seed=1001)
+    progs = W.gen_c1(512, seed=2001)
+    ds, ods = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, progs)
+    gpu_ctx.set_summation(True)
+    gpu_ctx.set_tuning(4, 4)
+    try:
+        mse = api.compute_mse(gpu_ctx, P, ds)
+    finally:
+        gpu_ctx.set_summation(False)
+        gpu_ctx.set_tuning(2, 8)
+    for i in np.random.default_rng(3).choice(len(progs), 8, replace=False):
+        assert mse[i] == O.compute_mse(progs[i], ods), i
+
+
+@pytest.mark.parametrize("reference", [False, True])
+def test_split_rounds_same_as_unsplit(gpu_ctx, reference):
+    """A small partials arena splits evaluations into tile ranges; results are unchanged."""
+    X, y, t = W.data_c2(5003, seed=51)
+    progs = W.gen_c2(200, seed=52)
+    ds, _ = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, progs)
+    gpu_ctx.set_summation(reference)
+    try:
+        m0 = api.compute_mse(gpu_ctx, P, ds, allow_rejected=True)
+        gm0, g0 = api.compute_mse_with_gradient(gpu_ctx, P, ds, allow_rejected=True)
+        gpu_ctx.set_partials_limit(64 * 1024)
+        m1 = api.compute_mse(gpu_ctx, P, ds, allow_rejected=True)
+        gm1, g1 = api.compute_mse_with_gradient(gpu_ctx, P, ds, allow_rejected=True)
+    finally:
+        gpu_ctx.set_partials_limit(4 << 30)
+        gpu_ctx.set_summation(False)
+    assert same_bits(m0, m1)[0]
+    assert same_bits(gm0, gm1)[0]
+    assert same_bits(g0, g1)[0]
