@@ -182,6 +182,27 @@ RSR_API int rsr_fit_lbfgs(rsr_context* ctx, rsr_programs* progs, const rsr_datas
 RSR_API int rsr_fit_cmaes(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, int max_iterations,
                           int lambda, int mu, uint64_t seed, const float* normals, const uint8_t* mask,
                           float* fitness_out, int32_t* iters_out);
+/* run_nelder_mead (nelder_mead.rs:9-165) over the Float constants (at most 32) of every
+ * program with mask[i] != 0; programs without Float constants are left alone (fitness
+ * NaN).  iters_out (optional): iterations run. */
+RSR_API int rsr_fit_nelder_mead(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, int max_iterations,
+                                const uint8_t* mask, float* fitness_out, int32_t* iters_out);
+/* Individual::optimize_constants (individual.rs:65-75) for every program with mask[i] != 0,
+ * followed by the caller's calculate_mse (island.rs:233-236): the dispatch of
+ * optimize_individual_constants (optimize/mod.rs:10-59) -- L-BFGS (15 iterations, M = 6,
+ * reference mode) for programs without a matrix inverse, else CMA-ES (16 / 8,
+ * max(max_iterations / 16, 2) iterations, Philox stream `seed`) for programs with a
+ * vector or matrix constant, else Nelder-Mead (max_iterations) -- then every constant
+ * component with magnitude below `threshold` (reference: 0.05) set to zero, then the
+ * MSE of the result into mse_out (NaN for programs not optimized).  The reference's
+ * simplify_ast of the snapped tree is not applied.  method_out (optional): RSR_FIT_*. */
+#define RSR_FIT_NONE 0
+#define RSR_FIT_LBFGS 1
+#define RSR_FIT_CMAES 2
+#define RSR_FIT_NELDER_MEAD 3
+RSR_API int rsr_optimize_constants(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, int max_iterations,
+                                   uint64_t seed, float threshold, const uint8_t* mask, float* mse_out,
+                                   uint8_t* method_out);
 /* The standard normal the CMA-ES sampler draws for (seed, program, iteration,
  * candidate, dimension): Philox4x32-10 -> two 24-bit uniforms -> Box-Muller as
  * cmaes.rs:13-17. */
@@ -206,6 +227,8 @@ typedef struct rsr_server rsr_server;
 #define RSR_REQ_GRAD 1   /* compute_mse_with_gradient: out = mse[n], out2 = grad[n][32] */
 #define RSR_REQ_LBFGS 2  /* rsr_fit_lbfgs: out = fitness[n]; constants updated in place */
 #define RSR_REQ_CMAES 3  /* rsr_fit_cmaes: out = fitness[n], iters_out[n] (optional) */
+#define RSR_REQ_NELDER_MEAD 4 /* rsr_fit_nelder_mead: out = fitness[n], iters_out[n] (optional) */
+#define RSR_REQ_OPTIMIZE 5    /* rsr_optimize_constants (threshold 0.05): out = mse[n] after the fit */
 typedef struct rsr_request {
     int kind;                    /* RSR_REQ_* */
     rsr_programs* programs;      /* not used by any other call until the request completes */

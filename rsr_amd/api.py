@@ -381,6 +381,37 @@ def fit_cmaes(ctx: Context, progs: Programs, ds: Dataset, iterations: int, lam: 
     return fit[: progs.n], its[: progs.n]
 
 
+def fit_nelder_mead(ctx: Context, progs: Programs, ds: Dataset, iterations: int = 100, mask=None):
+    """Batched run_nelder_mead (nelder_mead.rs:9-165): returns (fitness[n], iterations_run[n])."""
+    l = lib()
+    l.rsr_fit_nelder_mead.restype = C.c_int
+    l.rsr_fit_nelder_mead.argtypes = [_P, _P, _P, C.c_int, _u8p, _f32p, _i32p]
+    fit = np.zeros(max(progs.n, 1), np.float32)
+    its = np.zeros(max(progs.n, 1), np.int32)
+    m = _mask(progs, mask)
+    _check(l.rsr_fit_nelder_mead(ctx.h, progs.h, ds.h, iterations, _ptr(m, _u8p) if m is not None else None,
+                                 _ptr(fit, _f32p), _ptr(its, _i32p)))
+    return fit[: progs.n], its[: progs.n]
+
+
+FIT_NONE, FIT_LBFGS, FIT_CMAES, FIT_NELDER_MEAD = 0, 1, 2, 3
+
+
+def optimize_constants(ctx: Context, progs: Programs, ds: Dataset, iterations: int = 100, seed: int = 0,
+                       threshold: float = 0.05, mask=None):
+    """Individual::optimize_constants (individual.rs:65-75) + calculate_mse, batched:
+    returns (mse_after[n], method[n]) with method one of FIT_*."""
+    l = lib()
+    l.rsr_optimize_constants.restype = C.c_int
+    l.rsr_optimize_constants.argtypes = [_P, _P, _P, C.c_int, C.c_uint64, C.c_float, _u8p, _f32p, _u8p]
+    mse = np.zeros(max(progs.n, 1), np.float32)
+    meth = np.zeros(max(progs.n, 1), np.uint8)
+    m = _mask(progs, mask)
+    _check(l.rsr_optimize_constants(ctx.h, progs.h, ds.h, iterations, seed, threshold,
+                                    _ptr(m, _u8p) if m is not None else None, _ptr(mse, _f32p), _ptr(meth, _u8p)))
+    return mse[: progs.n], meth[: progs.n]
+
+
 def cmaes_normal(seed: int, program: int, iteration: int, candidate: int, dim: int) -> float:
     """The normal the CMA-ES sampler draws (host restatement of the device function)."""
     return _bind_fit().rsr_cmaes_normal(seed, program, iteration, candidate, dim)
@@ -389,7 +420,7 @@ def cmaes_normal(seed: int, program: int, iteration: int, candidate: int, dim: i
 # ---------------------------------------------------------------------------------
 # serving: submit / poll / wait (rsr_server_*)
 # ---------------------------------------------------------------------------------
-REQ_EVAL, REQ_GRAD, REQ_LBFGS, REQ_CMAES = 0, 1, 2, 3
+REQ_EVAL, REQ_GRAD, REQ_LBFGS, REQ_CMAES, REQ_NELDER_MEAD, REQ_OPTIMIZE = 0, 1, 2, 3, 4, 5
 RSR_E_NOT_READY = -10
 RSR_E_TIMEOUT = -9
 
@@ -453,7 +484,7 @@ class Ticket:
             return self.out[: self.n]
         if self.kind == REQ_GRAD:
             return self.out[: self.n], self.out2[: self.n]
-        if self.kind == REQ_CMAES:
+        if self.kind in (REQ_CMAES, REQ_NELDER_MEAD):
             return self.out[: self.n], self.iters[: self.n]
         return self.out[: self.n]
 
@@ -480,7 +511,7 @@ class Server:
         n = progs.n
         out = np.zeros(max(n, 1), np.float32)
         out2 = np.zeros((max(n, 1), 32), np.float32) if kind == REQ_GRAD else None
-        iters = np.zeros(max(n, 1), np.int32) if kind == REQ_CMAES else None
+        iters = np.zeros(max(n, 1), np.int32) if kind in (REQ_CMAES, REQ_NELDER_MEAD) else None
         m = _mask(progs, mask)
         r = _Request(kind, progs.h.value if isinstance(progs.h, C.c_void_p) else progs.h,
                      ds.h.value if isinstance(ds.h, C.c_void_p) else ds.h,
@@ -504,6 +535,14 @@ class Server:
     def submit_cmaes(self, progs: Programs, ds: Dataset, iterations: int, lam: int = 16, mu: int = 8,
                      seed: int = 0, mask=None) -> Ticket:
         return self._submit(REQ_CMAES, progs, ds, mask, iterations=iterations, lam=lam, mu=mu, seed=seed)
+
+    def submit_nelder_mead(self, progs: Programs, ds: Dataset, iterations: int = 100, mask=None) -> Ticket:
+        return self._submit(REQ_NELDER_MEAD, progs, ds, mask, iterations=iterations)
+
+    def submit_optimize(self, progs: Programs, ds: Dataset, iterations: int = 100, seed: int = 0,
+                        mask=None) -> Ticket:
+        """optimize_constants + calculate_mse (threshold 0.05): the ticket yields mse[n]."""
+        return self._submit(REQ_OPTIMIZE, progs, ds, mask, iterations=iterations, seed=seed)
 
     def poll(self, t: Ticket) -> bool:
         st = C.c_int()

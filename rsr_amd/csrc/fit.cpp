@@ -39,6 +39,31 @@ extern "C" int rsr_fit_cmaes(rsr_context* ctx, rsr_programs* p, const rsr_datase
     return run_job(ctx, *job);
 }
 
+// run_nelder_mead (src/engine/optimize/nelder_mead.rs:9-165) for every program with mask[i] != 0.
+extern "C" int rsr_fit_nelder_mead(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int max_iterations,
+                                   const uint8_t* mask, float* fitness_out, int32_t* iters_out) {
+    if (!ctx || !p || !ds || (p->n && !fitness_out)) return set_error(RSR_E_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    std::unique_ptr<Job> job;
+    int rc = make_nm_job(ctx, p, ds, max_iterations, mask, fitness_out, iters_out, &job);
+    if (rc) return rc;
+    return run_job(ctx, *job);
+}
+
+// Individual::optimize_constants (src/engine/search/individual.rs:65-75) + calculate_mse.
+extern "C" int rsr_optimize_constants(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int max_iterations,
+                                      uint64_t seed, float threshold, const uint8_t* mask, float* mse_out,
+                                      uint8_t* method_out) {
+    if (!ctx || !p || !ds || (p->n && !mse_out)) return set_error(RSR_E_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    std::unique_ptr<Job> job;
+    int rc = make_optimize_job(ctx, p, ds, max_iterations, seed, threshold, mask, mse_out, method_out, &job);
+    if (rc) return rc;
+    return run_job(ctx, *job);
+}
+
 extern "C" float rsr_cmaes_normal(uint64_t seed, uint32_t individual, uint32_t iteration, uint32_t candidate,
                                   uint32_t dim) {
     return cma_normal_host(seed, individual, iteration, candidate, dim);

@@ -346,6 +346,158 @@ class CmaesJob final : public Job {
     size_t device_bytes() const override { return jb.bytes(); }
 };
 
+// ---------------------------------------------------------------------------------
+// run_nelder_mead (src/engine/optimize/nelder_mead.rs:9-165), one state machine for the
+// batch: every tick evaluates the points each individual requested (its reflected,
+// expanded or contracted point, or its shrunk simplex) through 33 candidate slots per
+// individual; the step kernel consumes them and requests the next.
+// ---------------------------------------------------------------------------------
+class NmJob final : public Job {
+   public:
+    explicit NmJob(rsr_context* c) : ctx(c), jb(c) {}
+    rsr_context* ctx;
+    rsr_programs* p = nullptr;
+    const rsr_dataset* ds = nullptr;
+    Lowering* Lp = nullptr;
+    JobBuffers jb;
+    NmDev st;
+    std::vector<int> prog;
+    float *d_cmse = nullptr, *d_cparams = nullptr, *d_fit = nullptr;
+    int* d_iters = nullptr;
+    DevPrograms CP{};
+    float* fitness_out = nullptr;
+    int32_t* iters_out = nullptr;
+    bool started = false;
+    int working = 0;
+    uint64_t mean_nodes = 0, mean_dim = 0;
+
+    int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>&) override {
+        if (!started) {
+            HIP_TRY(nm_begin(st, (const float*)p->dparams.ptr, d_cparams, ctx->stream), "nm_begin");
+            node_evals += prog.size() * (mean_dim + 1) * mean_nodes * (uint64_t)ds->num_samples;
+        } else {
+            node_evals += (uint64_t)working * mean_nodes * (uint64_t)ds->num_samples;
+        }
+        EvalStep s;
+        s.P = CP;
+        s.D = dev_view(ds);
+        s.active = st.cand_active;
+        s.mse_out = d_cmse;
+        mse.push_back(s);
+        return RSR_OK;
+    }
+    int post() override {
+        HIP_TRY(nm_step(st, d_cparams, d_cmse, d_counters, ctx->stream), "nm_step");
+        return RSR_OK;
+    }
+    int advance(const int* c) override {
+        started = true;
+        working = c[0];
+        if (working == 0) done = true;
+        return RSR_OK;
+    }
+    int finish() override {
+        if (prog.empty()) return RSR_OK;
+        HIP_TRY(nm_finish(st, (float*)p->dparams.ptr, d_fit, d_iters, ctx->stream), "nm_finish");
+        std::vector<float> fit(prog.size());
+        std::vector<int> its(prog.size());
+        HIP_TRY(hipMemcpyAsync(fit.data(), d_fit, fit.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+                "fit");
+        HIP_TRY(hipMemcpyAsync(its.data(), d_iters, its.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+                "iters");
+        int rc = download_params(p);  // synchronises the stream
+        if (rc) return rc;
+        for (size_t k = 0; k < prog.size(); ++k) {
+            fitness_out[prog[k]] = fit[k];
+            if (iters_out) iters_out[prog[k]] = its[k];
+        }
+        return RSR_OK;
+    }
+    size_t device_bytes() const override { return jb.bytes(); }
+};
+
+// ---------------------------------------------------------------------------------
+// optimize_constants (src/engine/search/individual.rs:65-75 with
+// src/engine/optimize/mod.rs:10-59): per program, L-BFGS (15 iterations) when it has no
+// matrix inverse, else CMA-ES (max(iterations / 16, 2)) when it has a vector or matrix
+// constant, else Nelder-Mead (iterations); the three fits run as parts of one job (their
+// steps share ticks).  Then every constant component below the threshold in magnitude
+// is set to zero and the MSE of the resulting programs is evaluated (the caller's
+// calculate_mse after optimize_constants, island.rs:233-236).
+// ---------------------------------------------------------------------------------
+class OptimizeJob final : public Job {
+   public:
+    explicit OptimizeJob(rsr_context* c) : ctx(c) {}
+    rsr_context* ctx;
+    rsr_programs* p = nullptr;
+    const rsr_dataset* ds = nullptr;
+    std::vector<std::unique_ptr<Job>> fits;
+    std::unique_ptr<Job> eval;
+    std::vector<float> fit_tmp, mse_tmp;
+    std::vector<int32_t> iters_tmp;
+    std::vector<uint8_t> chosen;
+    float threshold = 0.05f;
+    float* mse_out = nullptr;
+    int first_rc = RSR_OK;
+
+    int plan(std::vector<EvalStep>&, std::vector<EvalStep>&) override { return RSR_OK; }
+    int advance(const int*) override { return RSR_OK; }
+    void leaves(std::vector<Job*>& out) override {
+        if (eval) {
+            eval->leaves(out);
+            return;
+        }
+        for (auto& f : fits) f->leaves(out);
+    }
+    int refresh() override {
+        if (eval) {
+            done = eval->done;
+            return RSR_OK;
+        }
+        for (auto& f : fits)
+            if (!f->done) return RSR_OK;
+        for (auto& f : fits) {
+            const int rc = f->finish();
+            if (rc && !first_rc) first_rc = rc;
+        }
+        // apply_threshold (scalar.rs:60-100) on every constant component of the fitted programs
+        bool changed = false;
+        for (int i = 0; i < p->n; ++i) {
+            if (!chosen[i]) continue;
+            for (uint32_t f = p->param_off[i]; f < p->param_off[i + 1]; ++f) {
+                if (__builtin_fabsf(p->params[f]) < threshold && p->params[f] != 0.0f) {
+                    p->params[f] = 0.0f;
+                    changed = true;
+                }
+            }
+            int64_t v = p->value_off[i];
+            uint32_t f = p->param_off[i];
+            for (int64_t c = p->const_off[i]; c < p->const_off[i + 1]; ++c) {
+                const int s = vt_size(p->ctypes[c]);
+                for (int k = 0; k < s; ++k) p->values[v + k] = p->params[f + k];
+                v += s ? s : 1;
+                f += s;
+            }
+        }
+        int rc = changed ? upload_params(p) : RSR_OK;
+        if (rc) return rc;
+        rc = make_eval_job(ctx, p, ds, chosen.data(), mse_tmp.data(), &eval);
+        if (rc) return rc;
+        return RSR_OK;
+    }
+    int finish() override {
+        const int rc = eval ? eval->finish() : RSR_OK;
+        for (int i = 0; i < p->n; ++i) mse_out[i] = chosen[i] ? mse_tmp[i] : NAN;
+        if (first_rc) return first_rc;
+        return rc == RSR_E_PROGRAM ? RSR_OK : rc;  // rejected programs are reported per program (NaN)
+    }
+    size_t device_bytes() const override {
+        size_t b = 0;
+        for (const auto& f : fits) b += f->device_bytes();
+        return b + (eval ? eval->device_bytes() : 0);
+    }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------
@@ -626,6 +778,182 @@ int make_cmaes_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
     return RSR_OK;
 }
 
+int make_nm_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, const uint8_t* mask,
+                float* fitness_out, int32_t* iters_out, std::unique_ptr<Job>* out) {
+    if (iterations < 0) return set_error(RSR_E_INVALID_ARG, "iterations must be >= 0");
+    auto j = std::make_unique<NmJob>(ctx);
+    j->p = p;
+    j->ds = ds;
+    j->fitness_out = fitness_out;
+    j->iters_out = iters_out;
+    int rc = get_lowering(p, ds->num_features, false, &j->Lp);
+    if (rc) return rc;
+    const Lowering* Lp = j->Lp;
+    std::vector<int> ndim;
+    std::vector<std::vector<int32_t>> offs;
+    std::vector<uint32_t> poff;
+    uint64_t nodes = 0, dims = 0;
+    for (int i = 0; i < p->n; ++i) {
+        fitness_out[i] = NAN;
+        if (iters_out) iters_out[i] = 0;
+        if (mask && !mask[i]) continue;
+        if (Lp->status[i] != PROG_OK) continue;
+        // the Float constants, in constant order, at most 32 (nelder_mead.rs:19-30)
+        std::vector<int32_t> o;
+        uint32_t f = 0;
+        for (int64_t c = p->const_off[i]; c < p->const_off[i + 1]; ++c) {
+            const int s = vt_size(p->ctypes[c]);
+            if (p->ctypes[c] == VtFloat && o.size() < 32) o.push_back((int32_t)f);
+            f += s;
+        }
+        if (o.empty()) continue;  // nelder_mead.rs:32-34
+        j->prog.push_back(i);
+        ndim.push_back((int)o.size());
+        offs.push_back(std::move(o));
+        poff.push_back(p->param_off[i]);
+        nodes += prog_nodes(p, i);
+        dims += ndim.back();
+    }
+    const int n = (int)j->prog.size();
+    if (n == 0) {
+        j->done = true;
+        *out = std::move(j);
+        return RSR_OK;
+    }
+    j->mean_nodes = nodes / n;
+    j->mean_dim = dims / n;
+    NmDev& st = j->st;
+    JobBuffers& sc = j->jb;
+    st.n = n;
+    st.max_iter = iterations;
+    std::vector<int32_t> opt((size_t)kMaxP * n, 0);
+    for (int k = 0; k < n; ++k)
+        for (size_t jj = 0; jj < offs[k].size(); ++jj) opt[jj * n + k] = offs[k][jj];
+    // candidate slots: full copies of each individual's parameter block
+    std::vector<int64_t> cand_off((size_t)n * kNmSlots);
+    std::vector<float> cand_params;
+    for (int k = 0; k < n; ++k) {
+        const int i = j->prog[k];
+        const float* pp = &p->params[p->param_off[i]];
+        const int pc = (int)(p->param_off[i + 1] - p->param_off[i]);
+        for (int c = 0; c < kNmSlots; ++c) {
+            cand_off[(size_t)k * kNmSlots + c] = (int64_t)cand_params.size();
+            cand_params.insert(cand_params.end(), pp, pp + pc);
+        }
+    }
+    cand_params.push_back(0.0f);
+    int *d_nd;
+    int32_t* d_opt;
+    uint32_t* d_poff;
+    int64_t* d_coff;
+    if ((rc = sc.put(ndim, &d_nd)) || (rc = sc.put(opt, &d_opt)) || (rc = sc.put(poff, &d_poff)) ||
+        (rc = sc.put(cand_off, &d_coff)) || (rc = sc.put(cand_params, &j->d_cparams)))
+        return rc;
+    st.ndim = d_nd;
+    st.opt_off = d_opt;
+    st.param_off = d_poff;
+    st.cand_off = d_coff;
+    if ((rc = sc.get((size_t)kNmSlots * n, &st.fit)) || (rc = sc.get((size_t)kNmSlots * n, &st.mse)) ||
+        (rc = sc.get((size_t)kNmSlots * kMaxP * n, &st.pt)) || (rc = sc.get((size_t)kMaxP * n, &st.cen)) ||
+        (rc = sc.get((size_t)kMaxP * n, &st.refl)) || (rc = sc.get((size_t)kMaxP * n, &st.trial)) ||
+        (rc = sc.get((size_t)n, &st.rf)) || (rc = sc.get((size_t)n, &st.rm)) || (rc = sc.get((size_t)n, &st.lim)) ||
+        (rc = sc.get((size_t)kNmSlots * n, &st.idx)) || (rc = sc.get((size_t)n, &st.phase)) ||
+        (rc = sc.get((size_t)n, &st.iter)) || (rc = sc.get((size_t)n * kNmSlots, &st.cand_active)))
+        return rc;
+    std::vector<ProgDesc> hdesc((size_t)n * kNmSlots);
+    for (int k = 0; k < n; ++k)
+        for (int c = 0; c < kNmSlots; ++c) {
+            ProgDesc d = Lp->desc[j->prog[k]];
+            d.param_off = (uint32_t)cand_off[(size_t)k * kNmSlots + c];
+            hdesc[(size_t)k * kNmSlots + c] = d;
+        }
+    std::vector<int> rank(p->n);
+    for (int r = 0; r < p->n; ++r) rank[Lp->order[r]] = r;
+    std::vector<int32_t> corder((size_t)n * kNmSlots);
+    std::iota(corder.begin(), corder.end(), 0);
+    const std::vector<int>& prog = j->prog;
+    std::stable_sort(corder.begin(), corder.end(),
+                     [&](int a, int b) { return rank[prog[a / kNmSlots]] < rank[prog[b / kNmSlots]]; });
+    ProgDesc* d_desc;
+    int32_t* d_order;
+    if ((rc = sc.put(hdesc, &d_desc)) || (rc = sc.put(corder, &d_order)) ||
+        (rc = sc.get((size_t)n * kNmSlots, &j->d_cmse)) || (rc = sc.get((size_t)n, &j->d_fit)) ||
+        (rc = sc.get((size_t)n, &j->d_iters)))
+        return rc;
+    j->CP.code = (const UInstr*)Lp->dcode.ptr;
+    j->CP.progs = d_desc;
+    j->CP.params = j->d_cparams;
+    j->CP.order = d_order;
+    j->CP.n_progs = n * kNmSlots;
+    j->CP.spill_slots = Lp->spill_slots;
+    *out = std::move(j);
+    return RSR_OK;
+}
+
+int make_optimize_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, uint64_t seed,
+                      float threshold, const uint8_t* mask, float* mse_out, uint8_t* method_out,
+                      std::unique_ptr<Job>* out) {
+    if (iterations < 0) return set_error(RSR_E_INVALID_ARG, "iterations must be >= 0");
+    if (!(threshold >= 0.0f)) return set_error(RSR_E_INVALID_ARG, "threshold must be >= 0");
+    auto j = std::make_unique<OptimizeJob>(ctx);
+    j->p = p;
+    j->ds = ds;
+    j->threshold = threshold;
+    j->mse_out = mse_out;
+    Lowering* Lp;
+    int rc = get_lowering(p, ds->num_features, false, &Lp);
+    if (rc) return rc;
+    std::vector<uint8_t> ml(p->n, 0), mc(p->n, 0), mn(p->n, 0);
+    j->chosen.assign(p->n, 0);
+    bool any_l = false, any_c = false, any_n = false;
+    for (int i = 0; i < p->n; ++i) {
+        if (method_out) method_out[i] = RSR_FIT_NONE;
+        if ((mask && !mask[i]) || Lp->status[i] != PROG_OK) continue;
+        bool vecmat = false;
+        for (int64_t c = p->const_off[i]; c < p->const_off[i + 1]; ++c)
+            if (p->ctypes[c] != VtFloat) vecmat = true;
+        int m;
+        if (Lp->differentiable[i]) {
+            ml[i] = 1;
+            any_l = true;
+            m = RSR_FIT_LBFGS;
+        } else if (vecmat) {
+            mc[i] = 1;
+            any_c = true;
+            m = RSR_FIT_CMAES;
+        } else {
+            mn[i] = 1;
+            any_n = true;
+            m = RSR_FIT_NELDER_MEAD;
+        }
+        j->chosen[i] = 1;
+        if (method_out) method_out[i] = (uint8_t)m;
+    }
+    j->fit_tmp.assign(p->n, NAN);
+    j->mse_tmp.assign(p->n, NAN);
+    j->iters_tmp.assign(p->n, 0);
+    std::unique_ptr<Job> f;
+    if (any_l) {
+        if ((rc = make_lbfgs_job(ctx, p, ds, 15, 6, RSR_LBFGS_REFERENCE, ml.data(), j->fit_tmp.data(), &f))) return rc;
+        j->fits.push_back(std::move(f));
+    }
+    if (any_c) {
+        const int ci = std::max(iterations / 16, 2);
+        if ((rc = make_cmaes_job(ctx, p, ds, ci, 16, 8, seed, nullptr, mc.data(), j->fit_tmp.data(),
+                                 j->iters_tmp.data(), &f)))
+            return rc;
+        j->fits.push_back(std::move(f));
+    }
+    if (any_n) {
+        if ((rc = make_nm_job(ctx, p, ds, iterations, mn.data(), j->fit_tmp.data(), j->iters_tmp.data(), &f)))
+            return rc;
+        j->fits.push_back(std::move(f));
+    }
+    if ((rc = j->refresh())) return rc;  // no fits at all: straight to the final evaluation
+    *out = std::move(j);
+    return RSR_OK;
+}
+
 // ---------------------------------------------------------------------------------
 // ticks
 // ---------------------------------------------------------------------------------
@@ -740,9 +1068,17 @@ int run_grad(rsr_context* ctx, std::vector<EvalStep>& steps) {
 
 }  // namespace
 
-int run_tick(rsr_context* ctx, const std::vector<Job*>& jobs) {
+int run_tick(rsr_context* ctx, const std::vector<Job*>& top) {
+    std::vector<Job*> jobs;
+    for (Job* j : top) j->leaves(jobs);
     const size_t nj = jobs.size();
-    if (nj == 0) return RSR_OK;
+    if (nj == 0) {
+        for (Job* j : top) {
+            const int rc = j->refresh();
+            if (rc) return rc;
+        }
+        return RSR_OK;
+    }
     bool counters = false;
     for (Job* j : jobs) counters = counters || j->needs_counters();
     int rc;
@@ -771,6 +1107,10 @@ int run_tick(rsr_context* ctx, const std::vector<Job*>& jobs) {
     for (size_t k = 0; k < nj; ++k) {
         jobs[k]->ticks++;
         if ((rc = jobs[k]->advance(hc ? hc + 2 * k : nullptr))) return rc;
+    }
+    for (Job* j : top) {
+        if (std::find(jobs.begin(), jobs.end(), j) == jobs.end()) j->ticks++;
+        if ((rc = j->refresh())) return rc;
     }
     return RSR_OK;
 }

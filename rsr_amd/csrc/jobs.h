@@ -61,6 +61,12 @@ class Job {
     virtual size_t device_bytes() const = 0;
     // Whether advance() reads the tick's counters (single-step jobs do not).
     virtual bool needs_counters() const { return true; }
+    // The jobs a tick advances on behalf of this one (a composite job lists its parts).
+    virtual void leaves(std::vector<Job*>& out) {
+        if (!done) out.push_back(this);
+    }
+    // Called after every tick (a composite job moves between its stages here).
+    virtual int refresh() { return RSR_OK; }
     bool done = false;
     int* d_counters = nullptr;  // two device ints owned by the tick driver
     uint64_t node_evals = 0;    // node-evals launched so far (statistics; fits estimate from active counts)
@@ -80,6 +86,16 @@ int make_lbfgs_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
 int make_cmaes_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, int lambda, int mu,
                    uint64_t seed, const float* normals, const uint8_t* mask, float* fitness_out, int32_t* iters_out,
                    std::unique_ptr<Job>* out);
+
+// run_nelder_mead over the masked programs with Float constants; fitness_out [n].
+int make_nm_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, const uint8_t* mask,
+                float* fitness_out, int32_t* iters_out, std::unique_ptr<Job>* out);
+// optimize_constants: per-program dispatch to L-BFGS / CMA-ES / Nelder-Mead, constants
+// below `threshold` in magnitude set to zero, then the MSE of the result; mse_out [n],
+// method_out [n] (RSR_FIT_*, optional).
+int make_optimize_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, uint64_t seed,
+                      float threshold, const uint8_t* mask, float* mse_out, uint8_t* method_out,
+                      std::unique_ptr<Job>* out);
 
 // Advance every job by one tick (caller holds ctx->mu).
 int run_tick(rsr_context* ctx, const std::vector<Job*>& jobs);

@@ -169,36 +169,45 @@ __global__ void k_finalize_mse(const SegTable T) {
 // ---------------------------------------------------------------------------------
 // MSE, reference-order summation from per-point terms
 // ---------------------------------------------------------------------------------
-// Eight threads per (segment, scheduling position): thread l is the reference's lane
-// accumulator l and adds the terms of points l, l+8, l+16, ... of the launch's range in
-// order (evaluator.rs:159-170), starting from the carried accumulator of earlier ranges.
-// The last range reduces the eight accumulators in f32x8::reduce_add order.
-__global__ void k_fold_terms(const SegTable T) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int si = find_seg_thread(T, t);
+// One wave per (segment, scheduling position).  Lanes 0..7 are the reference's lane
+// accumulators: accumulator l adds the terms of points l, l+8, l+16, ... of the launch's
+// range in order (evaluator.rs:159-170), starting from the state carried from earlier
+// ranges.  The wave reads 64 consecutive terms per step (coalesced) and folds its eight
+// 8-point batches one after the other; the last range reduces the eight accumulators
+// in f32x8::reduce_add order.  (first_thread / find_seg_thread count fold waves here.)
+__global__ __launch_bounds__(256) void k_fold_terms(const SegTable T) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+    const int si = find_seg_thread(T, wave);
     const SegDesc& S = T.s[si];
-    const int64_t idx = t - S.first_thread;
-    const int64_t sp = idx >> 3;
-    const int l = (int)(idx & 7);
+    const int64_t sp = wave - S.first_thread;
     if (sp >= S.n_progs) return;
     if (S.live != nullptr && *S.live == 0) return;
-    const int prog = S.order[sp];
+    const int prog = (int)uni((uint32_t)S.order[sp]);
     if (S.active != nullptr && !S.active[prog]) return;
     const int64_t p0 = S.tile0 * S.tile_pts;
     const int64_t p1 = min(p0 + S.n_tiles * S.tile_pts, S.count8);
     const float* row = S.partials + sp * S.n_tiles * S.tile_pts - p0;
-    float acc = S.first ? 0.0f : S.kstate[sp * 8 + l];
-    for (int64_t q = p0 + l; q < p1; q += 8) acc = acc + row[q];
+    float acc = (!S.first && lane < 8) ? S.kstate[sp * 8 + lane] : 0.0f;
+    int64_t q = p0;
+    for (; q + 256 <= p1; q += 256) {  // four 64-point steps with their loads issued together
+        const float v0 = row[q + lane], v1 = row[q + 64 + lane], v2 = row[q + 128 + lane], v3 = row[q + 192 + lane];
+        acc = fold_batches(acc, v0, lane);
+        acc = fold_batches(acc, v1, lane);
+        acc = fold_batches(acc, v2, lane);
+        acc = fold_batches(acc, v3, lane);
+    }
+    for (; q < p1; q += 64) {
+        const int nb = (int)min<int64_t>(8, (p1 - q) >> 3);  // whole 8-point batches left
+        const float v = q + lane < p1 ? row[q + lane] : 0.0f;
+        for (int k = 0; k < nb; ++k) acc = acc + __shfl(v, (lane & 7) + 8 * k, 64);
+    }
     if (!S.last) {
-        S.kstate[sp * 8 + l] = acc;
+        if (lane < 8) S.kstate[sp * 8 + lane] = acc;
         return;
     }
-    // the eight threads of a program are consecutive and aligned to 8 (width-8 shuffles)
-    const float a0 = __shfl(acc, 0, 8), a1 = __shfl(acc, 1, 8), a2 = __shfl(acc, 2, 8), a3 = __shfl(acc, 3, 8);
-    const float a4 = __shfl(acc, 4, 8), a5 = __shfl(acc, 5, 8), a6 = __shfl(acc, 6, 8), a7 = __shfl(acc, 7, 8);
-    const float s0 = a0 + a4, s1 = a1 + a5, s2 = a2 + a6, s3 = a3 + a7;
-    const float mse = ((s0 + s2) + (s1 + s3)) / S.n_samples;
-    if (l == 0) S.mse_out[prog] = __builtin_isfinite(mse) ? mse : FLT_MAX;
+    const float mse = reduce_lanes8(acc) / S.n_samples;
+    if (lane == 0) S.mse_out[prog] = __builtin_isfinite(mse) ? mse : FLT_MAX;
 }
 
 // ---------------------------------------------------------------------------------
@@ -435,7 +444,7 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
             d.first_wave = waves;
             d.first_thread = threads;
             waves += d.n_tiles * d.n_groups;
-            threads += (int64_t)d.n_progs * (terms ? 8 : 1);
+            threads += d.n_progs;  // finalize threads, or fold waves
             spill = max(spill, st.P.spill_slots);
             T.s[T.n++] = d;
         }
@@ -454,7 +463,8 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
         if (timer && base + cnt >= n) (void)hipEventRecord(timer->end, s);
         const int tb = 256;
         if (terms)
-            hipLaunchKernelGGL(k_fold_terms, dim3((unsigned)((threads + tb - 1) / tb)), dim3(tb), 0, s, T);
+            hipLaunchKernelGGL(k_fold_terms, dim3((unsigned)((threads + kWaves - 1) / kWaves)), dim3(64 * kWaves), 0,
+                               s, T);
         else
             hipLaunchKernelGGL(k_finalize_mse, dim3((unsigned)((threads + tb - 1) / tb)), dim3(tb), 0, s, T);
         if ((e = hipGetLastError()) != hipSuccess) return e;

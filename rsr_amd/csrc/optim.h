@@ -55,4 +55,26 @@ hipError_t cmaes_update(const CmaDev& st, const float* cand_mse, int* counter, h
 hipError_t cmaes_finish(const CmaDev& st, float* params, float* fitness, int* iters_out, hipStream_t s);
 float cma_normal_host(uint64_t seed, uint32_t ind, uint32_t it, uint32_t cand, uint32_t j);
 
+// Nelder-Mead over the Float constants of n individuals (nelder_mead.rs:9-165).
+// Simplex records r = 0..ndim hold (fit, mse, point); idx is the records' sorted order.
+// Every individual evaluates its requested points through candidate slots
+// (slot s of individual i = candidate i * kNmSlots + s, with its own parameter block).
+constexpr int kNmSlots = 33;
+enum : int { NM_INIT = 0, NM_REFLECT, NM_EXPAND, NM_CONTRACT, NM_SHRINK, NM_DONE };
+struct NmDev {
+    int n = 0, max_iter = 0;
+    const int* ndim;            // Float constants fitted (1..32)
+    const int32_t* opt_off;     // [32][n] offset of Float constant j in the parameter block
+    const uint32_t* param_off;  // the individual's block in the batch parameters
+    const int64_t* cand_off;    // [n][33] candidate parameter blocks
+    float *fit, *mse, *pt;      // [33][n], [33][n], [33][32][n]
+    float *cen, *refl, *trial;  // [32][n]
+    float *rf, *rm, *lim;       // [n]
+    int *idx, *phase, *iter;    // [33][n], [n], [n]
+    uint8_t* cand_active;       // [n][33]
+};
+hipError_t nm_begin(const NmDev& st, const float* params, float* cand_params, hipStream_t s);
+hipError_t nm_step(const NmDev& st, float* cand_params, const float* cand_mse, int* counter, hipStream_t s);
+hipError_t nm_finish(const NmDev& st, float* params, float* fitness, int* iters_out, hipStream_t s);
+
 }  // namespace rsr

@@ -422,4 +422,241 @@ float cma_normal_host(uint64_t seed, uint32_t ind, uint32_t it, uint32_t cand, u
     return cma_normal(seed, ind, it, cand, j);
 }
 
+// ------------------------------------------------------------------------------------
+// Nelder-Mead (nelder_mead.rs:9-165), one thread per individual.  Fitness of a point =
+// mse + 0.01 * sum|v| (the L1 summed in dimension order); alpha 1, gamma 2, rho 0.5,
+// sigma 0.5; stop when |worst - best| < 1e-7 or best mse < 1e-8.  The simplex is sorted
+// by a stable insertion sort of record indices (the reference's sort_unstable_by leaves
+// the order of equal fitnesses unspecified).
+// ------------------------------------------------------------------------------------
+namespace {
+
+struct NmView {
+    const NmDev& st;
+    int i;
+    __device__ float& fit(int r) const { return st.fit[(int64_t)r * st.n + i]; }
+    __device__ float& mse(int r) const { return st.mse[(int64_t)r * st.n + i]; }
+    __device__ float& pt(int r, int j) const { return st.pt[((int64_t)r * kMaxP + j) * st.n + i]; }
+    __device__ float& cen(int j) const { return st.cen[(int64_t)j * st.n + i]; }
+    __device__ float& refl(int j) const { return st.refl[(int64_t)j * st.n + i]; }
+    __device__ float& trial(int j) const { return st.trial[(int64_t)j * st.n + i]; }
+    __device__ int& idx(int s) const { return st.idx[(int64_t)s * st.n + i]; }
+    __device__ int off(int j) const { return st.opt_off[(int64_t)j * st.n + i]; }
+};
+
+__device__ float nm_fitness(float m, const float* v, int64_t stride, int nd) {
+    float l1 = 0.0f;
+    for (int j = 0; j < nd; ++j) l1 += __builtin_fabsf(v[(int64_t)j * stride]);
+    return m + 0.01f * l1;
+}
+
+__device__ void nm_sort(const NmView& V, int nd) {
+    for (int a = 1; a <= nd; ++a) {
+        const int k = V.idx(a);
+        const float kf = V.fit(k);
+        int b = a - 1;
+        while (b >= 0 && V.fit(V.idx(b)) > kf) {
+            V.idx(b + 1) = V.idx(b);
+            --b;
+        }
+        V.idx(b + 1) = k;
+    }
+}
+
+}  // namespace
+
+__global__ void k_nm_begin(NmDev st, const float* __restrict__ params, float* __restrict__ cand_params) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    const NmView V{st, i};
+    const int nd = st.ndim[i];
+    const float* pp = params + st.param_off[i];
+    for (int r = 0; r < kNmSlots; ++r) {
+        const int64_t c = (int64_t)i * kNmSlots + r;
+        st.cand_active[c] = r <= nd ? 1 : 0;
+        if (r > nd) continue;
+        float* cp = cand_params + st.cand_off[c];
+        for (int j = 0; j < nd; ++j) {
+            float v = pp[V.off(j)];
+            if (r == j + 1) v += __builtin_fabsf(v) < 1e-4f ? 0.01f : v * 0.10f;
+            V.pt(r, j) = v;
+            cp[V.off(j)] = v;
+        }
+        V.idx(r) = r;
+    }
+    st.phase[i] = NM_INIT;
+    st.iter[i] = 0;
+}
+
+__global__ void k_nm_step(NmDev st, float* __restrict__ cand_params, const float* __restrict__ cand_mse,
+                          int* __restrict__ n_working) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    const NmView V{st, i};
+    const int nd = st.ndim[i];
+    const int64_t c0 = (int64_t)i * kNmSlots;
+    int ph = st.phase[i];
+    for (int r = 0; r < kNmSlots; ++r) st.cand_active[c0 + r] = 0;
+    if (ph == NM_DONE) return;
+    const int64_t S = st.n;  // field stride
+    bool iterate = false;
+    switch (ph) {
+        case NM_INIT:
+            for (int r = 0; r <= nd; ++r) {
+                const float m = cand_mse[c0 + r];
+                V.mse(r) = m;
+                V.fit(r) = nm_fitness(m, &V.pt(r, 0), S, nd);
+            }
+            iterate = true;
+            break;
+        case NM_REFLECT: {
+            const float m = cand_mse[c0];
+            const float rfv = nm_fitness(m, &V.refl(0), S, nd);
+            const float bf = V.fit(V.idx(0)), swf = V.fit(V.idx(nd - 1)), wf = V.fit(V.idx(nd));
+            const int w = V.idx(nd);
+            if (rfv >= bf && rfv < swf) {
+                V.fit(w) = rfv;
+                V.mse(w) = m;
+                for (int j = 0; j < nd; ++j) V.pt(w, j) = V.refl(j);
+                st.iter[i] += 1;
+                iterate = true;
+                break;
+            }
+            st.rf[i] = rfv;
+            st.rm[i] = m;
+            float* cp = cand_params + st.cand_off[c0];
+            if (rfv < bf) {
+                for (int j = 0; j < nd; ++j) {
+                    const float e = V.cen(j) + 2.0f * (V.refl(j) - V.cen(j));
+                    V.trial(j) = e;
+                    cp[V.off(j)] = e;
+                }
+                st.phase[i] = NM_EXPAND;
+            } else {
+                const bool inside = rfv < wf;
+                for (int j = 0; j < nd; ++j) {
+                    const float src = inside ? V.refl(j) : V.pt(w, j);
+                    const float cv = V.cen(j) + 0.5f * (src - V.cen(j));
+                    V.trial(j) = cv;
+                    cp[V.off(j)] = cv;
+                }
+                st.lim[i] = inside ? rfv : wf;
+                st.phase[i] = NM_CONTRACT;
+            }
+            st.cand_active[c0] = 1;
+            atomicAdd(n_working, 1);
+            return;
+        }
+        case NM_EXPAND: {
+            const float m = cand_mse[c0];
+            const float ef = nm_fitness(m, &V.trial(0), S, nd);
+            const int w = V.idx(nd);
+            const bool take = ef < st.rf[i];
+            V.fit(w) = take ? ef : st.rf[i];
+            V.mse(w) = take ? m : st.rm[i];
+            for (int j = 0; j < nd; ++j) V.pt(w, j) = take ? V.trial(j) : V.refl(j);
+            st.iter[i] += 1;
+            iterate = true;
+            break;
+        }
+        case NM_CONTRACT: {
+            const float m = cand_mse[c0];
+            const float cf = nm_fitness(m, &V.trial(0), S, nd);
+            if (cf < st.lim[i]) {
+                const int w = V.idx(nd);
+                V.fit(w) = cf;
+                V.mse(w) = m;
+                for (int j = 0; j < nd; ++j) V.pt(w, j) = V.trial(j);
+                st.iter[i] += 1;
+                iterate = true;
+                break;
+            }
+            // shrink every record towards the best one
+            const int b = V.idx(0);
+            for (int s = 1; s <= nd; ++s) {
+                const int r = V.idx(s);
+                float* cp = cand_params + st.cand_off[c0 + s];
+                for (int j = 0; j < nd; ++j) {
+                    const float bp = V.pt(b, j);
+                    const float v = bp + 0.5f * (V.pt(r, j) - bp);
+                    V.pt(r, j) = v;
+                    cp[V.off(j)] = v;
+                }
+                st.cand_active[c0 + s] = 1;
+            }
+            st.phase[i] = NM_SHRINK;
+            atomicAdd(n_working, 1);
+            return;
+        }
+        case NM_SHRINK:
+            for (int s = 1; s <= nd; ++s) {
+                const int r = V.idx(s);
+                const float m = cand_mse[c0 + s];
+                V.mse(r) = m;
+                V.fit(r) = nm_fitness(m, &V.pt(r, 0), S, nd);
+            }
+            st.iter[i] += 1;
+            iterate = true;
+            break;
+        default: break;
+    }
+    if (!iterate) return;
+    // the top of the reference's iteration loop
+    if (st.iter[i] >= st.max_iter) {
+        st.phase[i] = NM_DONE;
+        return;
+    }
+    nm_sort(V, nd);
+    const float bf = V.fit(V.idx(0)), bm = V.mse(V.idx(0)), wf = V.fit(V.idx(nd));
+    if (__builtin_fabsf(wf - bf) < 1e-7f || bm < 1e-8f) {
+        st.phase[i] = NM_DONE;
+        return;
+    }
+    for (int j = 0; j < nd; ++j) V.cen(j) = 0.0f;
+    for (int s = 0; s < nd; ++s) {
+        const int r = V.idx(s);
+        for (int j = 0; j < nd; ++j) V.cen(j) += V.pt(r, j);
+    }
+    for (int j = 0; j < nd; ++j) V.cen(j) /= (float)nd;
+    const int w = V.idx(nd);
+    float* cp = cand_params + st.cand_off[c0];
+    for (int j = 0; j < nd; ++j) {
+        const float v = V.cen(j) + 1.0f * (V.cen(j) - V.pt(w, j));
+        V.refl(j) = v;
+        cp[V.off(j)] = v;
+    }
+    st.cand_active[c0] = 1;
+    st.phase[i] = NM_REFLECT;
+    atomicAdd(n_working, 1);
+}
+
+__global__ void k_nm_finish(NmDev st, float* __restrict__ params, float* __restrict__ fitness,
+                            int* __restrict__ iters_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    const NmView V{st, i};
+    const int nd = st.ndim[i];
+    nm_sort(V, nd);
+    const int b = V.idx(0);
+    float* pp = params + st.param_off[i];
+    for (int j = 0; j < nd; ++j) pp[V.off(j)] = V.pt(b, j);
+    fitness[i] = V.mse(b);
+    if (iters_out) iters_out[i] = st.iter[i];
+}
+
+hipError_t nm_begin(const NmDev& st, const float* params, float* cand_params, hipStream_t s) {
+    hipLaunchKernelGGL(k_nm_begin, RSR_BLK(st.n), 0, s, st, params, cand_params);
+    return hipGetLastError();
+}
+
+hipError_t nm_step(const NmDev& st, float* cand_params, const float* cand_mse, int* counter, hipStream_t s) {
+    hipLaunchKernelGGL(k_nm_step, RSR_BLK(st.n), 0, s, st, cand_params, cand_mse, counter);
+    return hipGetLastError();
+}
+
+hipError_t nm_finish(const NmDev& st, float* params, float* fitness, int* iters_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_nm_finish, RSR_BLK(st.n), 0, s, st, params, fitness, iters_out);
+    return hipGetLastError();
+}
+
 }  // namespace rsr

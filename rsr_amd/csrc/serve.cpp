@@ -69,6 +69,11 @@ int make_job(rsr_context* ctx, Pending& r, std::unique_ptr<Job>* out) {
         case RSR_REQ_CMAES:
             return make_cmaes_job(ctx, q.programs, q.dataset, q.iterations, q.lambda, q.mu, q.seed, nullptr, mask,
                                   q.out, q.iters_out, out);
+        case RSR_REQ_NELDER_MEAD:
+            return make_nm_job(ctx, q.programs, q.dataset, q.iterations, mask, q.out, q.iters_out, out);
+        case RSR_REQ_OPTIMIZE:
+            return make_optimize_job(ctx, q.programs, q.dataset, q.iterations, q.seed, 0.05f, mask, q.out, nullptr,
+                                     out);
         default: return set_error(RSR_E_INVALID_ARG, "unknown request kind %d", q.kind);
     }
 }
@@ -262,7 +267,7 @@ extern "C" int rsr_server_create(rsr_context* ctx, int max_inflight, uint64_t bu
 extern "C" int rsr_server_submit(rsr_server* s, const rsr_request* req, uint64_t* ticket) {
     if (!s || !req || !ticket) return set_error(RSR_E_INVALID_ARG, "null argument");
     if (!req->programs || !req->dataset) return set_error(RSR_E_INVALID_ARG, "request without programs or dataset");
-    if (req->kind < RSR_REQ_EVAL || req->kind > RSR_REQ_CMAES)
+    if (req->kind < RSR_REQ_EVAL || req->kind > RSR_REQ_OPTIMIZE)
         return set_error(RSR_E_INVALID_ARG, "unknown request kind %d", req->kind);
     const int n = req->programs->n;
     if (n > 0 && !req->out) return set_error(RSR_E_INVALID_ARG, "request without output buffer");

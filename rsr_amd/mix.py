@@ -66,6 +66,10 @@ def make_jobs(ctx, pools, n_jobs=64, rate=4.0, seed=4000, pop_range=(1000, 16384
         pool = pools[k]
         idx = rng.integers(0, len(pool), int(pops[j]))
         progs = [pool[i] for i in idx]
+        P = api.Programs(ctx, progs)
+        ok = P.status(ds) == 0  # an island never holds programs the evaluator rejects
+        P.close()
+        progs = [p for p, good in zip(progs, ok) if good]
         jobs.append(MixJob(j, k, float(arrivals[j]), int(npts[j]), progs, ds))
     return jobs
 

@@ -12,7 +12,7 @@ import pytest
 import oracle as O
 from rsr_amd import api
 from rsr_amd import workloads as W
-from rsr_amd.isa import Program, Scalar
+from rsr_amd.isa import Op, Program, Scalar, VT
 
 pytestmark = pytest.mark.gpu
 f32 = np.float32
@@ -122,3 +122,79 @@ def test_cmaes_normal_stream_matches_box_muller():
     z = np.array([api.cmaes_normal(123, 5, it, c, j) for it in range(3) for c in range(8) for j in range(16)])
     assert abs(z.mean()) < 0.15 and 0.85 < z.std() < 1.15
     assert api.cmaes_normal(123, 5, 0, 0, 0) == api.cmaes_normal(123, 5, 0, 0, 0)
+
+
+@pytest.mark.parametrize("gen,data,nf", [(W.gen_c0, W.data_c0, 1), (W.gen_c3, W.data_c3, 8)])
+def test_nelder_mead_reference_order_bit_exact(gpu_ctx, gen, data, nf):
+    X, y, t = data(403, seed=81)
+    progs = gen(48, seed=82)
+    gpu_ctx.set_summation(True)
+    try:
+        ds = api.Dataset.new(gpu_ctx, X, y, t)
+        P = api.Programs(gpu_ctx, progs)
+        st = P.status(ds)
+        fit, its = api.fit_nelder_mead(gpu_ctx, P, ds, iterations=60, mask=(st == 0).astype(np.uint8))
+        vals = P.get_constants()
+    finally:
+        gpu_ctx.set_summation(False)
+    ods = O.ODataset.new(X, y, nf)
+    off = 0
+    for i, p in enumerate(progs):
+        nv = sum(len(c.vals) for c in p.constants)
+        if st[i] == 0:
+            rf, rc = O.run_nelder_mead(p, ods, 60)
+            if rf is None:
+                assert np.isnan(fit[i]), i
+            else:
+                assert fit[i] == rf, (i, fit[i], rf)
+                assert np.array_equal(vals[off:off + nv].view(np.uint32), rc[:nv].view(np.uint32)), i
+        off += nv
+
+
+def _mat2_const_program():
+    """tr(A^-1 (B + C)) + c with a Mat2 constant C and a Float constant c."""
+    return Program([(Op.LoadVarM2, 0), (Op.InverseM2, 0), (Op.LoadVarM2, 4), (Op.LoadConstM2, 0), (Op.AddM2, 0),
+                    (Op.MulM2, 0), (Op.TraceM2, 0), (Op.LoadConstF, 1), (Op.AddF, 0)],
+                   [Scalar(VT.Mat2, (0.3, -0.2, 0.1, 0.04)), Scalar(VT.Float, (0.7,))])
+
+
+def test_optimize_constants_dispatch_snap_and_mse(gpu_ctx):
+    X, y, t = W.data_c3(600, seed=83)
+    progs = W.gen_c3(40, seed=84) + [_mat2_const_program(), _mat2_const_program()]
+    gpu_ctx.set_summation(True)
+    try:
+        ds = api.Dataset.new(gpu_ctx, X, y, t)
+        P = api.Programs(gpu_ctx, progs)
+        st = P.status(ds)
+        ok = (st == 0).astype(np.uint8)
+        before = P.get_constants().copy()
+        mse, meth = api.optimize_constants(gpu_ctx, P, ds, iterations=40, seed=5, mask=ok)
+        after = P.get_constants()
+        again = api.compute_mse(gpu_ctx, P, ds, allow_rejected=True)
+    finally:
+        gpu_ctx.set_summation(False)
+    ods = O.ODataset.new(X, y, 8)
+    off = 0
+    for i, p in enumerate(progs):
+        nv = sum(len(c.vals) for c in p.constants)
+        if not ok[i]:
+            assert meth[i] == api.FIT_NONE and np.isnan(mse[i])
+            off += nv
+            continue
+        has_inv = any(op == Op.InverseM2 for op, _ in p.code)
+        vecmat = any(c.vt != VT.Float for c in p.constants)
+        want = api.FIT_LBFGS if not has_inv else (api.FIT_CMAES if vecmat else api.FIT_NELDER_MEAD)
+        assert meth[i] == want, i
+        v = after[off:off + nv]
+        assert not np.any((np.abs(v) < 0.05) & (v != 0)), i  # apply_threshold
+        assert mse[i] == again[i], i  # calculate_mse of the snapped program
+        # deterministic fits follow the oracle exactly, then snap
+        if want in (api.FIT_LBFGS, api.FIT_NELDER_MEAD):
+            q = _with_consts(p, before[off:off + nv])
+            if want == api.FIT_LBFGS:
+                _, rc = O.run_lbfgs(q, ods, iters=15, m=6)
+            else:
+                _, rc = O.run_nelder_mead(q, ods, 40)
+            rc = np.where(np.abs(rc[:nv]) < 0.05, f32(0), rc[:nv]).astype(f32)
+            assert np.array_equal(v.view(np.uint32), rc.view(np.uint32)) or np.array_equal(v, rc), i
+        off += nv
