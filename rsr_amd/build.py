@@ -17,6 +17,9 @@ COMMON = [
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     # never emit packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # keep wave-uniform control flow (the interpreter's dispatch on the scalar opcode) as
+    # plain scalar branches instead of structurized exec-mask regions
+    "-mllvm", "-structurizecfg-skip-uniform-regions=true",
     "-Wno-unused-command-line-argument",
 ]
 

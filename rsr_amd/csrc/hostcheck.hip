@@ -17,11 +17,11 @@ using namespace rsr;
 static void run_host(const std::vector<UInstr>& code, const float* params, float* R, float* G, int k,
                      float* spill /* [slot][2] */) {
     for (const UInstr& u : code) {
-        const uint32_t op = u.w0 & 255u, d = (u.w0 >> 8) & 31u, a = (u.w0 >> 16) & 31u, b = (u.w0 >> 24) & 31u;
-        const uint32_t c = u.w1 & 31u, e = (u.w1 >> 8) & 31u;
+        const uint32_t op = u.w0 & 255u, x = u.w0 >> 8, d = u.w1, a = u.w2, b = u.w3;
+        const uint32_t c = ux_c(x), e = ux_e(x);
         float v = 0.0f, g = 0.0f;
         switch (op) {
-            case U_IMM: v = params[u.w1]; g = ((int)u.w1 == k) ? 1.0f : 0.0f; break;
+            case U_IMM: v = params[x]; g = ((int)x == k) ? 1.0f : 0.0f; break;
             case U_ADD: v = R[a] + R[b]; g = G[a] + G[b]; break;
             case U_SUB: v = R[a] - R[b]; g = G[a] - G[b]; break;
             case U_MUL: v = R[a] * R[b]; g = (R[a] * G[b]) + (G[a] * R[b]); break;
@@ -42,12 +42,12 @@ static void run_host(const std::vector<UInstr>& code, const float* params, float
             case U_ADD3: v = (R[a] + R[b]) + R[c]; g = (G[a] + G[b]) + G[c]; break;
             case U_RCPSAFE: v = 1.0f / ((fabsf(R[a]) < 1e-9f) ? 1.0f : R[a]); break;
             case U_INVSEL: {
-                const float x = (u.w1 & (1u << 16)) ? -R[a] : R[a];
-                v = (fabsf(R[c]) < 1e-9f) ? ((u.w1 & (2u << 16)) ? 1.0f : 0.0f) : x * R[b];
+                const float xa = (ux_flags(x) & kInvNeg) ? -R[a] : R[a];
+                v = (fabsf(R[c]) < 1e-9f) ? ((ux_flags(x) & kInvOne) ? 1.0f : 0.0f) : xa * R[b];
                 break;
             }
-            case U_SPILL: v = R[a]; g = G[a]; spill[2 * u.w1] = v; spill[2 * u.w1 + 1] = g; break;
-            case U_FILL: v = spill[2 * u.w1]; g = spill[2 * u.w1 + 1]; break;
+            case U_SPILL: v = R[a]; g = G[a]; spill[2 * x] = v; spill[2 * x + 1] = g; break;
+            case U_FILL: v = spill[2 * x]; g = spill[2 * x + 1]; break;
             default: break;
         }
         R[d] = v;

@@ -55,72 +55,114 @@ __device__ __forceinline__ void sfor(Fn&& fn) {
     }(std::make_integer_sequence<int, N>{});
 }
 
+// The micro-op at code[i], in SGPRs (one 16-byte scalar load).
+struct Uop {
+    uint32_t w0, d, a, b;
+};
+__device__ __forceinline__ Uop load_uop(const UInstr* __restrict__ code, uint32_t i) {
+    Uop u;
+    u.w0 = uni(code[i].w0);
+    u.d = uni(code[i].w1);
+    u.a = uni(code[i].w2);
+    u.b = uni(code[i].w3);
+    return u;
+}
+
 // Primal interpreter over PPL register files (one per point of the lane).
 // `lds` is this wave's spill area: slot s, point p at lds[(s * PPL + p) * 64 + lane].
+// Operands a and b of every micro-op are read for all points before the dispatch (one
+// indexed-read group per operand), the switch only computes, and the result is written
+// back for all points after it, so each operand costs one index-mode region however
+// many points a lane holds.
 template <int PPL>
 __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restrict__ code, uint32_t len,
                                             const float* __restrict__ params, float* __restrict__ lds, int lane) {
-    uint32_t nw0 = len ? uni(code[0].w0) : 0u, nw1 = len ? uni(code[0].w1) : 0u;
+    Uop nx = load_uop(code, 0);  // the code array ends with one padding entry
     for (uint32_t pc = 0; pc < len; ++pc) {
-        const uint32_t w0 = nw0, w1 = nw1;
-        // prefetch the next micro-op (the code array carries one padding entry)
-        nw0 = uni(code[pc + 1].w0);
-        nw1 = uni(code[pc + 1].w1);
-        const uint32_t op = w0 & 255u;
-        const uint32_t d = (w0 >> 8) & 31u, a = (w0 >> 16) & 31u, b = (w0 >> 24) & 31u;
-        const uint32_t c = w1 & 31u, e = (w1 >> 8) & 31u;
-        float res[PPL];
+        const Uop u = nx;
+        nx = load_uop(code, pc + 1);  // prefetch
+        const uint32_t op = u.w0 & 255u, x = u.w0 >> 8;
+        const uint32_t d = u.d, a = u.a, b = u.b;
+        __builtin_assume(d < 32u);
+        __builtin_assume(a < 32u);
+        __builtin_assume(b < 32u);
+        float xa[PPL], xb[PPL], res[PPL];
+        sfor<PPL>([&](auto P) { xa[P()] = fget<P(), PPL>(R)[a]; });
+        sfor<PPL>([&](auto P) { xb[P()] = fget<P(), PPL>(R)[b]; });
         switch (op) {
             case U_IMM: {
-                const float v = params[w1];
+                const float v = params[x];
                 sfor<PPL>([&](auto P) { res[P()] = v; });
                 break;
             }
-#define RSR_CASE(OPC, EXPR)                               \
-    case OPC: {                                           \
-        sfor<PPL>([&](auto P) {                           \
-            const v32f r = fget<P(), PPL>(R);             \
-            res[P()] = (EXPR);                            \
-        });                                               \
-        break;                                            \
-    }
-            RSR_CASE(U_ADD, r[a] + r[b])
-            RSR_CASE(U_SUB, r[a] - r[b])
-            RSR_CASE(U_MUL, r[a] * r[b])
-            RSR_CASE(U_DIV, r[a] / r[b])
-            RSR_CASE(U_SQR, r[a] * r[a])
-            RSR_CASE(U_SQRT, __builtin_sqrtf(r[a]))
-            RSR_CASE(U_SIN, dm_sin(r[a]))
-            RSR_CASE(U_COS, dm_cos(r[a]))
-            RSR_CASE(U_EXP, dm_exp(r[a]))
-            RSR_CASE(U_LN, dm_ln(r[a]))
-            RSR_CASE(U_NEG, -r[a])
-            RSR_CASE(U_MULADD, (r[a] * r[b]) + (r[c] * r[e]))
-            RSR_CASE(U_MULSUB, (r[a] * r[b]) - (r[c] * r[e]))
-            RSR_CASE(U_ADDMUL, r[a] + (r[b] * r[c]))
-            RSR_CASE(U_SQRTACC, __builtin_sqrtf(r[a] + (r[b] * r[b])))
-            RSR_CASE(U_ADD3, (r[a] + r[b]) + r[c])
-            RSR_CASE(U_RCPSAFE, 1.0f / ((__builtin_fabsf(r[a]) < 1e-9f) ? 1.0f : r[a]))
-            case U_INVSEL: {
-                const float k = (w1 & (2u << 16)) ? 1.0f : 0.0f;
-                const bool ng = (w1 & (1u << 16)) != 0u;
+#define RSR_CASE(OPC, EXPR)                                   \
+    case OPC:                                                 \
+        sfor<PPL>([&](auto P) {                               \
+            const float A = xa[P()], B = xb[P()];             \
+            (void)B;                                          \
+            res[P()] = (EXPR);                                \
+        });                                                   \
+        break;
+            RSR_CASE(U_ADD, A + B)
+            RSR_CASE(U_SUB, A - B)
+            RSR_CASE(U_MUL, A * B)
+            RSR_CASE(U_DIV, A / B)
+            RSR_CASE(U_SQR, A * A)
+            RSR_CASE(U_SQRT, __builtin_sqrtf(A))
+            RSR_CASE(U_SIN, dm_sin(A))
+            RSR_CASE(U_COS, dm_cos(A))
+            RSR_CASE(U_EXP, dm_exp(A))
+            RSR_CASE(U_LN, dm_ln(A))
+            RSR_CASE(U_NEG, -A)
+            RSR_CASE(U_SQRTACC, __builtin_sqrtf(A + (B * B)))
+            RSR_CASE(U_RCPSAFE, 1.0f / ((__builtin_fabsf(A) < 1e-9f) ? 1.0f : A))
+#undef RSR_CASE
+            case U_MULADD:
+            case U_MULSUB: {
+                const uint32_t c = x & 31u, e = (x >> 5) & 31u;
+                const bool sub = op == U_MULSUB;
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
-                    const float x = ng ? -r[a] : r[a];
-                    res[P()] = (__builtin_fabsf(r[c]) < 1e-9f) ? k : x * r[b];
+                    const float p1 = xa[P()] * xb[P()], p2 = r[c] * r[e];
+                    res[P()] = sub ? p1 - p2 : p1 + p2;
                 });
                 break;
             }
-#undef RSR_CASE
-            case U_SPILL:
+            case U_ADDMUL: {
+                const uint32_t c = x & 31u;
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
-                    lds[(w1 * PPL + P()) * 64u + lane] = r[a];
-                    res[P()] = r[a];
+                    res[P()] = xa[P()] + (xb[P()] * r[c]);
+                });
+                break;
+            }
+            case U_ADD3: {
+                const uint32_t c = x & 31u;
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    res[P()] = (xa[P()] + xb[P()]) + r[c];
+                });
+                break;
+            }
+            case U_INVSEL: {
+                const uint32_t c = x & 31u, fl = (x >> 10) & 3u;
+                const float k = (fl & kInvOne) ? 1.0f : 0.0f;
+                const bool ng = (fl & kInvNeg) != 0u;
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    const float xv = ng ? -xa[P()] : xa[P()];
+                    res[P()] = (__builtin_fabsf(r[c]) < 1e-9f) ? k : xv * xb[P()];
+                });
+                break;
+            }
+            case U_SPILL:
+                sfor<PPL>([&](auto P) {
+                    lds[(x * PPL + P()) * 64u + lane] = xa[P()];
+                    res[P()] = xa[P()];
                 });
                 break;
             case U_FILL:
-                sfor<PPL>([&](auto P) { res[P()] = lds[(w1 * PPL + P()) * 64u + lane]; });
+                sfor<PPL>([&](auto P) { res[P()] = lds[(x * PPL + P()) * 64u + lane]; });
                 break;
             default:
                 sfor<PPL>([&](auto P) { res[P()] = 0.0f; });
@@ -145,134 +187,141 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
         if constexpr (T >= 4) { RSR_TAN1(STMT, 2, c) RSR_TAN1(STMT, 3, d) } \
     } while (0)
 // Spill slot s, channel ch (0 = value, 1 + j = tangent j) at lds[(s * (1 + T) + ch) * 64 + lane].
+// As in the primal interpreter, operands a and b of every channel are read before the
+// dispatch and the results written after it.
 template <int T>
 __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restrict__ code, uint32_t len,
                                          const float* __restrict__ params, uint32_t k0, float* __restrict__ lds,
                                          int lane) {
-    uint32_t nw0 = len ? uni(code[0].w0) : 0u, nw1 = len ? uni(code[0].w1) : 0u;
+    Uop nx = load_uop(code, 0);
     for (uint32_t pc = 0; pc < len; ++pc) {
-        const uint32_t w0 = nw0, w1 = nw1;
-        nw0 = uni(code[pc + 1].w0);
-        nw1 = uni(code[pc + 1].w1);
-        const uint32_t op = w0 & 255u;
-        const uint32_t d = (w0 >> 8) & 31u, a = (w0 >> 16) & 31u, b = (w0 >> 24) & 31u;
-        const uint32_t c = w1 & 31u, e = (w1 >> 8) & 31u;
+        const Uop u = nx;
+        nx = load_uop(code, pc + 1);
+        const uint32_t op = u.w0 & 255u, x = u.w0 >> 8;
+        const uint32_t d = u.d, a = u.a, b = u.b;
+        __builtin_assume(d < 32u);
+        __builtin_assume(a < 32u);
+        __builtin_assume(b < 32u);
+        float va, vb, ga[T], gb[T];
+        {
+            const v32f Vv = G.v;
+            va = Vv[a];
+            RSR_TAN(ga[j] = Gj[a]);
+            vb = Vv[b];
+            RSR_TAN(gb[j] = Gj[b]);
+        }
         float rv;
         float rg[T];
-        const v32f Vv = G.v;
-#define V Vv
         switch (op) {
             case U_IMM:
-                rv = params[w1];
-                RSR_TAN(rg[j] = (w1 == k0 + (uint32_t)j) ? 1.0f : 0.0f);
+                rv = params[x];
+                RSR_TAN(rg[j] = (x == k0 + (uint32_t)j) ? 1.0f : 0.0f);
                 break;
             case U_ADD:
-                rv = V[a] + V[b];
-                RSR_TAN(rg[j] = Gj[a] + Gj[b]);
+                rv = va + vb;
+                RSR_TAN(rg[j] = ga[j] + gb[j]);
                 break;
             case U_SUB:
-                rv = V[a] - V[b];
-                RSR_TAN(rg[j] = Gj[a] - Gj[b]);
+                rv = va - vb;
+                RSR_TAN(rg[j] = ga[j] - gb[j]);
                 break;
-            case U_MUL: {
-                const float x = V[a], y = V[b];
-                rv = x * y;
-                RSR_TAN(rg[j] = (x * Gj[b]) + (Gj[a] * y));
+            case U_MUL:
+                rv = va * vb;
+                RSR_TAN(rg[j] = (va * gb[j]) + (ga[j] * vb));
                 break;
-            }
             case U_DIV: {
-                const float x = V[a], y = V[b];
-                rv = x / y;
-                const float yy = y * y;
-                RSR_TAN(rg[j] = ((Gj[a] * y) - (x * Gj[b])) / yy);
+                rv = va / vb;
+                const float yy = vb * vb;
+                RSR_TAN(rg[j] = ((ga[j] * vb) - (va * gb[j])) / yy);
                 break;
             }
             case U_SQR: {
-                const float x = V[a];
-                rv = x * x;
-                const float x2 = 2.0f * x;
-                RSR_TAN(rg[j] = x2 * Gj[a]);
+                rv = va * va;
+                const float x2 = 2.0f * va;
+                RSR_TAN(rg[j] = x2 * ga[j]);
                 break;
             }
             case U_SQRT: {
-                const float s = __builtin_sqrtf(V[a]);
-                rv = s;
-                const float s2 = 2.0f * s;
-                RSR_TAN(rg[j] = Gj[a] / s2);
+                const float sq = __builtin_sqrtf(va);
+                rv = sq;
+                const float s2 = 2.0f * sq;
+                RSR_TAN(rg[j] = ga[j] / s2);
                 break;
             }
             case U_SIN: {
-                const float x = V[a];
-                rv = dm_sin(x);
-                const float cs = dm_cos(x);
-                RSR_TAN(rg[j] = Gj[a] * cs);
+                rv = dm_sin(va);
+                const float cs = dm_cos(va);
+                RSR_TAN(rg[j] = ga[j] * cs);
                 break;
             }
             case U_COS: {
-                const float x = V[a];
-                rv = dm_cos(x);
-                const float ns = -dm_sin(x);
-                RSR_TAN(rg[j] = Gj[a] * ns);
+                rv = dm_cos(va);
+                const float ns = -dm_sin(va);
+                RSR_TAN(rg[j] = ga[j] * ns);
                 break;
             }
             case U_EXP: {
-                const float ex = dm_exp(V[a]);
+                const float ex = dm_exp(va);
                 rv = ex;
-                RSR_TAN(rg[j] = Gj[a] * ex);
+                RSR_TAN(rg[j] = ga[j] * ex);
                 break;
             }
-            case U_LN: {
-                const float x = V[a];
-                rv = dm_ln(x);
-                RSR_TAN(rg[j] = Gj[a] / x);
+            case U_LN:
+                rv = dm_ln(va);
+                RSR_TAN(rg[j] = ga[j] / va);
                 break;
-            }
             case U_NEG:
-                rv = -V[a];
-                RSR_TAN(rg[j] = -Gj[a]);
+                rv = -va;
+                RSR_TAN(rg[j] = -ga[j]);
                 break;
-            case U_MULADD: case U_MULSUB: {
-                const float xa = V[a], xb = V[b], xc = V[c], xe = V[e];
+            case U_MULADD:
+            case U_MULSUB: {
+                const uint32_t c = x & 31u, e = (x >> 5) & 31u;
+                const v32f Vv = G.v;
+                const float xc = Vv[c], xe = Vv[e];
                 const bool sub = op == U_MULSUB;
-                const float p1 = xa * xb, p2 = xc * xe;
+                const float p1 = va * vb, p2 = xc * xe;
                 rv = sub ? p1 - p2 : p1 + p2;
-                RSR_TAN(const float t1 = (xa * Gj[b]) + (Gj[a] * xb); const float t2 = (xc * Gj[e]) + (Gj[c] * xe);
+                RSR_TAN(const float t1 = (va * gb[j]) + (ga[j] * vb); const float t2 = (xc * Gj[e]) + (Gj[c] * xe);
                         rg[j] = sub ? t1 - t2 : t1 + t2);
                 break;
             }
             case U_ADDMUL: {
-                const float xb = V[b], xc = V[c];
-                rv = V[a] + (xb * xc);
-                RSR_TAN(rg[j] = Gj[a] + ((xb * Gj[c]) + (Gj[b] * xc)));
+                const uint32_t c = x & 31u;
+                const v32f Vv = G.v;
+                const float xc = Vv[c];
+                rv = va + (vb * xc);
+                RSR_TAN(rg[j] = ga[j] + ((vb * Gj[c]) + (gb[j] * xc)));
                 break;
             }
             case U_SQRTACC: {
-                const float xb = V[b];
-                const float s = __builtin_sqrtf(V[a] + (xb * xb));
-                rv = s;
-                const float s2 = 2.0f * s;
-                RSR_TAN(rg[j] = (Gj[a] + ((xb * Gj[b]) + (Gj[b] * xb))) / s2);
+                const float sq = __builtin_sqrtf(va + (vb * vb));
+                rv = sq;
+                const float s2 = 2.0f * sq;
+                RSR_TAN(rg[j] = (ga[j] + ((vb * gb[j]) + (gb[j] * vb))) / s2);
                 break;
             }
-            case U_ADD3:
-                rv = (V[a] + V[b]) + V[c];
-                RSR_TAN(rg[j] = (Gj[a] + Gj[b]) + Gj[c]);
+            case U_ADD3: {
+                const uint32_t c = x & 31u;
+                const v32f Vv = G.v;
+                rv = (va + vb) + Vv[c];
+                RSR_TAN(rg[j] = (ga[j] + gb[j]) + Gj[c]);
                 break;
+            }
             case U_SPILL:
-                rv = V[a];
-                lds[(w1 * (1 + T)) * 64u + lane] = rv;
-                RSR_TAN(rg[j] = Gj[a]; lds[(w1 * (1 + T) + 1 + j) * 64u + lane] = rg[j]);
+                rv = va;
+                lds[(x * (1 + T)) * 64u + lane] = rv;
+                RSR_TAN(rg[j] = ga[j]; lds[(x * (1 + T) + 1 + j) * 64u + lane] = rg[j]);
                 break;
             case U_FILL:
-                rv = lds[(w1 * (1 + T)) * 64u + lane];
-                RSR_TAN(rg[j] = lds[(w1 * (1 + T) + 1 + j) * 64u + lane]);
+                rv = lds[(x * (1 + T)) * 64u + lane];
+                RSR_TAN(rg[j] = lds[(x * (1 + T) + 1 + j) * 64u + lane]);
                 break;
             default:
                 rv = 0.0f;
                 RSR_TAN(rg[j] = 0.0f);
                 break;
         }
-#undef V
         {
             v32f t = G.v;
             t[d] = rv;

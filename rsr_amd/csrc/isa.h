@@ -47,13 +47,15 @@ static inline int vt_size(int vt) {
     }
 }
 
-// Device micro-ops.  Encoding (two dwords):
-//   w0 = op | d << 8 | a << 16 | b << 24
-//   w1 = c | e << 8 | flags << 16          (IMM: w1 = flat parameter index)
+// Device micro-ops.  Encoding (four dwords, one scalar load, no field extraction for
+// the register operands):
+//   w0 = op | x << 8     x = IMM: flat parameter index; SPILL / FILL: LDS slot;
+//                            otherwise c | e << 5 | flags << 10
+//   w1 = d, w2 = a, w3 = b   register indices (< 32; 0 when unused)
 // Semantics (every arithmetic step is one IEEE binary32 operation, left to
 // right, never contracted into an FMA):
 enum UOp : uint8_t {
-    U_IMM = 0,    // d = params[w1]
+    U_IMM = 0,    // d = params[x]
     U_ADD,        // d = a + b
     U_SUB,        // d = a - b
     U_MUL,        // d = a * b
@@ -72,8 +74,8 @@ enum UOp : uint8_t {
     U_ADD3,       // d = (a + b) + c
     U_RCPSAFE,    // d = 1 / ((|a| < 1e-9) ? 1 : a)
     U_INVSEL,     // d = (|c| < 1e-9) ? K : (A * b), A = flags&1 ? -a : a, K = flags&2 ? 1 : 0
-    U_SPILL,      // LDS slot w1 <- a (d = a: the register is rewritten with its own value)
-    U_FILL,       // d <- LDS slot w1
+    U_SPILL,      // LDS slot x <- a (d = a: the register is rewritten with its own value)
+    U_FILL,       // d <- LDS slot x
     U_COUNT
 };
 
@@ -95,9 +97,18 @@ enum ProgStatus : int32_t {
     PROG_REGS = -6,
 };
 
-struct UInstr {
-    uint32_t w0, w1;
+struct alignas(16) UInstr {
+    uint32_t w0, w1, w2, w3;
 };
+
+static inline UInstr make_uinstr(uint32_t op, uint32_t d, uint32_t a, uint32_t b, uint32_t x) {
+    return UInstr{op | (x << 8), d, a, b};
+}
+// operand fields of x
+static inline uint32_t ux_c(uint32_t x) { return x & 31u; }
+static inline uint32_t ux_e(uint32_t x) { return (x >> 5) & 31u; }
+static inline uint32_t ux_flags(uint32_t x) { return (x >> 10) & 3u; }
+constexpr uint32_t kInvNeg = 1u, kInvOne = 2u;  // INVSEL flags: negate a; K = 1
 
 // Per-program descriptor consumed by the kernels.
 struct ProgDesc {

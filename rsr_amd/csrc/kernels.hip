@@ -55,6 +55,14 @@ __device__ __forceinline__ int find_seg_thread(const SegTable& T, int64_t t) {
     return s;
 }
 
+// One step of Kahan-compensated summation.
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+    const float yv = v - c;
+    const float tv = s + yv;
+    c = (tv - s) - yv;
+    s = tv;
+}
+
 // f32x8::reduce_add order: (lo4 + hi4), then ((0 + 2) + (1 + 3)); result in every lane.
 __device__ __forceinline__ float reduce_lanes8(float acc) {
     const float a0 = __shfl(acc, 0, 64), a1 = __shfl(acc, 1, 64), a2 = __shfl(acc, 2, 64), a3 = __shfl(acc, 3, 64);
@@ -151,12 +159,17 @@ __global__ void k_finalize_mse(const SegTable T) {
         s = S.kstate[2 * sp];
         c = S.kstate[2 * sp + 1];
     }
-    for (int64_t lt = 0; lt < S.n_tiles; ++lt) {
-        const float yv = S.partials[lt * S.n_progs + sp] - c;
-        const float tv = s + yv;
-        c = (tv - s) - yv;
-        s = tv;
+    const float* src = S.partials + sp;
+    const int64_t stride = S.n_progs;
+    int64_t lt = 0;
+    for (; lt + 16 <= S.n_tiles; lt += 16) {  // loads issued ahead of the serial compensated chain
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = src[(lt + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) kahan_add(s, c, v[u]);
     }
+    for (; lt < S.n_tiles; ++lt) kahan_add(s, c, src[lt * stride]);
     if (!S.last) {
         S.kstate[2 * sp] = s;
         S.kstate[2 * sp + 1] = c;
@@ -292,18 +305,26 @@ __global__ void k_finalize_grad(const SegTable T) {
         mc = ks[3];
     }
     const bool mine = slot == 0 || slot - 1 < np;
-    for (int64_t lt = 0; lt < S.n_tiles; ++lt) {
-        const float* row = S.partials + (lt * S.n_progs + sp) * S.slots;
-        if (mine) {
-            const float yv = row[slot] - c;
-            const float tv = s + yv;
-            c = (tv - s) - yv;
-            s = tv;
+    const float* src = S.partials + (int64_t)sp * S.slots;
+    const int64_t stride = (int64_t)S.n_progs * S.slots;
+    int64_t lt = 0;
+    for (; lt + 8 <= S.n_tiles; lt += 8) {  // loads issued ahead of the serial compensated chains
+        float v[8], v0[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            v0[u] = src[(lt + u) * stride];
+            v[u] = mine ? src[(lt + u) * stride + slot] : 0.0f;
         }
-        const float ym = row[0] - mc;
-        const float tm = m + ym;
-        mc = (tm - m) - ym;
-        m = tm;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (mine) kahan_add(s, c, v[u]);
+            kahan_add(m, mc, v0[u]);
+        }
+    }
+    for (; lt < S.n_tiles; ++lt) {
+        const float* row = src + lt * stride;
+        if (mine) kahan_add(s, c, row[slot]);
+        kahan_add(m, mc, row[0]);
     }
     if (!S.last) {
         ks[0] = s;

@@ -45,7 +45,7 @@ struct MI {
     UOp op;
     int ns;
     Src s[4];
-    uint32_t w1extra;  // IMM parameter index or flags (<< 16)
+    uint32_t extra;  // IMM parameter index, or INVSEL flags (kInvNeg | kInvOne)
 };
 
 constexpr int kMaxDepth = 32;
@@ -123,10 +123,7 @@ class Lowerer {
             if (v.spilled & (1u << k)) {
                 int r = alloc();
                 if (r < 0) { ok_ = false; return v; }
-                UInstr u;
-                u.w0 = (uint32_t)U_FILL | (uint32_t)r << 8;
-                u.w1 = v.slot[k];
-                out_.code.push_back(u);
+                out_.code.push_back(make_uinstr(U_FILL, (uint32_t)r, 0, 0, v.slot[k]));
                 slot_free_[v.slot[k]] = true;
                 v.reg[k] = (uint8_t)r;
                 v.spilled &= (uint16_t)~(1u << k);
@@ -157,10 +154,7 @@ class Lowerer {
         for (int k = 0; k < victim->n; ++k) {
             if ((victim->spilled & (1u << k)) || !is_temp(victim->reg[k])) continue;
             const int r = victim->reg[k];
-            UInstr u;
-            u.w0 = (uint32_t)U_SPILL | (uint32_t)r << 8 | (uint32_t)r << 16;
-            u.w1 = (uint32_t)slot;
-            out_.code.push_back(u);
+            out_.code.push_back(make_uinstr(U_SPILL, (uint32_t)r, (uint32_t)r, 0, (uint32_t)slot));
             slot_free_[slot] = false;
             if ((uint32_t)slot + 1 > out_.spill_slots) out_.spill_slots = slot + 1;
             victim->slot[k] = (uint8_t)slot;
@@ -227,11 +221,8 @@ class Lowerer {
                 const Src& s = m.s[k];
                 return (uint32_t)(s.virt ? vphys[s.id] : s.id);
             };
-            UInstr u;
-            u.w0 = (uint32_t)m.op | (uint32_t)d << 8 | reg(0) << 16 | reg(1) << 24;
-            if (m.op == U_IMM) u.w1 = m.w1extra;
-            else u.w1 = reg(2) | reg(3) << 8 | m.w1extra;
-            out_.code.push_back(u);
+            const uint32_t x = m.op == U_IMM ? m.extra : (reg(2) | reg(3) << 5 | m.extra << 10);
+            out_.code.push_back(make_uinstr(m.op, (uint32_t)d, reg(0), reg(1), x));
         }
         // virtual values with no reads and no output role (none are generated) are freed
         for (int v = 0; v < n; ++v)
@@ -250,7 +241,7 @@ class Lowerer {
         m.ns = (int)s.size();
         int k = 0;
         for (const Src& x : s) m.s[k++] = x;
-        m.w1extra = extra;
+        m.extra = extra;
         return m;
     }
 
@@ -494,10 +485,10 @@ class Lowerer {
                 return produce(VtMat2,
                                {mi(U_MULSUB, {M(0), M(3), M(1), M(2)}),          // v0 det
                                 mi(U_RCPSAFE, {V(0)}),                            // v1 inv_d
-                                mi(U_INVSEL, {M(3), V(1), V(0)}, 2u << 16),       // K=1
-                                mi(U_INVSEL, {M(1), V(1), V(0)}, 1u << 16),       // -m1, K=0
-                                mi(U_INVSEL, {M(2), V(1), V(0)}, 1u << 16),       // -m2, K=0
-                                mi(U_INVSEL, {M(0), V(1), V(0)}, 2u << 16)},      // K=1
+                                mi(U_INVSEL, {M(3), V(1), V(0)}, kInvOne),       // K=1
+                                mi(U_INVSEL, {M(1), V(1), V(0)}, kInvNeg),       // -m1, K=0
+                                mi(U_INVSEL, {M(2), V(1), V(0)}, kInvNeg),       // -m2, K=0
+                                mi(U_INVSEL, {M(0), V(1), V(0)}, kInvOne)},      // K=1
                                regs_of({&m}), {V(2), V(3), V(4), V(5)});
             }
             case MulM3: {  // out[r+3c] = (a[r]*b[3c] + a[r+3]*b[3c+1]) + a[r+6]*b[3c+2]
@@ -579,7 +570,7 @@ class Lowerer {
                 for (int k = 0; k < 9; ++k) {
                     uint32_t fl = (neg[k] ? 1u : 0u) | ((k == 0 || k == 4 || k == 8) ? 2u : 0u);
                     outs.push_back(V((int)seq.size()));
-                    seq.push_back(mi(U_INVSEL, {V(num[k]), V(5), V(4)}, fl << 16));
+                    seq.push_back(mi(U_INVSEL, {V(num[k]), V(5), V(4)}, fl));
                 }
                 return produce(VtMat3, seq, regs_of({&m}), outs);
             }

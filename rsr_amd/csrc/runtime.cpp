@@ -633,7 +633,7 @@ int rsr::get_lowering(rsr_programs* p, int F, bool dual, Lowering** out) {
             L->n_invalid++;
         }
     }
-    code.push_back(UInstr{0u, 0u});  // prefetch padding
+    code.push_back(UInstr{0u, 0u, 0u, 0u});  // prefetch padding
     // scheduling order: most expensive first, so groups are balanced and the tail is short
     L->order.resize(p->n);
     std::iota(L->order.begin(), L->order.end(), 0);
