@@ -48,6 +48,39 @@ static void run_host(const std::vector<UInstr>& code, const float* params, float
             }
             case U_SPILL: v = R[a]; g = G[a]; spill[2 * x] = v; spill[2 * x + 1] = g; break;
             case U_FILL: v = spill[2 * x]; g = spill[2 * x + 1]; break;
+            case U_DOT3:
+                v = ((R[a] * R[b]) + (R[a + 1] * R[b + 1])) + (R[a + 2] * R[b + 2]);
+                g = (((R[a] * G[b]) + (G[a] * R[b])) + ((R[a + 1] * G[b + 1]) + (G[a + 1] * R[b + 1]))) +
+                    ((R[a + 2] * G[b + 2]) + (G[a + 2] * R[b + 2]));
+                break;
+            case U_NORM3: {
+                v = sqrtf(((R[a] * R[a]) + (R[a + 1] * R[a + 1])) + (R[a + 2] * R[a + 2]));
+                const float t0 = ((R[a] * G[a]) + (G[a] * R[a])) + ((R[a + 1] * G[a + 1]) + (G[a + 1] * R[a + 1]));
+                g = (t0 + ((R[a + 2] * G[a + 2]) + (G[a + 2] * R[a + 2]))) / (2.0f * v);
+                break;
+            }
+            case U_ADDV3: case U_SUBV3: case U_SCALEV3: case U_IMMV3: case U_CROSS3: {
+                float rv[3], rg[3];
+                for (int q = 0; q < 3; ++q) {
+                    switch (op) {
+                        case U_ADDV3: rv[q] = R[a + q] + R[b + q]; rg[q] = G[a + q] + G[b + q]; break;
+                        case U_SUBV3: rv[q] = R[a + q] - R[b + q]; rg[q] = G[a + q] - G[b + q]; break;
+                        case U_SCALEV3: rv[q] = R[a + q] * R[b]; rg[q] = (R[a + q] * G[b]) + (G[a + q] * R[b]); break;
+                        case U_IMMV3: rv[q] = params[x + q]; rg[q] = ((int)(x + q) == k) ? 1.0f : 0.0f; break;
+                        default: {  // cross: component q = (a[i]*b[j]) - (a[j]*b[i])
+                            const int i = (q + 1) % 3, j = (q + 2) % 3;
+                            rv[q] = (R[a + i] * R[b + j]) - (R[a + j] * R[b + i]);
+                            rg[q] = ((R[a + i] * G[b + j]) + (G[a + i] * R[b + j])) -
+                                    ((R[a + j] * G[b + i]) + (G[a + j] * R[b + i]));
+                        }
+                    }
+                }
+                for (int q = 0; q < 3; ++q) {
+                    R[d + q] = rv[q];
+                    G[d + q] = rg[q];
+                }
+                continue;
+            }
             default: break;
         }
         R[d] = v;

@@ -164,6 +164,64 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             case U_FILL:
                 sfor<PPL>([&](auto P) { res[P()] = lds[(x * PPL + P()) * 64u + lane]; });
                 break;
+            // Vec3 block ops: components 1 and 2 are read here and written before the common
+            // write-back of component 0 (every read happens before the first write)
+#define RSR_V3CASE(OPC, E0, E1, E2)                                             \
+    case OPC: {                                                                 \
+        __builtin_assume(a < 30u);                                              \
+        __builtin_assume(b < 30u);                                              \
+        __builtin_assume(d < 30u);                                              \
+        float r1[PPL], r2[PPL];                                                 \
+        sfor<PPL>([&](auto P) {                                                 \
+            const v32f r = fget<P(), PPL>(R);                                   \
+            const float a0 = xa[P()], a1 = r[a + 1], a2 = r[a + 2];             \
+            const float b0 = xb[P()], b1 = r[b + 1], b2 = r[b + 2];             \
+            (void)b1; (void)b2;                                                 \
+            res[P()] = (E0);                                                    \
+            r1[P()] = (E1);                                                     \
+            r2[P()] = (E2);                                                     \
+        });                                                                     \
+        sfor<PPL>([&](auto P) {                                                 \
+            v32f t = fget<P(), PPL>(R);                                         \
+            t[d + 1] = r1[P()];                                                 \
+            t[d + 2] = r2[P()];                                                 \
+            fget<P(), PPL>(R) = t;                                              \
+        });                                                                     \
+        break;                                                                  \
+    }
+            RSR_V3CASE(U_ADDV3, a0 + b0, a1 + b1, a2 + b2)
+            RSR_V3CASE(U_SUBV3, a0 - b0, a1 - b1, a2 - b2)
+            RSR_V3CASE(U_SCALEV3, a0 * b0, a1 * b0, a2 * b0)
+            RSR_V3CASE(U_CROSS3, (a1 * b2) - (a2 * b1), (a2 * b0) - (a0 * b2), (a0 * b1) - (a1 * b0))
+#undef RSR_V3CASE
+            case U_IMMV3: {
+                __builtin_assume(d < 30u);
+                const float v0 = params[x], v1 = params[x + 1], v2 = params[x + 2];
+                sfor<PPL>([&](auto P) {
+                    res[P()] = v0;
+                    v32f t = fget<P(), PPL>(R);
+                    t[d + 1] = v1;
+                    t[d + 2] = v2;
+                    fget<P(), PPL>(R) = t;
+                });
+                break;
+            }
+            case U_DOT3:
+                __builtin_assume(a < 30u);
+                __builtin_assume(b < 30u);
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    res[P()] = ((xa[P()] * xb[P()]) + (r[a + 1] * r[b + 1])) + (r[a + 2] * r[b + 2]);
+                });
+                break;
+            case U_NORM3:
+                __builtin_assume(a < 30u);
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    const float a1 = r[a + 1], a2 = r[a + 2];
+                    res[P()] = __builtin_sqrtf(((xa[P()] * xa[P()]) + (a1 * a1)) + (a2 * a2));
+                });
+                break;
             default:
                 sfor<PPL>([&](auto P) { res[P()] = 0.0f; });
                 break;
@@ -317,6 +375,70 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                 rv = lds[(x * (1 + T)) * 64u + lane];
                 RSR_TAN(rg[j] = lds[(x * (1 + T) + 1 + j) * 64u + lane]);
                 break;
+            case U_DOT3: {
+                __builtin_assume(a < 30u);
+                __builtin_assume(b < 30u);
+                const v32f Vv = G.v;
+                const float a1 = Vv[a + 1], a2 = Vv[a + 2], b1 = Vv[b + 1], b2 = Vv[b + 2];
+                rv = ((va * vb) + (a1 * b1)) + (a2 * b2);
+                RSR_TAN(rg[j] = (((va * gb[j]) + (ga[j] * vb)) + ((a1 * Gj[b + 1]) + (Gj[a + 1] * b1))) +
+                                ((a2 * Gj[b + 2]) + (Gj[a + 2] * b2)));
+                break;
+            }
+            case U_NORM3: {
+                __builtin_assume(a < 30u);
+                const v32f Vv = G.v;
+                const float a1 = Vv[a + 1], a2 = Vv[a + 2];
+                const float sq = __builtin_sqrtf(((va * va) + (a1 * a1)) + (a2 * a2));
+                rv = sq;
+                const float s2 = 2.0f * sq;
+                RSR_TAN(rg[j] = ((((va * ga[j]) + (ga[j] * va)) + ((a1 * Gj[a + 1]) + (Gj[a + 1] * a1))) +
+                                 ((a2 * Gj[a + 2]) + (Gj[a + 2] * a2))) / s2);
+                break;
+            }
+            case U_ADDV3: case U_SUBV3: case U_SCALEV3: case U_CROSS3: case U_IMMV3: {
+                __builtin_assume(a < 30u);
+                __builtin_assume(b < 30u);
+                __builtin_assume(d < 30u);
+                // components 1 and 2 of the value and of every tangent channel
+                const v32f Vv = G.v;
+                const float a1 = Vv[a + 1], a2 = Vv[a + 2], b1 = Vv[b + 1], b2 = Vv[b + 2];
+                float v1, v2, g1[T], g2[T];
+                if (op == U_ADDV3) {
+                    rv = va + vb; v1 = a1 + b1; v2 = a2 + b2;
+                    RSR_TAN(rg[j] = ga[j] + gb[j]; g1[j] = Gj[a + 1] + Gj[b + 1]; g2[j] = Gj[a + 2] + Gj[b + 2]);
+                } else if (op == U_SUBV3) {
+                    rv = va - vb; v1 = a1 - b1; v2 = a2 - b2;
+                    RSR_TAN(rg[j] = ga[j] - gb[j]; g1[j] = Gj[a + 1] - Gj[b + 1]; g2[j] = Gj[a + 2] - Gj[b + 2]);
+                } else if (op == U_SCALEV3) {
+                    rv = va * vb; v1 = a1 * vb; v2 = a2 * vb;
+                    RSR_TAN(rg[j] = (va * gb[j]) + (ga[j] * vb); g1[j] = (a1 * gb[j]) + (Gj[a + 1] * vb);
+                            g2[j] = (a2 * gb[j]) + (Gj[a + 2] * vb));
+                } else if (op == U_CROSS3) {
+                    rv = (a1 * b2) - (a2 * b1); v1 = (a2 * vb) - (va * b2); v2 = (va * b1) - (a1 * vb);
+                    RSR_TAN(rg[j] = ((a1 * Gj[b + 2]) + (Gj[a + 1] * b2)) - ((a2 * Gj[b + 1]) + (Gj[a + 2] * b1));
+                            g1[j] = ((a2 * gb[j]) + (Gj[a + 2] * vb)) - ((va * Gj[b + 2]) + (ga[j] * b2));
+                            g2[j] = ((va * Gj[b + 1]) + (ga[j] * b1)) - ((a1 * gb[j]) + (Gj[a + 1] * vb)));
+                } else {
+                    rv = params[x]; v1 = params[x + 1]; v2 = params[x + 2];
+                    RSR_TAN(rg[j] = (x == k0 + (uint32_t)j) ? 1.0f : 0.0f;
+                            g1[j] = (x + 1u == k0 + (uint32_t)j) ? 1.0f : 0.0f;
+                            g2[j] = (x + 2u == k0 + (uint32_t)j) ? 1.0f : 0.0f);
+                }
+                {
+                    v32f t = G.v;
+                    t[d + 1] = v1;
+                    t[d + 2] = v2;
+                    G.v = t;
+                }
+                { v32f t = G.a; t[d + 1] = g1[0]; t[d + 2] = g2[0]; G.a = t; }
+                if constexpr (T >= 2) { v32f t = G.b; t[d + 1] = g1[1]; t[d + 2] = g2[1]; G.b = t; }
+                if constexpr (T >= 4) {
+                    { v32f t = G.c; t[d + 1] = g1[2]; t[d + 2] = g2[2]; G.c = t; }
+                    { v32f t = G.d; t[d + 1] = g1[3]; t[d + 2] = g2[3]; G.d = t; }
+                }
+                break;
+            }
             default:
                 rv = 0.0f;
                 RSR_TAN(rg[j] = 0.0f);

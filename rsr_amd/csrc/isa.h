@@ -76,6 +76,16 @@ enum UOp : uint8_t {
     U_INVSEL,     // d = (|c| < 1e-9) ? K : (A * b), A = flags&1 ? -a : a, K = flags&2 ? 1 : 0
     U_SPILL,      // LDS slot x <- a (d = a: the register is rewritten with its own value)
     U_FILL,       // d <- LDS slot x
+    // Vec3 micro-ops on three consecutive registers (operands a..a+2, b..b+2, results
+    // d..d+2); every operand component is read before any result is written, so a
+    // result block may overlap its operands
+    U_ADDV3,      // d+k = a+k + b+k
+    U_SUBV3,      // d+k = a+k - b+k
+    U_SCALEV3,    // d+k = a+k * b                    (b: one register)
+    U_IMMV3,      // d+k = params[x + k]
+    U_CROSS3,     // d = (a1*b2) - (a2*b1), d+1 = (a2*b0) - (a0*b2), d+2 = (a0*b1) - (a1*b0)
+    U_DOT3,       // d = ((a0*b0) + (a1*b1)) + (a2*b2) (one result register)
+    U_NORM3,      // d = sqrt(((a0*a0) + (a1*a1)) + (a2*a2))
     U_COUNT
 };
 

@@ -43,16 +43,26 @@ __device__ __forceinline__ void load_features(Files<PPL>& R, const float* __rest
     }
 }
 
-// Segment owning global work index w (first_wave / first_thread are prefix sums).
+// Segment owning a work index (first_wave / first_thread are increasing prefix sums).
+// A fixed chain of uniform selects over the table, no data-dependent loop.
 __device__ __forceinline__ int find_seg_wave(const SegTable& T, int64_t w) {
-    int s = 0;
-    while (s + 1 < T.n && w >= T.s[s + 1].first_wave) ++s;
-    return s;
+    int si = 0;
+#pragma unroll
+    for (int s = 1; s < kMaxSegs; ++s) si = (s < T.n && w >= T.s[s].first_wave) ? s : si;
+    return si;
 }
-__device__ __forceinline__ int find_seg_thread(const SegTable& T, int64_t t) {
-    int s = 0;
-    while (s + 1 < T.n && t >= T.s[s + 1].first_thread) ++s;
-    return s;
+__device__ __forceinline__ int find_seg_first_thread(const SegTable& T, int64_t t) {
+    int si = 0;
+#pragma unroll
+    for (int s = 1; s < kMaxSegs; ++s) si = (s < T.n && t >= T.s[s].first_thread) ? s : si;
+    return si;
+}
+// Finalize kernels: every segment's thread range starts on a wave boundary (first_thread
+// is a multiple of 64), so the segment follows from the wave's first thread and is
+// wave-uniform.
+__device__ __forceinline__ int find_seg_thread(const SegTable& T) {
+    const int64_t t0 = (int64_t)uni(blockIdx.x) * blockDim.x + ((int64_t)uni(threadIdx.x >> 6) << 6);
+    return find_seg_first_thread(T, t0);
 }
 
 // One step of Kahan-compensated summation.
@@ -147,7 +157,7 @@ __global__ __launch_bounds__(256) void k_eval_mse(const SegTable T, int spill_sl
 // One thread per (segment, scheduling position): Kahan sum of the tile partials.
 __global__ void k_finalize_mse(const SegTable T) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int si = find_seg_thread(T, t);
+    const int si = find_seg_thread(T);
     const SegDesc& S = T.s[si];
     const int64_t sp = t - S.first_thread;
     if (sp >= S.n_progs) return;
@@ -187,11 +197,11 @@ __global__ void k_finalize_mse(const SegTable T) {
 // range in order (evaluator.rs:159-170), starting from the state carried from earlier
 // ranges.  The wave reads 64 consecutive terms per step (coalesced) and folds its eight
 // 8-point batches one after the other; the last range reduces the eight accumulators
-// in f32x8::reduce_add order.  (first_thread / find_seg_thread count fold waves here.)
+// in f32x8::reduce_add order.  (first_thread counts fold waves here.)
 __global__ __launch_bounds__(256) void k_fold_terms(const SegTable T) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
-    const int si = find_seg_thread(T, wave);
+    const int si = find_seg_first_thread(T, wave);
     const SegDesc& S = T.s[si];
     const int64_t sp = wave - S.first_thread;
     if (sp >= S.n_progs) return;
@@ -285,7 +295,7 @@ __global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_s
 // in the same order, so all threads of a program agree.  kstate: 4 floats per (sp, slot).
 __global__ void k_finalize_grad(const SegTable T) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int si = find_seg_thread(T, t);
+    const int si = find_seg_thread(T);
     const SegDesc& S = T.s[si];
     const int64_t idx = t - S.first_thread;
     const int sp = (int)(idx / S.slots);
@@ -465,7 +475,8 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
             d.first_wave = waves;
             d.first_thread = threads;
             waves += d.n_tiles * d.n_groups;
-            threads += d.n_progs;  // finalize threads, or fold waves
+            // finalize threads (whole waves per segment), or fold waves
+            threads += terms ? (int64_t)d.n_progs : ((int64_t)d.n_progs + 63) / 64 * 64;
             spill = max(spill, st.P.spill_slots);
             T.s[T.n++] = d;
         }
@@ -507,7 +518,7 @@ hipError_t launch_grad_steps(const EvalStep* steps, int n, bool reference_order,
             d.first_wave = waves;
             d.first_thread = threads;
             waves += reference_order ? (int64_t)d.n_progs * d.n_pass : d.n_tiles * d.n_progs * d.n_pass;
-            threads += (int64_t)d.n_progs * d.slots;
+            threads += ((int64_t)d.n_progs * d.slots + 63) / 64 * 64;  // whole waves per segment
             spill = max(spill, st.P.spill_slots);
             T.s[T.n++] = d;
         }

@@ -183,6 +183,53 @@ class Lowerer {
         }
     }
 
+    // A value held in consecutive registers (never true for a partly spilled value).
+    static bool contig(const Val& v) {
+        if (v.spilled) return false;
+        for (int k = 1; k < v.n; ++k)
+            if (v.reg[k] != v.reg[0] + k) return false;
+        return true;
+    }
+
+    // First block of n consecutive temporaries that are free or in `dying`; -1 if none.
+    int find_block(int n, const std::vector<int>& dying) const {
+        bool f[kRegs];
+        for (int r = 0; r < kRegs; ++r) f[r] = free_[r];
+        for (int r : dying)
+            if (is_temp(r)) f[r] = true;
+        for (int r = F_; r + n <= kRegs; ++r) {
+            bool ok = true;
+            for (int k = 0; k < n && ok; ++k) ok = f[r + k];
+            if (ok) return r;
+        }
+        return -1;
+    }
+
+    // One block micro-op: operands a (and b) consumed, result in n_out consecutive
+    // registers (n_out = 1: any register).  The operand temporaries are released before
+    // the result is placed, so the result may reuse them (the device reads every operand
+    // component first).  Returns false, with nothing emitted, when no block can be found
+    // even after spilling; the caller then lowers the op component by component.
+    bool block_op(UOp op, int n_out, const Val& A, const Val* B, uint32_t x, Val& result) {
+        std::vector<int> dying;
+        for (int k = 0; k < A.n; ++k) dying.push_back(A.reg[k]);
+        if (B)
+            for (int k = 0; k < B->n; ++k) dying.push_back(B->reg[k]);
+        int d = find_block(n_out, dying);
+        while (d < 0) {
+            if (!spill_one()) return false;
+            d = find_block(n_out, dying);
+        }
+        for (int r : dying) release(r);
+        for (int k = 0; k < n_out; ++k) free_[d + k] = false;
+        if ((uint32_t)(d + n_out) > out_.max_reg) out_.max_reg = d + n_out;
+        out_.code.push_back(make_uinstr(op, (uint32_t)d, (uint32_t)A.reg[0], B ? (uint32_t)B->reg[0] : 0u, x));
+        result = Val{};
+        result.n = (uint8_t)n_out;
+        for (int k = 0; k < n_out; ++k) result.reg[k] = (uint8_t)(d + k);
+        return true;
+    }
+
     // Emit a micro-op sequence.  `owned` lists the operand components consumed by the
     // reference op; `outs` names the result components (virtual ids or physical
     // pass-through registers).  Returns false when the register file is exhausted.
@@ -267,6 +314,11 @@ class Lowerer {
         int n = vt_size(vt);
         if (vt != VtFloat) out_.has_vecmat_const = true;
         else out_.n_float_consts++;
+        if (vt == VtVec3) {
+            Val r, none;
+            none.n = 0;
+            if (block_op(U_IMMV3, 3, none, nullptr, (uint32_t)cflat_[c], r)) return push(vt, r) ? PROG_OK : PROG_STACK;
+        }
         std::vector<MI> seq;
         std::vector<Src> outs;
         for (int k = 0; k < n; ++k) {
@@ -282,6 +334,10 @@ class Lowerer {
     int cwise(int vt, UOp op) {
         if (!can_pop(vt, 2)) return PROG_STACK;
         Val b = pop(vt), a = pop(vt);
+        if (vt == VtVec3 && contig(a) && contig(b) && (op == U_ADD || op == U_SUB)) {
+            Val r;
+            if (block_op(op == U_ADD ? U_ADDV3 : U_SUBV3, 3, a, &b, 0, r)) return push(vt, r) ? PROG_OK : PROG_STACK;
+        }
         std::vector<MI> seq;
         std::vector<Src> outs;
         for (int k = 0; k < a.n; ++k) {
@@ -297,6 +353,10 @@ class Lowerer {
     int scale(int vt) {
         if (!can_pop(VtFloat) || !can_pop(vt)) return PROG_STACK;
         Val s = pop(VtFloat), v = pop(vt);
+        if (vt == VtVec3 && contig(v) && !s.spilled) {
+            Val r;
+            if (block_op(U_SCALEV3, 3, v, &s, 0, r)) return push(vt, r) ? PROG_OK : PROG_STACK;
+        }
         std::vector<MI> seq;
         std::vector<Src> outs;
         for (int k = 0; k < v.n; ++k) {
@@ -394,6 +454,10 @@ class Lowerer {
             case DotV3: {  // ((a0*b0) + (a1*b1)) + (a2*b2)
                 if (!can_pop(VtVec3, 2)) return PROG_STACK;
                 Val b = pop(VtVec3), a = pop(VtVec3);
+                if (contig(a) && contig(b)) {
+                    Val r;
+                    if (block_op(U_DOT3, 1, a, &b, 0, r)) return push(VtFloat, r) ? PROG_OK : PROG_STACK;
+                }
                 return produce(VtFloat,
                                {mi(U_MULADD, {P(a.reg[0]), P(b.reg[0]), P(a.reg[1]), P(b.reg[1])}),
                                 mi(U_ADDMUL, {V(0), P(a.reg[2]), P(b.reg[2])})},
@@ -410,6 +474,10 @@ class Lowerer {
             case NormV3: {  // sqrt(((v0*v0) + (v1*v1)) + (v2*v2))
                 if (!can_pop(VtVec3)) return PROG_STACK;
                 Val v = pop(VtVec3);
+                if (contig(v)) {
+                    Val r;
+                    if (block_op(U_NORM3, 1, v, nullptr, 0, r)) return push(VtFloat, r) ? PROG_OK : PROG_STACK;
+                }
                 return produce(VtFloat,
                                {mi(U_MULADD, {P(v.reg[0]), P(v.reg[0]), P(v.reg[1]), P(v.reg[1])}),
                                 mi(U_SQRTACC, {V(0), P(v.reg[2])})},
@@ -418,6 +486,10 @@ class Lowerer {
             case CrossV3: {
                 if (!can_pop(VtVec3, 2)) return PROG_STACK;
                 Val b = pop(VtVec3), a = pop(VtVec3);
+                if (contig(a) && contig(b)) {
+                    Val r;
+                    if (block_op(U_CROSS3, 3, a, &b, 0, r)) return push(VtVec3, r) ? PROG_OK : PROG_STACK;
+                }
                 auto A = [&](int k) { return P(a.reg[k]); };
                 auto B = [&](int k) { return P(b.reg[k]); };
                 return produce(VtVec3,
