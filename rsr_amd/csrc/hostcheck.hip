@@ -59,13 +59,93 @@ static void run_host(const std::vector<UInstr>& code, const float* params, float
                 g = (t0 + ((R[a + 2] * G[a + 2]) + (G[a + 2] * R[a + 2]))) / (2.0f * v);
                 break;
             }
-            case U_ADDV3: case U_SUBV3: case U_SCALEV3: case U_IMMV3: case U_CROSS3: {
+            case U_ADDK: case U_SUBK: case U_KSUB: case U_MULK: case U_DIVK: case U_KDIV: {
+                const float p = params[x], gp = ((int)x == k) ? 1.0f : 0.0f;
+                switch (op) {
+                    case U_ADDK: v = R[a] + p; g = G[a] + gp; break;
+                    case U_SUBK: v = R[a] - p; g = G[a] - gp; break;
+                    case U_KSUB: v = p - R[a]; g = gp - G[a]; break;
+                    case U_MULK: v = R[a] * p; g = (R[a] * gp) + (G[a] * p); break;
+                    case U_DIVK: v = R[a] / p; g = ((G[a] * p) - (R[a] * gp)) / (p * p); break;
+                    default: v = p / R[a]; g = ((gp * R[a]) - (p * G[a])) / (R[a] * R[a]); break;
+                }
+                break;
+            }
+            case U_MULM3V3: {
+                float rv[3], rg[3];
+                for (int i = 0; i < 3; ++i) {
+                    const float t = (R[a + i] * R[b]) + (R[a + 3 + i] * R[b + 1]);
+                    const float tg = ((R[a + i] * G[b]) + (G[a + i] * R[b])) +
+                                     ((R[a + 3 + i] * G[b + 1]) + (G[a + 3 + i] * R[b + 1]));
+                    rv[i] = t + (R[a + 6 + i] * R[b + 2]);
+                    rg[i] = tg + ((R[a + 6 + i] * G[b + 2]) + (G[a + 6 + i] * R[b + 2]));
+                }
+                for (int i = 0; i < 3; ++i) {
+                    R[d + i] = rv[i];
+                    G[d + i] = rg[i];
+                }
+                continue;
+            }
+            case U_SCALEM3: case U_SCALEM3K: {
+                const bool kk = op == U_SCALEM3K;
+                const float sv = kk ? params[x] : R[b], gs = kk ? (((int)x == k) ? 1.0f : 0.0f) : G[b];
+                float rv[9], rg[9];
+                for (int q = 0; q < 9; ++q) {
+                    rv[q] = R[a + q] * sv;
+                    rg[q] = (R[a + q] * gs) + (G[a + q] * sv);
+                }
+                for (int q = 0; q < 9; ++q) {
+                    R[d + q] = rv[q];
+                    G[d + q] = rg[q];
+                }
+                continue;
+            }
+            case U_ADDV4: case U_SUBV4: case U_SCALEV4: case U_SCALEV4K: case U_MULM2: case U_INVM2: {
+                float rv[4], rg[4];
+                const float p = params[x], gp = ((int)x == k) ? 1.0f : 0.0f;
+                for (int q = 0; q < 4; ++q) {
+                    switch (op) {
+                        case U_ADDV4: rv[q] = R[a + q] + R[b + q]; rg[q] = G[a + q] + G[b + q]; break;
+                        case U_SUBV4: rv[q] = R[a + q] - R[b + q]; rg[q] = G[a + q] - G[b + q]; break;
+                        case U_SCALEV4: rv[q] = R[a + q] * R[b]; rg[q] = (R[a + q] * G[b]) + (G[a + q] * R[b]); break;
+                        case U_SCALEV4K: rv[q] = R[a + q] * p; rg[q] = (R[a + q] * gp) + (G[a + q] * p); break;
+                        case U_MULM2: {
+                            const int r = q & 1, c = q >> 1;
+                            rv[q] = (R[a + r] * R[b + 2 * c]) + (R[a + r + 2] * R[b + 2 * c + 1]);
+                            rg[q] = ((R[a + r] * G[b + 2 * c]) + (G[a + r] * R[b + 2 * c])) +
+                                    ((R[a + r + 2] * G[b + 2 * c + 1]) + (G[a + r + 2] * R[b + 2 * c + 1]));
+                            break;
+                        }
+                        default: {  // inverse (linalg/eval.rs:316-329); tangents unused (never lowered dual)
+                            const float det = (R[a] * R[a + 3]) - (R[a + 1] * R[a + 2]);
+                            const bool sing = fabsf(det) < 1e-9f;
+                            const float inv = 1.0f / (sing ? 1.0f : det);
+                            static const int src[4] = {3, 1, 2, 0};
+                            const float m = (q == 1 || q == 2) ? -R[a + src[q]] : R[a + src[q]];
+                            rv[q] = sing ? ((q == 0 || q == 3) ? 1.0f : 0.0f) : m * inv;
+                            rg[q] = 0.0f;
+                        }
+                    }
+                }
+                for (int q = 0; q < 4; ++q) {
+                    R[d + q] = rv[q];
+                    G[d + q] = rg[q];
+                }
+                continue;
+            }
+            case U_ADDV3: case U_SUBV3: case U_SCALEV3: case U_SCALEV3K: case U_IMMV3: case U_CROSS3: {
                 float rv[3], rg[3];
                 for (int q = 0; q < 3; ++q) {
                     switch (op) {
                         case U_ADDV3: rv[q] = R[a + q] + R[b + q]; rg[q] = G[a + q] + G[b + q]; break;
                         case U_SUBV3: rv[q] = R[a + q] - R[b + q]; rg[q] = G[a + q] - G[b + q]; break;
                         case U_SCALEV3: rv[q] = R[a + q] * R[b]; rg[q] = (R[a + q] * G[b]) + (G[a + q] * R[b]); break;
+                        case U_SCALEV3K: {
+                            const float p = params[x], gp = ((int)x == k) ? 1.0f : 0.0f;
+                            rv[q] = R[a + q] * p;
+                            rg[q] = (R[a + q] * gp) + (G[a + q] * p);
+                            break;
+                        }
                         case U_IMMV3: rv[q] = params[x + q]; rg[q] = ((int)(x + q) == k) ? 1.0f : 0.0f; break;
                         default: {  // cross: component q = (a[i]*b[j]) - (a[j]*b[i])
                             const int i = (q + 1) % 3, j = (q + 2) % 3;

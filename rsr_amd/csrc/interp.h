@@ -194,6 +194,131 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             RSR_V3CASE(U_SCALEV3, a0 * b0, a1 * b0, a2 * b0)
             RSR_V3CASE(U_CROSS3, (a1 * b2) - (a2 * b1), (a2 * b0) - (a0 * b2), (a0 * b1) - (a1 * b0))
 #undef RSR_V3CASE
+#define RSR_KCASE(OPC, EXPR)                               \
+    case OPC: {                                            \
+        const float K = params[x];                         \
+        sfor<PPL>([&](auto P) {                            \
+            const float A = xa[P()];                       \
+            res[P()] = (EXPR);                             \
+        });                                                \
+        break;                                             \
+    }
+            RSR_KCASE(U_ADDK, A + K)
+            RSR_KCASE(U_SUBK, A - K)
+            RSR_KCASE(U_KSUB, K - A)
+            RSR_KCASE(U_MULK, A * K)
+            RSR_KCASE(U_DIVK, A / K)
+            RSR_KCASE(U_KDIV, K / A)
+#undef RSR_KCASE
+            case U_SCALEV3K: {
+                __builtin_assume(a < 30u);
+                __builtin_assume(d < 30u);
+                const float K = params[x];
+                float r1[PPL], r2[PPL];
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    const float a1 = r[a + 1], a2 = r[a + 2];
+                    res[P()] = xa[P()] * K;
+                    r1[P()] = a1 * K;
+                    r2[P()] = a2 * K;
+                });
+                sfor<PPL>([&](auto P) {
+                    v32f t = fget<P(), PPL>(R);
+                    t[d + 1] = r1[P()];
+                    t[d + 2] = r2[P()];
+                    fget<P(), PPL>(R) = t;
+                });
+                break;
+            }
+            case U_MULM3V3: {
+                __builtin_assume(a < 23u);
+                __builtin_assume(b < 30u);
+                __builtin_assume(d < 30u);
+                float r1[PPL], r2[PPL];
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    const float v0 = xb[P()], v1 = r[b + 1], v2 = r[b + 2];
+                    const float m0 = xa[P()], m1 = r[a + 1], m2 = r[a + 2], m3 = r[a + 3], m4 = r[a + 4];
+                    const float m5 = r[a + 5], m6 = r[a + 6], m7 = r[a + 7], m8 = r[a + 8];
+                    res[P()] = ((m0 * v0) + (m3 * v1)) + (m6 * v2);
+                    r1[P()] = ((m1 * v0) + (m4 * v1)) + (m7 * v2);
+                    r2[P()] = ((m2 * v0) + (m5 * v1)) + (m8 * v2);
+                });
+                sfor<PPL>([&](auto P) {
+                    v32f t = fget<P(), PPL>(R);
+                    t[d + 1] = r1[P()];
+                    t[d + 2] = r2[P()];
+                    fget<P(), PPL>(R) = t;
+                });
+                break;
+            }
+            case U_SCALEM3:
+            case U_SCALEM3K: {
+                __builtin_assume(a < 23u);
+                __builtin_assume(d < 23u);
+                const float K = params[x];
+                const bool kk = op == U_SCALEM3K;
+                // per point: all nine reads, then the writes (files of different points are disjoint)
+                sfor<PPL>([&](auto P) {
+                    v32f t = fget<P(), PPL>(R);
+                    const float sv = kk ? K : xb[P()];
+                    float rr[8];
+#pragma unroll
+                    for (int q = 1; q < 9; ++q) rr[q - 1] = t[a + q] * sv;
+                    res[P()] = xa[P()] * sv;
+#pragma unroll
+                    for (int q = 1; q < 9; ++q) t[d + q] = rr[q - 1];
+                    fget<P(), PPL>(R) = t;
+                });
+                break;
+            }
+            case U_ADDV4:
+            case U_SUBV4:
+            case U_SCALEV4:
+            case U_SCALEV4K:
+            case U_MULM2:
+            case U_INVM2: {
+                __builtin_assume(a < 29u);
+                __builtin_assume(b < 29u);
+                __builtin_assume(d < 29u);
+                const float K = params[x];
+                sfor<PPL>([&](auto P) {
+                    v32f t = fget<P(), PPL>(R);
+                    const float A0 = xa[P()], A1 = t[a + 1], A2 = t[a + 2], A3 = t[a + 3];
+                    float o0, o1, o2, o3;
+                    if (op == U_ADDV4 || op == U_SUBV4) {
+                        const float B0 = xb[P()], B1 = t[b + 1], B2 = t[b + 2], B3 = t[b + 3];
+                        if (op == U_ADDV4) {
+                            o0 = A0 + B0; o1 = A1 + B1; o2 = A2 + B2; o3 = A3 + B3;
+                        } else {
+                            o0 = A0 - B0; o1 = A1 - B1; o2 = A2 - B2; o3 = A3 - B3;
+                        }
+                    } else if (op == U_MULM2) {
+                        const float B0 = xb[P()], B1 = t[b + 1], B2 = t[b + 2], B3 = t[b + 3];
+                        o0 = (A0 * B0) + (A2 * B1);
+                        o1 = (A1 * B0) + (A3 * B1);
+                        o2 = (A0 * B2) + (A2 * B3);
+                        o3 = (A1 * B2) + (A3 * B3);
+                    } else if (op == U_INVM2) {
+                        const float det = (A0 * A3) - (A1 * A2);
+                        const bool sing = __builtin_fabsf(det) < 1e-9f;
+                        const float inv = 1.0f / (sing ? 1.0f : det);
+                        o0 = sing ? 1.0f : A3 * inv;
+                        o1 = sing ? 0.0f : (-A1) * inv;
+                        o2 = sing ? 0.0f : (-A2) * inv;
+                        o3 = sing ? 1.0f : A0 * inv;
+                    } else {
+                        const float sv = op == U_SCALEV4K ? K : xb[P()];
+                        o0 = A0 * sv; o1 = A1 * sv; o2 = A2 * sv; o3 = A3 * sv;
+                    }
+                    res[P()] = o0;
+                    t[d + 1] = o1;
+                    t[d + 2] = o2;
+                    t[d + 3] = o3;
+                    fget<P(), PPL>(R) = t;
+                });
+                break;
+            }
             case U_IMMV3: {
                 __builtin_assume(d < 30u);
                 const float v0 = params[x], v1 = params[x + 1], v2 = params[x + 2];
@@ -396,7 +521,165 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                                  ((a2 * Gj[a + 2]) + (Gj[a + 2] * a2))) / s2);
                 break;
             }
-            case U_ADDV3: case U_SUBV3: case U_SCALEV3: case U_CROSS3: case U_IMMV3: {
+            case U_ADDK: case U_SUBK: case U_KSUB: case U_MULK: case U_DIVK: case U_KDIV: {
+                const float K = params[x];
+                float gk[T];
+                RSR_TAN(gk[j] = (x == k0 + (uint32_t)j) ? 1.0f : 0.0f);
+                if (op == U_ADDK) {
+                    rv = va + K;
+                    RSR_TAN(rg[j] = ga[j] + gk[j]);
+                } else if (op == U_SUBK) {
+                    rv = va - K;
+                    RSR_TAN(rg[j] = ga[j] - gk[j]);
+                } else if (op == U_KSUB) {
+                    rv = K - va;
+                    RSR_TAN(rg[j] = gk[j] - ga[j]);
+                } else if (op == U_MULK) {
+                    rv = va * K;
+                    RSR_TAN(rg[j] = (va * gk[j]) + (ga[j] * K));
+                } else if (op == U_DIVK) {
+                    rv = va / K;
+                    const float kk = K * K;
+                    RSR_TAN(rg[j] = ((ga[j] * K) - (va * gk[j])) / kk);
+                } else {
+                    rv = K / va;
+                    const float aa = va * va;
+                    RSR_TAN(rg[j] = ((gk[j] * va) - (K * ga[j])) / aa);
+                }
+                break;
+            }
+            case U_MULM3V3: {
+                __builtin_assume(a < 23u);
+                __builtin_assume(b < 30u);
+                __builtin_assume(d < 30u);
+                const v32f Vv = G.v;
+                float out[3], og[3][T];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const float m0 = Vv[a + i], m1 = Vv[a + 3 + i], m2 = Vv[a + 6 + i];
+                    const float v0 = Vv[b], v1 = Vv[b + 1], v2 = Vv[b + 2];
+                    const float t = (m0 * v0) + (m1 * v1);
+                    out[i] = t + (m2 * v2);
+                    RSR_TAN(const float tg = ((m0 * Gj[b]) + (Gj[a + i] * v0)) + ((m1 * Gj[b + 1]) + (Gj[a + 3 + i] * v1));
+                            og[i][j] = tg + ((m2 * Gj[b + 2]) + (Gj[a + 6 + i] * v2)));
+                }
+                rv = out[0];
+                RSR_TAN(rg[j] = og[0][j]);
+                {
+                    v32f t = G.v;
+                    t[d + 1] = out[1];
+                    t[d + 2] = out[2];
+                    G.v = t;
+                }
+                { v32f t = G.a; t[d + 1] = og[1][0]; t[d + 2] = og[2][0]; G.a = t; }
+                if constexpr (T >= 2) { v32f t = G.b; t[d + 1] = og[1][1]; t[d + 2] = og[2][1]; G.b = t; }
+                if constexpr (T >= 4) {
+                    { v32f t = G.c; t[d + 1] = og[1][2]; t[d + 2] = og[2][2]; G.c = t; }
+                    { v32f t = G.d; t[d + 1] = og[1][3]; t[d + 2] = og[2][3]; G.d = t; }
+                }
+                break;
+            }
+            case U_SCALEM3:
+            case U_SCALEM3K: {
+                __builtin_assume(a < 23u);
+                __builtin_assume(d < 23u);
+                const v32f Vv = G.v;
+                const bool kk = op == U_SCALEM3K;
+                const float sv = kk ? params[x] : vb;
+                float gs[T];
+                RSR_TAN(gs[j] = kk ? ((x == k0 + (uint32_t)j) ? 1.0f : 0.0f) : gb[j]);
+                float out[9], og[9][T];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    const float m = Vv[a + q];
+                    out[q] = m * sv;
+                    RSR_TAN(og[q][j] = (m * gs[j]) + (Gj[a + q] * sv));
+                }
+                rv = out[0];
+                RSR_TAN(rg[j] = og[0][j]);
+                {
+                    v32f t = G.v;
+#pragma unroll
+                    for (int q = 1; q < 9; ++q) t[d + q] = out[q];
+                    G.v = t;
+                }
+                {
+                    v32f t = G.a;
+#pragma unroll
+                    for (int q = 1; q < 9; ++q) t[d + q] = og[q][0];
+                    G.a = t;
+                }
+                if constexpr (T >= 2) {
+                    v32f t = G.b;
+#pragma unroll
+                    for (int q = 1; q < 9; ++q) t[d + q] = og[q][1];
+                    G.b = t;
+                }
+                if constexpr (T >= 4) {
+                    {
+                        v32f t = G.c;
+#pragma unroll
+                        for (int q = 1; q < 9; ++q) t[d + q] = og[q][2];
+                        G.c = t;
+                    }
+                    {
+                        v32f t = G.d;
+#pragma unroll
+                        for (int q = 1; q < 9; ++q) t[d + q] = og[q][3];
+                        G.d = t;
+                    }
+                }
+                break;
+            }
+            case U_ADDV4: case U_SUBV4: case U_SCALEV4: case U_SCALEV4K: case U_MULM2: {
+                __builtin_assume(a < 29u);
+                __builtin_assume(b < 29u);
+                __builtin_assume(d < 29u);
+                const v32f Vv = G.v;
+                float A[4], B[4], out[4], og[4][T];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    A[q] = Vv[a + q];
+                    B[q] = Vv[b + q];
+                }
+                const bool kk = op == U_SCALEV4K;
+                const float sv = kk ? params[x] : B[0];
+                float gs[T];
+                RSR_TAN(gs[j] = kk ? ((x == k0 + (uint32_t)j) ? 1.0f : 0.0f) : gb[j]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (op == U_ADDV4) {
+                        out[q] = A[q] + B[q];
+                        RSR_TAN(og[q][j] = Gj[a + q] + Gj[b + q]);
+                    } else if (op == U_SUBV4) {
+                        out[q] = A[q] - B[q];
+                        RSR_TAN(og[q][j] = Gj[a + q] - Gj[b + q]);
+                    } else if (op == U_MULM2) {
+                        const int r = q & 1, c = q >> 1;
+                        out[q] = (A[r] * B[2 * c]) + (A[r + 2] * B[2 * c + 1]);
+                        RSR_TAN(og[q][j] = ((A[r] * Gj[b + 2 * c]) + (Gj[a + r] * B[2 * c])) +
+                                           ((A[r + 2] * Gj[b + 2 * c + 1]) + (Gj[a + r + 2] * B[2 * c + 1])));
+                    } else {
+                        out[q] = A[q] * sv;
+                        RSR_TAN(og[q][j] = (A[q] * gs[j]) + (Gj[a + q] * sv));
+                    }
+                }
+                rv = out[0];
+                RSR_TAN(rg[j] = og[0][j]);
+                {
+                    v32f t = G.v;
+                    t[d + 1] = out[1]; t[d + 2] = out[2]; t[d + 3] = out[3];
+                    G.v = t;
+                }
+                { v32f t = G.a; t[d + 1] = og[1][0]; t[d + 2] = og[2][0]; t[d + 3] = og[3][0]; G.a = t; }
+                if constexpr (T >= 2) { v32f t = G.b; t[d + 1] = og[1][1]; t[d + 2] = og[2][1]; t[d + 3] = og[3][1]; G.b = t; }
+                if constexpr (T >= 4) {
+                    { v32f t = G.c; t[d + 1] = og[1][2]; t[d + 2] = og[2][2]; t[d + 3] = og[3][2]; G.c = t; }
+                    { v32f t = G.d; t[d + 1] = og[1][3]; t[d + 2] = og[2][3]; t[d + 3] = og[3][3]; G.d = t; }
+                }
+                break;
+            }
+            case U_ADDV3: case U_SUBV3: case U_SCALEV3: case U_SCALEV3K: case U_CROSS3: case U_IMMV3: {
                 __builtin_assume(a < 30u);
                 __builtin_assume(b < 30u);
                 __builtin_assume(d < 30u);
@@ -410,10 +693,15 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                 } else if (op == U_SUBV3) {
                     rv = va - vb; v1 = a1 - b1; v2 = a2 - b2;
                     RSR_TAN(rg[j] = ga[j] - gb[j]; g1[j] = Gj[a + 1] - Gj[b + 1]; g2[j] = Gj[a + 2] - Gj[b + 2]);
-                } else if (op == U_SCALEV3) {
-                    rv = va * vb; v1 = a1 * vb; v2 = a2 * vb;
-                    RSR_TAN(rg[j] = (va * gb[j]) + (ga[j] * vb); g1[j] = (a1 * gb[j]) + (Gj[a + 1] * vb);
-                            g2[j] = (a2 * gb[j]) + (Gj[a + 2] * vb));
+                } else if (op == U_SCALEV3 || op == U_SCALEV3K) {
+                    // the scalar: register b, or the parameter x (tangent 1 on its own channel)
+                    const bool kk = op == U_SCALEV3K;
+                    const float sv = kk ? params[x] : vb;
+                    float gs[T];
+                    RSR_TAN(gs[j] = kk ? ((x == k0 + (uint32_t)j) ? 1.0f : 0.0f) : gb[j]);
+                    rv = va * sv; v1 = a1 * sv; v2 = a2 * sv;
+                    RSR_TAN(rg[j] = (va * gs[j]) + (ga[j] * sv); g1[j] = (a1 * gs[j]) + (Gj[a + 1] * sv);
+                            g2[j] = (a2 * gs[j]) + (Gj[a + 2] * sv));
                 } else if (op == U_CROSS3) {
                     rv = (a1 * b2) - (a2 * b1); v1 = (a2 * vb) - (va * b2); v2 = (va * b1) - (a1 * vb);
                     RSR_TAN(rg[j] = ((a1 * Gj[b + 2]) + (Gj[a + 1] * b2)) - ((a2 * Gj[b + 1]) + (Gj[a + 2] * b1));

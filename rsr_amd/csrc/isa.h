@@ -86,6 +86,26 @@ enum UOp : uint8_t {
     U_CROSS3,     // d = (a1*b2) - (a2*b1), d+1 = (a2*b0) - (a0*b2), d+2 = (a0*b1) - (a1*b0)
     U_DOT3,       // d = ((a0*b0) + (a1*b1)) + (a2*b2) (one result register)
     U_NORM3,      // d = sqrt(((a0*a0) + (a1*a1)) + (a2*a2))
+    // a Float constant as operand, read from the parameters (p = params[x]); the
+    // constant is never materialised in a register
+    U_ADDK,       // d = a + p
+    U_SUBK,       // d = a - p
+    U_KSUB,       // d = p - a
+    U_MULK,       // d = a * p
+    U_DIVK,       // d = a / p
+    U_KDIV,       // d = p / a
+    U_SCALEV3K,   // d+k = a+k * p
+    // Mat3 (column-major, nine consecutive registers) block ops
+    U_MULM3V3,    // d+i = ((a+i * b) + (a+3+i * b+1)) + (a+6+i * b+2), i < 3
+    U_SCALEM3,    // d+k = a+k * b, k < 9
+    U_SCALEM3K,   // d+k = a+k * p
+    // Mat2 (column-major, four consecutive registers) block ops
+    U_ADDV4,      // d+k = a+k + b+k, k < 4
+    U_SUBV4,      // d+k = a+k - b+k
+    U_SCALEV4,    // d+k = a+k * b
+    U_SCALEV4K,   // d+k = a+k * p
+    U_MULM2,      // d+r+2c = (a+r * b+2c) + (a+r+2 * b+2c+1)
+    U_INVM2,      // inverse with the singular -> identity blend (no dual form: never lowered dual)
     U_COUNT
 };
 

@@ -34,6 +34,7 @@ struct Val {
     uint8_t slot[9] = {0};   // LDS slot of a spilled component
     uint16_t spilled = 0;    // bit k: component k lives in LDS slot[k]
     uint32_t seq = 0;        // push order (older values are consumed later)
+    int32_t konst = -1;      // >= 0: a Float constant (flat parameter index) not yet in a register
 };
 
 struct Src {
@@ -116,7 +117,23 @@ class Lowerer {
     bool can_pop(int vt, int n = 1) { return (int)stk_[sidx(vt)].size() >= n; }
     // Pop a value and bring spilled components back into registers.  A failed fill
     // leaves ok_ false (reported as PROG_REGS).
+    // Pop a value; a pending Float constant is materialised into a register.
     Val pop(int vt) {
+        Val v = pop_raw(vt);
+        if (v.konst >= 0) {
+            const int r = alloc();
+            if (r < 0) {
+                ok_ = false;
+                return v;
+            }
+            out_.code.push_back(make_uinstr(U_IMM, (uint32_t)r, 0, 0, (uint32_t)v.konst));
+            v.reg[0] = (uint8_t)r;
+            v.konst = -1;
+        }
+        return v;
+    }
+    // Pop a value as it is (a pending constant stays pending).
+    Val pop_raw(int vt) {
         Val v = stk_[sidx(vt)].back();
         stk_[sidx(vt)].pop_back();
         for (int k = 0; k < v.n; ++k)
@@ -141,6 +158,7 @@ class Lowerer {
         Val* victim = nullptr;
         for (auto& st : stk_)
             for (auto& v : st) {
+                if (v.konst >= 0) continue;  // a pending constant holds no register
                 bool has_temp = false;
                 for (int k = 0; k < v.n; ++k)
                     if (!(v.spilled & (1u << k)) && is_temp(v.reg[k])) has_temp = true;
@@ -314,6 +332,12 @@ class Lowerer {
         int n = vt_size(vt);
         if (vt != VtFloat) out_.has_vecmat_const = true;
         else out_.n_float_consts++;
+        if (vt == VtFloat) {  // pending: its consumer reads it from the parameters
+            Val r;
+            r.n = 1;
+            r.konst = cflat_[c];
+            return push(vt, r) ? PROG_OK : PROG_STACK;
+        }
         if (vt == VtVec3) {
             Val r, none;
             none.n = 0;
@@ -334,9 +358,10 @@ class Lowerer {
     int cwise(int vt, UOp op) {
         if (!can_pop(vt, 2)) return PROG_STACK;
         Val b = pop(vt), a = pop(vt);
-        if (vt == VtVec3 && contig(a) && contig(b) && (op == U_ADD || op == U_SUB)) {
+        if ((vt == VtVec3 || vt == VtMat2) && contig(a) && contig(b) && (op == U_ADD || op == U_SUB)) {
+            const UOp bop = vt == VtVec3 ? (op == U_ADD ? U_ADDV3 : U_SUBV3) : (op == U_ADD ? U_ADDV4 : U_SUBV4);
             Val r;
-            if (block_op(op == U_ADD ? U_ADDV3 : U_SUBV3, 3, a, &b, 0, r)) return push(vt, r) ? PROG_OK : PROG_STACK;
+            if (block_op(bop, a.n, a, &b, 0, r)) return push(vt, r) ? PROG_OK : PROG_STACK;
         }
         std::vector<MI> seq;
         std::vector<Src> outs;
@@ -352,7 +377,18 @@ class Lowerer {
     // scale: (F s, T v) -> T, out_k = v_k * s (linalg/eval.rs:106-118, 163-175, 245-257, 384-406)
     int scale(int vt) {
         if (!can_pop(VtFloat) || !can_pop(vt)) return PROG_STACK;
-        Val s = pop(VtFloat), v = pop(vt);
+        Val s = pop_raw(VtFloat), v = pop(vt);
+        if (s.konst >= 0 && (vt == VtVec3 || vt == VtMat2 || vt == VtMat3) && contig(v)) {
+            const UOp bop = vt == VtVec3 ? U_SCALEV3K : vt == VtMat2 ? U_SCALEV4K : U_SCALEM3K;
+            Val r;
+            if (block_op(bop, v.n, v, nullptr, (uint32_t)s.konst, r)) return push(vt, r) ? PROG_OK : PROG_STACK;
+        }
+        if (s.konst >= 0 && !materialize(s)) return PROG_REGS;
+        if ((vt == VtMat2 || vt == VtMat3) && contig(v) && !s.spilled) {
+            Val r;
+            if (block_op(vt == VtMat2 ? U_SCALEV4 : U_SCALEM3, v.n, v, &s, 0, r))
+                return push(vt, r) ? PROG_OK : PROG_STACK;
+        }
         if (vt == VtVec3 && contig(v) && !s.spilled) {
             Val r;
             if (block_op(U_SCALEV3, 3, v, &s, 0, r)) return push(vt, r) ? PROG_OK : PROG_STACK;
@@ -376,7 +412,42 @@ class Lowerer {
         return push(VtFloat, r) ? PROG_OK : PROG_STACK;
     }
 
-    int binary_f(UOp op) { return cwise(VtFloat, op); }
+    // Float binary op; a pending constant operand becomes a parameter operand (the
+    // commutative add and multiply take it on either side: IEEE + and * are commutative).
+    int binary_f(UOp op) {
+        if (!can_pop(VtFloat, 2)) return PROG_STACK;
+        Val b = pop_raw(VtFloat), a = pop_raw(VtFloat);
+        if (a.konst >= 0 && b.konst >= 0 && !materialize(a)) return PROG_REGS;
+        if (a.konst < 0 && b.konst < 0) {
+            push_back_raw(a);
+            push_back_raw(b);
+            return cwise(VtFloat, op);
+        }
+        const bool kb = b.konst >= 0;
+        const Val& v = kb ? a : b;  // the register operand
+        const uint32_t x = (uint32_t)(kb ? b.konst : a.konst);
+        UOp k;
+        switch (op) {
+            case U_ADD: k = U_ADDK; break;
+            case U_MUL: k = U_MULK; break;
+            case U_SUB: k = kb ? U_SUBK : U_KSUB; break;
+            default: k = kb ? U_DIVK : U_KDIV; break;
+        }
+        Val r;
+        if (!block_op(k, 1, v, nullptr, x, r)) return PROG_REGS;
+        return push(VtFloat, r) ? PROG_OK : PROG_STACK;
+    }
+    // Put a popped value back unchanged (same age).
+    void push_back_raw(const Val& v) { stk_[sidx(VtFloat)].push_back(v); }
+    bool materialize(Val& v) {
+        if (v.konst < 0) return true;
+        const int r = alloc();
+        if (r < 0) return false;
+        out_.code.push_back(make_uinstr(U_IMM, (uint32_t)r, 0, 0, (uint32_t)v.konst));
+        v.reg[0] = (uint8_t)r;
+        v.konst = -1;
+        return true;
+    }
 
     // generic "consume operands, produce one value" helper
     int produce(int out_vt, const std::vector<MI>& seq, const std::vector<int>& owned,
@@ -516,6 +587,10 @@ class Lowerer {
             case MulM2: {  // out[r+2c] = a[r]*b[2c] + a[r+2]*b[2c+1]
                 if (!can_pop(VtMat2, 2)) return PROG_STACK;
                 Val b = pop(VtMat2), a = pop(VtMat2);
+                if (contig(a) && contig(b)) {
+                    Val r;
+                    if (block_op(U_MULM2, 4, a, &b, 0, r)) return push(VtMat2, r) ? PROG_OK : PROG_STACK;
+                }
                 std::vector<MI> seq;
                 for (int c = 0; c < 2; ++c)
                     for (int r = 0; r < 2; ++r)
@@ -552,6 +627,10 @@ class Lowerer {
                 if (!can_pop(VtMat2)) return PROG_STACK;
                 if (dual_) return PROG_OK;  // dual inverse: stack unchanged
                 Val m = pop(VtMat2);
+                if (contig(m)) {
+                    Val r;
+                    if (block_op(U_INVM2, 4, m, nullptr, 0, r)) return push(VtMat2, r) ? PROG_OK : PROG_STACK;
+                }
                 auto M = [&](int k) { return P(m.reg[k]); };
                 // det, inv_d, then blended entries (linalg/eval.rs:316-329)
                 return produce(VtMat2,
@@ -580,6 +659,10 @@ class Lowerer {
             case MulM3V3: {
                 if (!can_pop(VtMat3) || !can_pop(VtVec3)) return PROG_STACK;
                 Val m = pop(VtMat3), v = pop(VtVec3);
+                if (contig(m) && contig(v)) {
+                    Val r;
+                    if (block_op(U_MULM3V3, 3, m, &v, 0, r)) return push(VtVec3, r) ? PROG_OK : PROG_STACK;
+                }
                 std::vector<MI> seq;
                 std::vector<Src> outs;
                 for (int r = 0; r < 3; ++r) {
