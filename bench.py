@@ -119,6 +119,60 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": n * 4}), flush=True)
 
 
+def _timed(ctx, fn):
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    ctx.synchronize()
+    return out, time.perf_counter() - t0
+
+
+def run_configs(ctx, log):
+    """The matrix workloads with constant fitting: evaluation, gradients and the fits."""
+    out = {}
+    log("[bench] Mat3 workload: 16384 programs x 20000 points, gradients, L-BFGS 20 iterations m=5 FP64")
+    X, y, t = W.data_c2(20_000, seed=1002)
+    progs = W.gen_c2(16384, seed=2002)
+    ds = api.Dataset.new(ctx, X, y, t)
+    P = api.Programs(ctx, progs)
+    ok = P.status(ds) == 0
+    ne = node_evals([p for p, g in zip(progs, ok) if g], len(y))
+    api.compute_mse(ctx, P, ds, allow_rejected=True)
+    _, te = _timed(ctx, lambda: api.compute_mse(ctx, P, ds, allow_rejected=True))
+    api.compute_mse_with_gradient(ctx, P, ds, allow_rejected=True)
+    _, tg = _timed(ctx, lambda: api.compute_mse_with_gradient(ctx, P, ds, allow_rejected=True))
+    fit, tl = _timed(ctx, lambda: api.fit_lbfgs(ctx, P, ds, iterations=20, history=5, mode=api.LBFGS_FP64,
+                                                mask=ok.astype(np.uint8)))
+    out["mat3_lbfgs"] = {"programs": len(progs), "valid_programs": int(ok.sum()), "points": len(y),
+                         "eval_ms": te * 1e3, "eval_node_evals_per_s": ne / te,
+                         "mse_and_gradient_ms": tg * 1e3, "lbfgs_ms": tl * 1e3,
+                         "lbfgs": {"iterations": 20, "history": 5, "mode": "fp64"},
+                         "fitted": int(np.isfinite(fit).sum())}
+    P.close()
+    ds.close()
+    log("[bench] Mat2 workload: 4096 programs x 50000 points, CMA-ES lambda 32 x 30 iterations")
+    X, y, t = W.data_c3(50_000, seed=1003)
+    progs = W.gen_c3(4096, seed=2003)
+    ds = api.Dataset.new(ctx, X, y, t)
+    P = api.Programs(ctx, progs)
+    ok = P.status(ds) == 0
+    ne = node_evals([p for p, g in zip(progs, ok) if g], len(y))
+    api.compute_mse(ctx, P, ds, allow_rejected=True)
+    _, te = _timed(ctx, lambda: api.compute_mse(ctx, P, ds, allow_rejected=True))
+    (fit, its), tc = _timed(ctx, lambda: api.fit_cmaes(ctx, P, ds, iterations=30, lam=32, mu=16, seed=7,
+                                                       mask=ok.astype(np.uint8)))
+    cand = int(ok.sum()) * 32
+    out["mat2_cmaes"] = {"programs": len(progs), "valid_programs": int(ok.sum()), "points": len(y),
+                         "eval_ms": te * 1e3, "eval_node_evals_per_s": ne / te,
+                         "cmaes_ms": tc * 1e3, "cmaes": {"lambda": 32, "mu": 16, "iterations": 30},
+                         "candidate_evals_per_iteration": cand,
+                         "candidate_node_evals_per_s": ne * 32 * float(np.mean(its[ok])) / tc if ok.any() else None,
+                         "mean_iterations_run": float(np.mean(its[ok])) if ok.any() else None}
+    P.close()
+    ds.close()
+    return out
+
+
 def load_traffic():
     try:
         with open(TRAFFIC_FILE) as f:
@@ -141,6 +195,7 @@ def main():
     ap.add_argument("--fp64-programs", type=int, default=64)
     ap.add_argument("--copy-probe", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving job mix")
+    ap.add_argument("--no-configs", action="store_true", help="skip the matrix / fitting workloads")
     ap.add_argument("--serve-jobs", type=int, default=64)
     ap.add_argument("--serve-gens", type=int, default=3)
     ap.add_argument("--serve-rate", type=float, default=4.0)
@@ -196,6 +251,10 @@ def main():
         plain = {"value": node_evals(p0, 400) / sec0, "unit": "node-evals/s",
                  "workload": "1000 scalar programs x 400 points, one launch per population",
                  "ms_per_step": sec0 * 1e3, "kernel_ms": kms0}
+
+    per_config = None
+    if not args.no_configs:
+        per_config = run_configs(ctx, log)
 
     serving = None
     if not args.no_serve:
@@ -277,6 +336,7 @@ def main():
                   for m, v in runs.items()},
         "accuracy": accuracy or None,
         "plain_gpu_eval": plain,
+        "fit_workloads": per_config,
         "serving": serving,
         "batched_over_plain": value / plain["value"] if plain else None,
         "roofline": {"bound": "mfma", "kind": "fp32 vector ALU (no MFMA: not a contraction)",

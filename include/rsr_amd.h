@@ -136,7 +136,7 @@ RSR_API int rsr_programs_create(rsr_context* ctx, int n_programs, const uint8_t*
                                 const int64_t* value_offsets, rsr_programs** out);
 RSR_API int rsr_programs_count(const rsr_programs* progs, int* n_programs, int64_t* n_values);
 /* Constants in the create-time layout (what the optimizers write back,
- * lbfgs.rs:219-229, cmaes.rs:181-191). */
+ * lbfgs.rs:154-164, cmaes.rs:181-191). */
 RSR_API int rsr_programs_get_constants(const rsr_programs* progs, float* const_values_out);
 RSR_API int rsr_programs_set_constants(rsr_programs* progs, const float* const_values);
 /* Lowering status of every program against a dataset's feature count. */
@@ -167,7 +167,7 @@ RSR_API int rsr_eval_points_dual(rsr_context* ctx, rsr_programs* progs, int prog
 /* ---- constant fitting (src/engine/optimize/) ----
  * Fitted constants are written back into the program batch (device and host copies;
  * read them with rsr_programs_get_constants), like the reference's write-back into
- * Individual.constants (lbfgs.rs:219-229, cmaes.rs:181-191).  fitness_out[i] is the
+ * Individual.constants (lbfgs.rs:154-164, cmaes.rs:181-191).  fitness_out[i] is the
  * optimizer's final MSE, NaN for programs not fitted (mask[i] == 0, rejected, or no
  * parameters).  mask may be NULL (fit every program). */
 #define RSR_LBFGS_REFERENCE 0  /* run_lbfgs (lbfgs.rs:9-165): f32, M = history, its quirks */
@@ -207,6 +207,23 @@ RSR_API int rsr_optimize_constants(rsr_context* ctx, rsr_programs* progs, const 
  * candidate, dimension): Philox4x32-10 -> two 24-bit uniforms -> Box-Muller as
  * cmaes.rs:13-17. */
 RSR_API float rsr_cmaes_normal(uint64_t seed, uint32_t program, uint32_t iteration, uint32_t candidate, uint32_t dim);
+
+/* ---- the search loop's scoring and selection (host code, no device needed) ----
+ * Individual::complexity (individual.rs:108-110): the sum of node weights (node.rs:21-33,
+ * basic/mod.rs:156-163, linalg/mod.rs:434-478) of each program in the wire format
+ * (ops, code_offsets[n+1] as for rsr_programs_create); NegF weighs 1. */
+RSR_API int rsr_complexity(int n_programs, const uint8_t* ops, const int64_t* code_offsets, int32_t* complexity_out);
+/* Fitness of island.rs:241-255: mse + complexity * parsimony (reference default 5e-6,
+ * config.rs:48), FLT_MAX for a non-finite MSE. */
+RSR_API int rsr_fitness(int n, const float* mse, const int32_t* complexity, float parsimony, float* fitness_out);
+/* assign_rank_and_crowding_distance (operators/selection.rs:39-134): non-dominated fronts
+ * on (fitness, age), crowding distance over fitness then age. */
+RSR_API int rsr_assign_rank_and_crowding(int n, const float* fitness, const uint64_t* age, uint32_t* rank_out,
+                                         float* crowding_out);
+/* tournament_selection_pareto (selection.rs:13-26) given the k drawn indices (the caller's
+ * random stream): the winner under nsga2_compare (lower rank, then larger crowding). */
+RSR_API int rsr_tournament(int n, const uint32_t* rank, const float* crowding, int k, const uint32_t* draws,
+                           uint32_t* winner_out);
 
 /* ---------------------------------------------------------------------------------
  * Serving (the "async submit/poll" of the boundary; the reference's callers are rayon

@@ -241,6 +241,11 @@ def tournament(cands: Sequence[int], rank: np.ndarray, crowd: np.ndarray) -> int
                                 _p(np.ascontiguousarray(crowd, np.float32), _f32p))
 
 
+def fitness(mse: float, complexity: int, parsimony: float) -> float:
+    """island.rs:241-255: mse + complexity * parsimony, f32::MAX when mse is not finite."""
+    return lib().orc_fitness(mse, complexity, parsimony)
+
+
 def sinf(x: float) -> float:
     return lib().orc_sinf(x)
 

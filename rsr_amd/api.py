@@ -412,6 +412,59 @@ def optimize_constants(ctx: Context, progs: Programs, ds: Dataset, iterations: i
     return mse[: progs.n], meth[: progs.n]
 
 
+# ---------------------------------------------------------------------------------
+# scoring and selection (host functions; selection.rs, individual.rs:108-110, island.rs:241-255)
+# ---------------------------------------------------------------------------------
+def complexity(progs: Sequence[Program]) -> np.ndarray:
+    """Individual::complexity of every program (sum of node weights)."""
+    l = lib()
+    l.rsr_complexity.restype = C.c_int
+    l.rsr_complexity.argtypes = [C.c_int, _u8p, _i64p, _i32p]
+    ops, _, co, *_ = pack_programs(progs)
+    out = np.zeros(max(len(progs), 1), np.int32)
+    _check(l.rsr_complexity(len(progs), _ptr(ops, _u8p), _ptr(co, _i64p), _ptr(out, _i32p)))
+    return out[: len(progs)]
+
+
+def fitness(mse, complexity_, parsimony: float = 5e-6) -> np.ndarray:
+    l = lib()
+    l.rsr_fitness.restype = C.c_int
+    l.rsr_fitness.argtypes = [C.c_int, _f32p, _i32p, C.c_float, _f32p]
+    m = np.ascontiguousarray(mse, np.float32)
+    c = np.ascontiguousarray(complexity_, np.int32)
+    out = np.zeros(max(len(m), 1), np.float32)
+    _check(l.rsr_fitness(len(m), _ptr(m, _f32p), _ptr(c, _i32p), parsimony, _ptr(out, _f32p)))
+    return out[: len(m)]
+
+
+def assign_rank_and_crowding(fitness_, age):
+    l = lib()
+    l.rsr_assign_rank_and_crowding.restype = C.c_int
+    l.rsr_assign_rank_and_crowding.argtypes = [C.c_int, _f32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), _f32p]
+    f = np.ascontiguousarray(fitness_, np.float32)
+    a = np.ascontiguousarray(age, np.uint64)
+    n = len(f)
+    r = np.zeros(max(n, 1), np.uint32)
+    c = np.zeros(max(n, 1), np.float32)
+    _check(l.rsr_assign_rank_and_crowding(n, _ptr(f, _f32p), a.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          r.ctypes.data_as(C.POINTER(C.c_uint32)), _ptr(c, _f32p)))
+    return r[:n], c[:n]
+
+
+def tournament(draws, rank, crowding) -> int:
+    l = lib()
+    l.rsr_tournament.restype = C.c_int
+    l.rsr_tournament.argtypes = [C.c_int, C.POINTER(C.c_uint32), _f32p, C.c_int, C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32)]
+    r = np.ascontiguousarray(rank, np.uint32)
+    c = np.ascontiguousarray(crowding, np.float32)
+    d = np.ascontiguousarray(draws, np.uint32)
+    w = C.c_uint32()
+    _check(l.rsr_tournament(len(r), r.ctypes.data_as(C.POINTER(C.c_uint32)), _ptr(c, _f32p), len(d),
+                            d.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(w)))
+    return int(w.value)
+
+
 def cmaes_normal(seed: int, program: int, iteration: int, candidate: int, dim: int) -> float:
     """The normal the CMA-ES sampler draws (host restatement of the device function)."""
     return _bind_fit().rsr_cmaes_normal(seed, program, iteration, candidate, dim)

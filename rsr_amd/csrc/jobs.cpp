@@ -64,6 +64,16 @@ int download_params(rsr_programs* p) {
     return RSR_OK;
 }
 
+// The lowering's cost-sorted scheduling order restricted to the programs with on[i] != 0,
+// uploaded into the job's buffers: steps over a subset launch work for the subset only.
+int subset_order(JobBuffers& jb, const Lowering* L, const std::vector<uint8_t>& on, int32_t** d_order, int* count) {
+    std::vector<int32_t> o;
+    for (int32_t i : L->order)
+        if (on[i]) o.push_back(i);
+    *count = (int)o.size();
+    return jb.put(o, d_order);
+}
+
 EvalStep mse_step(const rsr_programs* p, const Lowering* L, const rsr_dataset* ds, const uint8_t* active,
                   float* d_mse) {
     EvalStep st;
@@ -96,13 +106,20 @@ class EvalJob final : public Job {
     JobBuffers jb;
     float* d_mse = nullptr;
     uint8_t* d_act = nullptr;
+    int32_t* d_order = nullptr;  // subset order when not every program is evaluated
+    int n_order = 0;
     float* out = nullptr;
     std::vector<uint8_t> on;
     int n_bad = 0;
     uint64_t nodes = 0;
 
     int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>&) override {
-        mse.push_back(mse_step(p, L, ds, d_act, d_mse));
+        EvalStep st = mse_step(p, L, ds, d_act, d_mse);
+        if (d_order) {
+            st.P.order = d_order;
+            st.P.n_progs = n_order;
+        }
+        mse.push_back(st);
         node_evals += nodes * (uint64_t)ds->num_samples;
         return RSR_OK;
     }
@@ -137,6 +154,8 @@ class GradJob final : public Job {
     float *d_mse = nullptr, *d_grad = nullptr;
     uint8_t *d_act0 = nullptr, *d_act1 = nullptr;
     bool any_zero = false, any_pos = false;
+    int32_t *d_order0 = nullptr, *d_order1 = nullptr;
+    int n_order0 = 0, n_order1 = 0;
     float *mse_out = nullptr, *grad_out = nullptr;
     std::vector<uint8_t> on;
     int n_bad = 0;
@@ -144,8 +163,18 @@ class GradJob final : public Job {
 
     int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>& grad) override {
         // zero-parameter programs are scored by compute_mse on the primal lowering (evaluator.rs:187-189)
-        if (any_zero) mse.push_back(mse_step(p, Lp, ds, d_act0, d_mse));
-        if (any_pos) grad.push_back(grad_step(p, Ld, ds, d_act1, d_mse, d_grad));
+        if (any_zero) {
+            EvalStep st = mse_step(p, Lp, ds, d_act0, d_mse);
+            st.P.order = d_order0;
+            st.P.n_progs = n_order0;
+            mse.push_back(st);
+        }
+        if (any_pos) {
+            EvalStep st = grad_step(p, Ld, ds, d_act1, d_mse, d_grad);
+            st.P.order = d_order1;
+            st.P.n_progs = n_order1;
+            grad.push_back(st);
+        }
         node_evals += nodes * (uint64_t)ds->num_samples;
         return RSR_OK;
     }
@@ -189,6 +218,7 @@ class LbfgsJob final : public Job {
     std::vector<int> prog;
     uint8_t *act_fit = nullptr, *act_ls = nullptr, *act_grad = nullptr;
     float *d_mse = nullptr, *d_grad = nullptr, *d_fit = nullptr;
+    int32_t *order_p = nullptr, *order_d = nullptr;  // scheduling orders over the fitted programs
     float* fitness_out = nullptr;
     int iters = 0, it = 0, ls_round = 0, working = 0;
     Stage stage = INIT;
@@ -197,10 +227,17 @@ class LbfgsJob final : public Job {
     int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>& grad) override {
         const uint64_t n_eval = stage == INIT ? prog.size() : (uint64_t)working;
         node_evals += n_eval * mean_nodes * (uint64_t)ds->num_samples;
-        if (stage == SEARCH)
-            mse.push_back(mse_step(p, Lp, ds, act_ls, d_mse));
-        else
-            grad.push_back(grad_step(p, Ld, ds, stage == INIT ? act_fit : act_grad, d_mse, d_grad));
+        if (stage == SEARCH) {
+            EvalStep st = mse_step(p, Lp, ds, act_ls, d_mse);
+            st.P.order = order_p;
+            st.P.n_progs = (int)prog.size();
+            mse.push_back(st);
+        } else {
+            EvalStep st = grad_step(p, Ld, ds, stage == INIT ? act_fit : act_grad, d_mse, d_grad);
+            st.P.order = order_d;
+            st.P.n_progs = (int)prog.size();
+            grad.push_back(st);
+        }
         return RSR_OK;
     }
     int post() override {
@@ -282,6 +319,7 @@ class CmaesJob final : public Job {
     CmaDev st;
     std::vector<int> prog;
     uint8_t* act_fit = nullptr;
+    int32_t* order_fit = nullptr;
     float *d_mse0 = nullptr, *d_cmse = nullptr, *d_cparams = nullptr, *d_fit = nullptr;
     int* d_iters = nullptr;
     DevPrograms CP{};
@@ -293,7 +331,10 @@ class CmaesJob final : public Job {
 
     int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>&) override {
         if (stage == INIT) {
-            mse.push_back(mse_step(p, Lp, ds, act_fit, d_mse0));
+            EvalStep st = mse_step(p, Lp, ds, act_fit, d_mse0);
+            st.P.order = order_fit;
+            st.P.n_progs = (int)prog.size();
+            mse.push_back(st);
             node_evals += prog.size() * mean_nodes * (uint64_t)ds->num_samples;
             return RSR_OK;
         }
@@ -522,7 +563,8 @@ int make_eval_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, cons
         if (j->on[i]) j->nodes += prog_nodes(p, i);
     }
     if ((rc = j->jb.get((size_t)p->n, &j->d_mse))) return rc;
-    if (!all && (rc = j->jb.put(j->on, &j->d_act))) return rc;
+    if (!all && ((rc = j->jb.put(j->on, &j->d_act)) || (rc = subset_order(j->jb, j->L, j->on, &j->d_order, &j->n_order))))
+        return rc;
     *out = std::move(j);
     return RSR_OK;
 }
@@ -561,7 +603,9 @@ int make_grad_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, cons
         }
     }
     if ((rc = j->jb.get((size_t)p->n, &j->d_mse)) || (rc = j->jb.get((size_t)p->n * 32, &j->d_grad)) ||
-        (rc = j->jb.put(act0, &j->d_act0)) || (rc = j->jb.put(act1, &j->d_act1)))
+        (rc = j->jb.put(act0, &j->d_act0)) || (rc = j->jb.put(act1, &j->d_act1)) ||
+        (rc = subset_order(j->jb, Lp, act0, &j->d_order0, &j->n_order0)) ||
+        (rc = subset_order(j->jb, Ld, act1, &j->d_order1, &j->n_order1)))
         return rc;
     *out = std::move(j);
     return RSR_OK;
@@ -625,9 +669,12 @@ int make_lbfgs_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
         return rc;
     std::vector<uint8_t> fitmask(p->n, 0);
     for (int i : j->prog) fitmask[i] = 1;
+    int cnt;
     if ((rc = sc.put(fitmask, &j->act_fit)) || (rc = sc.get((size_t)p->n, &j->act_ls)) ||
         (rc = sc.get((size_t)p->n, &j->act_grad)) || (rc = sc.get((size_t)p->n, &j->d_mse)) ||
-        (rc = sc.get((size_t)p->n * 32, &j->d_grad)) || (rc = sc.get((size_t)n, &j->d_fit)))
+        (rc = sc.get((size_t)p->n * 32, &j->d_grad)) || (rc = sc.get((size_t)n, &j->d_fit)) ||
+        (rc = subset_order(sc, j->Lp, fitmask, &j->order_p, &cnt)) ||
+        (rc = subset_order(sc, j->Ld, fitmask, &j->order_d, &cnt)))
         return rc;
     *out = std::move(j);
     return RSR_OK;
@@ -711,7 +758,10 @@ int make_cmaes_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
     CmaDev& st = j->st;
     std::vector<uint8_t> fitmask(p->n, 0);
     for (int i : prog) fitmask[i] = 1;
-    if ((rc = sc.put(fitmask, &j->act_fit)) || (rc = sc.get((size_t)p->n, &j->d_mse0))) return rc;
+    int cnt;
+    if ((rc = sc.put(fitmask, &j->act_fit)) || (rc = sc.get((size_t)p->n, &j->d_mse0)) ||
+        (rc = subset_order(sc, Lp, fitmask, &j->order_fit, &cnt)))
+        return rc;
     st.n = n;
     st.lambda = lambda;
     st.mu = mu;

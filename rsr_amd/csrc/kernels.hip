@@ -109,6 +109,13 @@ __global__ __launch_bounds__(256) void k_eval_mse(const SegTable T, int spill_sl
     if (lt >= S.n_tiles) return;
     if (S.live != nullptr && *S.live == 0) return;
     const int g = (int)(w - lt * S.n_groups);
+    const int first = g * S.group;
+    const int last = min(first + S.group, S.n_progs);
+    if (S.active != nullptr) {  // nothing to evaluate in this group: skip the feature loads too
+        bool any = false;
+        for (int sp = first; sp < last; ++sp) any = any || S.active[uni((uint32_t)S.order[sp])] != 0;
+        if (!any) return;  // the finalize and fold kernels never read inactive programs' partials
+    }
     float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * PPL * 64;
     const int64_t base = (S.tile0 + lt) * (64 * PPL);
 
@@ -123,8 +130,6 @@ __global__ __launch_bounds__(256) void k_eval_mse(const SegTable T, int spill_sl
         y[p] = S.target[pt];
         valid[p] = pt < S.count8;
     }
-    const int first = g * S.group;
-    const int last = min(first + S.group, S.n_progs);
     for (int sp = first; sp < last; ++sp) {
         const int prog = (int)uni((uint32_t)S.order[sp]);
         float s = 0.0f;
