@@ -193,6 +193,7 @@ def main():
     ap.add_argument("--summation", choices=["auto", "kahan", "reference"], default="auto")
     ap.add_argument("--cpu-programs", type=int, default=8192)
     ap.add_argument("--fp64-programs", type=int, default=64)
+    ap.add_argument("--cpu-programs-1t", type=int, default=512)
     ap.add_argument("--copy-probe", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving job mix")
     ap.add_argument("--no-configs", action="store_true", help="skip the matrix / fitting workloads")
@@ -290,6 +291,17 @@ def main():
         cpu = {"value": cne / cdt, "unit": "node-evals/s", "cores": threads, "kind": "port",
                "sample": f"{len(sample)} of the 8192 programs x 100000 points "
                          f"({cne:.3g} node-evals, {cdt:.1f} s wall)"}
+        one = sample[:args.cpu_programs_1t]
+        log(f"[bench] cpu baseline, 1 thread: {len(one)} programs")
+        _, dt1 = cpu_run(O.compute_mse, one, ods, 1)
+        ne1 = node_evals(one, len(y))
+        cpu["single_thread"] = {"value": ne1 / dt1, "cores": 1,
+                                "sample": f"first {len(one)} programs x 100000 points ({dt1:.1f} s wall)"}
+        try:
+            cpu["host_cpu"] = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                                   if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
         k = min(args.fp64_programs, len(sample))
         log(f"[bench] fp64 oracle on {k} programs")
         idx = np.random.default_rng(0).choice(len(sample), k, replace=False)
@@ -312,8 +324,8 @@ def main():
     bytes_launch = (X.shape[1] + 1) * 4 * count8 + len(progs) * 4  # compulsory: features + targets + outputs
     traffic = load_traffic()
     traffic_bytes = None
-    if traffic and traffic.get("summation") == head:
-        traffic_bytes = traffic.get("hbm_bytes_per_launch")
+    if traffic:  # HBM bytes per launch of the same kernel and workload, from FETCH_SIZE / WRITE_SIZE runs
+        traffic_bytes = traffic.get("modes", {}).get(head, {}).get("hbm_bytes_per_launch")
 
     out = {
         "metric": "node-evals/s + p99 gen latency, served vs plain GPU eval, MSE rel err \u22641e-5",
