@@ -100,6 +100,12 @@ RSR_API int rsr_context_synchronize(rsr_context* ctx);
 #define RSR_SUM_KAHAN 0
 #define RSR_SUM_REFERENCE 1
 RSR_API int rsr_context_set_summation(rsr_context* ctx, int mode);
+/* Graph mode (default off): rsr_eval_mse captures the evaluation of a program batch over a
+ * dataset (its launches and the copy of the results) into a hipGraph on first use and
+ * replays it on later calls with the same dataset, summation and tuning; new constants
+ * (rsr_programs_set_constants, the fitters) are picked up.  Evaluations that need more than
+ * one launch round run eagerly. */
+RSR_API int rsr_context_set_graphs(rsr_context* ctx, int enable);
 /* Upper bound (bytes, >= 4096) of the device arena that holds the per-tile Kahan partials
  * (or, in reference summation, the per-point terms) of one launch; larger evaluations are
  * split into tile ranges with the summation state carried between them, which gives the

@@ -48,6 +48,7 @@ SIGNATURES = {
     "rsr_context_synchronize": (C.c_int, [_P]),
     "rsr_context_set_summation": (C.c_int, [_P, C.c_int]),
     "rsr_context_set_partials_limit": (C.c_int, [_P, C.c_uint64]),
+    "rsr_context_set_graphs": (C.c_int, [_P, C.c_int]),
     "rsr_dataset_create": (C.c_int, [_P, _f32p, _f32p, C.c_int64, _u8p, C.c_int, C.c_int, _PP]),
     "rsr_dataset_create_aosoa": (C.c_int, [_P, _f32p, _f32p, C.c_int, C.c_int64, C.c_int64, _PP]),
     "rsr_dataset_subset": (C.c_int, [_P, _P, C.c_int64, _PP]),
@@ -144,6 +145,10 @@ class Context:
     def set_summation(self, reference_order: bool):
         """False: pairwise + Kahan (default); True: the reference's 8-lane order (bit-identical)."""
         _check(lib().rsr_context_set_summation(self.h, 1 if reference_order else 0))
+
+    def set_graphs(self, enable: bool):
+        """compute_mse replays a captured hipGraph per (programs, dataset, settings)."""
+        _check(lib().rsr_context_set_graphs(self.h, 1 if enable else 0))
 
     def set_partials_limit(self, nbytes: int):
         """Bound the Kahan-partials arena of one launch (larger evaluations split into tile ranges)."""

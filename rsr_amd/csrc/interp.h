@@ -6,9 +6,9 @@
 // dispatched by a uniform branch tree: no lane ever diverges.  Register operands are
 // wave-uniform indices into a per-lane register file held in VGPRs
 // (ext_vector_type(32) indexed by an SGPR compiles to s_set_gpr_idx_on + v_mov, no
-// scratch).  Every iteration reads its operands inside the switch and writes exactly
-// one register component at a single point after it, which keeps the register file in
-// place (no vector copies).
+// scratch).  Every iteration reads operands a and b before the switch, computes inside
+// it, and writes result component 0 after it (block ops write their further components
+// inside the switch), which keeps the register file in place (no vector copies).
 //
 // The arithmetic of every micro-op is the reference formula step by step (see isa.h);
 // the dual variant carries T tangent channels and applies the forward-mode rules of
@@ -21,6 +21,13 @@
 #include "devmath.h"
 
 namespace rsr {
+
+// Program constants are read through the constant address space: the pointer and the
+// index are wave-uniform, so the read is one s_load (waited on with lgkmcnt) instead of a
+// vector load whose vmcnt(0) wait would also drain the lane's outstanding dataset loads.
+// The buffer is only written between launches, never during one.
+typedef const __attribute__((address_space(4))) float* kptr;
+__device__ __forceinline__ float kld(const float* p, uint32_t i) { return ((kptr)p)[i]; }
 
 typedef float v32f __attribute__((ext_vector_type(32)));
 
@@ -82,16 +89,16 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
         const Uop u = nx;
         nx = load_uop(code, pc + 1);  // prefetch
         const uint32_t op = u.w0 & 255u, x = u.w0 >> 8;
-        const uint32_t d = u.d, a = u.a, b = u.b;
-        __builtin_assume(d < 32u);
-        __builtin_assume(a < 32u);
-        __builtin_assume(b < 32u);
+        // The masks (no-ops on valid code) give instruction selection the known-zero high
+        // bits it needs to fold the +1, +2, ... of block components into the base register
+        // of one indexed-move region instead of computing a new index per component.
+        const uint32_t d = u.d & 31u, a = u.a & 31u, b = u.b & 31u;
         float xa[PPL], xb[PPL], res[PPL];
         sfor<PPL>([&](auto P) { xa[P()] = fget<P(), PPL>(R)[a]; });
         sfor<PPL>([&](auto P) { xb[P()] = fget<P(), PPL>(R)[b]; });
         switch (op) {
             case U_IMM: {
-                const float v = params[x];
+                const float v = kld(params, x);
                 sfor<PPL>([&](auto P) { res[P()] = v; });
                 break;
             }
@@ -196,7 +203,7 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
 #undef RSR_V3CASE
 #define RSR_KCASE(OPC, EXPR)                               \
     case OPC: {                                            \
-        const float K = params[x];                         \
+        const float K = kld(params, x);                         \
         sfor<PPL>([&](auto P) {                            \
             const float A = xa[P()];                       \
             res[P()] = (EXPR);                             \
@@ -213,7 +220,7 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             case U_SCALEV3K: {
                 __builtin_assume(a < 30u);
                 __builtin_assume(d < 30u);
-                const float K = params[x];
+                const float K = kld(params, x);
                 float r1[PPL], r2[PPL];
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
@@ -256,7 +263,7 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             case U_SCALEM3K: {
                 __builtin_assume(a < 23u);
                 __builtin_assume(d < 23u);
-                const float K = params[x];
+                const float K = kld(params, x);
                 const bool kk = op == U_SCALEM3K;
                 // per point: all nine reads, then the writes (files of different points are disjoint)
                 sfor<PPL>([&](auto P) {
@@ -281,7 +288,7 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
                 __builtin_assume(a < 29u);
                 __builtin_assume(b < 29u);
                 __builtin_assume(d < 29u);
-                const float K = params[x];
+                const float K = kld(params, x);
                 sfor<PPL>([&](auto P) {
                     v32f t = fget<P(), PPL>(R);
                     const float A0 = xa[P()], A1 = t[a + 1], A2 = t[a + 2], A3 = t[a + 3];
@@ -321,7 +328,7 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             }
             case U_IMMV3: {
                 __builtin_assume(d < 30u);
-                const float v0 = params[x], v1 = params[x + 1], v2 = params[x + 2];
+                const float v0 = kld(params, x), v1 = kld(params, x + 1), v2 = kld(params, x + 2);
                 sfor<PPL>([&](auto P) {
                     res[P()] = v0;
                     v32f t = fget<P(), PPL>(R);
@@ -381,10 +388,10 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
         const Uop u = nx;
         nx = load_uop(code, pc + 1);
         const uint32_t op = u.w0 & 255u, x = u.w0 >> 8;
-        const uint32_t d = u.d, a = u.a, b = u.b;
-        __builtin_assume(d < 32u);
-        __builtin_assume(a < 32u);
-        __builtin_assume(b < 32u);
+        // The masks (no-ops on valid code) give instruction selection the known-zero high
+        // bits it needs to fold the +1, +2, ... of block components into the base register
+        // of one indexed-move region instead of computing a new index per component.
+        const uint32_t d = u.d & 31u, a = u.a & 31u, b = u.b & 31u;
         float va, vb, ga[T], gb[T];
         {
             const v32f Vv = G.v;
@@ -397,7 +404,7 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
         float rg[T];
         switch (op) {
             case U_IMM:
-                rv = params[x];
+                rv = kld(params, x);
                 RSR_TAN(rg[j] = (x == k0 + (uint32_t)j) ? 1.0f : 0.0f);
                 break;
             case U_ADD:
@@ -522,7 +529,7 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                 break;
             }
             case U_ADDK: case U_SUBK: case U_KSUB: case U_MULK: case U_DIVK: case U_KDIV: {
-                const float K = params[x];
+                const float K = kld(params, x);
                 float gk[T];
                 RSR_TAN(gk[j] = (x == k0 + (uint32_t)j) ? 1.0f : 0.0f);
                 if (op == U_ADDK) {
@@ -585,7 +592,7 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                 __builtin_assume(d < 23u);
                 const v32f Vv = G.v;
                 const bool kk = op == U_SCALEM3K;
-                const float sv = kk ? params[x] : vb;
+                const float sv = kk ? kld(params, x) : vb;
                 float gs[T];
                 RSR_TAN(gs[j] = kk ? ((x == k0 + (uint32_t)j) ? 1.0f : 0.0f) : gb[j]);
                 float out[9], og[9][T];
@@ -643,7 +650,7 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                     B[q] = Vv[b + q];
                 }
                 const bool kk = op == U_SCALEV4K;
-                const float sv = kk ? params[x] : B[0];
+                const float sv = kk ? kld(params, x) : B[0];
                 float gs[T];
                 RSR_TAN(gs[j] = kk ? ((x == k0 + (uint32_t)j) ? 1.0f : 0.0f) : gb[j]);
 #pragma unroll
@@ -696,7 +703,7 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                 } else if (op == U_SCALEV3 || op == U_SCALEV3K) {
                     // the scalar: register b, or the parameter x (tangent 1 on its own channel)
                     const bool kk = op == U_SCALEV3K;
-                    const float sv = kk ? params[x] : vb;
+                    const float sv = kk ? kld(params, x) : vb;
                     float gs[T];
                     RSR_TAN(gs[j] = kk ? ((x == k0 + (uint32_t)j) ? 1.0f : 0.0f) : gb[j]);
                     rv = va * sv; v1 = a1 * sv; v2 = a2 * sv;
@@ -708,7 +715,7 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                             g1[j] = ((a2 * gb[j]) + (Gj[a + 2] * vb)) - ((va * Gj[b + 2]) + (ga[j] * b2));
                             g2[j] = ((va * Gj[b + 1]) + (ga[j] * b1)) - ((a1 * gb[j]) + (Gj[a + 1] * vb)));
                 } else {
-                    rv = params[x]; v1 = params[x + 1]; v2 = params[x + 2];
+                    rv = kld(params, x); v1 = kld(params, x + 1); v2 = kld(params, x + 2);
                     RSR_TAN(rg[j] = (x == k0 + (uint32_t)j) ? 1.0f : 0.0f;
                             g1[j] = (x + 1u == k0 + (uint32_t)j) ? 1.0f : 0.0f;
                             g2[j] = (x + 2u == k0 + (uint32_t)j) ? 1.0f : 0.0f);

@@ -1165,6 +1165,78 @@ int run_tick(rsr_context* ctx, const std::vector<Job*>& top) {
     return RSR_OK;
 }
 
+// compute_mse of every program through a captured graph (built on first use for this
+// dataset and these settings, replayed afterwards).  Returns RSR_E_UNSUPPORTED without
+// side effects when the evaluation does not fit one launch round; the caller then runs it
+// eagerly.
+int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, float* out) {
+    Lowering* L;
+    int rc = get_lowering(p, ds->num_features, false, &L);
+    if (rc) return rc;
+    const int summ = ctx->summation;
+    const bool terms = summ == RSR_SUM_REFERENCE;
+    EvalTuning t = ctx->tuning;
+    while (t.ppl > 1 && eval_mse_lds_bytes(L->spill_slots, t.ppl) > ctx->max_lds) t.ppl /= 2;
+    if (eval_mse_lds_bytes(L->spill_slots, t.ppl) > ctx->max_lds)
+        return set_error(RSR_E_UNSUPPORTED, "spill area exceeds the workgroup LDS limit");
+    EvalGraph* g = nullptr;
+    for (auto& e : p->graphs)
+        if (e->ds_id == ds->id && e->summation == summ && e->ppl == t.ppl && e->group == t.group) g = e.get();
+    if (!g) {
+        std::vector<int32_t> order;
+        for (int32_t i : L->order)
+            if (L->status[i] == PROG_OK) order.push_back(i);
+        EvalStep st = mse_step(p, L, ds, nullptr, nullptr);
+        st.P.n_progs = (int)order.size();
+        const size_t need = mse_partials_floats(st, t.ppl, terms) * sizeof(float);
+        if (need > ctx->arena_bytes) return RSR_E_UNSUPPORTED;
+        auto ng = std::make_unique<EvalGraph>();
+        ng->ds_id = ds->id;
+        ng->summation = summ;
+        ng->ppl = t.ppl;
+        ng->group = t.group;
+        if ((rc = ng->d_mse.alloc(ctx, std::max<size_t>(p->n, 1) * sizeof(float))) ||
+            (rc = ng->d_order.upload(ctx, order.data(), std::max<size_t>(order.size(), 1) * sizeof(int32_t))) ||
+            (rc = ng->arena.alloc(ctx, std::max<size_t>(need, 16))))
+            return rc;
+        HIP_TRY(hipHostMalloc((void**)&ng->host, std::max<size_t>(p->n, 1) * sizeof(float), hipHostMallocDefault),
+                "graph output");
+        HIP_TRY(hipMemsetAsync(ng->d_mse.ptr, 0, (size_t)p->n * sizeof(float), ctx->stream), "graph output clear");
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "graph set-up");  // uploads done before capture
+        st.P.order = (const int32_t*)ng->d_order.ptr;
+        st.mse_out = (float*)ng->d_mse.ptr;
+        st.partials = (float*)ng->arena.ptr;
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed), "graph capture");
+        hipError_t e = launch_mse_steps(&st, 1, t, terms, ctx->stream, nullptr);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ng->host, ng->d_mse.ptr, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost,
+                               ctx->stream);
+        hipGraph_t graph = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return hip_error(e != hipSuccess ? e : e2, "graph capture");
+        }
+        ng->graph = graph;
+        HIP_TRY(hipGraphInstantiate(&ng->exec, graph, nullptr, nullptr, 0), "graph instantiate");
+        g = ng.get();
+        p->graphs.push_back(std::move(ng));
+    }
+    HIP_TRY(hipGraphLaunch(g->exec, ctx->stream), "graph launch");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "graph replay");
+    int bad = 0;
+    for (int i = 0; i < p->n; ++i) {
+        if (L->status[i] != PROG_OK) {
+            out[i] = NAN;
+            bad++;
+        } else {
+            out[i] = g->host[i];
+        }
+    }
+    if (bad) return set_error(RSR_E_PROGRAM, "%d program(s) rejected by lowering", bad);
+    return RSR_OK;
+}
+
 int run_job(rsr_context* ctx, Job& job) {
     std::vector<Job*> one{&job};
     while (!job.done) {

@@ -101,5 +101,8 @@ int make_optimize_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, 
 int run_tick(rsr_context* ctx, const std::vector<Job*>& jobs);
 // Run one job to completion and finish it (caller holds ctx->mu).
 int run_job(rsr_context* ctx, Job& job);
+// compute_mse through a cached captured graph (caller holds ctx->mu); RSR_E_UNSUPPORTED
+// (nothing done) when the evaluation needs more than one launch round.
+int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, float* mse_out);
 
 }  // namespace rsr

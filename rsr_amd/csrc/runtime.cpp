@@ -1,6 +1,7 @@
 // runtime.cpp -- C ABI implementation: contexts, device-resident datasets and program
 // batches, lowering cache, and the evaluation entry points.
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <float.h>
 #include <math.h>
 #include <string.h>
@@ -275,6 +276,24 @@ extern "C" int rsr_context_set_summation(rsr_context* ctx, int mode) {
     if (mode != RSR_SUM_KAHAN && mode != RSR_SUM_REFERENCE) return set_error(RSR_E_INVALID_ARG, "bad summation mode");
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->summation = mode;
+    return RSR_OK;
+}
+
+uint64_t rsr::next_object_id() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1);
+}
+
+rsr::EvalGraph::~EvalGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (host) (void)hipHostFree(host);
+}
+
+extern "C" int rsr_context_set_graphs(rsr_context* ctx, int enable) {
+    if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->graphs = enable != 0;
     return RSR_OK;
 }
 
@@ -693,6 +712,10 @@ extern "C" int rsr_eval_mse(rsr_context* ctx, rsr_programs* p, const rsr_dataset
     if (p->n == 0) return RSR_OK;
     std::lock_guard<std::mutex> g(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    if (ctx->graphs) {
+        const int rc = eval_mse_graph(ctx, p, ds, mse_out);
+        if (rc != RSR_E_UNSUPPORTED) return rc;
+    }
     std::unique_ptr<Job> job;
     int rc = make_eval_job(ctx, p, ds, nullptr, mse_out, &job);
     if (rc) return rc;

@@ -46,6 +46,19 @@ struct Lowering {
 
 int set_error(int code, const char* fmt, ...);
 int hip_error(hipError_t e, const char* what);
+uint64_t next_object_id();  // process-unique, never reused
+
+// A captured evaluation of one program batch over one dataset (graph mode): the graph
+// owns every buffer its nodes touch, so later allocations cannot invalidate it.
+struct EvalGraph {
+    uint64_t ds_id = 0;
+    int summation = 0, ppl = 0, group = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DevBuf d_mse, d_order, arena;
+    float* host = nullptr;  // pinned copy of the MSEs
+    ~EvalGraph();
+};
 
 }  // namespace rsr
 
@@ -63,6 +76,7 @@ struct rsr_context {
     std::mutex mu;
     rsr::EvalTuning tuning;
     int summation = 0;  // 0: pairwise + Kahan (fast), 1: reference order (bit-identical to the CPU evaluator)
+    bool graphs = false;  // rsr_eval_mse replays a captured graph per (programs, dataset, settings)
     rsr::DevBuf scratch_partials, scratch_gpartials, scratch_state, scratch_gstate, scratch_out, scratch_out2, scratch_act;
     size_t arena_bytes = (size_t)4 << 30;  // summation partials / terms per launch round
     // per-job tick counters: device array and its pinned host mirror
@@ -80,6 +94,7 @@ struct rsr_context {
 
 struct rsr_dataset {
     rsr_context* ctx = nullptr;
+    uint64_t id = rsr::next_object_id();
     int num_features = 0;
     int64_t num_samples = 0;
     int64_t count8 = 0;
@@ -104,6 +119,7 @@ struct rsr_programs {
     std::vector<uint32_t> param_off;
     rsr::DevBuf dparams;
     std::map<int, std::unique_ptr<rsr::Lowering>> lowerings;
+    std::vector<std::unique_ptr<rsr::EvalGraph>> graphs;
 };
 
 namespace rsr {
