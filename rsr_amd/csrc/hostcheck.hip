@@ -14,16 +14,25 @@
 
 using namespace rsr;
 
-static void run_host(const std::vector<UInstr>& code, const float* params, float* R, float* G, int k,
-                     float* spill /* [slot][2] */) {
-    for (const UInstr& u : code) {
-        const uint32_t op = u.w0 & 255u, x = u.w0 >> 8, d = u.w1, a = u.w2, b = u.w3;
-        const uint32_t c = ux_c(x), e = ux_e(x);
+// One micro-op on the host register files (values R, tangents G with respect to flat
+// parameter k).
+static void run_one(uint32_t op, uint32_t x, uint32_t d, uint32_t a, uint32_t b, uint32_t c, uint32_t e,
+                    const float* params, float* R, float* G, int k, float* spill) {
+    {
         float v = 0.0f, g = 0.0f;
         switch (op) {
             case U_IMM: v = params[x]; g = ((int)x == k) ? 1.0f : 0.0f; break;
             case U_ADD: v = R[a] + R[b]; g = G[a] + G[b]; break;
             case U_SUB: v = R[a] - R[b]; g = G[a] - G[b]; break;
+            // operand-swapped forms: value code only (no tangents)
+            case U_RSUB: v = R[b] - R[a]; break;
+            case U_RDIV: v = R[b] / R[a]; break;
+            case U_MULM3V3R: {
+                float rv[3];
+                for (int i = 0; i < 3; ++i) rv[i] = ((R[b + i] * R[a]) + (R[b + 3 + i] * R[a + 1])) + (R[b + 6 + i] * R[a + 2]);
+                for (int i = 0; i < 3; ++i) R[d + i] = rv[i];
+                return;
+            }
             case U_MUL: v = R[a] * R[b]; g = (R[a] * G[b]) + (G[a] * R[b]); break;
             case U_DIV: v = R[a] / R[b]; g = ((G[a] * R[b]) - (R[a] * G[b])) / (R[b] * R[b]); break;
             case U_SQR: v = R[a] * R[a]; g = (2.0f * R[a]) * G[a]; break;
@@ -84,7 +93,7 @@ static void run_host(const std::vector<UInstr>& code, const float* params, float
                     R[d + i] = rv[i];
                     G[d + i] = rg[i];
                 }
-                continue;
+                return;
             }
             case U_SCALEM3: case U_SCALEM3K: {
                 const bool kk = op == U_SCALEM3K;
@@ -98,7 +107,7 @@ static void run_host(const std::vector<UInstr>& code, const float* params, float
                     R[d + q] = rv[q];
                     G[d + q] = rg[q];
                 }
-                continue;
+                return;
             }
             case U_ADDV4: case U_SUBV4: case U_SCALEV4: case U_SCALEV4K: case U_MULM2: case U_INVM2: {
                 float rv[4], rg[4];
@@ -131,7 +140,18 @@ static void run_host(const std::vector<UInstr>& code, const float* params, float
                     R[d + q] = rv[q];
                     G[d + q] = rg[q];
                 }
-                continue;
+                return;
+            }
+            case U_RSUBV3: case U_RCROSS3: case U_SCALEV3S: {
+                float rv[3];
+                for (int q = 0; q < 3; ++q) {
+                    const int i = (q + 1) % 3, j = (q + 2) % 3;
+                    rv[q] = op == U_RSUBV3 ? R[b + q] - R[a + q]
+                          : op == U_SCALEV3S ? R[b + q] * R[a]
+                                             : (R[b + i] * R[a + j]) - (R[b + j] * R[a + i]);
+                }
+                for (int q = 0; q < 3; ++q) R[d + q] = rv[q];
+                return;
             }
             case U_ADDV3: case U_SUBV3: case U_SCALEV3: case U_SCALEV3K: case U_IMMV3: case U_CROSS3: {
                 float rv[3], rg[3];
@@ -159,12 +179,42 @@ static void run_host(const std::vector<UInstr>& code, const float* params, float
                     R[d + q] = rv[q];
                     G[d + q] = rg[q];
                 }
-                continue;
+                return;
             }
             default: break;
         }
         R[d] = v;
         G[d] = g;
+    }
+}
+
+// R and G hold kRegs + 3 entries: registers [kRegs, kRegs + 3) stand for the
+// interpreter's accumulator, so a kFwd operand is read from there.  A kNoWb result is copied there and
+// its file registers are overwritten with NaN, so any read of the file that the
+// forwarding pass wrongly declared dead poisons the program's result.
+static void run_host(const std::vector<UInstr>& code, const float* params, float* R, float* G, int k,
+                     float* spill /* [slot][2] */) {
+    float Fv[3] = {0.0f, 0.0f, 0.0f}, Fg[3] = {0.0f, 0.0f, 0.0f};
+    for (const UInstr& u : code) {
+        const uint32_t op = u.w0 & 255u, x = u.w0 >> 8, d = u.w1 & 31u;
+        for (int q = 0; q < 3; ++q) {
+            R[kRegs + q] = Fv[q];
+            G[kRegs + q] = Fg[q];
+        }
+        const uint32_t a = (u.w2 & kFwd) ? (uint32_t)kRegs : (u.w2 & 31u);
+        const uint32_t b = u.w3 & 31u;
+        const uint32_t c = ux_c(x), e = ux_e(x);
+        const int nd = ugroups(op).nd;
+        run_one(op, x, d, a, b, c, e, params, R, G, k, spill);
+        for (int q = 0; q < nd && q < 3; ++q) {
+            Fv[q] = R[d + q];
+            Fg[q] = G[d + q];
+        }
+        if (u.w1 & kNoWb)
+            for (int q = 0; q < nd; ++q) {
+                R[d + q] = __builtin_nanf("");
+                G[d + q] = __builtin_nanf("");
+            }
     }
 }
 
@@ -186,9 +236,9 @@ extern "C" RSR_API int rsr_lower_check(const uint8_t* ops, const uint16_t* args,
     std::vector<float> params;
     flatten_params(p, params);
     params.push_back(0.0f);
-    float R[kRegs], G[kRegs], spill[2 * kMaxSpillSlots];
+    float R[kRegs + 3], G[kRegs + 3], spill[2 * kMaxSpillSlots];
     for (int64_t i = 0; i < n; ++i) {
-        for (int r = 0; r < kRegs; ++r) { R[r] = 0.0f; G[r] = 0.0f; }
+        for (int r = 0; r < kRegs + 3; ++r) { R[r] = 0.0f; G[r] = 0.0f; }
         for (int f = 0; f < num_features; ++f) R[f] = feat[(int64_t)f * n + i];
         run_host(lw.code, params.data(), R, G, dual ? active_param : -1, spill);
         if (vals_out) vals_out[i] = R[lw.out_reg];

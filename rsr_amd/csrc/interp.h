@@ -77,14 +77,20 @@ __device__ __forceinline__ Uop load_uop(const UInstr* __restrict__ code, uint32_
 
 // Primal interpreter over PPL register files (one per point of the lane).
 // `lds` is this wave's spill area: slot s, point p at lds[(s * PPL + p) * 64 + lane].
-// Operands a and b of every micro-op are read for all points before the dispatch (one
-// indexed-read group per operand), the switch only computes, and the result is written
-// back for all points after it, so each operand costs one index-mode region however
-// many points a lane holds.
+// The interpreter is an accumulator machine over the register file: A0..A2 hold the
+// last result (one or three components) across iterations.  Operand a is staged into
+// them unless it is that result (kFwd), operand b into B0..B2, both before the dispatch
+// and one indexed-read group per operand; the switch computes the new result into A;
+// the write-back after it stores A into the file unless kNoWb.  Results with four or
+// nine components write their components 1.. to the file inside the switch.  After a
+// one-component result A1 and A2 are dead (no consumer reads them), which the cases
+// state explicitly so that they hold no registers across the scalar cases.
 template <int PPL>
 __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restrict__ code, uint32_t len,
                                             const float* __restrict__ params, float* __restrict__ lds, int lane) {
     Uop nx = load_uop(code, 0);  // the code array ends with one padding entry
+    float A0[PPL], A1[PPL], A2[PPL];
+    sfor<PPL>([&](auto P) { A0[P()] = 0.0f; A1[P()] = 0.0f; A2[P()] = 0.0f; });
     for (uint32_t pc = 0; pc < len; ++pc) {
         const Uop u = nx;
         nx = load_uop(code, pc + 1);  // prefetch
@@ -93,22 +99,50 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
         // bits it needs to fold the +1, +2, ... of block components into the base register
         // of one indexed-move region instead of computing a new index per component.
         const uint32_t d = u.d & 31u, a = u.a & 31u, b = u.b & 31u;
-        float xa[PPL], xb[PPL], res[PPL];
-        sfor<PPL>([&](auto P) { xa[P()] = fget<P(), PPL>(R)[a]; });
-        sfor<PPL>([&](auto P) { xb[P()] = fget<P(), PPL>(R)[b]; });
+        float B0[PPL], B1[PPL], B2[PPL];
+        if (u.b & kVec3) {
+            sfor<PPL>([&](auto P) {
+                const v32f r = fget<P(), PPL>(R);
+                B0[P()] = r[b];
+                B1[P()] = r[b + 1];
+                B2[P()] = r[b + 2];
+            });
+        } else if (!(u.b & kNone)) {
+            sfor<PPL>([&](auto P) { B0[P()] = fget<P(), PPL>(R)[b]; });
+        }
+        auto scalar_result = [&]() {
+            sfor<PPL>([&](auto P) {
+                A1[P()] = __builtin_nondeterministic_value(A1[P()]);
+                A2[P()] = __builtin_nondeterministic_value(A2[P()]);
+            });
+        };
+        if (!(u.a & (kFwd | kNone))) {
+            if (u.a & kVec3) {
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    A0[P()] = r[a];
+                    A1[P()] = r[a + 1];
+                    A2[P()] = r[a + 2];
+                });
+            } else {
+                sfor<PPL>([&](auto P) { A0[P()] = fget<P(), PPL>(R)[a]; });
+            }
+        }
         switch (op) {
             case U_IMM: {
                 const float v = kld(params, x);
-                sfor<PPL>([&](auto P) { res[P()] = v; });
+                sfor<PPL>([&](auto P) { A0[P()] = v; });
+                scalar_result();
                 break;
             }
 #define RSR_CASE(OPC, EXPR)                                   \
     case OPC:                                                 \
         sfor<PPL>([&](auto P) {                               \
-            const float A = xa[P()], B = xb[P()];             \
+            const float A = A0[P()], B = B0[P()];             \
             (void)B;                                          \
-            res[P()] = (EXPR);                                \
+            A0[P()] = (EXPR);                                 \
         });                                                   \
+        scalar_result();                                      \
         break;
             RSR_CASE(U_ADD, A + B)
             RSR_CASE(U_SUB, A - B)
@@ -123,6 +157,8 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             RSR_CASE(U_NEG, -A)
             RSR_CASE(U_SQRTACC, __builtin_sqrtf(A + (B * B)))
             RSR_CASE(U_RCPSAFE, 1.0f / ((__builtin_fabsf(A) < 1e-9f) ? 1.0f : A))
+            RSR_CASE(U_RSUB, B - A)
+            RSR_CASE(U_RDIV, B / A)
 #undef RSR_CASE
             case U_MULADD:
             case U_MULSUB: {
@@ -130,25 +166,28 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
                 const bool sub = op == U_MULSUB;
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
-                    const float p1 = xa[P()] * xb[P()], p2 = r[c] * r[e];
-                    res[P()] = sub ? p1 - p2 : p1 + p2;
+                    const float p1 = A0[P()] * B0[P()], p2 = r[c] * r[e];
+                    A0[P()] = sub ? p1 - p2 : p1 + p2;
                 });
+                scalar_result();
                 break;
             }
             case U_ADDMUL: {
                 const uint32_t c = x & 31u;
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
-                    res[P()] = xa[P()] + (xb[P()] * r[c]);
+                    A0[P()] = A0[P()] + (B0[P()] * r[c]);
                 });
+                scalar_result();
                 break;
             }
             case U_ADD3: {
                 const uint32_t c = x & 31u;
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
-                    res[P()] = (xa[P()] + xb[P()]) + r[c];
+                    A0[P()] = (A0[P()] + B0[P()]) + r[c];
                 });
+                scalar_result();
                 break;
             }
             case U_INVSEL: {
@@ -157,57 +196,49 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
                 const bool ng = (fl & kInvNeg) != 0u;
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
-                    const float xv = ng ? -xa[P()] : xa[P()];
-                    res[P()] = (__builtin_fabsf(r[c]) < 1e-9f) ? k : xv * xb[P()];
+                    const float xv = ng ? -A0[P()] : A0[P()];
+                    A0[P()] = (__builtin_fabsf(r[c]) < 1e-9f) ? k : xv * B0[P()];
                 });
+                scalar_result();
                 break;
             }
             case U_SPILL:
-                sfor<PPL>([&](auto P) {
-                    lds[(x * PPL + P()) * 64u + lane] = xa[P()];
-                    res[P()] = xa[P()];
-                });
+                sfor<PPL>([&](auto P) { lds[(x * PPL + P()) * 64u + lane] = A0[P()]; });
+                scalar_result();
                 break;
             case U_FILL:
-                sfor<PPL>([&](auto P) { res[P()] = lds[(x * PPL + P()) * 64u + lane]; });
+                sfor<PPL>([&](auto P) { A0[P()] = lds[(x * PPL + P()) * 64u + lane]; });
+                scalar_result();
                 break;
-            // Vec3 block ops: components 1 and 2 are read here and written before the common
-            // write-back of component 0 (every read happens before the first write)
+            // Vec3 block ops: every operand component is staged, the result replaces A
 #define RSR_V3CASE(OPC, E0, E1, E2)                                             \
-    case OPC: {                                                                 \
-        __builtin_assume(a < 30u);                                              \
-        __builtin_assume(b < 30u);                                              \
-        __builtin_assume(d < 30u);                                              \
-        float r1[PPL], r2[PPL];                                                 \
+    case OPC:                                                                   \
         sfor<PPL>([&](auto P) {                                                 \
-            const v32f r = fget<P(), PPL>(R);                                   \
-            const float a0 = xa[P()], a1 = r[a + 1], a2 = r[a + 2];             \
-            const float b0 = xb[P()], b1 = r[b + 1], b2 = r[b + 2];             \
+            const float a0 = A0[P()], a1 = A1[P()], a2 = A2[P()];               \
+            const float b0 = B0[P()], b1 = B1[P()], b2 = B2[P()];               \
             (void)b1; (void)b2;                                                 \
-            res[P()] = (E0);                                                    \
-            r1[P()] = (E1);                                                     \
-            r2[P()] = (E2);                                                     \
+            const float t0 = (E0), t1 = (E1), t2 = (E2);                        \
+            A0[P()] = t0;                                                       \
+            A1[P()] = t1;                                                       \
+            A2[P()] = t2;                                                       \
         });                                                                     \
-        sfor<PPL>([&](auto P) {                                                 \
-            v32f t = fget<P(), PPL>(R);                                         \
-            t[d + 1] = r1[P()];                                                 \
-            t[d + 2] = r2[P()];                                                 \
-            fget<P(), PPL>(R) = t;                                              \
-        });                                                                     \
-        break;                                                                  \
-    }
+        break;
             RSR_V3CASE(U_ADDV3, a0 + b0, a1 + b1, a2 + b2)
             RSR_V3CASE(U_SUBV3, a0 - b0, a1 - b1, a2 - b2)
             RSR_V3CASE(U_SCALEV3, a0 * b0, a1 * b0, a2 * b0)
             RSR_V3CASE(U_CROSS3, (a1 * b2) - (a2 * b1), (a2 * b0) - (a0 * b2), (a0 * b1) - (a1 * b0))
+            RSR_V3CASE(U_RSUBV3, b0 - a0, b1 - a1, b2 - a2)
+            RSR_V3CASE(U_RCROSS3, (b1 * a2) - (b2 * a1), (b2 * a0) - (b0 * a2), (b0 * a1) - (b1 * a0))
+            RSR_V3CASE(U_SCALEV3S, b0 * a0, b1 * a0, b2 * a0)
 #undef RSR_V3CASE
 #define RSR_KCASE(OPC, EXPR)                               \
     case OPC: {                                            \
-        const float K = kld(params, x);                         \
+        const float K = kld(params, x);                    \
         sfor<PPL>([&](auto P) {                            \
-            const float A = xa[P()];                       \
-            res[P()] = (EXPR);                             \
+            const float A = A0[P()];                       \
+            A0[P()] = (EXPR);                              \
         });                                                \
+        scalar_result();                                   \
         break;                                             \
     }
             RSR_KCASE(U_ADDK, A + K)
@@ -218,44 +249,43 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             RSR_KCASE(U_KDIV, K / A)
 #undef RSR_KCASE
             case U_SCALEV3K: {
-                __builtin_assume(a < 30u);
-                __builtin_assume(d < 30u);
                 const float K = kld(params, x);
-                float r1[PPL], r2[PPL];
                 sfor<PPL>([&](auto P) {
-                    const v32f r = fget<P(), PPL>(R);
-                    const float a1 = r[a + 1], a2 = r[a + 2];
-                    res[P()] = xa[P()] * K;
-                    r1[P()] = a1 * K;
-                    r2[P()] = a2 * K;
-                });
-                sfor<PPL>([&](auto P) {
-                    v32f t = fget<P(), PPL>(R);
-                    t[d + 1] = r1[P()];
-                    t[d + 2] = r2[P()];
-                    fget<P(), PPL>(R) = t;
+                    A0[P()] = A0[P()] * K;
+                    A1[P()] = A1[P()] * K;
+                    A2[P()] = A2[P()] * K;
                 });
                 break;
             }
             case U_MULM3V3: {
                 __builtin_assume(a < 23u);
-                __builtin_assume(b < 30u);
-                __builtin_assume(d < 30u);
-                float r1[PPL], r2[PPL];
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
-                    const float v0 = xb[P()], v1 = r[b + 1], v2 = r[b + 2];
-                    const float m0 = xa[P()], m1 = r[a + 1], m2 = r[a + 2], m3 = r[a + 3], m4 = r[a + 4];
-                    const float m5 = r[a + 5], m6 = r[a + 6], m7 = r[a + 7], m8 = r[a + 8];
-                    res[P()] = ((m0 * v0) + (m3 * v1)) + (m6 * v2);
-                    r1[P()] = ((m1 * v0) + (m4 * v1)) + (m7 * v2);
-                    r2[P()] = ((m2 * v0) + (m5 * v1)) + (m8 * v2);
+                    const float v0 = B0[P()], v1 = B1[P()], v2 = B2[P()];
+                    const float o0 = ((A0[P()] * v0) + (r[a + 3] * v1)) + (r[a + 6] * v2);
+                    const float o1 = ((r[a + 1] * v0) + (r[a + 4] * v1)) +
+                                     (r[a + 7] * v2);
+                    const float o2 = ((r[a + 2] * v0) + (r[a + 5] * v1)) +
+                                     (r[a + 8] * v2);
+                    A0[P()] = o0;
+                    A1[P()] = o1;
+                    A2[P()] = o2;
                 });
+                break;
+            }
+            case U_MULM3V3R: {
+                __builtin_assume(b < 23u);
                 sfor<PPL>([&](auto P) {
-                    v32f t = fget<P(), PPL>(R);
-                    t[d + 1] = r1[P()];
-                    t[d + 2] = r2[P()];
-                    fget<P(), PPL>(R) = t;
+                    const v32f r = fget<P(), PPL>(R);
+                    const float v0 = A0[P()], v1 = A1[P()], v2 = A2[P()];
+                    const float o0 = ((B0[P()] * v0) + (r[b + 3] * v1)) + (r[b + 6] * v2);
+                    const float o1 = ((r[b + 1] * v0) + (r[b + 4] * v1)) +
+                                     (r[b + 7] * v2);
+                    const float o2 = ((r[b + 2] * v0) + (r[b + 5] * v1)) +
+                                     (r[b + 8] * v2);
+                    A0[P()] = o0;
+                    A1[P()] = o1;
+                    A2[P()] = o2;
                 });
                 break;
             }
@@ -268,101 +298,118 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
                 // per point: all nine reads, then the writes (files of different points are disjoint)
                 sfor<PPL>([&](auto P) {
                     v32f t = fget<P(), PPL>(R);
-                    const float sv = kk ? K : xb[P()];
+                    const float sv = kk ? K : B0[P()];
                     float rr[8];
 #pragma unroll
                     for (int q = 1; q < 9; ++q) rr[q - 1] = t[a + q] * sv;
-                    res[P()] = xa[P()] * sv;
+                    A0[P()] = A0[P()] * sv;
 #pragma unroll
                     for (int q = 1; q < 9; ++q) t[d + q] = rr[q - 1];
                     fget<P(), PPL>(R) = t;
                 });
+                scalar_result();
                 break;
             }
-            case U_ADDV4:
-            case U_SUBV4:
-            case U_SCALEV4:
-            case U_SCALEV4K:
-            case U_MULM2:
+            // Mat2 / four-component block ops: components 1..3 are read and written here,
+            // component 0 goes through the accumulator
+#define RSR_V4CASE(OPC, E0, E1, E2, E3, READB)                                  \
+    case OPC: {                                                                 \
+        __builtin_assume(a < 29u);                                              \
+        __builtin_assume(b < 29u);                                              \
+        __builtin_assume(d < 29u);                                              \
+        const float K = kld(params, x);                                         \
+        (void)K;                                                                \
+        sfor<PPL>([&](auto P) {                                                 \
+            v32f t = fget<P(), PPL>(R);                                         \
+            const float a0 = A0[P()], a1 = t[a + 1], a2 = t[a + 2], a3 = t[a + 3]; \
+            const float b0 = B0[P()];                                           \
+            float b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;                              \
+            if (READB) { b1 = t[b + 1]; b2 = t[b + 2]; b3 = t[b + 3]; }         \
+            (void)b0; (void)b1; (void)b2; (void)b3;                             \
+            const float o0 = (E0), o1 = (E1), o2 = (E2), o3 = (E3);             \
+            A0[P()] = o0;                                                       \
+            t[d + 1] = o1;                                                      \
+            t[d + 2] = o2;                                                      \
+            t[d + 3] = o3;                                                      \
+            fget<P(), PPL>(R) = t;                                              \
+        });                                                                     \
+        scalar_result();                                                        \
+        break;                                                                  \
+    }
+            RSR_V4CASE(U_ADDV4, a0 + b0, a1 + b1, a2 + b2, a3 + b3, true)
+            RSR_V4CASE(U_SUBV4, a0 - b0, a1 - b1, a2 - b2, a3 - b3, true)
+            RSR_V4CASE(U_SCALEV4, a0 * b0, a1 * b0, a2 * b0, a3 * b0, false)
+            RSR_V4CASE(U_SCALEV4K, a0 * K, a1 * K, a2 * K, a3 * K, false)
+            RSR_V4CASE(U_MULM2, (a0 * b0) + (a2 * b1), (a1 * b0) + (a3 * b1), (a0 * b2) + (a2 * b3),
+                       (a1 * b2) + (a3 * b3), true)
+#undef RSR_V4CASE
             case U_INVM2: {
                 __builtin_assume(a < 29u);
-                __builtin_assume(b < 29u);
                 __builtin_assume(d < 29u);
-                const float K = kld(params, x);
                 sfor<PPL>([&](auto P) {
                     v32f t = fget<P(), PPL>(R);
-                    const float A0 = xa[P()], A1 = t[a + 1], A2 = t[a + 2], A3 = t[a + 3];
-                    float o0, o1, o2, o3;
-                    if (op == U_ADDV4 || op == U_SUBV4) {
-                        const float B0 = xb[P()], B1 = t[b + 1], B2 = t[b + 2], B3 = t[b + 3];
-                        if (op == U_ADDV4) {
-                            o0 = A0 + B0; o1 = A1 + B1; o2 = A2 + B2; o3 = A3 + B3;
-                        } else {
-                            o0 = A0 - B0; o1 = A1 - B1; o2 = A2 - B2; o3 = A3 - B3;
-                        }
-                    } else if (op == U_MULM2) {
-                        const float B0 = xb[P()], B1 = t[b + 1], B2 = t[b + 2], B3 = t[b + 3];
-                        o0 = (A0 * B0) + (A2 * B1);
-                        o1 = (A1 * B0) + (A3 * B1);
-                        o2 = (A0 * B2) + (A2 * B3);
-                        o3 = (A1 * B2) + (A3 * B3);
-                    } else if (op == U_INVM2) {
-                        const float det = (A0 * A3) - (A1 * A2);
-                        const bool sing = __builtin_fabsf(det) < 1e-9f;
-                        const float inv = 1.0f / (sing ? 1.0f : det);
-                        o0 = sing ? 1.0f : A3 * inv;
-                        o1 = sing ? 0.0f : (-A1) * inv;
-                        o2 = sing ? 0.0f : (-A2) * inv;
-                        o3 = sing ? 1.0f : A0 * inv;
-                    } else {
-                        const float sv = op == U_SCALEV4K ? K : xb[P()];
-                        o0 = A0 * sv; o1 = A1 * sv; o2 = A2 * sv; o3 = A3 * sv;
-                    }
-                    res[P()] = o0;
-                    t[d + 1] = o1;
-                    t[d + 2] = o2;
-                    t[d + 3] = o3;
+                    const float a0 = A0[P()], a1 = t[a + 1], a2 = t[a + 2], a3 = t[a + 3];
+                    const float det = (a0 * a3) - (a1 * a2);
+                    const bool sing = __builtin_fabsf(det) < 1e-9f;
+                    const float inv = 1.0f / (sing ? 1.0f : det);
+                    A0[P()] = sing ? 1.0f : a3 * inv;
+                    t[d + 1] = sing ? 0.0f : (-a1) * inv;
+                    t[d + 2] = sing ? 0.0f : (-a2) * inv;
+                    t[d + 3] = sing ? 1.0f : a0 * inv;
                     fget<P(), PPL>(R) = t;
                 });
+                scalar_result();
                 break;
             }
             case U_IMMV3: {
-                __builtin_assume(d < 30u);
                 const float v0 = kld(params, x), v1 = kld(params, x + 1), v2 = kld(params, x + 2);
                 sfor<PPL>([&](auto P) {
-                    res[P()] = v0;
-                    v32f t = fget<P(), PPL>(R);
-                    t[d + 1] = v1;
-                    t[d + 2] = v2;
-                    fget<P(), PPL>(R) = t;
+                    A0[P()] = v0;
+                    A1[P()] = v1;
+                    A2[P()] = v2;
                 });
                 break;
             }
             case U_DOT3:
-                __builtin_assume(a < 30u);
-                __builtin_assume(b < 30u);
                 sfor<PPL>([&](auto P) {
-                    const v32f r = fget<P(), PPL>(R);
-                    res[P()] = ((xa[P()] * xb[P()]) + (r[a + 1] * r[b + 1])) + (r[a + 2] * r[b + 2]);
+                    A0[P()] = ((A0[P()] * B0[P()]) + (A1[P()] * B1[P()])) + (A2[P()] * B2[P()]);
                 });
+                scalar_result();
                 break;
             case U_NORM3:
-                __builtin_assume(a < 30u);
                 sfor<PPL>([&](auto P) {
-                    const v32f r = fget<P(), PPL>(R);
-                    const float a1 = r[a + 1], a2 = r[a + 2];
-                    res[P()] = __builtin_sqrtf(((xa[P()] * xa[P()]) + (a1 * a1)) + (a2 * a2));
+                    const float a0 = A0[P()], a1 = A1[P()], a2 = A2[P()];
+                    A0[P()] = __builtin_sqrtf(((a0 * a0) + (a1 * a1)) + (a2 * a2));
                 });
+                scalar_result();
                 break;
             default:
-                sfor<PPL>([&](auto P) { res[P()] = 0.0f; });
+                sfor<PPL>([&](auto P) { A0[P()] = 0.0f; });
+                scalar_result();
                 break;
         }
-        sfor<PPL>([&](auto P) {
-            v32f t = fget<P(), PPL>(R);
-            t[d] = res[P()];
-            fget<P(), PPL>(R) = t;
-        });
+        // The write-back index also depends on the a and b words (their bits 16 and up
+        // are zero), which keeps them live to the end of the iteration: otherwise the
+        // compiler moves the prefetched next words into their registers right after the
+        // staging, which waits for the prefetch and exposes its latency on every op.
+        const uint32_t dw = (d + ((u.a | u.b) >> 16)) & 31u;
+        if (!(u.d & kNoWb)) {
+            if (u.d & kVec3) {
+                sfor<PPL>([&](auto P) {
+                    v32f t = fget<P(), PPL>(R);
+                    t[dw] = A0[P()];
+                    t[dw + 1] = A1[P()];
+                    t[dw + 2] = A2[P()];
+                    fget<P(), PPL>(R) = t;
+                });
+            } else {
+                sfor<PPL>([&](auto P) {
+                    v32f t = fget<P(), PPL>(R);
+                    t[dw] = A0[P()];
+                    fget<P(), PPL>(R) = t;
+                });
+            }
+        }
     }
 }
 

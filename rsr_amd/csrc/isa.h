@@ -51,7 +51,8 @@ static inline int vt_size(int vt) {
 // the register operands):
 //   w0 = op | x << 8     x = IMM: flat parameter index; SPILL / FILL: LDS slot;
 //                            otherwise c | e << 5 | flags << 10
-//   w1 = d, w2 = a, w3 = b   register indices (< 32; 0 when unused)
+//   w1 = d, w2 = a, w3 = b   register indices (< 32; 0 when unused) plus flags
+//                            (kFwd, kVec3, kNoWb below)
 // Semantics (every arithmetic step is one IEEE binary32 operation, left to
 // right, never contracted into an FMA):
 enum UOp : uint8_t {
@@ -106,6 +107,13 @@ enum UOp : uint8_t {
     U_SCALEV4K,   // d+k = a+k * p
     U_MULM2,      // d+r+2c = (a+r * b+2c) + (a+r+2 * b+2c+1)
     U_INVM2,      // inverse with the singular -> identity blend (no dual form: never lowered dual)
+    // operand-swapped forms, emitted so that a forwarded operand is always a (value code only)
+    U_RSUB,       // d = b - a
+    U_RDIV,       // d = b / a
+    U_RSUBV3,     // d+k = b+k - a+k
+    U_RCROSS3,    // cross(b, a): d = (b1*a2) - (b2*a1), d+1 = (b2*a0) - (b0*a2), d+2 = (b0*a1) - (b1*a0)
+    U_SCALEV3S,   // d+k = b+k * a                    (a: one register, b: Vec3)
+    U_MULM3V3R,   // d+i = ((b+i * a) + (b+3+i * a+1)) + (b+6+i * a+2), i < 3   (a: Vec3, b: Mat3)
     U_COUNT
 };
 
@@ -134,6 +142,52 @@ struct alignas(16) UInstr {
 static inline UInstr make_uinstr(uint32_t op, uint32_t d, uint32_t a, uint32_t b, uint32_t x) {
     return UInstr{op | (x << 8), d, a, b};
 }
+// Operand and result flags in the high bits of w1 (d), w2 (a) and w3 (b).  Register
+// indices are the low five bits.  Only the value lowering sets them; the dual
+// interpreter ignores them.
+//   kFwd  on a: operand a is the whole result of the previous micro-op of the same
+//         program, still held in the interpreter's accumulator (never reread from the
+//         register file).
+//   kVec3 on a or b: the operand has three components (a..a+2), staged before the
+//         dispatch; on d: the result has three components (d..d+2), written after it.
+//   kNone on a or b: the micro-op has no such operand (nothing is staged).
+//   kNoWb on d: the result is not written to the register file.  Set only when the
+//         next micro-op takes it through kFwd and no later micro-op reads it.
+constexpr uint32_t kFwd = 1u << 8, kVec3 = 1u << 9, kNone = 1u << 10, kNoWb = 1u << 11;
+
+// Register groups a micro-op reads and writes (component counts; 0 = unused).
+struct UGroups {
+    uint8_t na, nb, nc, ne, nd;
+};
+static inline UGroups ugroups(uint32_t op) {
+    switch (op) {
+        case U_IMM: case U_FILL: return {0, 0, 0, 0, 1};
+        case U_ADD: case U_SUB: case U_MUL: case U_DIV: case U_SQRTACC: return {1, 1, 0, 0, 1};
+        case U_SQR: case U_SQRT: case U_SIN: case U_COS: case U_EXP: case U_LN: case U_NEG: case U_RCPSAFE:
+        case U_SPILL: case U_ADDK: case U_SUBK: case U_KSUB: case U_MULK: case U_DIVK: case U_KDIV:
+            return {1, 0, 0, 0, 1};
+        case U_MULADD: case U_MULSUB: return {1, 1, 1, 1, 1};
+        case U_ADDMUL: case U_ADD3: case U_INVSEL: return {1, 1, 1, 0, 1};
+        case U_ADDV3: case U_SUBV3: case U_CROSS3: return {3, 3, 0, 0, 3};
+        case U_SCALEV3: return {3, 1, 0, 0, 3};
+        case U_SCALEV3K: return {3, 0, 0, 0, 3};
+        case U_IMMV3: return {0, 0, 0, 0, 3};
+        case U_DOT3: return {3, 3, 0, 0, 1};
+        case U_NORM3: return {3, 0, 0, 0, 1};
+        case U_MULM3V3: return {9, 3, 0, 0, 3};
+        case U_SCALEM3: return {9, 1, 0, 0, 9};
+        case U_SCALEM3K: return {9, 0, 0, 0, 9};
+        case U_ADDV4: case U_SUBV4: case U_MULM2: return {4, 4, 0, 0, 4};
+        case U_SCALEV4: return {4, 1, 0, 0, 4};
+        case U_SCALEV4K: case U_INVM2: return {4, 0, 0, 0, 4};
+        case U_RSUB: case U_RDIV: return {1, 1, 0, 0, 1};
+        case U_RSUBV3: case U_RCROSS3: return {3, 3, 0, 0, 3};
+        case U_SCALEV3S: return {1, 3, 0, 0, 3};
+        case U_MULM3V3R: return {3, 9, 0, 0, 3};
+        default: return {0, 0, 0, 0, 0};
+    }
+}
+
 // operand fields of x
 static inline uint32_t ux_c(uint32_t x) { return x & 31u; }
 static inline uint32_t ux_e(uint32_t x) { return (x >> 5) & 31u; }

@@ -97,8 +97,11 @@ __device__ __forceinline__ float fold_batches(float acc, float term, int lane) {
 // ---------------------------------------------------------------------------------
 // MSE, Kahan summation
 // ---------------------------------------------------------------------------------
+// At four points per lane the register file alone is 128 VGPRs; the occupancy hint
+// makes the allocator fit the kernel in 168 (three waves per SIMD).
 template <int PPL>
-__global__ __launch_bounds__(256) void k_eval_mse(const SegTable T, int spill_slots) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_eval_mse(const SegTable T,
+                                                                                           int spill_slots) {
     extern __shared__ float smem[];
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
@@ -122,14 +125,9 @@ __global__ __launch_bounds__(256) void k_eval_mse(const SegTable T, int spill_sl
     Files<PPL> R;
     sfor<PPL>([&](auto P) { fget<P(), PPL>(R) = (v32f)(0.0f); });
     load_features<PPL>(R, S.feat, S.F, S.npad, base, lane);
-    float y[PPL];
     bool valid[PPL];
 #pragma unroll
-    for (int p = 0; p < PPL; ++p) {
-        const int64_t pt = base + p * 64 + lane;
-        y[p] = S.target[pt];
-        valid[p] = pt < S.count8;
-    }
+    for (int p = 0; p < PPL; ++p) valid[p] = base + p * 64 + lane < S.count8;
     for (int sp = first; sp < last; ++sp) {
         const int prog = (int)uni((uint32_t)S.order[sp]);
         float s = 0.0f;
@@ -137,6 +135,10 @@ __global__ __launch_bounds__(256) void k_eval_mse(const SegTable T, int spill_sl
             const ProgDesc pd = S.progs[prog];
             run_program<PPL>(R, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), lds, lane);
             const uint32_t o = uni(pd.out_reg) & 31u;
+            // targets are re-read per program (cache hits) rather than held in registers
+            // through the interpreter loop
+            float y[PPL];
+            sfor<PPL>([&](auto P) { y[P()] = S.target[base + P() * 64 + lane]; });
             if (S.terms) {
                 // reference order: keep every per-point term; the fold kernel adds them
                 float* row = S.partials + (int64_t)sp * S.n_tiles * (64 * PPL) + lt * (64 * PPL);

@@ -52,6 +52,97 @@ struct MI {
 constexpr int kMaxDepth = 32;
 constexpr int kInf = 1 << 30;
 
+using RegSet = uint32_t;  // bit r: register r
+RegSet range_bits(uint32_t r, int n) {
+    return n ? (((n >= 32 ? ~0u : ((1u << n) - 1u)) << r)) : 0u;
+}
+
+// Operand-swapped form of a micro-op whose forwarded operand is b, so that it becomes
+// a: the same op for the commutative ones (IEEE + and * are commutative, and every
+// step of these formulas is symmetric in a and b, INVSEL's negation included since
+// (-a) * b == (-b) * a), a reversed op otherwise, U_COUNT when there is none.
+UOp swapped_op(uint32_t op) {
+    switch (op) {
+        case U_ADD: case U_MUL: case U_MULADD: case U_MULSUB: case U_ADD3: case U_INVSEL:
+        case U_ADDV3: case U_DOT3:
+            return (UOp)op;
+        case U_SUB: return U_RSUB;
+        case U_DIV: return U_RDIV;
+        case U_SUBV3: return U_RSUBV3;
+        case U_CROSS3: return U_RCROSS3;
+        case U_SCALEV3: return U_SCALEV3S;
+        case U_MULM3V3: return U_MULM3V3R;
+        default: return U_COUNT;
+    }
+}
+
+// Value-code post-pass.  A postfix program consumes most results in the very next
+// operation, so the interpreter keeps the last result's components in an accumulator:
+// a consumer whose whole operand a is exactly the previous result takes it from there
+// (kFwd; a consumer of it through b is first rewritten to its operand-swapped form),
+// and a forwarded result that no later micro-op reads is not written to the register
+// file at all (kNoWb).  kVec3 and kNone describe the operands and the result for the
+// staging and the write-back.  Liveness is exact: a backward scan over register reads
+// and writes, with the program result live at the end.
+void annotate_forwarding(std::vector<UInstr>& code, uint32_t out_reg) {
+    const int n = (int)code.size();
+    // 1. swap forwarded b operands into a
+    for (int i = 1; i < n; ++i) {
+        UInstr& u = code[i];
+        const UGroups gp = ugroups(code[i - 1].w0 & 255u), g = ugroups(u.w0 & 255u);
+        const uint32_t d = code[i - 1].w1 & 31u;
+        if (gp.nd != 1 && gp.nd != 3) continue;
+        if (g.na == gp.nd && (u.w2 & 31u) == d) continue;
+        if (!(g.nb == gp.nd && (u.w3 & 31u) == d)) continue;
+        const UOp s = swapped_op(u.w0 & 255u);
+        if (s == U_COUNT) continue;
+        u.w0 = (u.w0 & ~255u) | s;
+        std::swap(u.w2, u.w3);
+    }
+    // 2. liveness
+    std::vector<RegSet> live_after(n);
+    auto reads = [&](const UInstr& u, bool skip_a) {
+        const UGroups g = ugroups(u.w0 & 255u);
+        const uint32_t x = u.w0 >> 8;
+        RegSet s = 0;
+        if (!skip_a) s |= range_bits(u.w2 & 31u, g.na);
+        s |= range_bits(u.w3 & 31u, g.nb);
+        if (g.nc) s |= range_bits(ux_c(x), 1);
+        if (g.ne) s |= range_bits(ux_e(x), 1);
+        return s;
+    };
+    RegSet live = range_bits(out_reg, 1);
+    for (int i = n - 1; i >= 0; --i) {
+        live_after[i] = live;
+        const UGroups g = ugroups(code[i].w0 & 255u);
+        live = (live & ~range_bits(code[i].w1 & 31u, g.nd)) | reads(code[i], false);
+    }
+    // 3. operand shapes
+    for (int i = 0; i < n; ++i) {
+        UInstr& u = code[i];
+        const UGroups g = ugroups(u.w0 & 255u);
+        if (g.na == 3) u.w2 |= kVec3;
+        if (g.na == 0) u.w2 |= kNone;
+        if (g.nb == 3) u.w3 |= kVec3;
+        if (g.nb == 0) u.w3 |= kNone;
+        if (g.nd == 3) u.w1 |= kVec3;
+    }
+    // 4. forwarding and write-back elision
+    for (int i = 1; i < n; ++i) {
+        UInstr& u = code[i];
+        UInstr& p = code[i - 1];
+        const UGroups gp = ugroups(p.w0 & 255u), g = ugroups(u.w0 & 255u);
+        if (gp.nd != 1 && gp.nd != 3) continue;
+        const uint32_t d = p.w1 & 31u;
+        if (!(g.na == gp.nd && (u.w2 & 31u) == d)) continue;
+        u.w2 |= kFwd;
+        // the old value is dead after the consumer where the consumer overwrites it
+        const RegSet w = range_bits(d, gp.nd);
+        const RegSet survives = w & ~range_bits(u.w1 & 31u, g.nd);
+        if (!(w & reads(u, true)) && !(survives & live_after[i])) p.w1 |= kNoWb;
+    }
+}
+
 class Lowerer {
    public:
     Lowerer(const RefProgram& p, int F, bool dual) : p_(p), F_(F), dual_(dual) {
@@ -84,6 +175,7 @@ class Lowerer {
         if (!ok_) return fail(PROG_REGS);
         out_.out_reg = root.reg[0];
         out_.status = PROG_OK;
+        if (!dual_) annotate_forwarding(out_.code, out_.out_reg);
         return std::move(out_);
     }
 
