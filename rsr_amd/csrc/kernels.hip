@@ -199,45 +199,51 @@ __global__ void k_finalize_mse(const SegTable T) {
 // ---------------------------------------------------------------------------------
 // MSE, reference-order summation from per-point terms
 // ---------------------------------------------------------------------------------
-// One wave per (segment, scheduling position).  Lanes 0..7 are the reference's lane
-// accumulators: accumulator l adds the terms of points l, l+8, l+16, ... of the launch's
-// range in order (evaluator.rs:159-170), starting from the state carried from earlier
-// ranges.  The wave reads 64 consecutive terms per step (coalesced) and folds its eight
-// 8-point batches one after the other; the last range reduces the eight accumulators
-// in f32x8::reduce_add order.  (first_thread counts fold waves here.)
+// Eight scheduling positions per wave, eight lanes each: lane 8j + l is the reference's
+// lane accumulator l of position j, adding the terms of points l, l+8, l+16, ... of the
+// launch's range in order (evaluator.rs:159-170), starting from the state carried from
+// earlier ranges.  Every lane runs its own serial chain (no shuffles); the loads are
+// issued 32 batches ahead of the adds.  The last range reduces each position's eight
+// accumulators in f32x8::reduce_add order.  (first_thread counts fold waves here.)
 __global__ __launch_bounds__(256) void k_fold_terms(const SegTable T) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
     const int si = find_seg_first_thread(T, wave);
     const SegDesc& S = T.s[si];
-    const int64_t sp = wave - S.first_thread;
-    if (sp >= S.n_progs) return;
+    const int64_t sp0 = (wave - S.first_thread) * 8;
+    if (sp0 >= S.n_progs) return;
     if (S.live != nullptr && *S.live == 0) return;
-    const int prog = (int)uni((uint32_t)S.order[sp]);
-    if (S.active != nullptr && !S.active[prog]) return;
+    const int j = lane >> 3, l = lane & 7;
+    const int64_t sp = sp0 + j;
+    const bool on = sp < S.n_progs && (S.active == nullptr || S.active[S.order[sp]]);
     const int64_t p0 = S.tile0 * S.tile_pts;
-    const int64_t p1 = min(p0 + S.n_tiles * S.tile_pts, S.count8);
-    const float* row = S.partials + sp * S.n_tiles * S.tile_pts - p0;
-    float acc = (!S.first && lane < 8) ? S.kstate[sp * 8 + lane] : 0.0f;
-    int64_t q = p0;
-    for (; q + 256 <= p1; q += 256) {  // four 64-point steps with their loads issued together
-        const float v0 = row[q + lane], v1 = row[q + 64 + lane], v2 = row[q + 128 + lane], v3 = row[q + 192 + lane];
-        acc = fold_batches(acc, v0, lane);
-        acc = fold_batches(acc, v1, lane);
-        acc = fold_batches(acc, v2, lane);
-        acc = fold_batches(acc, v3, lane);
-    }
-    for (; q < p1; q += 64) {
-        const int nb = (int)min<int64_t>(8, (p1 - q) >> 3);  // whole 8-point batches left
-        const float v = q + lane < p1 ? row[q + lane] : 0.0f;
-        for (int k = 0; k < nb; ++k) acc = acc + __shfl(v, (lane & 7) + 8 * k, 64);
+    const int64_t p1 = min(p0 + S.n_tiles * S.tile_pts, S.count8);  // whole 8-point batches
+    const float* row = S.partials + (on ? sp : 0) * S.n_tiles * S.tile_pts - p0 + l;
+    float acc = (on && !S.first) ? S.kstate[sp * 8 + l] : 0.0f;
+    if (on) {
+        constexpr int kAhead = 32;
+        int64_t q = p0;
+        for (; q + 8 * kAhead <= p1; q += 8 * kAhead) {
+            float v[kAhead];
+#pragma unroll
+            for (int k = 0; k < kAhead; ++k) v[k] = row[q + 8 * k];
+#pragma unroll
+            for (int k = 0; k < kAhead; ++k) acc = acc + v[k];
+        }
+        for (; q < p1; q += 8) acc = acc + row[q];
     }
     if (!S.last) {
-        if (lane < 8) S.kstate[sp * 8 + lane] = acc;
+        if (on) S.kstate[sp * 8 + l] = acc;
         return;
     }
-    const float mse = reduce_lanes8(acc) / S.n_samples;
-    if (lane == 0) S.mse_out[prog] = __builtin_isfinite(mse) ? mse : FLT_MAX;
+    // f32x8::reduce_add within each group of eight lanes: (lo4 + hi4), then ((0 + 2) + (1 + 3))
+    const int g = lane & ~7;
+    const float a0 = __shfl(acc, g + 0, 64), a1 = __shfl(acc, g + 1, 64), a2 = __shfl(acc, g + 2, 64);
+    const float a3 = __shfl(acc, g + 3, 64), a4 = __shfl(acc, g + 4, 64), a5 = __shfl(acc, g + 5, 64);
+    const float a6 = __shfl(acc, g + 6, 64), a7 = __shfl(acc, g + 7, 64);
+    const float s0 = a0 + a4, s1 = a1 + a5, s2 = a2 + a6, s3 = a3 + a7;
+    const float mse = ((s0 + s2) + (s1 + s3)) / S.n_samples;
+    if (on && l == 0) S.mse_out[S.order[sp]] = __builtin_isfinite(mse) ? mse : FLT_MAX;
 }
 
 // ---------------------------------------------------------------------------------
@@ -483,7 +489,7 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
             d.first_thread = threads;
             waves += d.n_tiles * d.n_groups;
             // finalize threads (whole waves per segment), or fold waves
-            threads += terms ? (int64_t)d.n_progs : ((int64_t)d.n_progs + 63) / 64 * 64;
+            threads += terms ? ((int64_t)d.n_progs + 7) / 8 : ((int64_t)d.n_progs + 63) / 64 * 64;
             spill = max(spill, st.P.spill_slots);
             T.s[T.n++] = d;
         }
