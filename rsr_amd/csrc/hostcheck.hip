@@ -37,6 +37,11 @@ static void run_one(uint32_t op, uint32_t x, uint32_t d, uint32_t a, uint32_t b,
             case U_DIV: v = R[a] / R[b]; g = ((G[a] * R[b]) - (R[a] * G[b])) / (R[b] * R[b]); break;
             case U_SQR: v = R[a] * R[a]; g = (2.0f * R[a]) * G[a]; break;
             case U_SQRT: v = sqrtf(R[a]); g = G[a] / (2.0f * v); break;
+            case U_SINU: v = dm_sin(R[a]); break;  // point-invariant forms: value code only
+            case U_COSU: v = dm_cos(R[a]); break;
+            case U_EXPU: v = dm_exp(R[a]); break;
+            case U_LNU: v = dm_ln(R[a]); break;
+            case U_SQRTU: v = sqrtf(R[a]); break;
             case U_SIN: v = dm_sin(R[a]); g = G[a] * dm_cos(R[a]); break;
             case U_COS: v = dm_cos(R[a]); g = G[a] * (-dm_sin(R[a])); break;
             case U_EXP: v = dm_exp(R[a]); g = G[a] * v; break;

@@ -160,6 +160,21 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             RSR_CASE(U_RSUB, B - A)
             RSR_CASE(U_RDIV, B / A)
 #undef RSR_CASE
+            // point-invariant operand: one evaluation, copied to the lane's other points
+#define RSR_UCASE(OPC, EXPR)                                  \
+    case OPC: {                                               \
+        const float A = A0[0];                                \
+        const float r0 = (EXPR);                              \
+        sfor<PPL>([&](auto P) { A0[P()] = r0; });             \
+        scalar_result();                                      \
+        break;                                                \
+    }
+            RSR_UCASE(U_SINU, dm_sin(A))
+            RSR_UCASE(U_COSU, dm_cos(A))
+            RSR_UCASE(U_EXPU, dm_exp(A))
+            RSR_UCASE(U_LNU, dm_ln(A))
+            RSR_UCASE(U_SQRTU, __builtin_sqrtf(A))
+#undef RSR_UCASE
             case U_MULADD:
             case U_MULSUB: {
                 const uint32_t c = x & 31u, e = (x >> 5) & 31u;

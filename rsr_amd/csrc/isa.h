@@ -114,6 +114,13 @@ enum UOp : uint8_t {
     U_RCROSS3,    // cross(b, a): d = (b1*a2) - (b2*a1), d+1 = (b2*a0) - (b0*a2), d+2 = (b0*a1) - (b1*a0)
     U_SCALEV3S,   // d+k = b+k * a                    (a: one register, b: Vec3)
     U_MULM3V3R,   // d+i = ((b+i * a) + (b+3+i * a+1)) + (b+6+i * a+2), i < 3   (a: Vec3, b: Mat3)
+    // an operand that holds the same value at every point (it derives from constants
+    // only): computed once per lane and copied to the lane's other points (value code only)
+    U_SINU,       // d = sin(a)
+    U_COSU,       // d = cos(a)
+    U_EXPU,       // d = exp(a)
+    U_LNU,        // d = ln(a)
+    U_SQRTU,      // d = sqrt(a)
     U_COUNT
 };
 
@@ -184,6 +191,7 @@ static inline UGroups ugroups(uint32_t op) {
         case U_RSUBV3: case U_RCROSS3: return {3, 3, 0, 0, 3};
         case U_SCALEV3S: return {1, 3, 0, 0, 3};
         case U_MULM3V3R: return {3, 9, 0, 0, 3};
+        case U_SINU: case U_COSU: case U_EXPU: case U_LNU: case U_SQRTU: return {1, 0, 0, 0, 1};
         default: return {0, 0, 0, 0, 0};
     }
 }

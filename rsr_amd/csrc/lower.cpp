@@ -86,6 +86,33 @@ UOp swapped_op(uint32_t op) {
 // and writes, with the program result live at the end.
 void annotate_forwarding(std::vector<UInstr>& code, uint32_t out_reg) {
     const int n = (int)code.size();
+    // 0. transcendentals of point-invariant values (derived from constants only) are
+    //    computed once per lane
+    {
+        RegSet uni = 0;  // registers holding a point-invariant value
+        for (int i = 0; i < n; ++i) {
+            UInstr& u = code[i];
+            const uint32_t op = u.w0 & 255u, x = u.w0 >> 8;
+            const UGroups g = ugroups(op);
+            bool inv = op != U_FILL;  // a filled value is not tracked
+            if (g.na) inv = inv && (range_bits(u.w2 & 31u, g.na) & ~uni) == 0;
+            if (g.nb) inv = inv && (range_bits(u.w3 & 31u, g.nb) & ~uni) == 0;
+            if (g.nc) inv = inv && (range_bits(ux_c(x), 1) & ~uni) == 0;
+            if (g.ne) inv = inv && (range_bits(ux_e(x), 1) & ~uni) == 0;
+            if (inv && g.na == 1) {
+                switch (op) {
+                    case U_SIN: u.w0 = (u.w0 & ~255u) | U_SINU; break;
+                    case U_COS: u.w0 = (u.w0 & ~255u) | U_COSU; break;
+                    case U_EXP: u.w0 = (u.w0 & ~255u) | U_EXPU; break;
+                    case U_LN: u.w0 = (u.w0 & ~255u) | U_LNU; break;
+                    case U_SQRT: u.w0 = (u.w0 & ~255u) | U_SQRTU; break;
+                    default: break;
+                }
+            }
+            const RegSet w = range_bits(u.w1 & 31u, g.nd);
+            uni = inv ? (uni | w) : (uni & ~w);
+        }
+    }
     // 1. swap forwarded b operands into a
     for (int i = 1; i < n; ++i) {
         UInstr& u = code[i];
