@@ -37,6 +37,13 @@ static void run_one(uint32_t op, uint32_t x, uint32_t d, uint32_t a, uint32_t b,
             case U_DIV: v = R[a] / R[b]; g = ((G[a] * R[b]) - (R[a] * G[b])) / (R[b] * R[b]); break;
             case U_SQR: v = R[a] * R[a]; g = (2.0f * R[a]) * G[a]; break;
             case U_SQRT: v = sqrtf(R[a]); g = G[a] / (2.0f * v); break;
+            case U_SINK: v = dm_sin(params[x]); break;  // constant operand: value code only
+            case U_COSK: v = dm_cos(params[x]); break;
+            case U_EXPK: v = dm_exp(params[x]); break;
+            case U_LNK: v = dm_ln(params[x]); break;
+            case U_SQRTK: v = sqrtf(params[x]); break;
+            case U_NEGK: v = -params[x]; break;
+            case U_SQRK: v = params[x] * params[x]; break;
             case U_SINU: v = dm_sin(R[a]); break;  // point-invariant forms: value code only
             case U_COSU: v = dm_cos(R[a]); break;
             case U_EXPU: v = dm_exp(R[a]); break;

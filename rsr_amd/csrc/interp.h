@@ -175,6 +175,23 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             RSR_UCASE(U_LNU, dm_ln(A))
             RSR_UCASE(U_SQRTU, __builtin_sqrtf(A))
 #undef RSR_UCASE
+            // unary op of a constant: one evaluation, copied to every point of the lane
+#define RSR_UKCASE(OPC, EXPR)                                 \
+    case OPC: {                                               \
+        const float A = kld(params, x);                       \
+        const float r0 = (EXPR);                              \
+        sfor<PPL>([&](auto P) { A0[P()] = r0; });             \
+        scalar_result();                                      \
+        break;                                                \
+    }
+            RSR_UKCASE(U_SINK, dm_sin(A))
+            RSR_UKCASE(U_COSK, dm_cos(A))
+            RSR_UKCASE(U_EXPK, dm_exp(A))
+            RSR_UKCASE(U_LNK, dm_ln(A))
+            RSR_UKCASE(U_SQRTK, __builtin_sqrtf(A))
+            RSR_UKCASE(U_NEGK, -A)
+            RSR_UKCASE(U_SQRK, A * A)
+#undef RSR_UKCASE
             case U_MULADD:
             case U_MULSUB: {
                 const uint32_t c = x & 31u, e = (x >> 5) & 31u;

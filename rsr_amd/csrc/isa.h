@@ -121,6 +121,14 @@ enum UOp : uint8_t {
     U_EXPU,       // d = exp(a)
     U_LNU,        // d = ln(a)
     U_SQRTU,      // d = sqrt(a)
+    // unary ops of a Float constant p = params[x], computed once per lane (value code only)
+    U_SINK,       // d = sin(p)
+    U_COSK,       // d = cos(p)
+    U_EXPK,       // d = exp(p)
+    U_LNK,        // d = ln(p)
+    U_SQRTK,      // d = sqrt(p)
+    U_NEGK,       // d = -p
+    U_SQRK,       // d = p * p
     U_COUNT
 };
 
@@ -192,6 +200,8 @@ static inline UGroups ugroups(uint32_t op) {
         case U_SCALEV3S: return {1, 3, 0, 0, 3};
         case U_MULM3V3R: return {3, 9, 0, 0, 3};
         case U_SINU: case U_COSU: case U_EXPU: case U_LNU: case U_SQRTU: return {1, 0, 0, 0, 1};
+        case U_SINK: case U_COSK: case U_EXPK: case U_LNK: case U_SQRTK: case U_NEGK: case U_SQRK:
+            return {0, 0, 0, 0, 1};
         default: return {0, 0, 0, 0, 0};
     }
 }

@@ -525,6 +525,26 @@ class Lowerer {
 
     int unary_f(UOp op) {
         if (!can_pop(VtFloat)) return PROG_STACK;
+        if (!dual_ && stk_[0].back().konst >= 0) {  // a constant operand: read from the parameters
+            UOp k = U_COUNT;
+            switch (op) {
+                case U_SIN: k = U_SINK; break;
+                case U_COS: k = U_COSK; break;
+                case U_EXP: k = U_EXPK; break;
+                case U_LN: k = U_LNK; break;
+                case U_SQRT: k = U_SQRTK; break;
+                case U_NEG: k = U_NEGK; break;
+                case U_SQR: k = U_SQRK; break;
+                default: break;
+            }
+            if (k != U_COUNT) {
+                const Val c = pop_raw(VtFloat);
+                Val none, r;
+                none.n = 0;
+                if (!block_op(k, 1, none, nullptr, (uint32_t)c.konst, r)) return PROG_REGS;
+                return push(VtFloat, r) ? PROG_OK : PROG_STACK;
+            }
+        }
         Val a = pop(VtFloat);
         Val r;
         if (!emit({mi(op, {P(a.reg[0])})}, regs_of({&a}), {V(0)}, r)) return PROG_REGS;
