@@ -31,6 +31,14 @@ __device__ __forceinline__ float kld(const float* p, uint32_t i) { return ((kptr
 
 typedef float v32f __attribute__((ext_vector_type(32)));
 
+// The lane index for spill-slot addressing, recomputed where it is used (two VALU ops).
+// It depends on the micro-op's slot field (x >> 24 is zero for every valid slot), so the
+// compiler cannot hoist it out of the interpreter loop and keep it in a VGPR for the
+// whole program, which at four points per lane pushed the kernel into scratch spills.
+__device__ __forceinline__ uint32_t spill_lane(uint32_t x) {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, x >> 24));
+}
+
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
 // N register files as distinct members (an array of vectors would become a stack
@@ -235,11 +243,11 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
                 break;
             }
             case U_SPILL:
-                sfor<PPL>([&](auto P) { lds[(x * PPL + P()) * 64u + lane] = A0[P()]; });
+                sfor<PPL>([&](auto P) { lds[(x * PPL + P()) * 64u + spill_lane(x)] = A0[P()]; });
                 scalar_result();
                 break;
             case U_FILL:
-                sfor<PPL>([&](auto P) { A0[P()] = lds[(x * PPL + P()) * 64u + lane]; });
+                sfor<PPL>([&](auto P) { A0[P()] = lds[(x * PPL + P()) * 64u + spill_lane(x)]; });
                 scalar_result();
                 break;
             // Vec3 block ops: every operand component is staged, the result replaces A
