@@ -127,6 +127,15 @@ def _timed(ctx, fn):
     return out, time.perf_counter() - t0
 
 
+def _timed_best(ctx, fn, reps=5):
+    """Fastest of `reps` calls (a single short call is at the mercy of host jitter)."""
+    best = None
+    for _ in range(reps):
+        out, t = _timed(ctx, fn)
+        best = t if best is None else min(best, t)
+    return out, best
+
+
 def run_configs(ctx, log):
     """The matrix workloads with constant fitting: evaluation, gradients and the fits."""
     out = {}
@@ -138,9 +147,9 @@ def run_configs(ctx, log):
     ok = P.status(ds) == 0
     ne = node_evals([p for p, g in zip(progs, ok) if g], len(y))
     api.compute_mse(ctx, P, ds, allow_rejected=True)
-    _, te = _timed(ctx, lambda: api.compute_mse(ctx, P, ds, allow_rejected=True))
+    _, te = _timed_best(ctx, lambda: api.compute_mse(ctx, P, ds, allow_rejected=True))
     api.compute_mse_with_gradient(ctx, P, ds, allow_rejected=True)
-    _, tg = _timed(ctx, lambda: api.compute_mse_with_gradient(ctx, P, ds, allow_rejected=True))
+    _, tg = _timed_best(ctx, lambda: api.compute_mse_with_gradient(ctx, P, ds, allow_rejected=True))
     fit, tl = _timed(ctx, lambda: api.fit_lbfgs(ctx, P, ds, iterations=20, history=5, mode=api.LBFGS_FP64,
                                                 mask=ok.astype(np.uint8)))
     out["mat3_lbfgs"] = {"programs": len(progs), "valid_programs": int(ok.sum()), "points": len(y),
@@ -158,7 +167,7 @@ def run_configs(ctx, log):
     ok = P.status(ds) == 0
     ne = node_evals([p for p, g in zip(progs, ok) if g], len(y))
     api.compute_mse(ctx, P, ds, allow_rejected=True)
-    _, te = _timed(ctx, lambda: api.compute_mse(ctx, P, ds, allow_rejected=True))
+    _, te = _timed_best(ctx, lambda: api.compute_mse(ctx, P, ds, allow_rejected=True))
     (fit, its), tc = _timed(ctx, lambda: api.fit_cmaes(ctx, P, ds, iterations=30, lam=32, mu=16, seed=7,
                                                        mask=ok.astype(np.uint8)))
     cand = int(ok.sum()) * 32

@@ -26,6 +26,7 @@ namespace rsr {
 
 static constexpr int kWaves = 4;  // waves per workgroup
 static constexpr int kTan = 4;    // tangent channels per dual pass
+static constexpr int kGradGroup = 4;  // programs per gradient work item (features loaded once)
 
 // Load feature columns [0, F) of the lane's points into the register files.
 template <int PPL>
@@ -256,48 +257,55 @@ __global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_s
     const int si = find_seg_wave(T, wave);
     const SegDesc& S = T.s[si];
     const int64_t w = wave - S.first_wave;
-    const int64_t per_tile = (int64_t)S.n_progs * S.n_pass;
-    const int64_t lt = w / per_tile;  // tile within the launch's range
+    const int64_t lt = w / S.n_groups;  // tile within the launch's range
     if (lt >= S.n_tiles) return;
     if (S.live != nullptr && *S.live == 0) return;
-    const int64_t rem = w - lt * per_tile;
-    const int sp = (int)(rem / S.n_pass);
-    const int pass = (int)(rem - (int64_t)sp * S.n_pass);
-    const int prog = (int)uni((uint32_t)S.order[sp]);
-    const int np = (int)uni(S.nparams[prog]);
-    if ((S.active != nullptr && !S.active[prog]) || np == 0 || pass * kTan >= np) return;
+    const int g = (int)(w - lt * S.n_groups);
+    const int first = g * S.group;
+    const int last = min(first + S.group, S.n_progs);
     float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * (1 + kTan) * 64;
     const int64_t base = (S.tile0 + lt) * 64;
     const int64_t pt = base + lane;
+    const bool valid = pt < S.count8;
 
+    // The features are loaded once for the group.  Every program and pass reuses the
+    // files: the code never writes a feature register, so the feature tangents stay zero,
+    // and every temporary is written before it is read.
     Files<1> R1;
     R1.a = (v32f)(0.0f);
     load_features<1>(R1, S.feat, S.F, S.npad, base, lane);
     DualFiles<kTan> G;
     G.v = R1.a;
     G.a = G.b = G.c = G.d = (v32f)(0.0f);
-    const ProgDesc pd = S.progs[prog];
-    const uint32_t k0 = (uint32_t)(pass * kTan);
-    run_dual<kTan>(G, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), k0, lds, lane);
-    const uint32_t o = uni(pd.out_reg) & 31u;
-    const bool valid = pt < S.count8;
-    const v32f vfile = G.v;
-    const float diff = vfile[o] - S.target[pt];
-    float* dst = S.partials + (lt * S.n_progs + sp) * S.slots;
-    if (pass == 0) {
-        const float sq = diff * diff;
-        const float s = wave_sum(valid ? sq : 0.0f);
-        if (lane == 0) dst[0] = s;
-    }
-    const float two_diff = 2.0f * diff;
-    const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
+    for (int sp = first; sp < last; ++sp) {
+        const int prog = (int)uni((uint32_t)S.order[sp]);
+        const int np = (int)uni(S.nparams[prog]);
+        if ((S.active != nullptr && !S.active[prog]) || np == 0) continue;
+        const ProgDesc pd = S.progs[prog];
+        const uint32_t o = uni(pd.out_reg) & 31u;
+        float* dst = S.partials + (lt * S.n_progs + sp) * S.slots;
+        for (int pass = 0; pass * kTan < np; ++pass) {
+            const uint32_t k0 = (uint32_t)(pass * kTan);
+            run_dual<kTan>(G, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), k0, lds,
+                           lane);
+            const v32f vfile = G.v;
+            const float diff = vfile[o] - S.target[pt];
+            if (pass == 0) {
+                const float sq = diff * diff;
+                const float s = wave_sum(valid ? sq : 0.0f);
+                if (lane == 0) dst[0] = s;
+            }
+            const float two_diff = 2.0f * diff;
+            const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
 #pragma unroll
-    for (int j = 0; j < kTan; ++j) {
-        const int k = (int)k0 + j;
-        if (k < np) {
-            const float gterm = two_diff * gf[j][o];
-            const float s = wave_sum(valid ? gterm : 0.0f);
-            if (lane == 0) dst[1 + k] = s;
+            for (int j = 0; j < kTan; ++j) {
+                const int k = (int)k0 + j;
+                if (k < np) {
+                    const float gterm = two_diff * gf[j][o];
+                    const float s = wave_sum(valid ? gterm : 0.0f);
+                    if (lane == 0) dst[1 + k] = s;
+                }
+            }
         }
     }
 }
@@ -527,10 +535,10 @@ hipError_t launch_grad_steps(const EvalStep* steps, int n, bool reference_order,
         for (int i = 0; i < cnt; ++i) {
             const EvalStep& st = steps[base + i];
             if (st.max_params <= 0 || st.P.n_progs <= 0) continue;
-            SegDesc d = make_seg(st, 1, 1);
+            SegDesc d = make_seg(st, 1, reference_order ? 1 : kGradGroup);
             d.first_wave = waves;
             d.first_thread = threads;
-            waves += reference_order ? (int64_t)d.n_progs * d.n_pass : d.n_tiles * d.n_progs * d.n_pass;
+            waves += reference_order ? (int64_t)d.n_progs * d.n_pass : d.n_tiles * d.n_groups;
             threads += ((int64_t)d.n_progs * d.slots + 63) / 64 * 64;  // whole waves per segment
             spill = max(spill, st.P.spill_slots);
             T.s[T.n++] = d;
