@@ -83,6 +83,390 @@ __device__ __forceinline__ Uop load_uop(const UInstr* __restrict__ code, uint32_
     return u;
 }
 
+// Primal interpreter at one point per lane (run_program below dispatches to it).
+// `lds` is this wave's spill area: slot s, point p at lds[(s * PPL + p) * 64 + lane].
+// The same micro-op semantics as run_program, with the accumulator machinery off (the
+// code is written for both and `kAcc` selects): at one point per lane a staging move or
+// a write-back costs one VALU op, less than the branch that would skip it.  Operands a
+// and b are always read from the file (one component; three-component operands read
+// their others in their case), and results are always written to it (three-component
+// results write components 1 and 2 in their case).  kFwd, kVec3, kNone and kNoWb are
+// ignored, which is exact since the file then holds every value.
+template <int PPL>
+__device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __restrict__ code, uint32_t len,
+                                               const float* __restrict__ params, float* __restrict__ lds, int lane) {
+    Uop nx = load_uop(code, 0);  // the code array ends with one padding entry
+    constexpr bool kAcc = false;
+    float A0[PPL], A1[PPL], A2[PPL];
+    sfor<PPL>([&](auto P) { A0[P()] = 0.0f; A1[P()] = 0.0f; A2[P()] = 0.0f; });
+    for (uint32_t pc = 0; pc < len; ++pc) {
+        const Uop u = nx;
+        nx = load_uop(code, pc + 1);  // prefetch
+        const uint32_t op = u.w0 & 255u, x = u.w0 >> 8;
+        // The masks (no-ops on valid code) give instruction selection the known-zero high
+        // bits it needs to fold the +1, +2, ... of block components into the base register
+        // of one indexed-move region instead of computing a new index per component.
+        const uint32_t d = u.d & 31u, a = u.a & 31u, b = u.b & 31u;
+        float B0[PPL], B1[PPL], B2[PPL];
+        if (kAcc && (u.b & kVec3)) {
+            sfor<PPL>([&](auto P) {
+                const v32f r = fget<P(), PPL>(R);
+                B0[P()] = r[b];
+                B1[P()] = r[b + 1];
+                B2[P()] = r[b + 2];
+            });
+        } else if (kAcc ? !(u.b & kNone) : true) {
+            sfor<PPL>([&](auto P) { B0[P()] = fget<P(), PPL>(R)[b]; });
+        }
+        // components 1 and 2 of a three-component operand, and the three-component result
+        auto a1 = [&](auto P) {
+            if constexpr (kAcc) return A1[P()];
+            else return fget<P(), PPL>(R)[a + 1];
+        };
+        auto a2 = [&](auto P) {
+            if constexpr (kAcc) return A2[P()];
+            else return fget<P(), PPL>(R)[a + 2];
+        };
+        auto b1 = [&](auto P) {
+            if constexpr (kAcc) return B1[P()];
+            else return fget<P(), PPL>(R)[b + 1];
+        };
+        auto b2 = [&](auto P) {
+            if constexpr (kAcc) return B2[P()];
+            else return fget<P(), PPL>(R)[b + 2];
+        };
+        auto put3 = [&](auto P, float t0, float t1, float t2) {
+            A0[P()] = t0;
+            if constexpr (kAcc) {
+                A1[P()] = t1;
+                A2[P()] = t2;
+            } else {
+                v32f t = fget<P(), PPL>(R);
+                t[d + 1] = t1;
+                t[d + 2] = t2;
+                fget<P(), PPL>(R) = t;
+            }
+        };
+        auto scalar_result = [&]() {
+            sfor<PPL>([&](auto P) {
+                A1[P()] = __builtin_nondeterministic_value(A1[P()]);
+                A2[P()] = __builtin_nondeterministic_value(A2[P()]);
+            });
+        };
+        if (kAcc ? !(u.a & (kFwd | kNone)) : true) {
+            if (kAcc && (u.a & kVec3)) {
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    A0[P()] = r[a];
+                    A1[P()] = r[a + 1];
+                    A2[P()] = r[a + 2];
+                });
+            } else {
+                sfor<PPL>([&](auto P) { A0[P()] = fget<P(), PPL>(R)[a]; });
+            }
+        }
+        switch (op) {
+            case U_IMM: {
+                const float v = kld(params, x);
+                sfor<PPL>([&](auto P) { A0[P()] = v; });
+                scalar_result();
+                break;
+            }
+#define RSR_CASE(OPC, EXPR)                                   \
+    case OPC:                                                 \
+        sfor<PPL>([&](auto P) {                               \
+            const float A = A0[P()], B = B0[P()];             \
+            (void)B;                                          \
+            A0[P()] = (EXPR);                                 \
+        });                                                   \
+        scalar_result();                                      \
+        break;
+            RSR_CASE(U_ADD, A + B)
+            RSR_CASE(U_SUB, A - B)
+            RSR_CASE(U_MUL, A * B)
+            RSR_CASE(U_DIV, A / B)
+            RSR_CASE(U_SQR, A * A)
+            RSR_CASE(U_SQRT, __builtin_sqrtf(A))
+            RSR_CASE(U_SIN, dm_sin(A))
+            RSR_CASE(U_COS, dm_cos(A))
+            RSR_CASE(U_EXP, dm_exp(A))
+            RSR_CASE(U_LN, dm_ln(A))
+            RSR_CASE(U_NEG, -A)
+            RSR_CASE(U_SQRTACC, __builtin_sqrtf(A + (B * B)))
+            RSR_CASE(U_RCPSAFE, 1.0f / ((__builtin_fabsf(A) < 1e-9f) ? 1.0f : A))
+            RSR_CASE(U_RSUB, B - A)
+            RSR_CASE(U_RDIV, B / A)
+#undef RSR_CASE
+            // point-invariant operand: one evaluation, copied to the lane's other points
+#define RSR_UCASE(OPC, EXPR)                                  \
+    case OPC: {                                               \
+        const float A = A0[0];                                \
+        const float r0 = (EXPR);                              \
+        sfor<PPL>([&](auto P) { A0[P()] = r0; });             \
+        scalar_result();                                      \
+        break;                                                \
+    }
+            RSR_UCASE(U_SINU, dm_sin(A))
+            RSR_UCASE(U_COSU, dm_cos(A))
+            RSR_UCASE(U_EXPU, dm_exp(A))
+            RSR_UCASE(U_LNU, dm_ln(A))
+            RSR_UCASE(U_SQRTU, __builtin_sqrtf(A))
+#undef RSR_UCASE
+            // unary op of a constant: one evaluation, copied to every point of the lane
+#define RSR_UKCASE(OPC, EXPR)                                 \
+    case OPC: {                                               \
+        const float A = kld(params, x);                       \
+        const float r0 = (EXPR);                              \
+        sfor<PPL>([&](auto P) { A0[P()] = r0; });             \
+        scalar_result();                                      \
+        break;                                                \
+    }
+            RSR_UKCASE(U_SINK, dm_sin(A))
+            RSR_UKCASE(U_COSK, dm_cos(A))
+            RSR_UKCASE(U_EXPK, dm_exp(A))
+            RSR_UKCASE(U_LNK, dm_ln(A))
+            RSR_UKCASE(U_SQRTK, __builtin_sqrtf(A))
+            RSR_UKCASE(U_NEGK, -A)
+            RSR_UKCASE(U_SQRK, A * A)
+#undef RSR_UKCASE
+            case U_MULADD:
+            case U_MULSUB: {
+                const uint32_t c = x & 31u, e = (x >> 5) & 31u;
+                const bool sub = op == U_MULSUB;
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    const float p1 = A0[P()] * B0[P()], p2 = r[c] * r[e];
+                    A0[P()] = sub ? p1 - p2 : p1 + p2;
+                });
+                scalar_result();
+                break;
+            }
+            case U_ADDMUL: {
+                const uint32_t c = x & 31u;
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    A0[P()] = A0[P()] + (B0[P()] * r[c]);
+                });
+                scalar_result();
+                break;
+            }
+            case U_ADD3: {
+                const uint32_t c = x & 31u;
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    A0[P()] = (A0[P()] + B0[P()]) + r[c];
+                });
+                scalar_result();
+                break;
+            }
+            case U_INVSEL: {
+                const uint32_t c = x & 31u, fl = (x >> 10) & 3u;
+                const float k = (fl & kInvOne) ? 1.0f : 0.0f;
+                const bool ng = (fl & kInvNeg) != 0u;
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    const float xv = ng ? -A0[P()] : A0[P()];
+                    A0[P()] = (__builtin_fabsf(r[c]) < 1e-9f) ? k : xv * B0[P()];
+                });
+                scalar_result();
+                break;
+            }
+            case U_SPILL:
+                sfor<PPL>([&](auto P) { lds[(x * PPL + P()) * 64u + spill_lane(x)] = A0[P()]; });
+                scalar_result();
+                break;
+            case U_FILL:
+                sfor<PPL>([&](auto P) { A0[P()] = lds[(x * PPL + P()) * 64u + spill_lane(x)]; });
+                scalar_result();
+                break;
+            // Vec3 block ops: every operand component is staged, the result replaces A
+#define RSR_V3CASE(OPC, E0, E1, E2)                                             \
+    case OPC:                                                                   \
+        sfor<PPL>([&](auto P) {                                                 \
+            const float x0 = A0[P()], x1 = a1(P), x2 = a2(P);                   \
+            const float y0 = B0[P()], y1 = b1(P), y2 = b2(P);                   \
+            (void)y1; (void)y2;                                                 \
+            put3(P, (E0), (E1), (E2));                                          \
+        });                                                                     \
+        break;
+            RSR_V3CASE(U_ADDV3, x0 + y0, x1 + y1, x2 + y2)
+            RSR_V3CASE(U_SUBV3, x0 - y0, x1 - y1, x2 - y2)
+            RSR_V3CASE(U_SCALEV3, x0 * y0, x1 * y0, x2 * y0)
+            RSR_V3CASE(U_CROSS3, (x1 * y2) - (x2 * y1), (x2 * y0) - (x0 * y2), (x0 * y1) - (x1 * y0))
+            RSR_V3CASE(U_RSUBV3, y0 - x0, y1 - x1, y2 - x2)
+            RSR_V3CASE(U_RCROSS3, (y1 * x2) - (y2 * x1), (y2 * x0) - (y0 * x2), (y0 * x1) - (y1 * x0))
+            RSR_V3CASE(U_SCALEV3S, y0 * x0, y1 * x0, y2 * x0)
+#undef RSR_V3CASE
+#define RSR_KCASE(OPC, EXPR)                               \
+    case OPC: {                                            \
+        const float K = kld(params, x);                    \
+        sfor<PPL>([&](auto P) {                            \
+            const float A = A0[P()];                       \
+            A0[P()] = (EXPR);                              \
+        });                                                \
+        scalar_result();                                   \
+        break;                                             \
+    }
+            RSR_KCASE(U_ADDK, A + K)
+            RSR_KCASE(U_SUBK, A - K)
+            RSR_KCASE(U_KSUB, K - A)
+            RSR_KCASE(U_MULK, A * K)
+            RSR_KCASE(U_DIVK, A / K)
+            RSR_KCASE(U_KDIV, K / A)
+#undef RSR_KCASE
+            case U_SCALEV3K: {
+                const float K = kld(params, x);
+                sfor<PPL>([&](auto P) { put3(P, A0[P()] * K, a1(P) * K, a2(P) * K); });
+                break;
+            }
+            case U_MULM3V3: {
+                __builtin_assume(a < 23u);
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    const float v0 = B0[P()], v1 = b1(P), v2 = b2(P);
+                    const float o0 = ((A0[P()] * v0) + (r[a + 3] * v1)) + (r[a + 6] * v2);
+                    const float o1 = ((r[a + 1] * v0) + (r[a + 4] * v1)) +
+                                     (r[a + 7] * v2);
+                    const float o2 = ((r[a + 2] * v0) + (r[a + 5] * v1)) +
+                                     (r[a + 8] * v2);
+                    put3(P, o0, o1, o2);
+                });
+                break;
+            }
+            case U_MULM3V3R: {
+                __builtin_assume(b < 23u);
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    const float v0 = A0[P()], v1 = a1(P), v2 = a2(P);
+                    const float o0 = ((B0[P()] * v0) + (r[b + 3] * v1)) + (r[b + 6] * v2);
+                    const float o1 = ((r[b + 1] * v0) + (r[b + 4] * v1)) +
+                                     (r[b + 7] * v2);
+                    const float o2 = ((r[b + 2] * v0) + (r[b + 5] * v1)) +
+                                     (r[b + 8] * v2);
+                    put3(P, o0, o1, o2);
+                });
+                break;
+            }
+            case U_SCALEM3:
+            case U_SCALEM3K: {
+                __builtin_assume(a < 23u);
+                __builtin_assume(d < 23u);
+                const float K = kld(params, x);
+                const bool kk = op == U_SCALEM3K;
+                // per point: all nine reads, then the writes (files of different points are disjoint)
+                sfor<PPL>([&](auto P) {
+                    v32f t = fget<P(), PPL>(R);
+                    const float sv = kk ? K : B0[P()];
+                    float rr[8];
+#pragma unroll
+                    for (int q = 1; q < 9; ++q) rr[q - 1] = t[a + q] * sv;
+                    A0[P()] = A0[P()] * sv;
+#pragma unroll
+                    for (int q = 1; q < 9; ++q) t[d + q] = rr[q - 1];
+                    fget<P(), PPL>(R) = t;
+                });
+                scalar_result();
+                break;
+            }
+            // Mat2 / four-component block ops: components 1..3 are read and written here,
+            // component 0 goes through the accumulator
+#define RSR_V4CASE(OPC, E0, E1, E2, E3, READB)                                  \
+    case OPC: {                                                                 \
+        __builtin_assume(a < 29u);                                              \
+        __builtin_assume(b < 29u);                                              \
+        __builtin_assume(d < 29u);                                              \
+        const float K = kld(params, x);                                         \
+        (void)K;                                                                \
+        sfor<PPL>([&](auto P) {                                                 \
+            v32f t = fget<P(), PPL>(R);                                         \
+            const float a0 = A0[P()], a1 = t[a + 1], a2 = t[a + 2], a3 = t[a + 3]; \
+            const float b0 = B0[P()];                                           \
+            float b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;                              \
+            if (READB) { b1 = t[b + 1]; b2 = t[b + 2]; b3 = t[b + 3]; }         \
+            (void)b0; (void)b1; (void)b2; (void)b3;                             \
+            const float o0 = (E0), o1 = (E1), o2 = (E2), o3 = (E3);             \
+            A0[P()] = o0;                                                       \
+            t[d + 1] = o1;                                                      \
+            t[d + 2] = o2;                                                      \
+            t[d + 3] = o3;                                                      \
+            fget<P(), PPL>(R) = t;                                              \
+        });                                                                     \
+        scalar_result();                                                        \
+        break;                                                                  \
+    }
+            RSR_V4CASE(U_ADDV4, a0 + b0, a1 + b1, a2 + b2, a3 + b3, true)
+            RSR_V4CASE(U_SUBV4, a0 - b0, a1 - b1, a2 - b2, a3 - b3, true)
+            RSR_V4CASE(U_SCALEV4, a0 * b0, a1 * b0, a2 * b0, a3 * b0, false)
+            RSR_V4CASE(U_SCALEV4K, a0 * K, a1 * K, a2 * K, a3 * K, false)
+            RSR_V4CASE(U_MULM2, (a0 * b0) + (a2 * b1), (a1 * b0) + (a3 * b1), (a0 * b2) + (a2 * b3),
+                       (a1 * b2) + (a3 * b3), true)
+#undef RSR_V4CASE
+            case U_INVM2: {
+                __builtin_assume(a < 29u);
+                __builtin_assume(d < 29u);
+                sfor<PPL>([&](auto P) {
+                    v32f t = fget<P(), PPL>(R);
+                    const float a0 = A0[P()], a1 = t[a + 1], a2 = t[a + 2], a3 = t[a + 3];
+                    const float det = (a0 * a3) - (a1 * a2);
+                    const bool sing = __builtin_fabsf(det) < 1e-9f;
+                    const float inv = 1.0f / (sing ? 1.0f : det);
+                    A0[P()] = sing ? 1.0f : a3 * inv;
+                    t[d + 1] = sing ? 0.0f : (-a1) * inv;
+                    t[d + 2] = sing ? 0.0f : (-a2) * inv;
+                    t[d + 3] = sing ? 1.0f : a0 * inv;
+                    fget<P(), PPL>(R) = t;
+                });
+                scalar_result();
+                break;
+            }
+            case U_IMMV3: {
+                const float v0 = kld(params, x), v1 = kld(params, x + 1), v2 = kld(params, x + 2);
+                sfor<PPL>([&](auto P) { put3(P, v0, v1, v2); });
+                break;
+            }
+            case U_DOT3:
+                sfor<PPL>([&](auto P) {
+                    A0[P()] = ((A0[P()] * B0[P()]) + (a1(P) * b1(P))) + (a2(P) * b2(P));
+                });
+                scalar_result();
+                break;
+            case U_NORM3:
+                sfor<PPL>([&](auto P) {
+                    const float x0 = A0[P()], x1 = a1(P), x2 = a2(P);
+                    A0[P()] = __builtin_sqrtf(((x0 * x0) + (x1 * x1)) + (x2 * x2));
+                });
+                scalar_result();
+                break;
+            default:
+                sfor<PPL>([&](auto P) { A0[P()] = 0.0f; });
+                scalar_result();
+                break;
+        }
+        // The write-back index also depends on the a and b words (their bits 16 and up
+        // are zero), which keeps them live to the end of the iteration: otherwise the
+        // compiler moves the prefetched next words into their registers right after the
+        // staging, which waits for the prefetch and exposes its latency on every op.
+        const uint32_t dw = (d + ((u.a | u.b) >> 16)) & 31u;
+        if (kAcc ? !(u.d & kNoWb) : true) {
+            if (kAcc && (u.d & kVec3)) {
+                sfor<PPL>([&](auto P) {
+                    v32f t = fget<P(), PPL>(R);
+                    t[dw] = A0[P()];
+                    t[dw + 1] = A1[P()];
+                    t[dw + 2] = A2[P()];
+                    fget<P(), PPL>(R) = t;
+                });
+            } else {
+                sfor<PPL>([&](auto P) {
+                    v32f t = fget<P(), PPL>(R);
+                    t[dw] = A0[P()];
+                    fget<P(), PPL>(R) = t;
+                });
+            }
+        }
+    }
+}
+
 // Primal interpreter over PPL register files (one per point of the lane).
 // `lds` is this wave's spill area: slot s, point p at lds[(s * PPL + p) * 64 + lane].
 // The interpreter is an accumulator machine over the register file: A0..A2 hold the
@@ -96,6 +480,10 @@ __device__ __forceinline__ Uop load_uop(const UInstr* __restrict__ code, uint32_
 template <int PPL>
 __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restrict__ code, uint32_t len,
                                             const float* __restrict__ params, float* __restrict__ lds, int lane) {
+    if constexpr (PPL == 1) {
+        run_program_p1<PPL>(R, code, len, params, lds, lane);
+        return;
+    }
     Uop nx = load_uop(code, 0);  // the code array ends with one padding entry
     float A0[PPL], A1[PPL], A2[PPL];
     sfor<PPL>([&](auto P) { A0[P()] = 0.0f; A1[P()] = 0.0f; A2[P()] = 0.0f; });
