@@ -299,6 +299,14 @@ RSR_API int rsr_lower_check(const uint8_t* ops, const uint16_t* args, int n_code
                             const float* feat, int64_t n, float* vals_out, float* tan_out, int32_t* n_uops,
                             int32_t* max_reg);
 
+/* Lowered micro-code (test hook, no reference counterpart, no GPU): the micro-ops that
+ * the evaluation entry points would run for one program, four words each (opcode and
+ * operand words with their flags), into words_out[4 * capacity].  *n_uops receives the
+ * full count.  Returns the RSR_PROG_* status. */
+RSR_API int rsr_lower_code(const uint8_t* ops, const uint16_t* args, int n_code, const uint8_t* const_types,
+                           const float* const_values, int n_consts, int num_features, int dual,
+                           uint32_t* words_out, int32_t capacity, int32_t* n_uops);
+
 #ifdef __cplusplus
 }
 #endif

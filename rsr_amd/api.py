@@ -308,6 +308,27 @@ def eval_points_dual(ctx: Context, progs: Programs, ds: Dataset, program: int, a
     return v, g
 
 
+def lower_code(prog: Program, num_features: int, dual: bool = False):
+    """Lowered micro-code of one program (test hook, no GPU): (status, words[n][4] uint32)."""
+    l = lib()
+    fn = l.rsr_lower_code
+    fn.restype = C.c_int
+    fn.argtypes = [_u8p, _u16p, C.c_int, _u8p, _f32p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32), C.c_int32,
+                   _i32p]
+    ops, args, co, ct, cof, cv, vo = pack_programs([prog])
+    ops = ops if ops.size else np.zeros(1, np.uint8)
+    args = args if args.size else np.zeros(1, np.uint16)
+    ct = ct if ct.size else np.zeros(1, np.uint8)
+    cv = cv if cv.size else np.zeros(1, np.float32)
+    cap = 4 * max(len(prog.code), 1) + 64
+    words = np.zeros((cap, 4), np.uint32)
+    n = C.c_int32()
+    st = fn(_ptr(ops, _u8p), _ptr(args, _u16p), len(prog.code), _ptr(ct, _u8p), _ptr(cv, _f32p),
+            len(prog.constants), num_features, int(dual), words.ctypes.data_as(C.POINTER(C.c_uint32)), cap,
+            C.byref(n))
+    return st, words[: min(n.value, cap)]
+
+
 def lower_check(prog: Program, rows_soa: np.ndarray, dual: bool = False, active_param: int = 0):
     """Lower one program and run its micro-code on the host (lowering self-check, no GPU).
 

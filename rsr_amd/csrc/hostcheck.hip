@@ -7,6 +7,7 @@
 // __host__ __device__), so this also pins the device transcendental code on the CPU.
 #include <math.h>
 #include <string.h>
+#include <algorithm>
 #include <vector>
 #include "../../include/rsr_amd.h"
 #include "devmath.h"
@@ -255,6 +256,25 @@ extern "C" RSR_API int rsr_lower_check(const uint8_t* ops, const uint16_t* args,
         run_host(lw.code, params.data(), R, G, dual ? active_param : -1, spill);
         if (vals_out) vals_out[i] = R[lw.out_reg];
         if (tan_out) tan_out[i] = G[lw.out_reg];
+    }
+    return 0;
+}
+
+// Lowered micro-code of one program (value code when dual == 0) as four words per
+// micro-op, for tests that check which micro-ops and flags the lowering emits.
+extern "C" RSR_API int rsr_lower_code(const uint8_t* ops, const uint16_t* args, int n_code, const uint8_t* ctypes,
+                                      const float* cvals, int n_consts, int num_features, int dual,
+                                      uint32_t* words_out, int32_t capacity, int32_t* n_uops) {
+    RefProgram p{ops, args, n_code, ctypes, cvals, n_consts};
+    Lowered lw = lower_program(p, num_features, dual != 0);
+    if (n_uops) *n_uops = (int32_t)lw.code.size();
+    if (lw.status != PROG_OK) return lw.status;
+    const int32_t n = std::min<int32_t>(capacity, (int32_t)lw.code.size());
+    for (int32_t i = 0; i < n; ++i) {
+        words_out[4 * i] = lw.code[i].w0;
+        words_out[4 * i + 1] = lw.code[i].w1;
+        words_out[4 * i + 2] = lw.code[i].w2;
+        words_out[4 * i + 3] = lw.code[i].w3;
     }
     return 0;
 }

@@ -203,3 +203,13 @@ def node_evals(progs: Sequence[Program], n_points: int) -> int:
 def alg_flops(progs: Sequence[Program]) -> int:
     """Algorithmic f32 operations per point summed over programs (+3 per program for the MSE)."""
     return sum(sum(ALG_OPS.get(Op(o), 0) for o, _ in p.code) + 3 for p in progs)
+
+
+# Device micro-op names in opcode order and the operand flags (rsr_amd/csrc/isa.h); for
+# tests that inspect the lowered code through api.lower_code.
+UOPS = ("IMM ADD SUB MUL DIV SQR SQRT SIN COS EXP LN NEG MULADD MULSUB ADDMUL SQRTACC ADD3 RCPSAFE INVSEL "
+        "SPILL FILL ADDV3 SUBV3 SCALEV3 IMMV3 CROSS3 DOT3 NORM3 ADDK SUBK KSUB MULK DIVK KDIV SCALEV3K "
+        "MULM3V3 SCALEM3 SCALEM3K ADDV4 SUBV4 SCALEV4 SCALEV4K MULM2 INVM2 RSUB RDIV RSUBV3 RCROSS3 "
+        "SCALEV3S MULM3V3R SINU COSU EXPU LNU SQRTU SINK COSK EXPK LNK SQRTK NEGK SQRK").split()
+UFLAG_FWD, UFLAG_VEC3, UFLAG_NONE, UFLAG_NOWB = 1 << 8, 1 << 9, 1 << 10, 1 << 11
+

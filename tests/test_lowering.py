@@ -149,3 +149,57 @@ def test_aliasing_ops_emit_no_code():
     p = Program([(Op.LoadVarF, 0), (Op.LoadVarF, 0), (Op.LoadVarF, 0), (Op.MakeVec3, 0), (Op.GetYV3, 0)], [])
     st, v, _, nu, _ = api.lower_check(p, np.array([[3.0, 4.0]], f32))
     assert st == 0 and nu == 0 and list(v) == [3.0, 4.0]
+
+
+def _uop_names(prog, F):
+    from rsr_amd.isa import UOPS
+    st, words = api.lower_code(prog, F)
+    assert st == 0
+    return [UOPS[w[0] & 255] for w in words], words
+
+
+def test_uop_name_table_matches_isa_header():
+    import os
+    import re
+    from rsr_amd.isa import UOPS
+    text = open(os.path.join(os.path.dirname(__file__), "..", "rsr_amd", "csrc", "isa.h")).read()
+    body = text[text.index("enum UOp"):text.index("U_COUNT")]
+    names = re.findall(r"\bU_(\w+)\s*(?:=\s*0)?\s*,", body)
+    assert names == UOPS
+
+
+@pytest.mark.parametrize("which", ["lorentz", "mat3"])
+def test_forwarding_paths_bit_exact_with_poisoned_file(which):
+    """Each program produces the micro-op it is built for, and its host execution (where a
+    dropped write-back leaves NaN in the register file) equals the oracle bit for bit."""
+    import forwarding_programs as FP
+    from rsr_amd.isa import UFLAG_FWD, UFLAG_NOWB
+    if which == "lorentz":
+        progs, (X, y, t) = FP.lorentz_programs(), W.data_c1(300, seed=61)
+    else:
+        progs, (X, y, t) = FP.mat3_programs(), W.data_c2(300, seed=62)
+    F = X.shape[1]
+    fwd = nowb = 0
+    for name, want, p in progs:
+        names, words = _uop_names(p, F)
+        assert want in names, (name, names)
+        fwd += int(((words[:, 2] & UFLAG_FWD) != 0).sum())
+        nowb += int(((words[:, 1] & UFLAG_NOWB) != 0).sum())
+        # dual code is never annotated
+        st, dwords = api.lower_code(p, F, dual=True)
+        assert st == 0 and not ((dwords[:, 1:] >> 8) & 0xF).any(), name
+    assert fwd > 0 and nowb > 0
+    assert check_programs([p for _, _, p in progs], X, y, dual_params=1) == 0
+
+
+def test_forwarded_operand_is_always_a():
+    """After the swap pass no micro-op takes the previous result through operand b."""
+    progs = W.gen_c1(200, seed=63)
+    for p in progs:
+        st, words = api.lower_code(p, 9)
+        if st != 0:
+            continue
+        for i in range(1, len(words)):
+            d_prev = words[i - 1][1] & 31
+            if words[i][2] & (1 << 8):
+                assert (words[i][2] & 31) == d_prev
