@@ -205,6 +205,7 @@ def main():
     ap.add_argument("--cpu-programs-1t", type=int, default=512)
     ap.add_argument("--copy-probe", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving job mix")
+    ap.add_argument("--no-graphs", action="store_true", help="skip the graph-replay modes")
     ap.add_argument("--no-configs", action="store_true", help="skip the matrix / fitting workloads")
     ap.add_argument("--serve-jobs", type=int, default=64)
     ap.add_argument("--serve-gens", type=int, default=3)
@@ -239,13 +240,20 @@ def main():
         ctx.set_tuning(2, 8)
     ne = node_evals(progs, len(y))
     modes = ["kahan", "reference"] if args.summation == "auto" else [args.summation]
+    if not args.no_graphs:  # the same launches replayed from a captured hipGraph
+        modes += [m + "+graph" for m in modes]
     runs = {}
     for m in modes:
-        log(f"[bench] timed steps, {m} summation")
-        ctx.set_summation(m == "reference")
+        base, graph = m.split("+")[0], m.endswith("+graph")
+        log(f"[bench] timed steps, {base} summation" + (", graph replay" if graph else ""))
+        ctx.set_summation(base == "reference")
+        ctx.set_graphs(graph)
         sec, kern_ms, launches, mse = time_steps(ctx, P, ds, args.steps, args.warmup)
+        if graph:  # the kernel timer does not see graph replays; the kernels are the eager ones
+            kern_ms, launches = runs[base]["kernel_ms"], runs[base]["launches"]
         runs[m] = {"sec": sec, "kernel_ms": kern_ms, "launches": launches, "mse": mse}
     ctx.set_summation(False)
+    ctx.set_graphs(False)
 
     plain = None
     if not args.no_plain:
@@ -334,7 +342,7 @@ def main():
     traffic = load_traffic()
     traffic_bytes = None
     if traffic:  # HBM bytes per launch of the same kernel and workload, from FETCH_SIZE / WRITE_SIZE runs
-        traffic_bytes = traffic.get("modes", {}).get(head, {}).get("hbm_bytes_per_launch")
+        traffic_bytes = traffic.get("modes", {}).get(head.split("+")[0], {}).get("hbm_bytes_per_launch")
 
     out = {
         "metric": "node-evals/s + p99 gen latency, served vs plain GPU eval, MSE rel err \u22641e-5",
@@ -366,7 +374,8 @@ def main():
                      "frac_of_nofma_ceiling": achieved / PEAK_FP32_NOFMA_TOPS,
                      "kernel_ms": r["kernel_ms"], "launches": r["launches"],
                      "flops_per_launch": flops_launch, "compulsory_bytes_per_launch": bytes_launch,
-                     "traffic": traffic_bytes},
+                     "traffic": traffic_bytes,
+                     "hbm_gbs": traffic_bytes / r["kernel_ms"] * 1e-6 if traffic_bytes else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
