@@ -1018,7 +1018,16 @@ int ensure_counters(rsr_context* ctx, size_t n_ints) {
         HIP_TRY(hipHostMalloc((void**)&ctx->host_counters, cap * sizeof(int), hipHostMallocDefault), "counters");
         ctx->host_counters_cap = cap;
     }
-    return ensure_scratch(ctx, ctx->dev_counters, n_ints * sizeof(int));
+    const size_t before = ctx->dev_counters.bytes;
+    const int rc = ensure_scratch(ctx, ctx->dev_counters, n_ints * sizeof(int));
+    if (rc) return rc;
+    if (ctx->dev_counters.bytes != before || ctx->counters_dirty) {
+        // a new buffer, or a tick that did not read its counters back: start from zero
+        HIP_TRY(hipMemsetAsync(ctx->dev_counters.ptr, 0, ctx->dev_counters.bytes, ctx->stream), "counter clear");
+        ctx->counters_seen.assign(ctx->dev_counters.bytes / sizeof(int), 0u);
+        ctx->counters_dirty = false;
+    }
+    return RSR_OK;
 }
 
 // Run steps of one kind (MSE or gradient) at one points-per-lane.  The per-tile Kahan
@@ -1136,7 +1145,7 @@ int run_tick(rsr_context* ctx, const std::vector<Job*>& top) {
     if (counters) {
         if ((rc = ensure_counters(ctx, 2 * nj))) return rc;
         dc = (int*)ctx->dev_counters.ptr;
-        HIP_TRY(hipMemsetAsync(dc, 0, 2 * nj * sizeof(int), ctx->stream), "counter clear");
+        ctx->counters_dirty = true;  // until this tick's increases are read back
     }
     std::vector<EvalStep> mse, grad;
     for (size_t k = 0; k < nj; ++k) {
@@ -1149,10 +1158,19 @@ int run_tick(rsr_context* ctx, const std::vector<Job*>& top) {
         if ((rc = j->post())) return rc;
     // without counters the next stream synchronisation is the job's own finish()
     int* hc = nullptr;
+    std::vector<int> delta;
     if (counters) {
-        hc = ctx->host_counters;
-        HIP_TRY(hipMemcpyAsync(hc, dc, 2 * nj * sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "counter download");
+        int* raw = ctx->host_counters;
+        HIP_TRY(hipMemcpyAsync(raw, dc, 2 * nj * sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "counter download");
         HIP_TRY(hipStreamSynchronize(ctx->stream), "tick");
+        delta.resize(2 * nj);
+        for (size_t i = 0; i < 2 * nj; ++i) {  // this tick's increase of each slot
+            const uint32_t now = (uint32_t)raw[i];
+            delta[i] = (int)(now - ctx->counters_seen[i]);
+            ctx->counters_seen[i] = now;
+        }
+        ctx->counters_dirty = false;
+        hc = delta.data();
     }
     for (size_t k = 0; k < nj; ++k) {
         jobs[k]->ticks++;

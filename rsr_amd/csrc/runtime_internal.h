@@ -83,6 +83,10 @@ struct rsr_context {
     rsr::DevBuf dev_counters;
     int* host_counters = nullptr;
     size_t host_counters_cap = 0;
+    // The device counters only ever grow (kernels atomically add to them); a tick reads
+    // each slot's increase since the last read instead of clearing the slots first.
+    std::vector<uint32_t> counters_seen;
+    bool counters_dirty = true;  // cleared on the device before the next use
     // profiling: events around every main evaluation kernel, accumulated on read
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
