@@ -18,12 +18,12 @@
 
 namespace rsr {
 
-int JobBuffers::raw(size_t bytes, void** out) {
+int JobBuffers::raw(size_t bytes, void** out, bool clear) {
     const size_t n = std::max<size_t>(bytes, 16);
     bufs.emplace_back(new DevBuf());
     int rc = bufs.back()->alloc(ctx, n);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(bufs.back()->ptr, 0, n, ctx->stream), "job buffer clear");
+    if (clear) HIP_TRY(hipMemsetAsync(bufs.back()->ptr, 0, n, ctx->stream), "job buffer clear");
     *out = bufs.back()->ptr;
     return RSR_OK;
 }

@@ -31,15 +31,17 @@ struct JobBuffers {
     std::vector<std::unique_ptr<DevBuf>> bufs;
     std::vector<std::vector<char>> keep;
     explicit JobBuffers(rsr_context* c) : ctx(c) {}
-    int raw(size_t bytes, void** out);
+    // A zero-filled buffer (clear = false: its contents are written before any read).
+    int raw(size_t bytes, void** out, bool clear = true);
     int upload(void* dst, const void* src, size_t bytes);
     template <typename T>
     int get(size_t count, T** out) {
         return raw(count * sizeof(T), (void**)out);
     }
+    // A buffer holding a copy of v (uploaded, so not cleared first).
     template <typename T>
     int put(const std::vector<T>& v, T** out) {
-        int rc = get<T>(v.size(), out);
+        int rc = raw(v.size() * sizeof(T), (void**)out, v.empty());
         if (rc || v.empty()) return rc;
         return upload(*out, v.data(), v.size() * sizeof(T));
     }
