@@ -111,8 +111,9 @@ class Context:
         self.h = h
 
     def close(self):
+        """Destroy the context (raises, and keeps it, while one of its servers is alive)."""
         if self.h:
-            lib().rsr_context_destroy(self.h)
+            _check(lib().rsr_context_destroy(self.h))
             self.h = None
 
     def __del__(self):

@@ -877,6 +877,25 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
         }
         float rv;
         float rg[T];
+        // components 1.. of a Vec3 / Mat2 result, in the value file and every tangent file
+        auto put3 = [&](float v1, float v2, const float (&g1)[T], const float (&g2)[T]) {
+            { v32f t = G.v; t[d + 1] = v1; t[d + 2] = v2; G.v = t; }
+            { v32f t = G.a; t[d + 1] = g1[0]; t[d + 2] = g2[0]; G.a = t; }
+            if constexpr (T >= 2) { v32f t = G.b; t[d + 1] = g1[1]; t[d + 2] = g2[1]; G.b = t; }
+            if constexpr (T >= 4) {
+                { v32f t = G.c; t[d + 1] = g1[2]; t[d + 2] = g2[2]; G.c = t; }
+                { v32f t = G.d; t[d + 1] = g1[3]; t[d + 2] = g2[3]; G.d = t; }
+            }
+        };
+        auto put4 = [&](const float (&out)[4], const float (&og)[4][T]) {
+            { v32f t = G.v; t[d + 1] = out[1]; t[d + 2] = out[2]; t[d + 3] = out[3]; G.v = t; }
+            { v32f t = G.a; t[d + 1] = og[1][0]; t[d + 2] = og[2][0]; t[d + 3] = og[3][0]; G.a = t; }
+            if constexpr (T >= 2) { v32f t = G.b; t[d + 1] = og[1][1]; t[d + 2] = og[2][1]; t[d + 3] = og[3][1]; G.b = t; }
+            if constexpr (T >= 4) {
+                { v32f t = G.c; t[d + 1] = og[1][2]; t[d + 2] = og[2][2]; t[d + 3] = og[3][2]; G.c = t; }
+                { v32f t = G.d; t[d + 1] = og[1][3]; t[d + 2] = og[2][3]; t[d + 3] = og[3][3]; G.d = t; }
+            }
+        };
         switch (op) {
             case U_IMM:
                 rv = kld(params, x);
@@ -1113,7 +1132,11 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                 }
                 break;
             }
-            case U_ADDV4: case U_SUBV4: case U_SCALEV4: case U_SCALEV4K: case U_MULM2: {
+            // Block ops write their components 1.. inside the case.  Ops whose operand b is
+            // a block (b + k inside the file) and ops whose b is a scalar (any register,
+            // 30 and 31 included) are separate cases, so that no component index is ever
+            // formed past the end of the file.
+            case U_ADDV4: case U_SUBV4: case U_MULM2: {
                 __builtin_assume(a < 29u);
                 __builtin_assume(b < 29u);
                 __builtin_assume(d < 29u);
@@ -1124,10 +1147,6 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                     A[q] = Vv[a + q];
                     B[q] = Vv[b + q];
                 }
-                const bool kk = op == U_SCALEV4K;
-                const float sv = kk ? kld(params, x) : B[0];
-                float gs[T];
-                RSR_TAN(gs[j] = kk ? ((x == k0 + (uint32_t)j) ? 1.0f : 0.0f) : gb[j]);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     if (op == U_ADDV4) {
@@ -1136,32 +1155,40 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                     } else if (op == U_SUBV4) {
                         out[q] = A[q] - B[q];
                         RSR_TAN(og[q][j] = Gj[a + q] - Gj[b + q]);
-                    } else if (op == U_MULM2) {
+                    } else {
                         const int r = q & 1, c = q >> 1;
                         out[q] = (A[r] * B[2 * c]) + (A[r + 2] * B[2 * c + 1]);
                         RSR_TAN(og[q][j] = ((A[r] * Gj[b + 2 * c]) + (Gj[a + r] * B[2 * c])) +
                                            ((A[r + 2] * Gj[b + 2 * c + 1]) + (Gj[a + r + 2] * B[2 * c + 1])));
-                    } else {
-                        out[q] = A[q] * sv;
-                        RSR_TAN(og[q][j] = (A[q] * gs[j]) + (Gj[a + q] * sv));
                     }
                 }
                 rv = out[0];
                 RSR_TAN(rg[j] = og[0][j]);
-                {
-                    v32f t = G.v;
-                    t[d + 1] = out[1]; t[d + 2] = out[2]; t[d + 3] = out[3];
-                    G.v = t;
-                }
-                { v32f t = G.a; t[d + 1] = og[1][0]; t[d + 2] = og[2][0]; t[d + 3] = og[3][0]; G.a = t; }
-                if constexpr (T >= 2) { v32f t = G.b; t[d + 1] = og[1][1]; t[d + 2] = og[2][1]; t[d + 3] = og[3][1]; G.b = t; }
-                if constexpr (T >= 4) {
-                    { v32f t = G.c; t[d + 1] = og[1][2]; t[d + 2] = og[2][2]; t[d + 3] = og[3][2]; G.c = t; }
-                    { v32f t = G.d; t[d + 1] = og[1][3]; t[d + 2] = og[2][3]; t[d + 3] = og[3][3]; G.d = t; }
-                }
+                put4(out, og);
                 break;
             }
-            case U_ADDV3: case U_SUBV3: case U_SCALEV3: case U_SCALEV3K: case U_CROSS3: case U_IMMV3: {
+            case U_SCALEV4: case U_SCALEV4K: {
+                __builtin_assume(a < 29u);
+                __builtin_assume(d < 29u);
+                const v32f Vv = G.v;
+                const bool kk = op == U_SCALEV4K;
+                // the scalar: register b, or the parameter x (tangent 1 on its own channel)
+                const float sv = kk ? kld(params, x) : vb;
+                float gs[T];
+                RSR_TAN(gs[j] = kk ? ((x == k0 + (uint32_t)j) ? 1.0f : 0.0f) : gb[j]);
+                float out[4], og[4][T];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float aq = Vv[a + q];
+                    out[q] = aq * sv;
+                    RSR_TAN(og[q][j] = (aq * gs[j]) + (Gj[a + q] * sv));
+                }
+                rv = out[0];
+                RSR_TAN(rg[j] = og[0][j]);
+                put4(out, og);
+                break;
+            }
+            case U_ADDV3: case U_SUBV3: case U_CROSS3: {
                 __builtin_assume(a < 30u);
                 __builtin_assume(b < 30u);
                 __builtin_assume(d < 30u);
@@ -1175,7 +1202,27 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                 } else if (op == U_SUBV3) {
                     rv = va - vb; v1 = a1 - b1; v2 = a2 - b2;
                     RSR_TAN(rg[j] = ga[j] - gb[j]; g1[j] = Gj[a + 1] - Gj[b + 1]; g2[j] = Gj[a + 2] - Gj[b + 2]);
-                } else if (op == U_SCALEV3 || op == U_SCALEV3K) {
+                } else {
+                    rv = (a1 * b2) - (a2 * b1); v1 = (a2 * vb) - (va * b2); v2 = (va * b1) - (a1 * vb);
+                    RSR_TAN(rg[j] = ((a1 * Gj[b + 2]) + (Gj[a + 1] * b2)) - ((a2 * Gj[b + 1]) + (Gj[a + 2] * b1));
+                            g1[j] = ((a2 * gb[j]) + (Gj[a + 2] * vb)) - ((va * Gj[b + 2]) + (ga[j] * b2));
+                            g2[j] = ((va * Gj[b + 1]) + (ga[j] * b1)) - ((a1 * gb[j]) + (Gj[a + 1] * vb)));
+                }
+                put3(v1, v2, g1, g2);
+                break;
+            }
+            case U_SCALEV3: case U_SCALEV3K: case U_IMMV3: {
+                __builtin_assume(a < 30u);
+                __builtin_assume(d < 30u);
+                float v1, v2, g1[T], g2[T];
+                if (op == U_IMMV3) {
+                    rv = kld(params, x); v1 = kld(params, x + 1); v2 = kld(params, x + 2);
+                    RSR_TAN(rg[j] = (x == k0 + (uint32_t)j) ? 1.0f : 0.0f;
+                            g1[j] = (x + 1u == k0 + (uint32_t)j) ? 1.0f : 0.0f;
+                            g2[j] = (x + 2u == k0 + (uint32_t)j) ? 1.0f : 0.0f);
+                } else {
+                    const v32f Vv = G.v;
+                    const float a1 = Vv[a + 1], a2 = Vv[a + 2];
                     // the scalar: register b, or the parameter x (tangent 1 on its own channel)
                     const bool kk = op == U_SCALEV3K;
                     const float sv = kk ? kld(params, x) : vb;
@@ -1184,29 +1231,8 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                     rv = va * sv; v1 = a1 * sv; v2 = a2 * sv;
                     RSR_TAN(rg[j] = (va * gs[j]) + (ga[j] * sv); g1[j] = (a1 * gs[j]) + (Gj[a + 1] * sv);
                             g2[j] = (a2 * gs[j]) + (Gj[a + 2] * sv));
-                } else if (op == U_CROSS3) {
-                    rv = (a1 * b2) - (a2 * b1); v1 = (a2 * vb) - (va * b2); v2 = (va * b1) - (a1 * vb);
-                    RSR_TAN(rg[j] = ((a1 * Gj[b + 2]) + (Gj[a + 1] * b2)) - ((a2 * Gj[b + 1]) + (Gj[a + 2] * b1));
-                            g1[j] = ((a2 * gb[j]) + (Gj[a + 2] * vb)) - ((va * Gj[b + 2]) + (ga[j] * b2));
-                            g2[j] = ((va * Gj[b + 1]) + (ga[j] * b1)) - ((a1 * gb[j]) + (Gj[a + 1] * vb)));
-                } else {
-                    rv = kld(params, x); v1 = kld(params, x + 1); v2 = kld(params, x + 2);
-                    RSR_TAN(rg[j] = (x == k0 + (uint32_t)j) ? 1.0f : 0.0f;
-                            g1[j] = (x + 1u == k0 + (uint32_t)j) ? 1.0f : 0.0f;
-                            g2[j] = (x + 2u == k0 + (uint32_t)j) ? 1.0f : 0.0f);
                 }
-                {
-                    v32f t = G.v;
-                    t[d + 1] = v1;
-                    t[d + 2] = v2;
-                    G.v = t;
-                }
-                { v32f t = G.a; t[d + 1] = g1[0]; t[d + 2] = g2[0]; G.a = t; }
-                if constexpr (T >= 2) { v32f t = G.b; t[d + 1] = g1[1]; t[d + 2] = g2[1]; G.b = t; }
-                if constexpr (T >= 4) {
-                    { v32f t = G.c; t[d + 1] = g1[2]; t[d + 2] = g2[2]; G.c = t; }
-                    { v32f t = G.d; t[d + 1] = g1[3]; t[d + 2] = g2[3]; G.d = t; }
-                }
+                put3(v1, v2, g1, g2);
                 break;
             }
             default:

@@ -236,6 +236,8 @@ extern "C" int rsr_context_synchronize(rsr_context* ctx) {
 
 extern "C" int rsr_context_destroy(rsr_context* ctx) {
     if (!ctx) return RSR_OK;
+    if (ctx->live_servers.load() > 0)
+        return set_error(RSR_E_INVALID_ARG, "the context still has a server; destroy the server first");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     drain_profile(ctx);

@@ -6,6 +6,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <atomic>
 #include <vector>
 
 #include "kernels.h"
@@ -86,6 +87,7 @@ struct rsr_context {
     // The device counters only ever grow (kernels atomically add to them); a tick reads
     // each slot's increase since the last read instead of clearing the slots first.
     std::vector<uint32_t> counters_seen;
+    std::atomic<int> live_servers{0};  // a context is not destroyed under a running server
     bool counters_dirty = true;  // cleared on the device before the next use
     // profiling: events around every main evaluation kernel, accumulated on read
     bool profiling = false;

@@ -260,6 +260,7 @@ extern "C" int rsr_server_create(rsr_context* ctx, int max_inflight, uint64_t bu
     } catch (...) {
         return set_error(RSR_E_UNSUPPORTED, "could not start the server thread");
     }
+    ctx->live_servers++;
     *out = s.release();
     return RSR_OK;
 }
@@ -362,6 +363,7 @@ extern "C" int rsr_server_destroy(rsr_server* s) {
     }
     s->cv.notify_all();
     if (s->th.joinable()) s->th.join();
+    s->ctx->live_servers--;
     delete s;
     return RSR_OK;
 }

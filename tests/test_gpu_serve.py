@@ -83,21 +83,23 @@ def test_mix_equals_alone(gpu_ctx, reference):
         reqs = setups(gpu_ctx)
         alone = [run_alone(gpu_ctx, *r) for r in reqs]
         srv = api.Server(gpu_ctx, max_inflight=8)
-        Ps = [api.Programs(gpu_ctx, r[2]) for r in reqs]
-        tickets = [submit(srv, r[0], r[1], P, r[3]) for r, P in zip(reqs[:3], Ps[:3])]
-        time.sleep(0.002)  # the rest arrive while the first three are on the GPU
-        tickets += [submit(srv, r[0], r[1], P, r[3]) for r, P in zip(reqs[3:], Ps[3:])]
-        for k, (tk, (res, consts)) in enumerate(zip(tickets, alone)):
-            got = tk.wait(60.0)
+        try:
+            Ps = [api.Programs(gpu_ctx, r[2]) for r in reqs]
+            tickets = [submit(srv, r[0], r[1], P, r[3]) for r, P in zip(reqs[:3], Ps[:3])]
+            time.sleep(0.002)  # the rest arrive while the first three are on the GPU
+            tickets += [submit(srv, r[0], r[1], P, r[3]) for r, P in zip(reqs[3:], Ps[3:])]
+            results = [tk.wait(60.0) for tk in tickets]
+            st = srv.stats()
+        finally:
+            srv.close()  # joins the loop thread even when a request failed
+        for k, (got, (res, consts)) in enumerate(zip(results, alone)):
             got = got if isinstance(got, tuple) else (got,)
             assert len(got) == len(res)
             for g, r in zip(got, res):
                 assert same(g, r), (k, reqs[k][0])
             assert same(Ps[k].get_constants(), consts), (k, reqs[k][0])
-        st = srv.stats()
         assert st["completed"] == len(reqs) and st["failed"] == 0
         assert st["max_inflight_seen"] >= 2  # requests overlapped on the device
-        srv.close()
     finally:
         gpu_ctx.set_summation(False)
 
@@ -107,26 +109,30 @@ def test_admission_slots_and_order(gpu_ctx):
     ds = api.Dataset.new(gpu_ctx, X, y, t)
     progs = valid_programs(gpu_ctx, W.gen_c0(300, seed=72), ds)
     srv = api.Server(gpu_ctx, max_inflight=2)
-    P = api.Programs(gpu_ctx, progs)
-    before = api.compute_mse(gpu_ctx, P, ds)
-    # same program batch: eval, fit, eval run in order; the last sees the fitted constants
-    t1 = srv.submit_eval(P, ds)
-    t2 = srv.submit_lbfgs(P, ds, iterations=5)
-    t3 = srv.submit_eval(P, ds)
-    others = [srv.submit_eval(api.Programs(gpu_ctx, progs), ds) for _ in range(5)]
-    assert same(t1.wait(60.0), before)
-    fit = t2.wait(60.0)
-    after = t3.wait(60.0)
+    try:
+        P = api.Programs(gpu_ctx, progs)
+        before = api.compute_mse(gpu_ctx, P, ds)
+        # same program batch: eval, fit, eval run in order; the last sees the fitted constants
+        t1 = srv.submit_eval(P, ds)
+        t2 = srv.submit_lbfgs(P, ds, iterations=5)
+        t3 = srv.submit_eval(P, ds)
+        others = [srv.submit_eval(api.Programs(gpu_ctx, progs), ds) for _ in range(5)]
+        r1 = t1.wait(60.0)
+        fit = t2.wait(60.0)
+        after = t3.wait(60.0)
+        rest = [o.wait(60.0) for o in others]
+        st = srv.stats()
+        q, r, ticks = t3.timing()
+    finally:
+        srv.close()  # joins the loop thread even when a request failed
+    assert same(r1, before)
     assert same(after, api.compute_mse(gpu_ctx, P, ds))
     assert np.isnan(fit).sum() < len(fit)  # some programs had constants to fit
-    for o in others:
-        assert same(o.wait(60.0), before)
-    st = srv.stats()
+    for o in rest:
+        assert same(o, before)
     assert st["max_inflight_seen"] <= 2
     assert st["completed"] == 8
-    q, r, ticks = t3.timing()
     assert q >= 0.0 and r >= 0.0 and ticks >= 1
-    srv.close()
 
 
 def test_budget_rejects_oversized_request(gpu_ctx):
@@ -134,10 +140,28 @@ def test_budget_rejects_oversized_request(gpu_ctx):
     ds = api.Dataset.new(gpu_ctx, X, y, t)
     progs = valid_programs(gpu_ctx, W.gen_c2(2048, seed=74), ds)
     srv = api.Server(gpu_ctx, max_inflight=4, budget_bytes=64 * 1024)
-    tk = srv.submit_lbfgs(api.Programs(gpu_ctx, progs), ds, iterations=3)
-    with pytest.raises(api.RsrError) as e:
-        tk.wait(60.0)
-    assert e.value.code == -4  # RSR_E_BUDGET
-    small = srv.submit_eval(api.Programs(gpu_ctx, progs[:16]), ds)
-    assert small.wait(60.0).shape == (16,)
-    srv.close()
+    try:
+        tk = srv.submit_lbfgs(api.Programs(gpu_ctx, progs), ds, iterations=3)
+        with pytest.raises(api.RsrError) as e:
+            tk.wait(60.0)
+        assert e.value.code == -4  # RSR_E_BUDGET
+        small = srv.submit_eval(api.Programs(gpu_ctx, progs[:16]), ds)
+        assert small.wait(60.0).shape == (16,)
+    finally:
+        srv.close()
+
+
+def test_context_is_not_destroyed_under_a_server():
+    """A context with a live server refuses destruction (the server's loop thread still
+    uses its stream and buffers); after the server is destroyed it goes normally."""
+    ctx = api.Context(0)
+    srv = api.Server(ctx, max_inflight=2)
+    try:
+        with pytest.raises(api.RsrError) as e:
+            ctx.close()
+        assert e.value.code == -1  # RSR_E_INVALID_ARG
+        assert ctx.h is not None
+    finally:
+        srv.close()
+    ctx.close()
+    assert ctx.h is None
