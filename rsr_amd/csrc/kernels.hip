@@ -248,7 +248,8 @@ __global__ __launch_bounds__(256) void k_fold_terms(const SegTable T) {
 }
 
 // ---------------------------------------------------------------------------------
-// Gradients (dual), Kahan summation: work item = (tile, program, pass)
+// Gradients (dual), Kahan summation: work item = (tile, group of kGradGroup programs),
+// every pass of four tangent channels of each program inside it
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_slots) {
     extern __shared__ float smem[];
