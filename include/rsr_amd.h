@@ -202,8 +202,10 @@ RSR_API int rsr_fit_nelder_mead(rsr_context* ctx, rsr_programs* progs, const rsr
  * max(max_iterations / 16, 2) iterations, Philox stream `seed`) for programs with a
  * vector or matrix constant, else Nelder-Mead (max_iterations) -- then every constant
  * component with magnitude below `threshold` (reference: 0.05) set to zero, then the
- * MSE of the result into mse_out (NaN for programs not optimized).  The reference's
- * simplify_ast of the snapped tree is not applied.  method_out (optional): RSR_FIT_*. */
+ * MSE of the result into mse_out (NaN for programs not optimized).  The snapped
+ * constants are written back to the batch; the reference's simplify_ast of the snapped
+ * tree is the separate host call rsr_simplify_programs (a simplified program is a new
+ * batch).  method_out (optional): RSR_FIT_*. */
 #define RSR_FIT_NONE 0
 #define RSR_FIT_LBFGS 1
 #define RSR_FIT_CMAES 2
@@ -224,6 +226,21 @@ RSR_API int rsr_complexity(int n_programs, const uint8_t* ops, const int64_t* co
 /* Fitness of island.rs:241-255: mse + complexity * parsimony (reference default 5e-6,
  * config.rs:48), FLT_MAX for a non-finite MSE. */
 RSR_API int rsr_fitness(int n, const float* mse, const int32_t* complexity, float parsimony, float* fitness_out);
+/* Individual::simplify (individual.rs:77-85) of each program, then Program::from_nodes:
+ * simplify_ast (src/engine/expr/simplify.rs:13-135) with the try_simplify rules of
+ * basic/mod.rs:45-115 and linalg/mod.rs:227-310, 728-890 (constant folding, identity
+ * and zero operands, x - x, x / x, ln(exp x), exp(ln x), sqrt(sqr x), sqr(sqrt x)).
+ * Input and output use the layout of rsr_programs_create; the outputs are the compiled
+ * programs (constants in postfix order, folded-away constants dropped).  Capacities:
+ * the simplified code is never longer than the input code, and code_capacity entries
+ * of constants with 9 * code_capacity payload floats always suffice.  Host code. */
+RSR_API int rsr_simplify_programs(int n_programs, const uint8_t* ops, const uint16_t* args,
+                                  const int64_t* code_offsets, const uint8_t* const_types,
+                                  const int64_t* const_offsets, const float* const_values,
+                                  const int64_t* value_offsets, int64_t code_capacity, int64_t const_capacity,
+                                  int64_t value_capacity, uint8_t* ops_out, uint16_t* args_out,
+                                  int64_t* code_offsets_out, uint8_t* const_types_out, int64_t* const_offsets_out,
+                                  float* const_values_out, int64_t* value_offsets_out);
 /* assign_rank_and_crowding_distance (operators/selection.rs:39-134): non-dominated fronts
  * on (fitness, age), crowding distance over fitness then age. */
 RSR_API int rsr_assign_rank_and_crowding(int n, const float* fitness, const uint64_t* age, uint32_t* rank_out,

@@ -11,11 +11,11 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
-from .isa import Program, VT, pack_programs, feature_count
+from .isa import Program, Scalar, VT, VT_SIZE, pack_programs, unpack_programs, feature_count
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librsr_amd.so")
@@ -439,6 +439,43 @@ def optimize_constants(ctx: Context, progs: Programs, ds: Dataset, iterations: i
     return mse[: progs.n], meth[: progs.n]
 
 
+def optimize_individuals(ctx: Context, programs: Sequence[Program], ds: Dataset, iterations: int = 100,
+                         seed: int = 0, threshold: float = 0.05, mask=None, parsimony: float = 5e-6):
+    """Individual::optimize_constants (individual.rs:65-75) as its callers use it
+    (island.rs:230-255, engine.rs:95-98): fit and snap the constants on the GPU, simplify
+    and recompile every optimized program (simplify_ast), then calculate_mse, complexity
+    and fitness of the result.  Programs with mask[i] == 0 come back unchanged with
+    their plain MSE.  Returns (programs[n], mse[n], complexity[n], fitness[n], method[n])."""
+    n = len(programs)
+    P = Programs(ctx, programs)
+    try:
+        _, meth = optimize_constants(ctx, P, ds, iterations, seed, threshold, mask)
+        vals = P.get_constants()
+    finally:
+        P.close()
+    todo = np.ones(n, bool) if mask is None else np.asarray(mask, np.uint8)[:n] != 0
+    fitted, off = [], 0
+    for p in programs:
+        cs = []
+        for c in p.constants:
+            k = VT_SIZE[c.vt] or 1
+            cs.append(Scalar(c.vt, tuple(float(v) for v in vals[off:off + k])))
+            off += k
+        fitted.append(Program(list(p.code), cs))
+    idx = [i for i in range(n) if todo[i]]
+    simplified = simplify_programs([fitted[i] for i in idx])
+    out = list(programs)
+    for i, q in zip(idx, simplified):
+        out[i] = q
+    Q = Programs(ctx, out)
+    try:
+        mse = compute_mse(ctx, Q, ds, allow_rejected=True)
+    finally:
+        Q.close()
+    comp = complexity(out)
+    return out, mse, comp, fitness(mse, comp, parsimony), meth
+
+
 # ---------------------------------------------------------------------------------
 # scoring and selection (host functions; selection.rs, individual.rs:108-110, island.rs:241-255)
 # ---------------------------------------------------------------------------------
@@ -462,6 +499,31 @@ def fitness(mse, complexity_, parsimony: float = 5e-6) -> np.ndarray:
     out = np.zeros(max(len(m), 1), np.float32)
     _check(l.rsr_fitness(len(m), _ptr(m, _f32p), _ptr(c, _i32p), parsimony, _ptr(out, _f32p)))
     return out[: len(m)]
+
+
+def simplify_programs(progs: Sequence[Program]) -> List[Program]:
+    """Individual::simplify + Program::from_nodes of every program (host, no GPU)."""
+    l = lib()
+    fn = l.rsr_simplify_programs
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int, _u8p, _u16p, _i64p, _u8p, _i64p, _f32p, _i64p, C.c_int64, C.c_int64, C.c_int64,
+                   _u8p, _u16p, _i64p, _u8p, _i64p, _f32p, _i64p]
+    n = len(progs)
+    ops, args, co, ct, cof, cv, vo = pack_programs(progs)
+    cap = max(int(co[-1]), 1)
+    ops_o = np.zeros(cap, np.uint8)
+    args_o = np.zeros(cap, np.uint16)
+    ct_o = np.zeros(cap, np.uint8)
+    cv_o = np.zeros(9 * cap, np.float32)
+    co_o = np.zeros(n + 1, np.int64)
+    cof_o = np.zeros(n + 1, np.int64)
+    vo_o = np.zeros(n + 1, np.int64)
+    pad = lambda a, t: a if a.size else np.zeros(1, t)
+    _check(fn(n, _ptr(pad(ops, np.uint8), _u8p), _ptr(pad(args, np.uint16), _u16p), _ptr(co, _i64p),
+              _ptr(pad(ct, np.uint8), _u8p), _ptr(cof, _i64p), _ptr(pad(cv, np.float32), _f32p), _ptr(vo, _i64p),
+              cap, cap, 9 * cap, _ptr(ops_o, _u8p), _ptr(args_o, _u16p), _ptr(co_o, _i64p), _ptr(ct_o, _u8p),
+              _ptr(cof_o, _i64p), _ptr(cv_o, _f32p), _ptr(vo_o, _i64p)))
+    return unpack_programs(ops_o, args_o, co_o, ct_o, cof_o, cv_o, vo_o)
 
 
 def assign_rank_and_crowding(fitness_, age):

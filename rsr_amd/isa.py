@@ -190,6 +190,22 @@ def pack_programs(progs: Sequence[Program]):
             np.asarray(ctypes, np.uint8), const_off, np.asarray(vals, np.float32), val_off)
 
 
+def unpack_programs(ops, args, code_off, ctypes, const_off, vals, val_off) -> List[Program]:
+    """Inverse of pack_programs."""
+    out = []
+    for i in range(len(code_off) - 1):
+        code = [(int(ops[k]), int(args[k])) for k in range(int(code_off[i]), int(code_off[i + 1]))]
+        consts = []
+        v = int(val_off[i])
+        for k in range(int(const_off[i]), int(const_off[i + 1])):
+            vt = VT(int(ctypes[k]))
+            n = VT_SIZE[vt] or 1
+            consts.append(Scalar(vt, tuple(float(x) for x in vals[v:v + n])))
+            v += n
+        out.append(Program(code, consts))
+    return out
+
+
 def feature_count(types: Sequence[int]) -> int:
     """Flat feature components of a schema (dataset.rs:46-55; Int/Bool count one)."""
     return sum(VT_SIZE[VT(t)] or 1 for t in types)

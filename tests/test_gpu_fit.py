@@ -198,3 +198,50 @@ def test_optimize_constants_dispatch_snap_and_mse(gpu_ctx):
             rc = np.where(np.abs(rc[:nv]) < 0.05, f32(0), rc[:nv]).astype(f32)
             assert np.array_equal(v.view(np.uint32), rc.view(np.uint32)) or np.array_equal(v, rc), i
         off += nv
+
+
+@pytest.mark.parametrize("which", ["scalar", "mat2"])
+def test_optimize_individuals_simplify_and_rescore(gpu_ctx, which):
+    """Individual::optimize_constants as the island uses it: fit and snap on the GPU, then
+    simplify_ast, recompile and rescore.  The simplified program equals the oracle's
+    simplification of the snapped program, and its MSE (reference order), complexity and
+    fitness equal the oracle's for that program."""
+    from oracle import simplify as OS
+    if which == "scalar":
+        X, y, t = W.data_c0(403, seed=91)
+        progs = W.gen_c0(80, seed=92)
+    else:
+        X, y, t = W.data_c3(500, seed=93)
+        progs = W.gen_c3(40, seed=94)
+    gpu_ctx.set_summation(True)
+    try:
+        ds = api.Dataset.new(gpu_ctx, X, y, t)
+        P = api.Programs(gpu_ctx, progs)
+        ok = (P.status(ds) == 0).astype(np.uint8)
+        P.close()
+        out, mse, comp, fit, meth = api.optimize_individuals(gpu_ctx, progs, ds, iterations=20, seed=3, mask=ok)
+        # the snapped constants of the same (deterministic) fit, for the oracle's simplification
+        P2 = api.Programs(gpu_ctx, progs)
+        _, meth2 = api.optimize_constants(gpu_ctx, P2, ds, iterations=20, seed=3, mask=ok)
+        snapped = P2.get_constants()
+    finally:
+        gpu_ctx.set_summation(False)
+    assert np.array_equal(meth, meth2)
+    ods = O.ODataset.new(X, y, X.shape[1])
+    off, shrunk = 0, 0
+    for i, p in enumerate(progs):
+        nv = sum(len(c.vals) for c in p.constants)
+        if ok[i]:
+            want = OS.simplify(_with_consts(p, snapped[off:off + nv]))
+            assert [(int(o), int(a)) for o, a in out[i].code] == [(int(o), int(a)) for o, a in want.code], i
+            assert [np.float32(c.vals).tobytes() for c in out[i].constants] == \
+                [np.float32(c.vals).tobytes() for c in want.constants], i
+            shrunk += len(want.code) < len(p.code)
+        else:
+            assert out[i] is progs[i]
+        m = O.compute_mse(out[i], ods)
+        assert np.float32(mse[i]).view(np.uint32) == np.float32(m).view(np.uint32), (i, mse[i], m)
+        assert fit[i] == np.float32(O.fitness(float(mse[i]), int(comp[i]), 5e-6)), i
+        off += nv
+    assert np.array_equal(comp, api.complexity(out))
+    assert shrunk > 0  # snapping to zero made some programs simplify
