@@ -241,6 +241,36 @@ RSR_API int rsr_simplify_programs(int n_programs, const uint8_t* ops, const uint
                                   int64_t value_capacity, uint8_t* ops_out, uint16_t* args_out,
                                   int64_t* code_offsets_out, uint8_t* const_types_out, int64_t* const_offsets_out,
                                   float* const_values_out, int64_t* value_offsets_out);
+/* Island::evaluate_buffer (island.rs:215-261) of n_islands islands in one call.  Island g's
+ * buffer is programs [island_offsets[g], island_offsets[g+1]) of the batch given in the
+ * rsr_programs_create layout; best_fitness[g] is its best individual's fitness and
+ * best_mini_mse[g] that individual's MSE on mini_batch (island.rs:217).  The island's
+ * uniform draws (its rng.random::<f32>() stream) are draws[draw_offsets[g] ..
+ * draw_offsets[g+1]), consumed in the reference's order: one when an individual's
+ * mini-batch MSE is not under twice the baseline (5% rescue), one for every promising
+ * individual (random fit with probability opt_prob).  Promising individuals are scored on
+ * dataset; those under the island's running best fitness, or drawn, with a finite MSE get
+ * Individual::optimize_constants (opt_iterations, threshold, CMA-ES stream seed), i.e. the
+ * fit, the snap and simplify_ast, and are rescored.  Outputs per individual: the buffer
+ * after the step (fitted individuals carry their simplified programs; capacities as for
+ * rsr_simplify_programs over the whole batch), fitness (mse + complexity * parsimony, or
+ * FLT_MAX), final MSE (the mini-batch MSE for individuals that were not promising, as in
+ * the reference), whether it was fitted; per island: the buffer index of the new best
+ * (-1: unchanged) and the draws consumed.  The islands' local hall of fame is the caller's:
+ * it updates from (complexity, final MSE) in buffer order.  No reference counterpart for
+ * the batching: the reference runs one island per rayon thread (engine.rs:52). */
+RSR_API int rsr_islands_evaluate(rsr_context* ctx, const rsr_dataset* dataset, const rsr_dataset* mini_batch,
+                                 int n_islands, const int64_t* island_offsets, const uint8_t* ops,
+                                 const uint16_t* args, const int64_t* code_offsets, const uint8_t* const_types,
+                                 const int64_t* const_offsets, const float* const_values,
+                                 const int64_t* value_offsets, const float* best_fitness, const float* best_mini_mse,
+                                 const float* draws, const int64_t* draw_offsets, float opt_prob, int opt_iterations,
+                                 float parsimony, float threshold, uint64_t seed, int64_t code_capacity,
+                                 int64_t const_capacity, int64_t value_capacity, uint8_t* ops_out, uint16_t* args_out,
+                                 int64_t* code_offsets_out, uint8_t* const_types_out, int64_t* const_offsets_out,
+                                 float* const_values_out, int64_t* value_offsets_out, float* fitness_out,
+                                 float* mse_out, uint8_t* optimized_out, int32_t* best_index_out,
+                                 int64_t* draws_used_out);
 /* assign_rank_and_crowding_distance (operators/selection.rs:39-134): non-dominated fronts
  * on (fitness, age), crowding distance over fitness then age. */
 RSR_API int rsr_assign_rank_and_crowding(int n, const float* fitness, const uint64_t* age, uint32_t* rank_out,

@@ -526,6 +526,54 @@ def simplify_programs(progs: Sequence[Program]) -> List[Program]:
     return unpack_programs(ops_o, args_o, co_o, ct_o, cof_o, cv_o, vo_o)
 
 
+def islands_evaluate(ctx: Context, dataset: Dataset, mini_batch: Dataset, buffers: Sequence[Sequence[Program]],
+                     best_fitness, best_mini_mse, draws: Sequence[np.ndarray], opt_prob: float = 0.01,
+                     opt_iterations: int = 100, parsimony: float = 5e-6, threshold: float = 0.05, seed: int = 0):
+    """Island::evaluate_buffer of every island (rsr_islands_evaluate; defaults of config.rs:40-66).
+
+    buffers[g]: island g's next-generation buffer; best_fitness[g] / best_mini_mse[g]: its
+    best individual's fitness and mini-batch MSE; draws[g]: its uniform f32 stream.
+    Returns dict(buffers, fitness, mse, optimized (per island lists / arrays),
+    best_index[g] (-1: unchanged), draws_used[g])."""
+    l = lib()
+    fn = l.rsr_islands_evaluate
+    fn.restype = C.c_int
+    fn.argtypes = ([_P, _P, _P, C.c_int, _i64p, _u8p, _u16p, _i64p, _u8p, _i64p, _f32p, _i64p, _f32p, _f32p, _f32p,
+                    _i64p, C.c_float, C.c_int, C.c_float, C.c_float, C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
+                    _u8p, _u16p, _i64p, _u8p, _i64p, _f32p, _i64p, _f32p, _f32p, _u8p, _i32p, _i64p])
+    G = len(buffers)
+    flat = [p for b in buffers for p in b]
+    n = len(flat)
+    isl = np.zeros(G + 1, np.int64)
+    isl[1:] = np.cumsum([len(b) for b in buffers])
+    ops, args, co, ct, cof, cv, vo = pack_programs(flat)
+    dr = [np.ascontiguousarray(d, np.float32) for d in draws]
+    doff = np.zeros(G + 1, np.int64)
+    doff[1:] = np.cumsum([d.size for d in dr])
+    dall = np.concatenate(dr) if dr and doff[-1] else np.zeros(1, np.float32)
+    bf = np.ascontiguousarray(best_fitness, np.float32)
+    bm = np.ascontiguousarray(best_mini_mse, np.float32)
+    cap = max(int(co[-1]), 1)
+    ops_o, args_o = np.zeros(cap, np.uint8), np.zeros(cap, np.uint16)
+    ct_o, cv_o = np.zeros(cap, np.uint8), np.zeros(9 * cap, np.float32)
+    co_o, cof_o, vo_o = np.zeros(n + 1, np.int64), np.zeros(n + 1, np.int64), np.zeros(n + 1, np.int64)
+    fit, mse = np.zeros(max(n, 1), np.float32), np.zeros(max(n, 1), np.float32)
+    opt = np.zeros(max(n, 1), np.uint8)
+    besti, used = np.zeros(max(G, 1), np.int32), np.zeros(max(G, 1), np.int64)
+    pad = lambda a, t: a if a.size else np.zeros(1, t)
+    _check(fn(ctx.h, dataset.h, mini_batch.h, G, _ptr(isl, _i64p), _ptr(pad(ops, np.uint8), _u8p),
+              _ptr(pad(args, np.uint16), _u16p), _ptr(co, _i64p), _ptr(pad(ct, np.uint8), _u8p), _ptr(cof, _i64p),
+              _ptr(pad(cv, np.float32), _f32p), _ptr(vo, _i64p), _ptr(pad(bf, np.float32), _f32p),
+              _ptr(pad(bm, np.float32), _f32p), _ptr(dall, _f32p), _ptr(doff, _i64p), opt_prob, opt_iterations,
+              parsimony, threshold, seed, cap, cap, 9 * cap, _ptr(ops_o, _u8p), _ptr(args_o, _u16p),
+              _ptr(co_o, _i64p), _ptr(ct_o, _u8p), _ptr(cof_o, _i64p), _ptr(cv_o, _f32p), _ptr(vo_o, _i64p),
+              _ptr(fit, _f32p), _ptr(mse, _f32p), _ptr(opt, _u8p), _ptr(besti, _i32p), _ptr(used, _i64p)))
+    progs = unpack_programs(ops_o, args_o, co_o, ct_o, cof_o, cv_o, vo_o)
+    split = lambda a: [a[isl[g]:isl[g + 1]] for g in range(G)]
+    return {"buffers": split(progs), "fitness": split(fit[:n]), "mse": split(mse[:n]),
+            "optimized": split(opt[:n].astype(bool)), "best_index": besti[:G], "draws_used": used[:G]}
+
+
 def assign_rank_and_crowding(fitness_, age):
     l = lib()
     l.rsr_assign_rank_and_crowding.restype = C.c_int
