@@ -179,6 +179,30 @@ def run_configs(ctx, log):
                          "mean_iterations_run": float(np.mean(its[ok])) if ok.any() else None}
     P.close()
     ds.close()
+    log("[bench] islands' evaluation step: 24 islands x 25 individuals, 400 points, mini-batch 64")
+    X, y, t = W.data_c0(400, seed=1000)
+    ds = api.Dataset.new(ctx, X, y, t)
+    mini = ds.subset(64)
+    pool = W.gen_c0(24 * 26, seed=2100)
+    buffers = [pool[g * 25:(g + 1) * 25] for g in range(24)]
+    bests = pool[24 * 25:]
+    Pb = api.Programs(ctx, bests)
+    best_mini = api.compute_mse(ctx, Pb, mini, allow_rejected=True)
+    best_fit = api.fitness(api.compute_mse(ctx, Pb, ds, allow_rejected=True), api.complexity(bests))
+    Pb.close()
+    rng = np.random.default_rng(7)
+    draws = [rng.random(100).astype(np.float32) for _ in range(24)]
+    step = lambda: api.islands_evaluate(ctx, ds, mini, buffers, best_fit, best_mini, draws)
+    step()
+    res, t_all = _timed_best(ctx, step, reps=3)
+    per_island = lambda: [api.islands_evaluate(ctx, ds, mini, [buffers[g]], best_fit[g:g + 1], best_mini[g:g + 1],
+                                               [draws[g]]) for g in range(24)]
+    _, t_per = _timed_best(ctx, per_island, reps=3)
+    out["islands_step"] = {"islands": 24, "island_size": 25, "points": 400, "mini_batch": 64,
+                           "batched_ms": t_all * 1e3, "one_call_per_island_ms": t_per * 1e3,
+                           "fitted": int(sum(int(o.sum()) for o in res["optimized"]))}
+    mini.close()
+    ds.close()
     return out
 
 
