@@ -127,6 +127,21 @@ RSR_API int rsr_dataset_create(rsr_context* ctx, const float* rows, const float*
  * target_batches, dataset.rs:17-34): feature_flat[(b*F + f)*8 + lane]. */
 RSR_API int rsr_dataset_create_aosoa(rsr_context* ctx, const float* feature_flat, const float* target_batches,
                                      int num_features, int64_t num_batches, int64_t num_samples, rsr_dataset** out);
+/* Datasets on disk.  Dataset::from_csv (dataset.rs:298-357): a header row, then records
+ * with the header's field count; the target in column target_col (the last when
+ * negative); the features in the other columns, left to right, as many as the feature
+ * types' flat size; numbers as f32::from_str parses them (no whitespace, correctly
+ * rounded).  Dataset::from_json (dataset.rs:265-291): [{"x": [...], "y": v}, ...] with
+ * other keys ignored.  rsr_read_* return the rows [n][F] and targets [n] (both pointers
+ * null: n only); rsr_dataset_from_file reads and then builds as rsr_dataset_create. */
+#define RSR_FILE_CSV 0
+#define RSR_FILE_JSON 1
+RSR_API int rsr_read_csv(const char* path, const uint8_t* feature_types, int n_feature_types, int target_col,
+                         float* rows_out, float* y_out, int64_t capacity, int64_t* n_out);
+RSR_API int rsr_read_json(const char* path, const uint8_t* feature_types, int n_feature_types, float* rows_out,
+                          float* y_out, int64_t capacity, int64_t* n_out);
+RSR_API int rsr_dataset_from_file(rsr_context* ctx, const char* path, int format, const uint8_t* feature_types,
+                                  int n_feature_types, int target_col, int normalize, rsr_dataset** out);
 /* Dataset::subset (dataset.rs:360-447): strided sample of target_samples points. */
 RSR_API int rsr_dataset_subset(rsr_context* ctx, const rsr_dataset* src, int64_t target_samples, rsr_dataset** out);
 RSR_API int rsr_dataset_info(const rsr_dataset* ds, int* num_features, int64_t* num_samples, int64_t* num_batches);

@@ -192,6 +192,20 @@ class Dataset:
                                               num_samples, C.byref(h)))
         return Dataset(ctx, h)
 
+    @staticmethod
+    def from_file(ctx: Context, path: str, feature_types: Sequence[int], fmt: str = "csv",
+                  target_col: Optional[int] = None, normalize: bool = False) -> "Dataset":
+        """Dataset::from_csv / from_json (dataset.rs:265-357) read straight into a device dataset."""
+        ft = np.asarray([int(t) for t in feature_types] or [0], np.uint8)
+        h = C.c_void_p()
+        l = lib()
+        l.rsr_dataset_from_file.restype = C.c_int
+        l.rsr_dataset_from_file.argtypes = [_P, C.c_char_p, C.c_int, _u8p, C.c_int, C.c_int, C.c_int, _PP]
+        _check(l.rsr_dataset_from_file(ctx.h, os.fsencode(path), 1 if fmt == "json" else 0, _ptr(ft, _u8p),
+                                       len(feature_types), -1 if target_col is None else int(target_col),
+                                       1 if normalize else 0, C.byref(h)))
+        return Dataset(ctx, h)
+
     def subset(self, target_samples: int) -> "Dataset":
         h = C.c_void_p()
         _check(lib().rsr_dataset_subset(self.ctx.h, self.h, target_samples, C.byref(h)))
@@ -499,6 +513,30 @@ def fitness(mse, complexity_, parsimony: float = 5e-6) -> np.ndarray:
     out = np.zeros(max(len(m), 1), np.float32)
     _check(l.rsr_fitness(len(m), _ptr(m, _f32p), _ptr(c, _i32p), parsimony, _ptr(out, _f32p)))
     return out[: len(m)]
+
+
+def read_dataset_file(path: str, feature_types: Sequence[int], fmt: str = "csv", target_col: Optional[int] = None):
+    """The rows [n][F] and targets [n] that Dataset::from_csv / from_json read (host, no GPU)."""
+    l = lib()
+    ft = np.asarray([int(t) for t in feature_types] or [0], np.uint8)
+    F = feature_count(feature_types)
+    n = C.c_int64()
+    if fmt == "json":
+        fn = l.rsr_read_json
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_char_p, _u8p, C.c_int, _f32p, _f32p, C.c_int64, _i64p]
+        call = lambda r, y, cap: fn(os.fsencode(path), _ptr(ft, _u8p), len(feature_types), r, y, cap, C.byref(n))
+    else:
+        fn = l.rsr_read_csv
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_char_p, _u8p, C.c_int, C.c_int, _f32p, _f32p, C.c_int64, _i64p]
+        tc = -1 if target_col is None else int(target_col)
+        call = lambda r, y, cap: fn(os.fsencode(path), _ptr(ft, _u8p), len(feature_types), tc, r, y, cap, C.byref(n))
+    _check(call(None, None, 0))
+    rows = np.zeros(max(n.value * F, 1), np.float32)
+    y = np.zeros(max(n.value, 1), np.float32)
+    _check(call(_ptr(rows, _f32p), _ptr(y, _f32p), n.value))
+    return rows[: n.value * F].reshape(n.value, F), y[: n.value]
 
 
 def simplify_programs(progs: Sequence[Program]) -> List[Program]:
