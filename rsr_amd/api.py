@@ -249,10 +249,12 @@ class Dataset:
 class Programs:
     """A device-resident batch of programs."""
 
-    def __init__(self, ctx: Context, progs: Sequence[Program]):
+    def __init__(self, ctx: Context, progs: Sequence[Program], packed=None):
+        """packed: the pack_programs arrays of progs, when the caller already has them
+        (progs is then only counted)."""
         self.ctx = ctx
         self.n = len(progs)
-        ops, args, co, ct, cof, cv, vo = pack_programs(progs)
+        ops, args, co, ct, cof, cv, vo = packed if packed is not None else pack_programs(progs)
         self._keep = (ops, args, co, ct, cof, cv, vo)
         h = C.c_void_p()
         _check(lib().rsr_programs_create(ctx.h, self.n, _ptr(ops, _u8p), _ptr(args, _u16p), _ptr(co, _i64p),

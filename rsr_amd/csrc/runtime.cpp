@@ -13,6 +13,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rsr_amd.h"
@@ -632,8 +633,26 @@ int rsr::get_lowering(rsr_programs* p, int F, bool dual, Lowering** out) {
     std::vector<ProgDesc> descs(p->n);
     std::vector<uint8_t> active(p->n, 0);
     std::vector<uint32_t> cost(p->n, 0);
+    // The programs lower independently: up to 16 host threads, a contiguous range each.
+    std::vector<Lowered> lws((size_t)p->n);
+    {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const int nt = std::min<int>({16, (int)hw, (p->n + 255) / 256});
+        auto work = [&](int t, int nthreads) {
+            const int a = (int)((int64_t)p->n * t / nthreads), b = (int)((int64_t)p->n * (t + 1) / nthreads);
+            for (int i = a; i < b; ++i) lws[(size_t)i] = lower_program(ref_program(p, i), F, dual);
+        };
+        if (nt <= 1) {
+            work(0, 1);
+        } else {
+            std::vector<std::thread> pool;
+            for (int t = 1; t < nt; ++t) pool.emplace_back(work, t, nt);
+            work(0, nt);
+            for (auto& th : pool) th.join();
+        }
+    }
     for (int i = 0; i < p->n; ++i) {
-        Lowered lw = lower_program(ref_program(p, i), F, dual);
+        Lowered& lw = lws[(size_t)i];
         L->status[i] = lw.status;
         L->nparams[i] = (uint8_t)std::min<uint32_t>(lw.n_params, 32);
         L->differentiable[i] = lw.differentiable ? 1 : 0;
