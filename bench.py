@@ -285,7 +285,8 @@ def main():
     from_host = None
     if not args.no_from_host:
         log("[bench] from host buffers: dataset + program upload, lowering, one evaluation")
-        ctx.set_summation(True)
+        fh_mode = "reference" if args.summation == "auto" else args.summation
+        ctx.set_summation(fh_mode == "reference")
         tp = time.perf_counter()
         packed = pack_programs(progs)  # the Python objects into the C-ABI arrays (not the library's work)
         pack_ms = (time.perf_counter() - tp) * 1e3
@@ -307,7 +308,7 @@ def main():
         ctx.set_summation(False)
         from_host = {"ms": best[0] * 1e3, "node_evals_per_s": ne / best[0],
                      "dataset_upload_ms": best[1] * 1e3, "program_upload_ms": best[2] * 1e3,
-                     "lowering_and_evaluation_ms": best[3] * 1e3, "summation": "reference",
+                     "lowering_and_evaluation_ms": best[3] * 1e3, "summation": fh_mode,
                      "python_packing_ms_not_included": pack_ms}
 
     plain = None
