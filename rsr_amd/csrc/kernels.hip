@@ -43,14 +43,17 @@ __device__ __forceinline__ int64_t xcd_block() {
 // contiguous range of work items.  Ordered tile-major, that range is a run of tiles with
 // every program group on each: each XCD's L2 then holds about an eighth of the dataset,
 // but the code of every program streams through it once per tile.  For segments with at
-// least 256 tiles the items are instead ordered in blocks of 16 tiles (block, then group,
-// then tile), so a group's code serves 16 consecutive tiles from the cache; L2 misses of
-// the Lorentz evaluation drop 12x and it runs about 1% faster.  Smaller segments keep the
-// tile-major order: there an XCD's range would cut through a block's groups, which are
-// sorted by cost, and unbalance the XCDs.  False for w past the segment's work.
+// least 256 tiles the items are instead ordered in blocks of KB tiles (block, then group,
+// then tile), so a group's code serves KB consecutive tiles from the cache (KB = 16 for the
+// evaluation, 8 for the gradients: the best of 1, 4, 8, 16, 32, 64 measured on the Lorentz
+// and Mat3 workloads; L2 misses of the Lorentz evaluation drop 12x against tile-major).
+// Smaller segments keep the tile-major order: there an XCD's range would cut through a
+// block's groups, which are sorted by cost, and unbalance the XCDs.  False for w past the
+// segment's work.
+template <int KB>
 __device__ __forceinline__ bool split_work(int64_t w, int64_t n_tiles, int n_groups, int64_t& lt, int& g) {
     if (w >= n_tiles * n_groups) return false;
-    const int K = n_tiles >= 256 ? 16 : 1;
+    const int K = n_tiles >= 256 ? KB : 1;
     const int64_t per = (int64_t)n_groups * K;
     const int64_t blk = w / per, r = w - blk * per;
     const int64_t t0 = blk * K;
@@ -143,7 +146,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int64_t w = wave - S.first_wave;
     int64_t lt;  // tile within the launch's range
     int g;
-    if (!split_work(w, S.n_tiles, S.n_groups, lt, g)) return;
+    if (!split_work<16>(w, S.n_tiles, S.n_groups, lt, g)) return;
     if (S.live != nullptr && *S.live == 0) return;
     const int first = g * S.group;
     const int last = min(first + S.group, S.n_progs);
@@ -292,7 +295,7 @@ __global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_s
     const int64_t w = wave - S.first_wave;
     int64_t lt;  // tile within the launch's range
     int g;
-    if (!split_work(w, S.n_tiles, S.n_groups, lt, g)) return;
+    if (!split_work<8>(w, S.n_tiles, S.n_groups, lt, g)) return;
     if (S.live != nullptr && *S.live == 0) return;
     const int first = g * S.group;
     const int last = min(first + S.group, S.n_progs);
