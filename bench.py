@@ -325,6 +325,8 @@ def main():
         plain = {"value": node_evals(p0, 400) / sec0, "unit": "node-evals/s",
                  "workload": "1000 scalar programs x 400 points, one launch per population",
                  "ms_per_step": sec0 * 1e3, "kernel_ms": kms0}
+        P0.close()
+        ds0.close()
 
     per_config = None
     if not args.no_configs:
@@ -436,6 +438,8 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    P.close()
+    ds.close()
     ctx.close()
 
 

@@ -79,8 +79,8 @@ RSR_API const char* rsr_version(void);
  * (0 = 90% of free device memory).  No reference counterpart (the reference is
  * in-process CPU code). */
 RSR_API int rsr_context_create(int device_ordinal, uint64_t hbm_budget_bytes, rsr_context** out);
-/* Fails with RSR_E_INVALID_ARG (and destroys nothing) while a server of the context is
- * alive: destroy the servers first. */
+/* Fails with RSR_E_INVALID_ARG (and destroys nothing) while a server, a dataset or a
+ * program batch of the context is alive: destroy those first. */
 RSR_API int rsr_context_destroy(rsr_context* ctx);
 RSR_API int rsr_context_memory(const rsr_context* ctx, uint64_t* used_bytes, uint64_t* budget_bytes);
 /* Profiling: when enabled, HIP events on the context's stream bracket every main

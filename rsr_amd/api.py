@@ -111,9 +111,15 @@ class Context:
         self.h = h
 
     def close(self):
-        """Destroy the context (raises, and keeps it, while one of its servers is alive)."""
+        """Destroy the context.  Raises, and keeps it, while one of its servers, datasets or
+        program batches is alive (after collecting garbage that may still hold some)."""
         if self.h:
-            _check(lib().rsr_context_destroy(self.h))
+            rc = lib().rsr_context_destroy(self.h)
+            if rc != RSR_OK:
+                import gc
+                gc.collect()
+                rc = lib().rsr_context_destroy(self.h)
+            _check(rc)
             self.h = None
 
     def __del__(self):

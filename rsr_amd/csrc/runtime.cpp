@@ -239,6 +239,9 @@ extern "C" int rsr_context_destroy(rsr_context* ctx) {
     if (!ctx) return RSR_OK;
     if (ctx->live_servers.load() > 0)
         return set_error(RSR_E_INVALID_ARG, "the context still has a server; destroy the server first");
+    if (ctx->live_objects.load() > 0)
+        return set_error(RSR_E_INVALID_ARG, "the context still has %d datasets or program batches; destroy them first",
+                         ctx->live_objects.load());
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     drain_profile(ctx);
@@ -400,6 +403,7 @@ extern "C" int rsr_dataset_create(rsr_context* ctx, const float* rows, const flo
     for (int64_t i = 0; i < num_samples; ++i) d->target[i] = (y[i] - d->tmean) / d->tstd;
     int rc = dataset_upload(ctx, d);
     if (rc) return rc;
+    ctx->live_objects++;
     *out = d.release();
     return RSR_OK;
 }
@@ -429,6 +433,7 @@ extern "C" int rsr_dataset_create_aosoa(rsr_context* ctx, const float* feature_f
         }
     int rc = dataset_upload(ctx, d);
     if (rc) return rc;
+    ctx->live_objects++;
     *out = d.release();
     return RSR_OK;
 }
@@ -467,6 +472,7 @@ extern "C" int rsr_dataset_subset(rsr_context* ctx, const rsr_dataset* src, int6
     }
     int rc = dataset_upload(ctx, d);
     if (rc) return rc;
+    ctx->live_objects++;
     *out = d.release();
     return RSR_OK;
 }
@@ -499,6 +505,7 @@ extern "C" int rsr_dataset_destroy(rsr_dataset* ds) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     delete ds;
+    ctx->live_objects--;
     return RSR_OK;
 }
 
@@ -579,6 +586,7 @@ extern "C" int rsr_programs_create(rsr_context* ctx, int n, const uint8_t* ops, 
         if (rc) return rc;
         HIP_TRY(hipStreamSynchronize(ctx->stream), "params upload");
     }
+    ctx->live_objects++;
     *out = p.release();
     return RSR_OK;
 }
@@ -722,6 +730,7 @@ extern "C" int rsr_programs_destroy(rsr_programs* p) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     delete p;
+    ctx->live_objects--;
     return RSR_OK;
 }
 

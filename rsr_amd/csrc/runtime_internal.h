@@ -88,6 +88,7 @@ struct rsr_context {
     // each slot's increase since the last read instead of clearing the slots first.
     std::vector<uint32_t> counters_seen;
     std::atomic<int> live_servers{0};  // a context is not destroyed under a running server
+    std::atomic<int> live_objects{0};  // ... nor while datasets or program batches use it
     bool counters_dirty = true;  // cleared on the device before the next use
     // profiling: events around every main evaluation kernel, accumulated on read
     bool profiling = false;

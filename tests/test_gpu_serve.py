@@ -165,3 +165,21 @@ def test_context_is_not_destroyed_under_a_server():
         srv.close()
     ctx.close()
     assert ctx.h is None
+
+
+def test_context_is_not_destroyed_under_its_datasets_and_programs():
+    """Datasets and program batches keep their context: destroying it first is refused,
+    and works once they are gone."""
+    ctx = api.Context(0)
+    X, y, t = W.data_c0(100, seed=3)
+    ds = api.Dataset.new(ctx, X, y, t)
+    P = api.Programs(ctx, W.gen_c0(4, seed=4))
+    with pytest.raises(api.RsrError):
+        ctx.close()
+    ds.close()
+    with pytest.raises(api.RsrError):
+        ctx.close()
+    assert ctx.h is not None
+    P.close()
+    ctx.close()
+    assert ctx.h is None
