@@ -61,3 +61,24 @@ def test_islands_evaluate_equals_sequential_loop(gpu_ctx, opt_prob):
             assert [_bits(c.vals).tolist() for c in a.constants] == [_bits(c.vals).tolist() for c in b.constants], g
         fitted += int(opt.sum())
     assert fitted > 0
+
+
+def test_islands_evaluate_edge_cases(gpu_ctx):
+    """An empty island, a one-individual island, and a draw stream that runs out (an
+    error naming the island, nothing partial returned)."""
+    X, y, t = W.data_c0(200, seed=98)
+    ds = api.Dataset.new(gpu_ctx, X, y, t)
+    mini = ds.subset(64)
+    progs = W.gen_c0(3, seed=99)
+    sq = W.square_program()  # the data's own formula: MSE 0 on both datasets
+    res = api.islands_evaluate(gpu_ctx, ds, mini, [[], [sq]], [1e30, 1e30], [1e30, 1e30],
+                               [np.zeros(0, f32), np.full(4, 0.5, f32)])
+    assert [len(b) for b in res["buffers"]] == [0, 1]
+    assert res["draws_used"].tolist() == [0, 1]  # promising by its mini MSE: one draw (no random fit)
+    assert res["best_index"].tolist() == [-1, 0]
+    assert res["optimized"][1].tolist() == [True]  # elite against the best's 1e30
+    assert res["mse"][1].tolist() == [0.0]
+    with pytest.raises(api.RsrError):
+        api.islands_evaluate(gpu_ctx, ds, mini, [progs], [1e30], [0.0], [np.full(2, 0.9, f32)])
+    mini.close()
+    ds.close()
