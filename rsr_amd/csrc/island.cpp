@@ -117,6 +117,13 @@ extern "C" int rsr_islands_evaluate(rsr_context* ctx, const rsr_dataset* dataset
     for (int g = 0; g < n_islands; ++g)
         if (island_offsets[g + 1] < island_offsets[g] || draw_offsets[g + 1] < draw_offsets[g])
             return set_error(RSR_E_INVALID_ARG, "island %d: decreasing offsets", g);
+    if ((n > 0 && (code_offsets[n] > code_offsets[0] && (!ops || !args))) ||
+        (n > 0 && const_offsets[n] > const_offsets[0] && (!const_types || !const_values)) ||
+        (draw_offsets[n_islands] > draw_offsets[0] && !draws))
+        return set_error(RSR_E_INVALID_ARG, "null program or draw arrays");
+    for (int64_t i = 0; i < n; ++i)
+        if (code_offsets[i + 1] < code_offsets[i] || const_offsets[i + 1] < const_offsets[i])
+            return set_error(RSR_E_INVALID_ARG, "individual %lld: decreasing offsets", (long long)i);
     Wire buf;
     for (int64_t i = 0; i < n; ++i)
         buf.add(ops, args, code_offsets, const_types, const_offsets, const_values, value_offsets, i);
