@@ -565,6 +565,8 @@ extern "C" int rsr_simplify_programs(int n, const uint8_t* ops, const uint16_t* 
         if (nc + (int64_t)s.code.size() > code_capacity || nk + (int64_t)s.consts.size() > const_capacity ||
             nv + vals > value_capacity)
             return set_error(RSR_E_INVALID_ARG, "program %d: output capacity exceeded", i);
+        if ((!s.code.empty() && (!ops_out || !args_out)) || (!s.consts.empty() && (!const_types_out || !const_values_out)))
+            return set_error(RSR_E_INVALID_ARG, "null output arrays");
         for (const Node& nd : s.code) {
             ops_out[nc] = nd.op;
             args_out[nc] = nd.arg;
