@@ -267,6 +267,8 @@ extern "C" int rsr_islands_evaluate(rsr_context* ctx, const rsr_dataset* dataset
         const int64_t v0 = src.vo[j], v1 = src.vo[j + 1];
         if (nc + (c1 - c0) > code_capacity || nk + (k1 - k0) > const_capacity || nv + (v1 - v0) > value_capacity)
             return set_error(RSR_E_INVALID_ARG, "individual %lld: output capacity exceeded", (long long)i);
+        if ((c1 > c0 && (!ops_out || !args_out)) || (k1 > k0 && (!const_types_out || !const_values_out)))
+            return set_error(RSR_E_INVALID_ARG, "null output program arrays");
         for (int64_t k = c0; k < c1; ++k, ++nc) {
             ops_out[nc] = src.ops[k];
             args_out[nc] = src.args[k];
