@@ -9,7 +9,9 @@ Individual::optimize_constants (individual.rs:65-75) is the oracle's optimizer d
 of optimize/mod.rs:10-59 (L-BFGS 15 iterations M = 6 for differentiable programs,
 Nelder-Mead otherwise; CMA-ES programs are not supported here), then the 0.05 snap
 (scalar.rs:60-80) and oracle.simplify.  Complexity is Node::weight (node.rs:21-33) with
-the operator weights of basic/mod.rs:156-163 and linalg/mod.rs:434-478.
+the operator weights of basic/mod.rs:156-163 and linalg/mod.rs:434-478.  Pinned by
+known answers worked out by hand (tests/test_island_oracle.py): the loop's draw
+consumption, rescue, random and elite fits and best updates, and the complexity table.
 """
 from __future__ import annotations
 
