@@ -17,7 +17,9 @@ state explicitly:
 - a kept argument keeps its constant mark (simplify.rs:108-110).
 Folded f32 arithmetic uses numpy float32 scalars (one IEEE rounding per operation);
 folded sin/cos/exp/ln call the C library's sinf/cosf/expf/logf, which is what Rust's
-f32 methods call on Linux.  Only well-formed programs are handled.
+f32 methods call on Linux.  Only well-formed programs are handled.  Pinned by 45 known
+answers worked out by hand from the rules, and by MSE preservation where the rules are
+exact (tests/test_simplify.py).
 """
 from __future__ import annotations
 
