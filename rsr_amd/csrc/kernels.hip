@@ -177,8 +177,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             sfor<PPL>([&](auto P) { y[P()] = S.target[base + P() * 64 + lane]; });
             if (S.terms) {
                 // reference order: keep every per-point term; the fold kernel adds them.
-                // Non-temporal stores: the 3.3 GB of terms per launch then do not evict the
-                // micro-code and dataset lines the other waves read from L2 (2% faster).
+                // The terms (3.3 GB per launch on the Lorentz workload) are only read back by
+                // the fold kernel: non-temporal stores, measured 2% faster per step.
                 float* row = S.partials + (int64_t)sp * S.n_tiles * (64 * PPL) + lt * (64 * PPL);
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
