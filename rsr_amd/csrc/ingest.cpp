@@ -14,6 +14,7 @@
 //
 // JSON, as serde reads [{"x": [f32, ...], "y": f32}, ...]: other keys are ignored; every
 // x must hold the schema's flat size.
+#include <ctype.h>
 #include <errno.h>
 #include <math.h>
 #include <stdint.h>
