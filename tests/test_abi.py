@@ -49,3 +49,23 @@ def test_no_cpu_fallback_in_product_bindings():
             if f.endswith((".py", ".cpp", ".hip", ".h")):
                 src = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in src and "from oracle" not in src and "liboracle" not in src, f
+
+
+def test_host_entry_points_reject_bad_arguments_without_gpu():
+    """Host-only entry points return RSR_E_INVALID_ARG (-1) for null or inconsistent
+    arguments instead of crashing, and name the problem."""
+    lib = ctypes.CDLL(LIB)
+    lib.rsr_last_error_message.restype = ctypes.c_char_p
+    i64 = ctypes.c_int64
+    # simplification: null offsets
+    assert lib.rsr_simplify_programs(1, None, None, None, None, None, None, None, i64(0), i64(0), i64(0),
+                                     None, None, None, None, None, None, None) == -1
+    # complexity: null arrays for a non-empty batch
+    assert lib.rsr_complexity(2, None, None, None) == -1
+    # file readers: a missing file, and a null count
+    n = i64()
+    assert lib.rsr_read_csv(b"/nonexistent/none.csv", None, 0, -1, None, None, i64(0), ctypes.byref(n)) == -1
+    assert b"cannot read" in lib.rsr_last_error_message()
+    assert lib.rsr_read_json(b"/nonexistent/none.json", None, 0, None, None, i64(0), None) == -1
+    # fitness: null arrays
+    assert lib.rsr_fitness(3, None, None, ctypes.c_float(5e-6), None) == -1
