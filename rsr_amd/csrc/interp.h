@@ -320,7 +320,7 @@ __device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __re
                 break;
             }
             case U_MULM3V3: {
-                __builtin_assume(a < 23u);
+                __builtin_assume(a < 24u);
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
                     const float v0 = B0[P()], v1 = b1(P), v2 = b2(P);
@@ -334,7 +334,7 @@ __device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __re
                 break;
             }
             case U_MULM3V3R: {
-                __builtin_assume(b < 23u);
+                __builtin_assume(b < 24u);
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
                     const float v0 = A0[P()], v1 = a1(P), v2 = a2(P);
@@ -349,8 +349,8 @@ __device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __re
             }
             case U_SCALEM3:
             case U_SCALEM3K: {
-                __builtin_assume(a < 23u);
-                __builtin_assume(d < 23u);
+                __builtin_assume(a < 24u);
+                __builtin_assume(d < 24u);
                 const float K = kld(params, x);
                 const bool kk = op == U_SCALEM3K;
                 // per point: all nine reads, then the writes (files of different points are disjoint)
@@ -686,7 +686,7 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
                 break;
             }
             case U_MULM3V3: {
-                __builtin_assume(a < 23u);
+                __builtin_assume(a < 24u);
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
                     const float v0 = B0[P()], v1 = B1[P()], v2 = B2[P()];
@@ -702,7 +702,7 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
                 break;
             }
             case U_MULM3V3R: {
-                __builtin_assume(b < 23u);
+                __builtin_assume(b < 24u);
                 sfor<PPL>([&](auto P) {
                     const v32f r = fget<P(), PPL>(R);
                     const float v0 = A0[P()], v1 = A1[P()], v2 = A2[P()];
@@ -719,8 +719,8 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             }
             case U_SCALEM3:
             case U_SCALEM3K: {
-                __builtin_assume(a < 23u);
-                __builtin_assume(d < 23u);
+                __builtin_assume(a < 24u);
+                __builtin_assume(d < 24u);
                 const float K = kld(params, x);
                 const bool kk = op == U_SCALEM3K;
                 // per point: all nine reads, then the writes (files of different points are disjoint)
@@ -1050,7 +1050,7 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                 break;
             }
             case U_MULM3V3: {
-                __builtin_assume(a < 23u);
+                __builtin_assume(a < 24u);
                 __builtin_assume(b < 30u);
                 __builtin_assume(d < 30u);
                 const v32f Vv = G.v;
@@ -1082,8 +1082,8 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
             }
             case U_SCALEM3:
             case U_SCALEM3K: {
-                __builtin_assume(a < 23u);
-                __builtin_assume(d < 23u);
+                __builtin_assume(a < 24u);
+                __builtin_assume(d < 24u);
                 const v32f Vv = G.v;
                 const bool kk = op == U_SCALEM3K;
                 const float sv = kk ? kld(params, x) : vb;

@@ -134,7 +134,8 @@ def test_all_tunings_bitwise_identical_mse(gpu_ctx):
     # same tile size -> same partial sums -> identical bits; across tile sizes within 1e-6
     for o in outs[1:]:
         assert check_mse(o, outs[0], 2e-6).all()
-    assert np.array_equal(outs[0], api.compute_mse(gpu_ctx, P, ds)) or True
+    # a repeat at the default tuning (2 points per lane, 8 programs per item) is bit-identical
+    assert np.array_equal(outs[1 * 3 + 1].view(np.uint32), api.compute_mse(gpu_ctx, P, ds).view(np.uint32))
 
 
 def test_deterministic_repeat(gpu_ctx):

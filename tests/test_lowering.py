@@ -11,6 +11,7 @@ import oracle as O
 from rsr_amd import api
 from rsr_amd import workloads as W
 from rsr_amd.isa import Op, Program, Scalar, VT, SIG
+import opcode_programs as OP
 
 f32 = np.float32
 
@@ -63,46 +64,13 @@ def test_c3_population_bit_exact():
     assert check_programs(W.gen_c3(150, seed=8), X, y, dual_params=6) == 0
 
 
-def _all_ops_programs(rng, n):
-    """Random programs drawing every operator of the instruction set."""
-    feature_leaves = {VT.Float: (Op.LoadVarF, [0, 1, 8]), VT.Vec2: (Op.LoadVarV2, [2, 7]),
-                      VT.Vec3: (Op.LoadVarV3, [4, 6]), VT.Mat2: (Op.LoadVarM2, [1, 5]), VT.Mat3: (Op.LoadVarM3, [0])}
-    by_type = {}
-    for op, (args, ret) in SIG.items():
-        by_type.setdefault(ret, []).append(op)
-    progs = []
-    for _ in range(n):
-        code, consts = [], []
-
-        def rec(t, d):
-            if d == 0 or rng.random() < 0.25:
-                if rng.random() < 0.3:
-                    size = {VT.Float: 1, VT.Vec2: 2, VT.Vec3: 3, VT.Mat2: 4, VT.Mat3: 9}[t]
-                    code.append(({VT.Float: Op.LoadConstF, VT.Vec2: Op.LoadConstV2, VT.Vec3: Op.LoadConstV3,
-                                  VT.Mat2: Op.LoadConstM2, VT.Mat3: Op.LoadConstM3}[t], len(consts)))
-                    consts.append(Scalar(t, tuple(float(f32(v)) for v in rng.uniform(-2, 2, size))))
-                else:
-                    op, idx = feature_leaves[t]
-                    code.append((op, int(rng.choice(idx))))
-                return
-            ops = by_type[t]
-            op = ops[rng.integers(len(ops))]
-            for a in SIG[op][0]:
-                rec(a, d - 1)
-            code.append((op, 0))
-
-        rec(VT.Float, int(rng.integers(1, 5)))
-        progs.append(Program(code, consts))
-    return progs
-
-
 def test_every_operator_bit_exact_incl_tangents():
     rng = np.random.default_rng(9)
     X = rng.standard_normal((64, 9)).astype(f32)
     X[5] = 0.0  # a zero row: exercises sqrt(0), division by 0, singular inverses
     X[6, 1:5] = [1, 2, 2, 4]  # singular Mat2
     y = rng.standard_normal(64).astype(f32)
-    progs = _all_ops_programs(rng, 600)
+    progs = OP.all_ops_programs(rng, 600)
     used = {Op(o) for p in progs for o, _ in p.code}
     assert set(SIG) <= used
     assert check_programs(progs, X, y, dual_params=3) == 0
@@ -203,3 +171,16 @@ def test_forwarded_operand_is_always_a():
             d_prev = words[i - 1][1] & 31
             if words[i][2] & (1 << 8):
                 assert (words[i][2] & 31) == d_prev
+
+
+def test_targeted_and_mat3_block_programs_bit_exact_incl_tangents():
+    """The hand-built operator programs and the Mat3 block programs whose base register is
+    23 (components 23..31) lower and run bit-exact on the host, tangents included."""
+    rng = np.random.default_rng(11)
+    X = OP.feature_rows(rng, 64)
+    y = rng.standard_normal(64).astype(f32)
+    assert check_programs(OP.targeted_programs(), X, y, dual_params=3) == 0
+    blocks = OP.mat3_block_programs(limit=8)
+    assert len(blocks) == 8
+    X2, y2, _ = W.data_c2(64, seed=12)
+    assert check_programs(blocks, X2, y2, dual_params=8) == 0
