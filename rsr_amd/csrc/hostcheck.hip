@@ -249,10 +249,12 @@ extern "C" RSR_API int rsr_lower_check(const uint8_t* ops, const uint16_t* args,
     std::vector<float> params;
     flatten_params(p, params);
     params.push_back(0.0f);
-    float R[kRegs + 3], G[kRegs + 3], spill[2 * kMaxSpillSlots];
+    float R[kRegs + 3], G[kRegs + 3], spill[2 * kMaxSlots];
     for (int64_t i = 0; i < n; ++i) {
-        for (int r = 0; r < kRegs + 3; ++r) { R[r] = 0.0f; G[r] = 0.0f; }
-        for (int f = 0; f < num_features; ++f) R[f] = feat[(int64_t)f * n + i];
+        // temporaries and spill slots start as NaN: a read before a write poisons the result
+        for (int r = 0; r < kRegs + 3; ++r) { R[r] = __builtin_nanf(""); G[r] = __builtin_nanf(""); }
+        for (int s = 0; s < 2 * kMaxSlots; ++s) spill[s] = __builtin_nanf("");
+        for (int f = 0; f < num_features; ++f) { R[f] = feat[(int64_t)f * n + i]; G[f] = 0.0f; }
         run_host(lw.code, params.data(), R, G, dual ? active_param : -1, spill);
         if (vals_out) vals_out[i] = R[lw.out_reg];
         if (tan_out) tan_out[i] = G[lw.out_reg];

@@ -84,7 +84,10 @@ __device__ __forceinline__ Uop load_uop(const UInstr* __restrict__ code, uint32_
 }
 
 // Primal interpreter at one point per lane (run_program below dispatches to it).
-// `lds` is this wave's spill area: slot s, point p at lds[(s * PPL + p) * 64 + lane].
+// `lds` is this wave's spill area: slot s, point p at lds[(s * PPL + p) * 64 + lane];
+// slots from kMaxSpillSlots up live in the wave's device-memory area `gsp` (slot
+// kMaxSpillSlots + g, point p at gsp[(g * PPL + p) * 64 + lane]), which only the launches
+// of big programs provide (null elsewhere: such slots then read as zero, never fault).
 // The same micro-op semantics as run_program, with the accumulator machinery off (the
 // code is written for both and `kAcc` selects): at one point per lane a staging move or
 // a write-back costs one VALU op, less than the branch that would skip it.  Operands a
@@ -94,7 +97,8 @@ __device__ __forceinline__ Uop load_uop(const UInstr* __restrict__ code, uint32_
 // ignored, which is exact since the file then holds every value.
 template <int PPL>
 __device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __restrict__ code, uint32_t len,
-                                               const float* __restrict__ params, float* __restrict__ lds, int lane) {
+                                               const float* __restrict__ params, float* __restrict__ lds,
+                                               float* __restrict__ gsp, int lane) {
     Uop nx = load_uop(code, 0);  // the code array ends with one padding entry
     constexpr bool kAcc = false;
     float A0[PPL], A1[PPL], A2[PPL];
@@ -272,11 +276,25 @@ __device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __re
                 break;
             }
             case U_SPILL:
-                sfor<PPL>([&](auto P) { lds[(x * PPL + P()) * 64u + spill_lane(x)] = A0[P()]; });
+                if (x < (uint32_t)kMaxSpillSlots) {
+                    sfor<PPL>([&](auto P) { lds[(x * PPL + P()) * 64u + spill_lane(x)] = A0[P()]; });
+                } else if (gsp != nullptr) {
+                    const uint32_t g = x - (uint32_t)kMaxSpillSlots;
+                    sfor<PPL>([&](auto P) { gsp[(g * PPL + P()) * 64u + spill_lane(x)] = A0[P()]; });
+                }
                 scalar_result();
                 break;
             case U_FILL:
-                sfor<PPL>([&](auto P) { A0[P()] = lds[(x * PPL + P()) * 64u + spill_lane(x)]; });
+                if (x < (uint32_t)kMaxSpillSlots) {
+                    sfor<PPL>([&](auto P) { A0[P()] = lds[(x * PPL + P()) * 64u + spill_lane(x)]; });
+                } else {
+                    const uint32_t g = x - (uint32_t)kMaxSpillSlots;
+                    if (gsp != nullptr) {
+                        sfor<PPL>([&](auto P) { A0[P()] = gsp[(g * PPL + P()) * 64u + spill_lane(x)]; });
+                    } else {
+                        sfor<PPL>([&](auto P) { A0[P()] = 0.0f; });
+                    }
+                }
                 scalar_result();
                 break;
             // Vec3 block ops: every operand component is staged, the result replaces A
@@ -479,9 +497,10 @@ __device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __re
 // state explicitly so that they hold no registers across the scalar cases.
 template <int PPL>
 __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restrict__ code, uint32_t len,
-                                            const float* __restrict__ params, float* __restrict__ lds, int lane) {
+                                            const float* __restrict__ params, float* __restrict__ lds,
+                                            float* __restrict__ gsp, int lane) {
     if constexpr (PPL == 1) {
-        run_program_p1<PPL>(R, code, len, params, lds, lane);
+        run_program_p1<PPL>(R, code, len, params, lds, gsp, lane);
         return;
     }
     Uop nx = load_uop(code, 0);  // the code array ends with one padding entry
@@ -631,11 +650,25 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
                 break;
             }
             case U_SPILL:
-                sfor<PPL>([&](auto P) { lds[(x * PPL + P()) * 64u + spill_lane(x)] = A0[P()]; });
+                if (x < (uint32_t)kMaxSpillSlots) {
+                    sfor<PPL>([&](auto P) { lds[(x * PPL + P()) * 64u + spill_lane(x)] = A0[P()]; });
+                } else if (gsp != nullptr) {
+                    const uint32_t g = x - (uint32_t)kMaxSpillSlots;
+                    sfor<PPL>([&](auto P) { gsp[(g * PPL + P()) * 64u + spill_lane(x)] = A0[P()]; });
+                }
                 scalar_result();
                 break;
             case U_FILL:
-                sfor<PPL>([&](auto P) { A0[P()] = lds[(x * PPL + P()) * 64u + spill_lane(x)]; });
+                if (x < (uint32_t)kMaxSpillSlots) {
+                    sfor<PPL>([&](auto P) { A0[P()] = lds[(x * PPL + P()) * 64u + spill_lane(x)]; });
+                } else {
+                    const uint32_t g = x - (uint32_t)kMaxSpillSlots;
+                    if (gsp != nullptr) {
+                        sfor<PPL>([&](auto P) { A0[P()] = gsp[(g * PPL + P()) * 64u + spill_lane(x)]; });
+                    } else {
+                        sfor<PPL>([&](auto P) { A0[P()] = 0.0f; });
+                    }
+                }
                 scalar_result();
                 break;
             // Vec3 block ops: every operand component is staged, the result replaces A
@@ -851,13 +884,14 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
         if constexpr (T >= 2) { RSR_TAN1(STMT, 1, b) }                 \
         if constexpr (T >= 4) { RSR_TAN1(STMT, 2, c) RSR_TAN1(STMT, 3, d) } \
     } while (0)
-// Spill slot s, channel ch (0 = value, 1 + j = tangent j) at lds[(s * (1 + T) + ch) * 64 + lane].
+// Spill slot s, channel ch (0 = value, 1 + j = tangent j) at lds[(s * (1 + T) + ch) * 64 + lane]
+// for s < kMaxSpillSlots, else at gsp[((s - kMaxSpillSlots) * (1 + T) + ch) * 64 + lane].
 // As in the primal interpreter, operands a and b of every channel are read before the
 // dispatch and the results written after it.
 template <int T>
 __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restrict__ code, uint32_t len,
                                          const float* __restrict__ params, uint32_t k0, float* __restrict__ lds,
-                                         int lane) {
+                                         float* __restrict__ gsp, int lane) {
     Uop nx = load_uop(code, 0);
     for (uint32_t pc = 0; pc < len; ++pc) {
         const Uop u = nx;
@@ -992,15 +1026,32 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                 RSR_TAN(rg[j] = (ga[j] + gb[j]) + Gj[c]);
                 break;
             }
-            case U_SPILL:
+            case U_SPILL: {
+                // slots below kMaxSpillSlots in LDS, the others in the wave's device area
+                const bool in_lds = x < (uint32_t)kMaxSpillSlots;
+                float* base = in_lds ? lds : gsp;
+                const uint32_t sx = in_lds ? x : x - (uint32_t)kMaxSpillSlots;
                 rv = va;
-                lds[(x * (1 + T)) * 64u + lane] = rv;
-                RSR_TAN(rg[j] = ga[j]; lds[(x * (1 + T) + 1 + j) * 64u + lane] = rg[j]);
+                RSR_TAN(rg[j] = ga[j]);
+                if (base != nullptr) {
+                    base[(sx * (1 + T)) * 64u + lane] = rv;
+                    RSR_TAN(base[(sx * (1 + T) + 1 + j) * 64u + lane] = rg[j]);
+                }
                 break;
-            case U_FILL:
-                rv = lds[(x * (1 + T)) * 64u + lane];
-                RSR_TAN(rg[j] = lds[(x * (1 + T) + 1 + j) * 64u + lane]);
+            }
+            case U_FILL: {
+                const bool in_lds = x < (uint32_t)kMaxSpillSlots;
+                const float* base = in_lds ? lds : gsp;
+                const uint32_t sx = in_lds ? x : x - (uint32_t)kMaxSpillSlots;
+                if (base != nullptr) {
+                    rv = base[(sx * (1 + T)) * 64u + lane];
+                    RSR_TAN(rg[j] = base[(sx * (1 + T) + 1 + j) * 64u + lane]);
+                } else {
+                    rv = 0.0f;
+                    RSR_TAN(rg[j] = 0.0f);
+                }
                 break;
+            }
             case U_DOT3: {
                 __builtin_assume(a < 30u);
                 __builtin_assume(b < 30u);

@@ -75,8 +75,8 @@ enum UOp : uint8_t {
     U_ADD3,       // d = (a + b) + c
     U_RCPSAFE,    // d = 1 / ((|a| < 1e-9) ? 1 : a)
     U_INVSEL,     // d = (|c| < 1e-9) ? K : (A * b), A = flags&1 ? -a : a, K = flags&2 ? 1 : 0
-    U_SPILL,      // LDS slot x <- a (d = a: the register is rewritten with its own value)
-    U_FILL,       // d <- LDS slot x
+    U_SPILL,      // spill slot x <- a (d = a: the register is rewritten with its own value)
+    U_FILL,       // d <- spill slot x
     // Vec3 micro-ops on three consecutive registers (operands a..a+2, b..b+2, results
     // d..d+2); every operand component is read before any result is written, so a
     // result block may overlap its operands
@@ -132,8 +132,16 @@ enum UOp : uint8_t {
     U_COUNT
 };
 
-// LDS spill slots available to one program (per point of a lane, per channel).
+// Spill slots of one program (per point of a lane, per channel).  Slots [0, kMaxSpillSlots)
+// live in the workgroup's LDS; slots [kMaxSpillSlots, kMaxSlots) live in a per-wave area of
+// device memory, which only the kernels for such programs ("big" programs, ProgDesc
+// out_reg & kBigProg) provide.  kMaxSlots covers the reference's five typed stacks at their
+// full depth of 32 (state.rs:12-37: 32 * (1 + 2 + 3 + 4 + 9) = 608 components) plus the
+// temporaries of one operator.
 constexpr int kMaxSpillSlots = 16;
+constexpr int kMaxSlots = 640;
+// ProgDesc.out_reg flag: the program spills beyond the LDS slots (see above).
+constexpr uint32_t kBigProg = 1u << 8;
 
 // Register file geometry: 32 f32 components per point per lane.  Dataset
 // feature components occupy registers [0, F); temporaries use [F, 32).
@@ -217,7 +225,7 @@ struct ProgDesc {
     uint32_t code_off;   // first micro-op in the code array
     uint32_t code_len;   // number of micro-ops
     uint32_t param_off;  // first flat parameter in the params array
-    uint32_t out_reg;    // register holding the Float result
+    uint32_t out_reg;    // register holding the Float result (low five bits), | kBigProg
 };
 
 }  // namespace rsr

@@ -66,12 +66,26 @@ int download_params(rsr_programs* p) {
 
 // The lowering's cost-sorted scheduling order restricted to the programs with on[i] != 0,
 // uploaded into the job's buffers: steps over a subset launch work for the subset only.
-int subset_order(JobBuffers& jb, const Lowering* L, const std::vector<uint8_t>& on, int32_t** d_order, int* count) {
+// The big programs stay last; *n_big counts them.
+int subset_order(JobBuffers& jb, const Lowering* L, const std::vector<uint8_t>& on, int32_t** d_order, int* count,
+                 int* n_big = nullptr) {
     std::vector<int32_t> o;
+    int nb = 0;
     for (int32_t i : L->order)
-        if (on[i]) o.push_back(i);
+        if (on[i]) {
+            o.push_back(i);
+            nb += L->big[i];
+        }
     *count = (int)o.size();
+    if (n_big) *n_big = nb;
     return jb.put(o, d_order);
+}
+
+// A step's view of a batch over a subset order.
+void set_order(EvalStep& st, const int32_t* order, int count, int n_big) {
+    st.P.order = order;
+    st.P.n_progs = count;
+    st.P.n_big = n_big;
 }
 
 EvalStep mse_step(const rsr_programs* p, const Lowering* L, const rsr_dataset* ds, const uint8_t* active,
@@ -107,7 +121,7 @@ class EvalJob final : public Job {
     float* d_mse = nullptr;
     uint8_t* d_act = nullptr;
     int32_t* d_order = nullptr;  // subset order when not every program is evaluated
-    int n_order = 0;
+    int n_order = 0, n_order_big = 0;
     float* out = nullptr;
     std::vector<uint8_t> on;
     int n_bad = 0;
@@ -115,10 +129,7 @@ class EvalJob final : public Job {
 
     int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>&) override {
         EvalStep st = mse_step(p, L, ds, d_act, d_mse);
-        if (d_order) {
-            st.P.order = d_order;
-            st.P.n_progs = n_order;
-        }
+        if (d_order) set_order(st, d_order, n_order, n_order_big);
         mse.push_back(st);
         node_evals += nodes * (uint64_t)ds->num_samples;
         return RSR_OK;
@@ -155,7 +166,7 @@ class GradJob final : public Job {
     uint8_t *d_act0 = nullptr, *d_act1 = nullptr;
     bool any_zero = false, any_pos = false;
     int32_t *d_order0 = nullptr, *d_order1 = nullptr;
-    int n_order0 = 0, n_order1 = 0;
+    int n_order0 = 0, n_order1 = 0, n_big0 = 0, n_big1 = 0;
     float *mse_out = nullptr, *grad_out = nullptr;
     std::vector<uint8_t> on;
     int n_bad = 0;
@@ -165,14 +176,12 @@ class GradJob final : public Job {
         // zero-parameter programs are scored by compute_mse on the primal lowering (evaluator.rs:187-189)
         if (any_zero) {
             EvalStep st = mse_step(p, Lp, ds, d_act0, d_mse);
-            st.P.order = d_order0;
-            st.P.n_progs = n_order0;
+            set_order(st, d_order0, n_order0, n_big0);
             mse.push_back(st);
         }
         if (any_pos) {
             EvalStep st = grad_step(p, Ld, ds, d_act1, d_mse, d_grad);
-            st.P.order = d_order1;
-            st.P.n_progs = n_order1;
+            set_order(st, d_order1, n_order1, n_big1);
             grad.push_back(st);
         }
         node_evals += nodes * (uint64_t)ds->num_samples;
@@ -219,6 +228,7 @@ class LbfgsJob final : public Job {
     uint8_t *act_fit = nullptr, *act_ls = nullptr, *act_grad = nullptr;
     float *d_mse = nullptr, *d_grad = nullptr, *d_fit = nullptr;
     int32_t *order_p = nullptr, *order_d = nullptr;  // scheduling orders over the fitted programs
+    int big_p = 0, big_d = 0;
     float* fitness_out = nullptr;
     int iters = 0, it = 0, ls_round = 0, working = 0;
     Stage stage = INIT;
@@ -229,13 +239,11 @@ class LbfgsJob final : public Job {
         node_evals += n_eval * mean_nodes * (uint64_t)ds->num_samples;
         if (stage == SEARCH) {
             EvalStep st = mse_step(p, Lp, ds, act_ls, d_mse);
-            st.P.order = order_p;
-            st.P.n_progs = (int)prog.size();
+            set_order(st, order_p, (int)prog.size(), big_p);
             mse.push_back(st);
         } else {
             EvalStep st = grad_step(p, Ld, ds, stage == INIT ? act_fit : act_grad, d_mse, d_grad);
-            st.P.order = order_d;
-            st.P.n_progs = (int)prog.size();
+            set_order(st, order_d, (int)prog.size(), big_d);
             grad.push_back(st);
         }
         return RSR_OK;
@@ -320,6 +328,7 @@ class CmaesJob final : public Job {
     std::vector<int> prog;
     uint8_t* act_fit = nullptr;
     int32_t* order_fit = nullptr;
+    int big_fit = 0;
     float *d_mse0 = nullptr, *d_cmse = nullptr, *d_cparams = nullptr, *d_fit = nullptr;
     int* d_iters = nullptr;
     DevPrograms CP{};
@@ -332,8 +341,7 @@ class CmaesJob final : public Job {
     int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>&) override {
         if (stage == INIT) {
             EvalStep st = mse_step(p, Lp, ds, act_fit, d_mse0);
-            st.P.order = order_fit;
-            st.P.n_progs = (int)prog.size();
+            set_order(st, order_fit, (int)prog.size(), big_fit);
             mse.push_back(st);
             node_evals += prog.size() * mean_nodes * (uint64_t)ds->num_samples;
             return RSR_OK;
@@ -563,7 +571,7 @@ int make_eval_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, cons
         if (j->on[i]) j->nodes += prog_nodes(p, i);
     }
     if ((rc = j->jb.get((size_t)p->n, &j->d_mse))) return rc;
-    if (!all && ((rc = j->jb.put(j->on, &j->d_act)) || (rc = subset_order(j->jb, j->L, j->on, &j->d_order, &j->n_order))))
+    if (!all && ((rc = j->jb.put(j->on, &j->d_act)) || (rc = subset_order(j->jb, j->L, j->on, &j->d_order, &j->n_order, &j->n_order_big))))
         return rc;
     *out = std::move(j);
     return RSR_OK;
@@ -604,8 +612,8 @@ int make_grad_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, cons
     }
     if ((rc = j->jb.get((size_t)p->n, &j->d_mse)) || (rc = j->jb.get((size_t)p->n * 32, &j->d_grad)) ||
         (rc = j->jb.put(act0, &j->d_act0)) || (rc = j->jb.put(act1, &j->d_act1)) ||
-        (rc = subset_order(j->jb, Lp, act0, &j->d_order0, &j->n_order0)) ||
-        (rc = subset_order(j->jb, Ld, act1, &j->d_order1, &j->n_order1)))
+        (rc = subset_order(j->jb, Lp, act0, &j->d_order0, &j->n_order0, &j->n_big0)) ||
+        (rc = subset_order(j->jb, Ld, act1, &j->d_order1, &j->n_order1, &j->n_big1)))
         return rc;
     *out = std::move(j);
     return RSR_OK;
@@ -673,8 +681,8 @@ int make_lbfgs_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
     if ((rc = sc.put(fitmask, &j->act_fit)) || (rc = sc.get((size_t)p->n, &j->act_ls)) ||
         (rc = sc.get((size_t)p->n, &j->act_grad)) || (rc = sc.get((size_t)p->n, &j->d_mse)) ||
         (rc = sc.get((size_t)p->n * 32, &j->d_grad)) || (rc = sc.get((size_t)n, &j->d_fit)) ||
-        (rc = subset_order(sc, j->Lp, fitmask, &j->order_p, &cnt)) ||
-        (rc = subset_order(sc, j->Ld, fitmask, &j->order_d, &cnt)))
+        (rc = subset_order(sc, j->Lp, fitmask, &j->order_p, &cnt, &j->big_p)) ||
+        (rc = subset_order(sc, j->Ld, fitmask, &j->order_d, &cnt, &j->big_d)))
         return rc;
     *out = std::move(j);
     return RSR_OK;
@@ -760,7 +768,7 @@ int make_cmaes_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
     for (int i : prog) fitmask[i] = 1;
     int cnt;
     if ((rc = sc.put(fitmask, &j->act_fit)) || (rc = sc.get((size_t)p->n, &j->d_mse0)) ||
-        (rc = subset_order(sc, Lp, fitmask, &j->order_fit, &cnt)))
+        (rc = subset_order(sc, Lp, fitmask, &j->order_fit, &cnt, &j->big_fit)))
         return rc;
     st.n = n;
     st.lambda = lambda;
@@ -824,6 +832,8 @@ int make_cmaes_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
     j->CP.order = d_order;
     j->CP.n_progs = n * lambda;
     j->CP.spill_slots = Lp->spill_slots;
+    j->CP.n_big = j->big_fit * lambda;  // the candidates of big parents sort last, as their parents
+    j->CP.gslots = Lp->gslots;
     *out = std::move(j);
     return RSR_OK;
 }
@@ -936,6 +946,10 @@ int make_nm_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int it
     j->CP.order = d_order;
     j->CP.n_progs = n * kNmSlots;
     j->CP.spill_slots = Lp->spill_slots;
+    int nb = 0;
+    for (int i : prog) nb += Lp->big[i];
+    j->CP.n_big = nb * kNmSlots;  // the candidates of big parents sort last, as their parents
+    j->CP.gslots = Lp->gslots;
     *out = std::move(j);
     return RSR_OK;
 }
@@ -1039,8 +1053,16 @@ int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int
     const bool ref = ctx->summation == RSR_SUM_REFERENCE;
     EvalTuning t = ctx->tuning;
     t.ppl = ppl;
+    int rc;
+    // device spill areas of the big-program launches (the same for every round)
+    BigSpill big;
+    big.floats = big_spill_floats(steps.data(), (int)steps.size(), ppl, grad);
+    if (big.floats) {
+        if ((rc = ensure_scratch(ctx, ctx->scratch_gspill, big.floats * sizeof(float)))) return rc;
+        big.ptr = (float*)ctx->scratch_gspill.ptr;
+    }
     if (ref && grad) {  // one wave per (program, tangent pass) walks the tiles in order
-        HIP_TRY(launch_grad_steps(steps.data(), (int)steps.size(), true, ctx->stream, ctx->next_timer()),
+        HIP_TRY(launch_grad_steps(steps.data(), (int)steps.size(), true, ctx->stream, ctx->next_timer(), big),
                 "reference-order gradient launch");
         return RSR_OK;
     }
@@ -1058,7 +1080,6 @@ int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int
     total *= sizeof(float);
     const size_t cap = std::max<size_t>(ctx->arena_bytes, 4096);
     const int64_t rounds = (int64_t)((total + cap - 1) / cap);
-    int rc;
     float* state = nullptr;
     if (rounds > 1) {
         size_t sf = 0;
@@ -1092,8 +1113,8 @@ int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int
             s.partials = arena;
             arena += part_floats(s);
         }
-        HIP_TRY(grad ? launch_grad_steps(round.data(), (int)round.size(), false, ctx->stream, ctx->next_timer())
-                     : launch_mse_steps(round.data(), (int)round.size(), t, terms, ctx->stream, ctx->next_timer()),
+        HIP_TRY(grad ? launch_grad_steps(round.data(), (int)round.size(), false, ctx->stream, ctx->next_timer(), big)
+                     : launch_mse_steps(round.data(), (int)round.size(), t, terms, ctx->stream, ctx->next_timer(), big),
                 "evaluation launch");
     }
     return RSR_OK;
@@ -1191,6 +1212,7 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
     Lowering* L;
     int rc = get_lowering(p, ds->num_features, false, &L);
     if (rc) return rc;
+    if (L->n_big > 0) return RSR_E_UNSUPPORTED;  // big programs run eagerly (context spill area)
     const int summ = ctx->summation;
     const bool terms = summ == RSR_SUM_REFERENCE;
     EvalTuning t = ctx->tuning;

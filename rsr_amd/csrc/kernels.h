@@ -26,8 +26,28 @@ struct DevPrograms {
     const float* params;
     const int32_t* order;  // scheduling position -> program index (cost-sorted)
     int n_progs;
-    int spill_slots;       // LDS spill slots per point per channel (max over the batch)
+    int spill_slots;       // LDS spill slots per point per channel (max over the batch, <= kMaxSpillSlots)
+    // Big programs (spill slots beyond the LDS ones, ProgDesc out_reg & kBigProg) sit at the
+    // last n_big scheduling positions; they run in bounded launches of their own whose
+    // waves each own `gslots` device-memory slots per point per channel.
+    int n_big = 0;
+    int gslots = 0;
 };
+
+// Device memory for the spill slots of big programs: one area per wave of their launches.
+struct BigSpill {
+    float* ptr = nullptr;
+    size_t floats = 0;
+};
+// Waves of a big-program launch (a multiple of four) whose per-wave area holds
+// (kMaxSpillSlots + gslots) * channels * 64 floats: those launches keep every spill slot in
+// device memory (channels: points per lane for values, 1 + tangent channels for gradients).
+int64_t big_waves(int gslots, int channels);
+// Floats of device memory a big-program launch of these per-point entry points needs.
+inline size_t big_points_floats(int gslots, int channels) {
+    return (size_t)big_waves(gslots, channels) * (size_t)(kMaxSpillSlots + gslots) * channels * 64;
+}
+
 
 struct EvalTuning {
     int ppl = 2;        // points per lane: 1, 2 or 4
@@ -61,6 +81,10 @@ struct EvalStep {
     bool first = true, last = true;
 };
 
+// Floats of device memory that the big-program launches of these steps need (0: none).
+// ppl: points per lane of an MSE launch; grad: a gradient launch (one point per lane).
+size_t big_spill_floats(const EvalStep* steps, int n, int ppl, bool grad);
+
 // Kernel-argument segment table (kernarg segment, read with scalar loads).
 struct SegDesc {
     const UInstr* code;
@@ -77,7 +101,9 @@ struct SegDesc {
     const uint8_t* nparams;
     float* kstate;
     int64_t npad, count8, tile0, n_tiles, first_wave, first_thread;
+    int64_t first_big;  // first work item of the segment in the big-program launch
     int n_progs, group, n_groups, F, n_pass, slots, first, last, terms, tile_pts;
+    int n_small;        // scheduling positions [0, n_small) run in the main launch
     float n_samples;
 };
 constexpr int kMaxSegs = 16;
@@ -104,19 +130,21 @@ size_t grad_state_floats(const EvalStep& st);
 // Kahan-compensated FP32 by the finalize kernel.  terms = true: per-point terms folded in
 // the reference's order (bit-identical to the CPU evaluator).
 hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune, bool terms, hipStream_t s,
-                            const KernelTimer* timer = nullptr);
+                            const KernelTimer* timer = nullptr, BigSpill big = {});
 // MSE + gradient (compute_mse_with_gradient, evaluator.rs:177-224) of every active
 // program with parameters, from the dual lowering.
 hipError_t launch_grad_steps(const EvalStep* steps, int n, bool reference_order, hipStream_t s,
-                             const KernelTimer* timer = nullptr);
+                             const KernelTimer* timer = nullptr, BigSpill big = {});
 
 // Per-point values of programs [first, first+count) (eval_simd over every batch);
 // out[(i - first) * count8 + point].
-hipError_t launch_eval_points(const DevPrograms& P, const DevDataset& D, int first, int count, float* out,
-                              hipStream_t s);
+// Big programs among them need big.floats >= big_points_floats(P.gslots, 1).
+hipError_t launch_eval_points(const DevPrograms& P, const DevDataset& D, int first, int count, bool any_big,
+                              float* out, hipStream_t s, BigSpill big = {});
 
 // Per-point value and tangent for one program and one active parameter (eval_simd_dual).
-hipError_t launch_eval_points_dual(const DevPrograms& P, const DevDataset& D, int prog, int active_param,
-                                   float* val_out, float* grad_out, hipStream_t s);
+// A big program needs big.floats >= big_points_floats(P.gslots, 2).
+hipError_t launch_eval_points_dual(const DevPrograms& P, const DevDataset& D, int prog, int active_param, bool is_big,
+                                   float* val_out, float* grad_out, hipStream_t s, BigSpill big = {});
 
 }  // namespace rsr

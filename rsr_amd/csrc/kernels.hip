@@ -19,6 +19,7 @@
 // Reference-order summation: one wave per (program[, pass]) walks the tiles in order
 // and folds each 8-point batch into 8 lane accumulators (see fold_batches).
 #include <float.h>
+#include <algorithm>
 #include "interp.h"
 #include "kernels.h"
 
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if (!split_work<16>(w, S.n_tiles, S.n_groups, lt, g)) return;
     if (S.live != nullptr && *S.live == 0) return;
     const int first = g * S.group;
-    const int last = min(first + S.group, S.n_progs);
+    const int last = min(first + S.group, S.n_small);
     if (S.active != nullptr) {  // nothing to evaluate in this group: skip the feature loads too
         bool any = false;
         for (int sp = first; sp < last; ++sp) any = any || S.active[uni((uint32_t)S.order[sp])] != 0;
@@ -177,7 +178,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         float s = 0.0f;
         if (S.active == nullptr || S.active[prog]) {
             const ProgDesc pd = S.progs[prog];
-            run_program<PPL>(R, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), lds, lane);
+            run_program<PPL>(R, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), lds,
+                             nullptr, lane);
             const uint32_t o = uni(pd.out_reg) & 31u;
             // targets are re-read per program (cache hits) rather than held in registers
             // through the interpreter loop
@@ -308,7 +310,7 @@ __global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_s
     if (!split_work<8>(w, S.n_tiles, S.n_groups, lt, g)) return;
     if (S.live != nullptr && *S.live == 0) return;
     const int first = g * S.group;
-    const int last = min(first + S.group, S.n_progs);
+    const int last = min(first + S.group, S.n_small);
     float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * (1 + kTan) * 64;
     const int64_t base = (S.tile0 + lt) * 64;
     const int64_t pt = base + lane;
@@ -333,7 +335,7 @@ __global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_s
         for (int pass = 0; pass * kTan < np; ++pass) {
             const uint32_t k0 = (uint32_t)(pass * kTan);
             run_dual<kTan>(G, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), k0, lds,
-                           lane);
+                           nullptr, lane);
             const v32f vfile = G.v;
             const float diff = vfile[o] - S.target[pt];
             if (pass == 0) {
@@ -420,19 +422,8 @@ __global__ void k_finalize_grad(const SegTable T) {
     }
 }
 
-// Reference-order gradient sums: one wave per (program, pass) walks the tiles in order.
-__global__ __launch_bounds__(256) void k_eval_grad_exact(const SegTable T, int spill_slots) {
-    extern __shared__ float smem[];
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
-    const int si = find_seg_wave(T, wave);
-    const SegDesc& S = T.s[si];
-    const int64_t w = wave - S.first_wave;
-    const int sp = (int)(w / S.n_pass);
-    const int pass = (int)(w - (int64_t)sp * S.n_pass);
-    if (sp >= S.n_progs) return;
-    if (S.live != nullptr && *S.live == 0) return;
-    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * (1 + kTan) * 64;
+// Reference-order gradient sums of one (program, pass): the wave walks the tiles in order.
+__device__ __forceinline__ void grad_exact_one(const SegDesc& S, int sp, int pass, float* lds, float* gsp, int lane) {
     const int prog = (int)uni((uint32_t)S.order[sp]);
     const int np = (int)uni(S.nparams[prog]);
     if ((S.active != nullptr && !S.active[prog]) || np == 0 || pass * kTan >= np) return;
@@ -448,7 +439,8 @@ __global__ __launch_bounds__(256) void k_eval_grad_exact(const SegTable T, int s
         DualFiles<kTan> G;
         G.v = R1.a;
         G.a = G.b = G.c = G.d = (v32f)(0.0f);
-        run_dual<kTan>(G, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), k0, lds, lane);
+        run_dual<kTan>(G, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), k0, lds, gsp,
+                       lane);
         const int64_t pt = base + lane;
         const bool valid = pt < S.count8;
         const v32f vf = G.v;
@@ -467,6 +459,162 @@ __global__ __launch_bounds__(256) void k_eval_grad_exact(const SegTable T, int s
         const int k = (int)k0 + j;
         const float g = reduce_lanes8(gacc[j]) / S.n_samples;
         if (lane == 0 && k < 32) S.grad_out[(int64_t)prog * 32 + k] = (fin && k < np) ? g : 0.0f;
+    }
+}
+
+// One wave per (program, pass) of the small positions.
+__global__ __launch_bounds__(256) void k_eval_grad_exact(const SegTable T, int spill_slots) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+    const int si = find_seg_wave(T, wave);
+    const SegDesc& S = T.s[si];
+    const int64_t w = wave - S.first_wave;
+    const int sp = (int)(w / S.n_pass);
+    const int pass = (int)(w - (int64_t)sp * S.n_pass);
+    if (sp >= S.n_small) return;
+    if (S.live != nullptr && *S.live == 0) return;
+    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * (1 + kTan) * 64;
+    grad_exact_one(S, sp, pass, lds, nullptr, lane);
+}
+
+// ---------------------------------------------------------------------------------
+// Big programs (spill slots in device memory): bounded launches, grid-stride over work
+// items; wave w owns the device area gspill + w * gwave_floats for its whole life, so no
+// two resident waves share one.  Work items are numbered per segment from first_big.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ int find_seg_big(const SegTable& T, int64_t it) {
+    int si = 0;
+#pragma unroll
+    for (int s = 1; s < kMaxSegs; ++s) si = (s < T.n && it >= T.s[s].first_big) ? s : si;
+    return si;
+}
+
+// MSE: item = (big position j, tile), program-major so that consecutive items of a wave
+// reuse the program's code.  Writes the same per-tile partial (or per-point terms) as the
+// main kernel would.
+template <int PPL>
+__global__ __launch_bounds__(256) void k_eval_mse_big(const SegTable T, int spill_slots, int64_t n_items,
+                                                      float* __restrict__ gspill, int64_t gwave_floats) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+    const int64_t n_waves = (int64_t)uni(gridDim.x) * kWaves;
+    // every spill slot in device memory: the first kMaxSpillSlots where the LDS ones would be
+    float* lds = gspill + wave * gwave_floats;
+    float* gsp = lds + kMaxSpillSlots * PPL * 64;
+    for (int64_t it = wave; it < n_items; it += n_waves) {
+        const int si = find_seg_big(T, it);
+        const SegDesc& S = T.s[si];
+        const int64_t w = it - S.first_big;
+        const int j = (int)(w / S.n_tiles);
+        const int64_t lt = w - (int64_t)j * S.n_tiles;
+        const int sp = S.n_small + j;
+        if (S.live != nullptr && *S.live == 0) continue;
+        const int prog = (int)uni((uint32_t)S.order[sp]);
+        const bool on = S.active == nullptr || S.active[prog];
+        const int64_t base = (S.tile0 + lt) * (64 * PPL);
+        float s = 0.0f;
+        if (on) {
+            Files<PPL> R;
+            sfor<PPL>([&](auto P) { fget<P(), PPL>(R) = (v32f)(0.0f); });
+            load_features<PPL>(R, S.feat, S.F, S.npad, base, lane);
+            const ProgDesc pd = S.progs[prog];
+            run_program<PPL>(R, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), lds, gsp,
+                             lane);
+            const uint32_t o = uni(pd.out_reg) & 31u;
+            if (S.terms) {
+                float* row = S.partials + (int64_t)sp * S.n_tiles * (64 * PPL) + lt * (64 * PPL);
+                sfor<PPL>([&](auto P) {
+                    const v32f r = fget<P(), PPL>(R);
+                    const float diff = r[o] - S.target[base + P() * 64 + lane];
+                    row[P() * 64 + lane] = diff * diff;
+                });
+                continue;
+            }
+            sfor<PPL>([&](auto P) {
+                const v32f r = fget<P(), PPL>(R);
+                const float diff = r[o] - S.target[base + P() * 64 + lane];
+                const float sq = diff * diff;
+                s = s + ((base + P() * 64 + lane < S.count8) ? sq : 0.0f);
+            });
+            s = wave_sum(s);
+        }
+        if (!S.terms && lane == 0) S.partials[lt * S.n_progs + sp] = s;
+    }
+}
+
+// Gradients, Kahan summation: item = (big position j, tile), every pass of the program.
+__global__ __launch_bounds__(256) void k_eval_grad_big(const SegTable T, int spill_slots, int64_t n_items,
+                                                       float* __restrict__ gspill, int64_t gwave_floats) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+    const int64_t n_waves = (int64_t)uni(gridDim.x) * kWaves;
+    float* lds = gspill + wave * gwave_floats;  // every spill slot in device memory
+    float* gsp = lds + kMaxSpillSlots * (1 + kTan) * 64;
+    for (int64_t it = wave; it < n_items; it += n_waves) {
+        const int si = find_seg_big(T, it);
+        const SegDesc& S = T.s[si];
+        const int64_t w = it - S.first_big;
+        const int j = (int)(w / S.n_tiles);
+        const int64_t lt = w - (int64_t)j * S.n_tiles;
+        const int sp = S.n_small + j;
+        if (S.live != nullptr && *S.live == 0) continue;
+        const int prog = (int)uni((uint32_t)S.order[sp]);
+        const int np = (int)uni(S.nparams[prog]);
+        if ((S.active != nullptr && !S.active[prog]) || np == 0) continue;
+        const int64_t base = (S.tile0 + lt) * 64;
+        const int64_t pt = base + lane;
+        const bool valid = pt < S.count8;
+        Files<1> R1;
+        R1.a = (v32f)(0.0f);
+        load_features<1>(R1, S.feat, S.F, S.npad, base, lane);
+        DualFiles<kTan> G;
+        G.v = R1.a;
+        G.a = G.b = G.c = G.d = (v32f)(0.0f);
+        const ProgDesc pd = S.progs[prog];
+        const uint32_t o = uni(pd.out_reg) & 31u;
+        float* dst = S.partials + (lt * S.n_progs + sp) * S.slots;
+        for (int pass = 0; pass * kTan < np; ++pass) {
+            const uint32_t k0 = (uint32_t)(pass * kTan);
+            run_dual<kTan>(G, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), k0, lds,
+                           gsp, lane);
+            const v32f vfile = G.v;
+            const float diff = vfile[o] - S.target[pt];
+            if (pass == 0) {
+                const float sq = diff * diff;
+                const float sum = wave_sum(valid ? sq : 0.0f);
+                if (lane == 0) dst[0] = sum;
+            }
+            const float two_diff = 2.0f * diff;
+            const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
+#pragma unroll
+            for (int q = 0; q < kTan; ++q) {
+                const int k = (int)k0 + q;
+                if (k < np) {
+                    const float sum = wave_sum(valid ? two_diff * gf[q][o] : 0.0f);
+                    if (lane == 0) dst[1 + k] = sum;
+                }
+            }
+        }
+    }
+}
+
+// Gradients, reference order: item = (big position j, pass), walking the tiles.
+__global__ __launch_bounds__(256) void k_eval_grad_exact_big(const SegTable T, int spill_slots, int64_t n_items,
+                                                             float* __restrict__ gspill, int64_t gwave_floats) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+    const int64_t n_waves = (int64_t)uni(gridDim.x) * kWaves;
+    float* lds = gspill + wave * gwave_floats;  // every spill slot in device memory
+    float* gsp = lds + kMaxSpillSlots * (1 + kTan) * 64;
+    for (int64_t it = wave; it < n_items; it += n_waves) {
+        const int si = find_seg_big(T, it);
+        const SegDesc& S = T.s[si];
+        const int64_t w = it - S.first_big;
+        const int j = (int)(w / S.n_pass);
+        const int pass = (int)(w - (int64_t)j * S.n_pass);
+        if (S.live != nullptr && *S.live == 0) continue;
+        grad_exact_one(S, S.n_small + j, pass, lds, gsp, lane);
     }
 }
 
@@ -518,21 +666,58 @@ static SegDesc make_seg(const EvalStep& st, int ppl, int group) {
     d.first = st.first ? 1 : 0;
     d.last = st.last ? 1 : 0;
     d.n_progs = st.P.n_progs;
+    d.n_small = st.P.n_progs - std::min(std::max(st.P.n_big, 0), st.P.n_progs);
     d.group = group;
-    d.n_groups = (st.P.n_progs + group - 1) / group;
+    d.n_groups = (d.n_small + group - 1) / group;
     d.F = st.D.num_features;
     d.n_pass = st.max_params > 0 ? (st.max_params + kTan - 1) / kTan : 1;
     d.n_samples = (float)st.D.num_samples;
     return d;
 }
 
+static int lds_slots(int spill_slots) { return std::min(spill_slots, kMaxSpillSlots); }
+
+// Floats of one wave's device spill area: every slot, the LDS-range ones included.
+static int64_t big_wave_floats(int gslots, int channels) {
+    return (int64_t)(kMaxSpillSlots + std::max(gslots, 0)) * channels * 64;
+}
+
+// At most 1024 waves (four per CU), fewer when the per-wave area is large: the launch's
+// device area stays under 256 MiB.  A multiple of the waves per workgroup.
+int64_t big_waves(int gslots, int channels) {
+    const int64_t per = big_wave_floats(gslots, channels) * (int64_t)sizeof(float);
+    const int64_t w = std::max<int64_t>(64, std::min<int64_t>(1024, ((int64_t)256 << 20) / per));
+    return w / kWaves * kWaves;
+}
+
+size_t big_spill_floats(const EvalStep* steps, int n, int ppl, bool grad) {
+    int g = -1;
+    for (int i = 0; i < n; ++i)
+        if (steps[i].P.n_big > 0 && steps[i].P.n_progs > 0) g = std::max(g, steps[i].P.gslots);
+    if (g < 0) return 0;
+    const int ch = grad ? 1 + kTan : ppl;
+    return (size_t)(big_waves(g, ch) * big_wave_floats(g, ch));
+}
+
+// Launch the big-program kernel K over `items` work items; each launched wave owns one
+// device spill area of big.ptr, which must hold every launched wave's area.
+template <typename K>
+static hipError_t launch_big(K kernel, const SegTable& T, int64_t items, int gslots, int channels, int spill,
+                             BigSpill big, hipStream_t s) {
+    const int64_t blocks = (std::min<int64_t>(big_waves(gslots, channels), items) + kWaves - 1) / kWaves;
+    const int64_t gwave = big_wave_floats(gslots, channels);
+    if (big.ptr == nullptr || (size_t)(blocks * kWaves * gwave) > big.floats) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(64 * kWaves), 0, s, T, spill, items, big.ptr, gwave);
+    return hipGetLastError();
+}
+
 hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune, bool terms, hipStream_t s,
-                            const KernelTimer* timer) {
+                            const KernelTimer* timer, BigSpill big) {
     for (int base = 0; base < n; base += kMaxSegs) {
         const int cnt = min(kMaxSegs, n - base);
         SegTable T{};
-        int64_t waves = 0, threads = 0;
-        int spill = 0;
+        int64_t waves = 0, threads = 0, big_items = 0;
+        int spill = 0, gslots = 0;
         for (int i = 0; i < cnt; ++i) {
             const EvalStep& st = steps[base + i];
             if (st.P.n_progs <= 0) continue;
@@ -541,24 +726,38 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
             d.terms = terms ? 1 : 0;
             d.first_wave = waves;
             d.first_thread = threads;
+            d.first_big = big_items;
             waves += d.n_tiles * d.n_groups;
+            big_items += d.n_tiles * (d.n_progs - d.n_small);
+            if (d.n_progs > d.n_small) gslots = max(gslots, st.P.gslots);
             // finalize threads (whole waves per segment), or fold waves
             threads += terms ? ((int64_t)d.n_progs + 7) / 8 : ((int64_t)d.n_progs + 63) / 64 * 64;
-            spill = max(spill, st.P.spill_slots);
+            spill = max(spill, lds_slots(st.P.spill_slots));
             T.s[T.n++] = d;
         }
-        if (waves == 0) continue;
+        if (waves == 0 && big_items == 0) continue;
         const int64_t blocks = (waves + kWaves - 1) / kWaves;
         if (blocks > INT32_MAX) return hipErrorInvalidValue;
         if (timer && base == 0) (void)hipEventRecord(timer->begin, s);
         const size_t lds = eval_mse_lds_bytes(spill, tune.ppl);
-        switch (tune.ppl) {
-            case 1: hipLaunchKernelGGL(k_eval_mse<1>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill); break;
-            case 4: hipLaunchKernelGGL(k_eval_mse<4>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill); break;
-            default: hipLaunchKernelGGL(k_eval_mse<2>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill); break;
+        if (waves > 0) {
+            switch (tune.ppl) {
+                case 1: hipLaunchKernelGGL(k_eval_mse<1>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill); break;
+                case 4: hipLaunchKernelGGL(k_eval_mse<4>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill); break;
+                default: hipLaunchKernelGGL(k_eval_mse<2>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill); break;
+            }
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
         }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        if (big_items > 0) {
+            hipError_t e;
+            switch (tune.ppl) {
+                case 1: e = launch_big(k_eval_mse_big<1>, T, big_items, gslots, 1, spill, big, s); break;
+                case 4: e = launch_big(k_eval_mse_big<4>, T, big_items, gslots, 4, spill, big, s); break;
+                default: e = launch_big(k_eval_mse_big<2>, T, big_items, gslots, 2, spill, big, s); break;
+            }
+            if (e != hipSuccess) return e;
+        }
         if (timer && base + cnt >= n) (void)hipEventRecord(timer->end, s);
         const int tb = 256;
         if (terms)
@@ -566,45 +765,59 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
                                s, T);
         else
             hipLaunchKernelGGL(k_finalize_mse, dim3((unsigned)((threads + tb - 1) / tb)), dim3(tb), 0, s, T);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
 hipError_t launch_grad_steps(const EvalStep* steps, int n, bool reference_order, hipStream_t s,
-                             const KernelTimer* timer) {
+                             const KernelTimer* timer, BigSpill big) {
     for (int base = 0; base < n; base += kMaxSegs) {
         const int cnt = min(kMaxSegs, n - base);
         SegTable T{};
-        int64_t waves = 0, threads = 0;
-        int spill = 0;
+        int64_t waves = 0, threads = 0, big_items = 0;
+        int spill = 0, gslots = 0;
         for (int i = 0; i < cnt; ++i) {
             const EvalStep& st = steps[base + i];
             if (st.max_params <= 0 || st.P.n_progs <= 0) continue;
             SegDesc d = make_seg(st, 1, reference_order ? 1 : grad_group(step_tiles(st, 1), st.P.n_progs));
             d.first_wave = waves;
             d.first_thread = threads;
-            waves += reference_order ? (int64_t)d.n_progs * d.n_pass : d.n_tiles * d.n_groups;
+            d.first_big = big_items;
+            const int nb = d.n_progs - d.n_small;
+            waves += reference_order ? (int64_t)d.n_small * d.n_pass : d.n_tiles * d.n_groups;
+            big_items += reference_order ? (int64_t)nb * d.n_pass : d.n_tiles * nb;
+            if (nb > 0) gslots = max(gslots, st.P.gslots);
             threads += ((int64_t)d.n_progs * d.slots + 63) / 64 * 64;  // whole waves per segment
-            spill = max(spill, st.P.spill_slots);
+            spill = max(spill, lds_slots(st.P.spill_slots));
             T.s[T.n++] = d;
         }
-        if (waves == 0) continue;
+        if (waves == 0 && big_items == 0) continue;
         const int64_t blocks = (waves + kWaves - 1) / kWaves;
         if (blocks > INT32_MAX) return hipErrorInvalidValue;
         const size_t lds = eval_grad_lds_bytes(spill);
         if (timer && base == 0) (void)hipEventRecord(timer->begin, s);
-        if (reference_order)
-            hipLaunchKernelGGL(k_eval_grad_exact, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
-        else
-            hipLaunchKernelGGL(k_eval_grad, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        if (waves > 0) {
+            if (reference_order)
+                hipLaunchKernelGGL(k_eval_grad_exact, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
+            else
+                hipLaunchKernelGGL(k_eval_grad, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        if (big_items > 0) {
+            const hipError_t e =
+                reference_order ? launch_big(k_eval_grad_exact_big, T, big_items, gslots, 1 + kTan, spill, big, s)
+                                : launch_big(k_eval_grad_big, T, big_items, gslots, 1 + kTan, spill, big, s);
+            if (e != hipSuccess) return e;
+        }
         if (timer && base + cnt >= n) (void)hipEventRecord(timer->end, s);
         if (!reference_order) {
             const int tb = 256;
             hipLaunchKernelGGL(k_finalize_grad, dim3((unsigned)((threads + tb - 1) / tb)), dim3(tb), 0, s, T);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
         }
     }
     return hipSuccess;
@@ -625,60 +838,120 @@ __global__ __launch_bounds__(256) void k_eval_points(const UInstr* __restrict__ 
     const int64_t tile = wave / count;
     const int i = (int)(wave - tile * count);
     if (tile >= n_tiles) return;
+    const ProgDesc pd = progs[first + i];
+    if (uni(pd.out_reg) & kBigProg) return;  // the big-program launch writes these rows
     const int64_t base = tile * 64;
     Files<1> R;
     R.a = (v32f)(0.0f);
     load_features<1>(R, feat, F, npad, base, lane);
-    const ProgDesc pd = progs[first + i];
-    run_program<1>(R, code + uni(pd.code_off), uni(pd.code_len), params + uni(pd.param_off), lds, lane);
+    run_program<1>(R, code + uni(pd.code_off), uni(pd.code_len), params + uni(pd.param_off), lds, nullptr, lane);
     const int64_t pt = base + lane;
     const v32f rf = R.a;
     if (pt < count8) out[(int64_t)i * count8 + pt] = rf[uni(pd.out_reg) & 31u];
 }
 
-hipError_t launch_eval_points(const DevPrograms& P, const DevDataset& D, int first, int count, float* out,
-                              hipStream_t s) {
-    if (count <= 0) return hipSuccess;
-    const int64_t n_tiles = eval_tiles(D, 1);
-    const int64_t blocks = (n_tiles * count + kWaves - 1) / kWaves;
-    const size_t lds_bytes = eval_mse_lds_bytes(P.spill_slots, 1);
-    hipLaunchKernelGGL(k_eval_points, dim3((unsigned)blocks), dim3(64 * kWaves), lds_bytes, s, P.code, P.progs,
-                       P.params, first, count, D.feat, D.num_features, D.npad, D.count8, n_tiles, out, P.spill_slots);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(64) void k_eval_points_dual(const UInstr* __restrict__ code, const ProgDesc* __restrict__ progs,
-                                                         const float* __restrict__ params, int prog, int active_param,
+// The big programs among [first, first + count): grid-stride over (program, tile) items,
+// skipping the others (every wave owns its device spill area).
+__global__ __launch_bounds__(256) void k_eval_points_big(const UInstr* __restrict__ code,
+                                                         const ProgDesc* __restrict__ progs,
+                                                         const float* __restrict__ params, int first, int count,
                                                          const float* __restrict__ feat, int F, int64_t npad,
-                                                         int64_t count8, float* __restrict__ val_out,
-                                                         float* __restrict__ grad_out) {
-    extern __shared__ float smem[];
+                                                         int64_t count8, int64_t n_tiles, float* __restrict__ out,
+                                                         int spill_slots, float* __restrict__ gspill,
+                                                         int64_t gwave_floats) {
     const int lane = threadIdx.x & 63;
-    const int64_t base = (int64_t)blockIdx.x * 64;
-    Files<1> R1;
-    R1.a = (v32f)(0.0f);
-    load_features<1>(R1, feat, F, npad, base, lane);
-    DualFiles<1> G;
-    G.v = R1.a;
-    G.a = (v32f)(0.0f);
-    const ProgDesc pd = progs[prog];
-    run_dual<1>(G, code + uni(pd.code_off), uni(pd.code_len), params + uni(pd.param_off), (uint32_t)active_param,
-                smem, lane);
-    const uint32_t o = uni(pd.out_reg) & 31u;
-    const int64_t pt = base + lane;
-    const v32f vf = G.v, gfl = G.a;
-    if (pt < count8) {
-        val_out[pt] = vf[o];
-        grad_out[pt] = gfl[o];
+    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+    const int64_t n_waves = (int64_t)uni(gridDim.x) * kWaves;
+    float* lds = gspill + wave * gwave_floats;  // every spill slot in device memory
+    float* gsp = lds + kMaxSpillSlots * 64;
+    const int64_t n_items = (int64_t)count * n_tiles;
+    for (int64_t it = wave; it < n_items; it += n_waves) {
+        const int i = (int)(it / n_tiles);
+        const int64_t tile = it - (int64_t)i * n_tiles;
+        const ProgDesc pd = progs[first + i];
+        if (!(uni(pd.out_reg) & kBigProg)) continue;
+        const int64_t base = tile * 64;
+        Files<1> R;
+        R.a = (v32f)(0.0f);
+        load_features<1>(R, feat, F, npad, base, lane);
+        run_program<1>(R, code + uni(pd.code_off), uni(pd.code_len), params + uni(pd.param_off), lds, gsp, lane);
+        const int64_t pt = base + lane;
+        const v32f rf = R.a;
+        if (pt < count8) out[(int64_t)i * count8 + pt] = rf[uni(pd.out_reg) & 31u];
     }
 }
 
-hipError_t launch_eval_points_dual(const DevPrograms& P, const DevDataset& D, int prog, int active_param,
-                                   float* val_out, float* grad_out, hipStream_t s) {
+hipError_t launch_eval_points(const DevPrograms& P, const DevDataset& D, int first, int count, bool any_big,
+                              float* out, hipStream_t s, BigSpill big) {
+    if (count <= 0) return hipSuccess;
     const int64_t n_tiles = eval_tiles(D, 1);
-    const size_t lds_bytes = (size_t)P.spill_slots * 2 * 64 * sizeof(float);
-    hipLaunchKernelGGL(k_eval_points_dual, dim3((unsigned)n_tiles), dim3(64), lds_bytes, s, P.code, P.progs, P.params,
-                       prog, active_param, D.feat, D.num_features, D.npad, D.count8, val_out, grad_out);
+    const int64_t blocks = (n_tiles * count + kWaves - 1) / kWaves;
+    const int spill = lds_slots(P.spill_slots);
+    const size_t lds_bytes = eval_mse_lds_bytes(spill, 1);
+    hipLaunchKernelGGL(k_eval_points, dim3((unsigned)blocks), dim3(64 * kWaves), lds_bytes, s, P.code, P.progs,
+                       P.params, first, count, D.feat, D.num_features, D.npad, D.count8, n_tiles, out, spill);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !any_big) return e;
+    const int64_t bblocks = (std::min<int64_t>(big_waves(P.gslots, 1), n_tiles * count) + kWaves - 1) / kWaves;
+    const int64_t gwave = big_wave_floats(P.gslots, 1);
+    if (big.ptr == nullptr || (size_t)(bblocks * kWaves * gwave) > big.floats) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_eval_points_big, dim3((unsigned)bblocks), dim3(64 * kWaves), 0, s, P.code, P.progs, P.params,
+                       first, count, D.feat, D.num_features, D.npad, D.count8, n_tiles, out, spill, big.ptr, gwave);
+    return hipGetLastError();
+}
+
+// One program's value and tangent per point; tiles grid-strided over the launch's waves
+// (one wave per workgroup), each wave with its own device spill area when gspill is set.
+__global__ __launch_bounds__(64) void k_eval_points_dual(const UInstr* __restrict__ code, const ProgDesc* __restrict__ progs,
+                                                         const float* __restrict__ params, int prog, int active_param,
+                                                         const float* __restrict__ feat, int F, int64_t npad,
+                                                         int64_t count8, int64_t n_tiles, float* __restrict__ val_out,
+                                                         float* __restrict__ grad_out, float* __restrict__ gspill,
+                                                         int64_t gwave_floats) {
+    extern __shared__ float smem[];
+    const int lane = threadIdx.x & 63;
+    // a big program keeps every spill slot in the block's device area
+    float* lds = gspill != nullptr ? gspill + (int64_t)blockIdx.x * gwave_floats : smem;
+    float* gsp = gspill != nullptr ? lds + kMaxSpillSlots * 2 * 64 : nullptr;
+    const ProgDesc pd = progs[prog];
+    const uint32_t o = uni(pd.out_reg) & 31u;
+    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const int64_t base = tile * 64;
+        Files<1> R1;
+        R1.a = (v32f)(0.0f);
+        load_features<1>(R1, feat, F, npad, base, lane);
+        DualFiles<1> G;
+        G.v = R1.a;
+        G.a = (v32f)(0.0f);
+        run_dual<1>(G, code + uni(pd.code_off), uni(pd.code_len), params + uni(pd.param_off), (uint32_t)active_param,
+                    lds, gsp, lane);
+        const int64_t pt = base + lane;
+        const v32f vf = G.v, gfl = G.a;
+        if (pt < count8) {
+            val_out[pt] = vf[o];
+            grad_out[pt] = gfl[o];
+        }
+    }
+}
+
+hipError_t launch_eval_points_dual(const DevPrograms& P, const DevDataset& D, int prog, int active_param, bool is_big,
+                                   float* val_out, float* grad_out, hipStream_t s, BigSpill big) {
+    const int64_t n_tiles = eval_tiles(D, 1);
+    const size_t lds_bytes = (size_t)lds_slots(P.spill_slots) * 2 * 64 * sizeof(float);
+    int64_t blocks = n_tiles;
+    float* g = nullptr;
+    int64_t gwave = 0;
+    size_t lds = lds_bytes;
+    if (is_big) {
+        blocks = std::min<int64_t>(big_waves(P.gslots, 2), n_tiles);
+        gwave = big_wave_floats(P.gslots, 2);
+        if (big.ptr == nullptr || (size_t)(blocks * gwave) > big.floats) return hipErrorInvalidValue;
+        g = big.ptr;
+        lds = 0;
+    }
+    hipLaunchKernelGGL(k_eval_points_dual, dim3((unsigned)blocks), dim3(64), lds, s, P.code, P.progs, P.params,
+                       prog, active_param, D.feat, D.num_features, D.npad, D.count8, n_tiles, val_out, grad_out, g,
+                       gwave);
     return hipGetLastError();
 }
 

@@ -15,8 +15,11 @@
 //     only after its last read inside the op, so multi-output ops (CrossV3, MulM3, ...)
 //     never read a clobbered input;
 //   * when the register file is full, the oldest value still on a stack (the one consumed
-//     last in postfix order) is spilled to a per-lane LDS slot and filled back when its
-//     consuming op pops it.
+//     last in postfix order) is spilled to a per-lane slot and filled back when its
+//     consuming op pops it; when the operands of one op do not fit the file, the op's
+//     micro-ops fill and spill its own values on demand.  The first kMaxSpillSlots slots
+//     are LDS, the rest (up to kMaxSlots, enough for the reference's 32-deep typed stacks)
+//     device memory, so every program within the reference's stack limits lowers.
 // Validation replaces the reference's undefined behaviour with status codes: stack
 // underflow or depth > 32 (state.rs:14), out-of-range feature or constant indices,
 // constant type mismatch, Bool/Int loads (silent no-ops in the reference,
@@ -28,11 +31,16 @@
 namespace rsr {
 namespace {
 
+// Component locations: [0, kRegs) a register; kSlotLoc + s spill slot s; kKonstLoc + p a
+// Float constant (flat parameter p) not yet in a register.
+constexpr int kSlotLoc = 64;
+constexpr int kKonstLoc = 1 << 14;
+inline bool in_reg(int l) { return l < kRegs; }
+inline bool in_slot(int l) { return l >= kSlotLoc && l < kKonstLoc; }
+
 struct Val {
     uint8_t n = 0;
-    uint8_t reg[9] = {0};
-    uint8_t slot[9] = {0};   // LDS slot of a spilled component
-    uint16_t spilled = 0;    // bit k: component k lives in LDS slot[k]
+    int16_t loc[9] = {0};    // location of each component (see above)
     uint32_t seq = 0;        // push order (older values are consumed later)
     int32_t konst = -1;      // >= 0: a Float constant (flat parameter index) not yet in a register
 };
@@ -46,7 +54,7 @@ struct MI {
     UOp op;
     int ns;
     Src s[4];
-    uint32_t extra;  // IMM parameter index, or INVSEL flags (kInvNeg | kInvOne)
+    uint32_t extra;  // parameter index (IMM and the constant-operand ops), or INVSEL flags
 };
 
 constexpr int kMaxDepth = 32;
@@ -174,7 +182,7 @@ class Lowerer {
    public:
     Lowerer(const RefProgram& p, int F, bool dual) : p_(p), F_(F), dual_(dual) {
         for (int r = F; r < kRegs; ++r) free_[r] = true;
-        for (int s = 0; s < kMaxSpillSlots; ++s) slot_free_[s] = true;
+        for (int s = 0; s < kMaxSlots; ++s) slot_free_[s] = true;
         // constant payload offsets and flat parameter starts
         int off = 0, flat = 0;
         coff_.resize(p.n_consts);
@@ -199,8 +207,14 @@ class Lowerer {
         if (stk_[0].size() != 1 || stk_[1].size() || stk_[2].size() || stk_[3].size() || stk_[4].size())
             return fail(PROG_ROOT);
         Val root = pop(VtFloat);  // fills the root if it was spilled
+        if (!in_reg(root.loc[0])) {  // nothing else is live: any free temporary will do
+            const int r = alloc();
+            if (r < 0) return fail(PROG_REGS);
+            to_reg(root.loc[0], r);
+            root.loc[0] = (int16_t)r;
+        }
         if (!ok_) return fail(PROG_REGS);
-        out_.out_reg = root.reg[0];
+        out_.out_reg = (uint32_t)root.loc[0];
         out_.status = PROG_OK;
         if (!dual_) annotate_forwarding(out_.code, out_.out_reg);
         return std::move(out_);
@@ -211,7 +225,7 @@ class Lowerer {
     int F_;
     bool dual_;
     bool free_[kRegs] = {false};
-    bool slot_free_[kMaxSpillSlots];
+    bool slot_free_[kMaxSlots];
     bool ok_ = true;
     uint32_t seq_ = 0;
     std::vector<Val> stk_[5];  // F, V2, V3, M2, M3
@@ -234,35 +248,54 @@ class Lowerer {
         }
     }
     bool can_pop(int vt, int n = 1) { return (int)stk_[sidx(vt)].size() >= n; }
-    // Pop a value and bring spilled components back into registers.  A failed fill
-    // leaves ok_ false (reported as PROG_REGS).
+
+    // Emit the micro-op that moves a component from a spill slot or a pending constant
+    // into register r, and free the slot.
+    void to_reg(int loc, int r) {
+        if (in_slot(loc)) {
+            const int slot = loc - kSlotLoc;
+            out_.code.push_back(make_uinstr(U_FILL, (uint32_t)r, 0, 0, (uint32_t)slot));
+            slot_free_[slot] = true;
+        } else {
+            out_.code.push_back(make_uinstr(U_IMM, (uint32_t)r, 0, 0, (uint32_t)(loc - kKonstLoc)));
+        }
+    }
+    int new_slot(int limit = kMaxSlots) {
+        for (int s = 0; s < limit; ++s)
+            if (slot_free_[s]) {
+                slot_free_[s] = false;
+                if ((uint32_t)s + 1 > out_.spill_slots) out_.spill_slots = s + 1;
+                return s;
+            }
+        return -1;
+    }
+    // Register r -> a new spill slot; returns the slot location, or -1 when every slot is taken.
+    int spill_reg(int r, int limit = kMaxSlots) {
+        const int slot = new_slot(limit);
+        if (slot < 0) return -1;
+        out_.code.push_back(make_uinstr(U_SPILL, (uint32_t)r, (uint32_t)r, 0, (uint32_t)slot));
+        free_[r] = true;
+        return kSlotLoc + slot;
+    }
+
     // Pop a value; a pending Float constant is materialised into a register.
     Val pop(int vt) {
         Val v = pop_raw(vt);
-        if (v.konst >= 0) {
-            const int r = alloc();
-            if (r < 0) {
-                ok_ = false;
-                return v;
-            }
-            out_.code.push_back(make_uinstr(U_IMM, (uint32_t)r, 0, 0, (uint32_t)v.konst));
-            v.reg[0] = (uint8_t)r;
-            v.konst = -1;
-        }
+        if (v.konst >= 0) materialize(v);
         return v;
     }
-    // Pop a value as it is (a pending constant stays pending).
+    // Pop a value as it is (a pending constant stays pending).  Spilled components are
+    // filled back into registers while registers can be found; the others stay in their
+    // slots, and the consuming op reads them from there (emit fills them on demand).
     Val pop_raw(int vt) {
         Val v = stk_[sidx(vt)].back();
         stk_[sidx(vt)].pop_back();
         for (int k = 0; k < v.n; ++k)
-            if (v.spilled & (1u << k)) {
-                int r = alloc();
-                if (r < 0) { ok_ = false; return v; }
-                out_.code.push_back(make_uinstr(U_FILL, (uint32_t)r, 0, 0, v.slot[k]));
-                slot_free_[v.slot[k]] = true;
-                v.reg[k] = (uint8_t)r;
-                v.spilled &= (uint16_t)~(1u << k);
+            if (in_slot(v.loc[k])) {
+                const int r = alloc();
+                if (r < 0) break;
+                to_reg(v.loc[k], r);
+                v.loc[k] = (int16_t)r;
             }
         return v;
     }
@@ -272,41 +305,34 @@ class Lowerer {
         stk_[sidx(vt)].push_back(v);
         return true;
     }
-    // Spill one temp component of the oldest stacked value; false if none can be spilled.
-    bool spill_one() {
+    // Spill one temporary component of the oldest stacked value (the one consumed last)
+    // into a slot below `limit`; false if no stacked value holds a temporary register or
+    // every such slot is taken.
+    bool spill_one(int limit = kMaxSlots) {
         Val* victim = nullptr;
         for (auto& st : stk_)
             for (auto& v : st) {
                 if (v.konst >= 0) continue;  // a pending constant holds no register
                 bool has_temp = false;
                 for (int k = 0; k < v.n; ++k)
-                    if (!(v.spilled & (1u << k)) && is_temp(v.reg[k])) has_temp = true;
+                    if (is_temp(v.loc[k])) has_temp = true;
                 if (has_temp && (!victim || v.seq < victim->seq)) victim = &v;
             }
         if (!victim) return false;
-        int slot = -1;
-        for (int s = 0; s < kMaxSpillSlots; ++s)
-            if (slot_free_[s]) { slot = s; break; }
-        if (slot < 0) return false;
         for (int k = 0; k < victim->n; ++k) {
-            if ((victim->spilled & (1u << k)) || !is_temp(victim->reg[k])) continue;
-            const int r = victim->reg[k];
-            out_.code.push_back(make_uinstr(U_SPILL, (uint32_t)r, (uint32_t)r, 0, (uint32_t)slot));
-            slot_free_[slot] = false;
-            if ((uint32_t)slot + 1 > out_.spill_slots) out_.spill_slots = slot + 1;
-            victim->slot[k] = (uint8_t)slot;
-            victim->spilled |= (uint16_t)(1u << k);
-            free_[r] = true;
+            if (!is_temp(victim->loc[k])) continue;
+            const int l = spill_reg(victim->loc[k], limit);
+            if (l < 0) return false;
+            victim->loc[k] = (int16_t)l;
             return true;  // one component per call
         }
         return false;
     }
-    bool is_temp(int r) const { return r >= F_; }
-    void release(int r) {
-        if (is_temp(r)) free_[r] = true;
-    }
-    void release_val(const Val& v) {
-        for (int k = 0; k < v.n; ++k) release(v.reg[k]);
+    bool is_temp(int l) const { return l >= F_ && l < kRegs; }
+    // A value component dies: its register or spill slot is free again.
+    void release(int l) {
+        if (is_temp(l)) free_[l] = true;
+        else if (in_slot(l)) slot_free_[l - kSlotLoc] = true;
     }
     int alloc() {
         for (;;) {
@@ -320,11 +346,10 @@ class Lowerer {
         }
     }
 
-    // A value held in consecutive registers (never true for a partly spilled value).
+    // A value held in consecutive registers.
     static bool contig(const Val& v) {
-        if (v.spilled) return false;
-        for (int k = 1; k < v.n; ++k)
-            if (v.reg[k] != v.reg[0] + k) return false;
+        for (int k = 0; k < v.n; ++k)
+            if (!in_reg(v.loc[k]) || v.loc[k] != v.loc[0] + k) return false;
         return true;
     }
 
@@ -343,77 +368,178 @@ class Lowerer {
     }
 
     // One block micro-op: operands a (and b) consumed, result in n_out consecutive
-    // registers (n_out = 1: any register).  The operand temporaries are released before
-    // the result is placed, so the result may reuse them (the device reads every operand
-    // component first).  Returns false, with nothing emitted, when no block can be found
-    // even after spilling; the caller then lowers the op component by component.
+    // registers (n_out = 1: any register).  The operands must be held in registers.  The
+    // operand temporaries are released before the result is placed, so the result may
+    // reuse them (the device reads every operand component first).  Returns false, with
+    // nothing emitted, when no block can be found even after spilling; the caller then
+    // lowers the op component by component.
     bool block_op(UOp op, int n_out, const Val& A, const Val* B, uint32_t x, Val& result) {
         std::vector<int> dying;
-        for (int k = 0; k < A.n; ++k) dying.push_back(A.reg[k]);
+        for (int k = 0; k < A.n; ++k) dying.push_back(A.loc[k]);
         if (B)
-            for (int k = 0; k < B->n; ++k) dying.push_back(B->reg[k]);
+            for (int k = 0; k < B->n; ++k) dying.push_back(B->loc[k]);
+        for (int l : dying)
+            if (!in_reg(l)) return false;
+        // room for the block is made with LDS slots only: where that is not enough, the
+        // component-wise form (which may spill to device memory) is the cheaper code
         int d = find_block(n_out, dying);
         while (d < 0) {
-            if (!spill_one()) return false;
+            if (!spill_one(kMaxSpillSlots)) return false;
             d = find_block(n_out, dying);
         }
         for (int r : dying) release(r);
         for (int k = 0; k < n_out; ++k) free_[d + k] = false;
         if ((uint32_t)(d + n_out) > out_.max_reg) out_.max_reg = d + n_out;
-        out_.code.push_back(make_uinstr(op, (uint32_t)d, (uint32_t)A.reg[0], B ? (uint32_t)B->reg[0] : 0u, x));
+        out_.code.push_back(make_uinstr(op, (uint32_t)d, (uint32_t)A.loc[0], B ? (uint32_t)B->loc[0] : 0u, x));
         result = Val{};
         result.n = (uint8_t)n_out;
-        for (int k = 0; k < n_out; ++k) result.reg[k] = (uint8_t)(d + k);
+        for (int k = 0; k < n_out; ++k) result.loc[k] = (int16_t)(d + k);
         return true;
     }
 
-    // Emit a micro-op sequence.  `owned` lists the operand components consumed by the
-    // reference op; `outs` names the result components (virtual ids or physical
-    // pass-through registers).  Returns false when the register file is exhausted.
+    static bool param_operand(UOp op) {
+        switch (op) {
+            case U_IMM: case U_ADDK: case U_SUBK: case U_KSUB: case U_MULK: case U_DIVK: case U_KDIV:
+            case U_SINK: case U_COSK: case U_EXPK: case U_LNK: case U_SQRTK: case U_NEGK: case U_SQRK:
+                return true;
+            default:
+                return false;
+        }
+    }
+
+    // Emit a micro-op sequence with its own register allocation.  `owned` lists the
+    // operand components consumed by the reference op (their locations); `outs` names the
+    // result components (virtual ids or operand locations passed through).  Sources are
+    // operand locations (P) or earlier results of the sequence (V).  Before each micro-op
+    // its sources are brought into registers (a FILL from a spill slot, an IMM for a
+    // pending constant); a register is found by taking a free one, else by spilling the
+    // oldest stacked value, else by spilling the live value of this op whose next read is
+    // furthest away (never a source of the current micro-op).  Sources are released after
+    // their last read, results that are still live may end in spill slots.  Returns false
+    // only when the register file cannot hold one micro-op's operands.
     bool emit(const std::vector<MI>& seq, const std::vector<int>& owned, const std::vector<Src>& outs,
               Val& result) {
         const int n = (int)seq.size();
-        std::vector<int> vlast(n, -1), vphys(n, -1);
-        std::array<int, kRegs> plast;
-        plast.fill(-2);  // -2: not owned
-        for (int r : owned)
-            if (is_temp(r)) plast[r] = -1;
+        // value table: operand components (by their location at the start) and the virtuals
+        struct Ent {
+            int loc;   // current location (register, slot or pending constant); -1 dead or unborn
+            int last;  // last read (kInf for a result)
+        };
+        std::vector<Ent> ent;
+        std::vector<int> opd_orig;  // entry k < opd_orig.size(): operand with start location opd_orig[k]
+        auto opd_entry = [&](int orig) {
+            for (size_t k = 0; k < opd_orig.size(); ++k)
+                if (opd_orig[k] == orig) return (int)k;
+            return -1;
+        };
+        for (int l : owned)
+            if (!in_reg(l) || is_temp(l)) {
+                if (opd_entry(l) >= 0) continue;
+                opd_orig.push_back(l);
+                ent.push_back(Ent{l, -1});
+            }
+        const int nopd = (int)opd_orig.size();
+        for (int i = 0; i < n; ++i) ent.push_back(Ent{-1, -1});
+        // entry of a source (-1: a feature register, read in place)
+        auto src_entry = [&](const Src& s) -> int {
+            if (s.virt) return nopd + s.id;
+            if (in_reg(s.id) && !is_temp(s.id)) return -1;
+            const int e = opd_entry(s.id);
+            return e;  // a source must be an owned operand component (or a feature)
+        };
         for (int i = 0; i < n; ++i)
             for (int k = 0; k < seq[i].ns; ++k) {
-                const Src& s = seq[i].s[k];
-                if (s.virt) vlast[s.id] = i;
-                else if (plast[s.id] != -2) plast[s.id] = i;
+                const int e = src_entry(seq[i].s[k]);
+                if (e >= 0) ent[e].last = i;
             }
         for (const Src& o : outs) {
-            if (o.virt) vlast[o.id] = kInf;
-            else if (plast[o.id] != -2) plast[o.id] = kInf;
+            const int e = src_entry(o);
+            if (e >= 0) ent[e].last = kInf;
         }
-        // owned registers never read and not passed through die immediately
-        for (int r = 0; r < kRegs; ++r)
-            if (plast[r] == -1) { free_[r] = true; plast[r] = -2; }
+        auto kill = [&](int e) {
+            release(ent[e].loc);
+            ent[e].loc = -1;
+        };
+        // owned components never read and not passed through die immediately
+        for (int e = 0; e < nopd; ++e)
+            if (ent[e].last == -1) kill(e);
+        auto next_read = [&](int e, int i) -> int {
+            if (ent[e].last == kInf) {
+                for (int j = i + 1; j < n; ++j)
+                    for (int k = 0; k < seq[j].ns; ++k)
+                        if (src_entry(seq[j].s[k]) == e) return j;
+                return kInf;
+            }
+            for (int j = i + 1; j <= ent[e].last && j < n; ++j)
+                for (int k = 0; k < seq[j].ns; ++k)
+                    if (src_entry(seq[j].s[k]) == e) return j;
+            return kInf;
+        };
+        // a register for micro-op i, never one in `pinned`
+        auto ealloc = [&](int i, RegSet pinned) -> int {
+            for (;;) {
+                for (int r = F_; r < kRegs; ++r)
+                    if (free_[r]) {
+                        free_[r] = false;
+                        if ((uint32_t)r + 1 > out_.max_reg) out_.max_reg = r + 1;
+                        return r;
+                    }
+                if (spill_one()) continue;
+                int victim = -1, far = -1;
+                for (int e = 0; e < (int)ent.size(); ++e) {
+                    const int l = ent[e].loc;
+                    if (!is_temp(l) || (pinned >> l) & 1u) continue;
+                    const int nr = next_read(e, i - 1);
+                    if (nr > far) {
+                        far = nr;
+                        victim = e;
+                    }
+                }
+                if (victim < 0) return -1;
+                const int l = spill_reg(ent[victim].loc);
+                if (l < 0) return -1;
+                ent[victim].loc = l;
+            }
+        };
         for (int i = 0; i < n; ++i) {
-            for (int r = 0; r < kRegs; ++r)
-                if (plast[r] == i) { free_[r] = true; plast[r] = -2; }
-            for (int v = 0; v < i; ++v)
-                if (vlast[v] == i && vphys[v] >= 0) free_[vphys[v]] = true;
-            int d = alloc();
-            if (d < 0) return false;
-            vphys[i] = d;
             const MI& m = seq[i];
-            auto reg = [&](int k) -> uint32_t {
-                if (k >= m.ns) return 0;
-                const Src& s = m.s[k];
-                return (uint32_t)(s.virt ? vphys[s.id] : s.id);
-            };
-            const uint32_t x = m.op == U_IMM ? m.extra : (reg(2) | reg(3) << 5 | m.extra << 10);
-            out_.code.push_back(make_uinstr(m.op, (uint32_t)d, reg(0), reg(1), x));
+            int se[4];
+            RegSet pinned = 0;
+            for (int k = 0; k < m.ns; ++k) {
+                se[k] = src_entry(m.s[k]);
+                if (se[k] >= 0 && in_reg(ent[se[k]].loc)) pinned |= 1u << ent[se[k]].loc;
+            }
+            for (int k = 0; k < m.ns; ++k) {
+                if (se[k] < 0 || in_reg(ent[se[k]].loc)) continue;
+                const int r = ealloc(i, pinned);
+                if (r < 0) return false;
+                to_reg(ent[se[k]].loc, r);
+                ent[se[k]].loc = r;
+                pinned |= 1u << r;
+            }
+            uint32_t rg[4] = {0, 0, 0, 0};
+            for (int k = 0; k < m.ns; ++k) rg[k] = (uint32_t)(se[k] >= 0 ? ent[se[k]].loc : m.s[k].id);
+            RegSet live_src = 0;
+            for (int k = 0; k < m.ns; ++k) {
+                if (se[k] < 0) continue;
+                if (ent[se[k]].last == i) {
+                    if (ent[se[k]].loc >= 0) kill(se[k]);
+                } else {
+                    live_src |= 1u << ent[se[k]].loc;
+                }
+            }
+            const int d = ealloc(i, live_src);
+            if (d < 0) return false;
+            ent[nopd + i].loc = d;
+            const uint32_t x = param_operand(m.op) ? m.extra : (rg[2] | rg[3] << 5 | m.extra << 10);
+            out_.code.push_back(make_uinstr(m.op, (uint32_t)d, rg[0], rg[1], x));
+            if (ent[nopd + i].last == -1) kill(nopd + i);  // no reader and no output role
         }
-        // virtual values with no reads and no output role (none are generated) are freed
-        for (int v = 0; v < n; ++v)
-            if (vlast[v] == -1 && vphys[v] >= 0) free_[vphys[v]] = true;
         result.n = (uint8_t)outs.size();
-        for (size_t k = 0; k < outs.size(); ++k)
-            result.reg[k] = (uint8_t)(outs[k].virt ? vphys[outs[k].id] : outs[k].id);
+        for (size_t k = 0; k < outs.size(); ++k) {
+            const int e = src_entry(outs[k]);
+            result.loc[k] = (int16_t)(e >= 0 ? ent[e].loc : outs[k].id);
+        }
         return true;
     }
 
@@ -432,7 +558,7 @@ class Lowerer {
     static std::vector<int> regs_of(std::initializer_list<const Val*> vs) {
         std::vector<int> r;
         for (const Val* v : vs)
-            for (int k = 0; k < v->n; ++k) r.push_back(v->reg[k]);
+            for (int k = 0; k < v->n; ++k) r.push_back(v->loc[k]);
         return r;
     }
 
@@ -441,7 +567,7 @@ class Lowerer {
         if (idx + n > F_) return PROG_FEATURE;
         Val v;
         v.n = (uint8_t)n;
-        for (int k = 0; k < n; ++k) v.reg[k] = (uint8_t)(idx + k);
+        for (int k = 0; k < n; ++k) v.loc[k] = (int16_t)(idx + k);
         return push(vt, v) ? PROG_OK : PROG_STACK;
     }
 
@@ -485,7 +611,7 @@ class Lowerer {
         std::vector<MI> seq;
         std::vector<Src> outs;
         for (int k = 0; k < a.n; ++k) {
-            seq.push_back(mi(op, {P(a.reg[k]), P(b.reg[k])}));
+            seq.push_back(mi(op, {P(a.loc[k]), P(b.loc[k])}));
             outs.push_back(V(k));
         }
         Val r;
@@ -502,20 +628,20 @@ class Lowerer {
             Val r;
             if (block_op(bop, v.n, v, nullptr, (uint32_t)s.konst, r)) return push(vt, r) ? PROG_OK : PROG_STACK;
         }
-        if (s.konst >= 0 && !materialize(s)) return PROG_REGS;
-        if ((vt == VtMat2 || vt == VtMat3) && contig(v) && !s.spilled) {
+        if (s.konst >= 0) materialize(s);
+        if ((vt == VtMat2 || vt == VtMat3) && contig(v) && in_reg(s.loc[0])) {
             Val r;
             if (block_op(vt == VtMat2 ? U_SCALEV4 : U_SCALEM3, v.n, v, &s, 0, r))
                 return push(vt, r) ? PROG_OK : PROG_STACK;
         }
-        if (vt == VtVec3 && contig(v) && !s.spilled) {
+        if (vt == VtVec3 && contig(v) && in_reg(s.loc[0])) {
             Val r;
             if (block_op(U_SCALEV3, 3, v, &s, 0, r)) return push(vt, r) ? PROG_OK : PROG_STACK;
         }
         std::vector<MI> seq;
         std::vector<Src> outs;
         for (int k = 0; k < v.n; ++k) {
-            seq.push_back(mi(U_MUL, {P(v.reg[k]), P(s.reg[0])}));
+            seq.push_back(mi(U_MUL, {P(v.loc[k]), P(s.loc[0])}));
             outs.push_back(V(k));
         }
         Val r;
@@ -541,13 +667,15 @@ class Lowerer {
                 const Val c = pop_raw(VtFloat);
                 Val none, r;
                 none.n = 0;
-                if (!block_op(k, 1, none, nullptr, (uint32_t)c.konst, r)) return PROG_REGS;
+                if (!block_op(k, 1, none, nullptr, (uint32_t)c.konst, r) &&
+                    !emit({mi(k, {}, (uint32_t)c.konst)}, {}, {V(0)}, r))
+                    return PROG_REGS;
                 return push(VtFloat, r) ? PROG_OK : PROG_STACK;
             }
         }
         Val a = pop(VtFloat);
         Val r;
-        if (!emit({mi(op, {P(a.reg[0])})}, regs_of({&a}), {V(0)}, r)) return PROG_REGS;
+        if (!emit({mi(op, {P(a.loc[0])})}, regs_of({&a}), {V(0)}, r)) return PROG_REGS;
         return push(VtFloat, r) ? PROG_OK : PROG_STACK;
     }
 
@@ -556,7 +684,7 @@ class Lowerer {
     int binary_f(UOp op) {
         if (!can_pop(VtFloat, 2)) return PROG_STACK;
         Val b = pop_raw(VtFloat), a = pop_raw(VtFloat);
-        if (a.konst >= 0 && b.konst >= 0 && !materialize(a)) return PROG_REGS;
+        if (a.konst >= 0 && b.konst >= 0) materialize(a);
         if (a.konst < 0 && b.konst < 0) {
             push_back_raw(a);
             push_back_raw(b);
@@ -573,19 +701,24 @@ class Lowerer {
             default: k = kb ? U_DIVK : U_KDIV; break;
         }
         Val r;
-        if (!block_op(k, 1, v, nullptr, x, r)) return PROG_REGS;
+        if (!block_op(k, 1, v, nullptr, x, r) && !emit({mi(k, {P(v.loc[0])}, x)}, regs_of({&v}), {V(0)}, r))
+            return PROG_REGS;
         return push(VtFloat, r) ? PROG_OK : PROG_STACK;
     }
     // Put a popped value back unchanged (same age).
     void push_back_raw(const Val& v) { stk_[sidx(VtFloat)].push_back(v); }
-    bool materialize(Val& v) {
-        if (v.konst < 0) return true;
+    // A pending Float constant into a register; when none can be found it stays a pending
+    // constant location that emit() loads where it is read.
+    void materialize(Val& v) {
+        if (v.konst < 0) return;
         const int r = alloc();
-        if (r < 0) return false;
-        out_.code.push_back(make_uinstr(U_IMM, (uint32_t)r, 0, 0, (uint32_t)v.konst));
-        v.reg[0] = (uint8_t)r;
+        if (r >= 0) {
+            out_.code.push_back(make_uinstr(U_IMM, (uint32_t)r, 0, 0, (uint32_t)v.konst));
+            v.loc[0] = (int16_t)r;
+        } else {
+            v.loc[0] = (int16_t)(kKonstLoc + v.konst);
+        }
         v.konst = -1;
-        return true;
     }
 
     // generic "consume operands, produce one value" helper
@@ -627,7 +760,7 @@ class Lowerer {
                 if (!can_pop(VtFloat, n)) return PROG_STACK;
                 Val r;
                 r.n = (uint8_t)n;
-                for (int k = n - 1; k >= 0; --k) r.reg[k] = pop(VtFloat).reg[0];
+                for (int k = n - 1; k >= 0; --k) r.loc[k] = pop(VtFloat).loc[0];
                 return push(op == MakeVec2 ? VtVec2 : VtVec3, r) ? PROG_OK : PROG_STACK;
             }
             case GetXV2: case GetYV2: case GetXV3: case GetYV3: case GetZV3: {
@@ -636,10 +769,10 @@ class Lowerer {
                 if (!can_pop(vt)) return PROG_STACK;
                 Val v = pop(vt);
                 for (int j = 0; j < v.n; ++j)
-                    if (j != k) release(v.reg[j]);
+                    if (j != k) release(v.loc[j]);
                 Val r;
                 r.n = 1;
-                r.reg[0] = v.reg[k];
+                r.loc[0] = v.loc[k];
                 return push(VtFloat, r) ? PROG_OK : PROG_STACK;
             }
             case AddV2: return cwise(VtVec2, U_ADD);
@@ -658,7 +791,7 @@ class Lowerer {
             case DotV2: {  // (a0*b0) + (a1*b1)
                 if (!can_pop(VtVec2, 2)) return PROG_STACK;
                 Val b = pop(VtVec2), a = pop(VtVec2);
-                return produce(VtFloat, {mi(U_MULADD, {P(a.reg[0]), P(b.reg[0]), P(a.reg[1]), P(b.reg[1])})},
+                return produce(VtFloat, {mi(U_MULADD, {P(a.loc[0]), P(b.loc[0]), P(a.loc[1]), P(b.loc[1])})},
                                regs_of({&a, &b}), {V(0)});
             }
             case DotV3: {  // ((a0*b0) + (a1*b1)) + (a2*b2)
@@ -669,15 +802,15 @@ class Lowerer {
                     if (block_op(U_DOT3, 1, a, &b, 0, r)) return push(VtFloat, r) ? PROG_OK : PROG_STACK;
                 }
                 return produce(VtFloat,
-                               {mi(U_MULADD, {P(a.reg[0]), P(b.reg[0]), P(a.reg[1]), P(b.reg[1])}),
-                                mi(U_ADDMUL, {V(0), P(a.reg[2]), P(b.reg[2])})},
+                               {mi(U_MULADD, {P(a.loc[0]), P(b.loc[0]), P(a.loc[1]), P(b.loc[1])}),
+                                mi(U_ADDMUL, {V(0), P(a.loc[2]), P(b.loc[2])})},
                                regs_of({&a, &b}), {V(1)});
             }
             case NormV2: {  // sqrt((v0*v0) + (v1*v1))
                 if (!can_pop(VtVec2)) return PROG_STACK;
                 Val v = pop(VtVec2);
                 return produce(VtFloat,
-                               {mi(U_MULADD, {P(v.reg[0]), P(v.reg[0]), P(v.reg[1]), P(v.reg[1])}),
+                               {mi(U_MULADD, {P(v.loc[0]), P(v.loc[0]), P(v.loc[1]), P(v.loc[1])}),
                                 mi(U_SQRT, {V(0)})},
                                regs_of({&v}), {V(1)});
             }
@@ -689,8 +822,8 @@ class Lowerer {
                     if (block_op(U_NORM3, 1, v, nullptr, 0, r)) return push(VtFloat, r) ? PROG_OK : PROG_STACK;
                 }
                 return produce(VtFloat,
-                               {mi(U_MULADD, {P(v.reg[0]), P(v.reg[0]), P(v.reg[1]), P(v.reg[1])}),
-                                mi(U_SQRTACC, {V(0), P(v.reg[2])})},
+                               {mi(U_MULADD, {P(v.loc[0]), P(v.loc[0]), P(v.loc[1]), P(v.loc[1])}),
+                                mi(U_SQRTACC, {V(0), P(v.loc[2])})},
                                regs_of({&v}), {V(1)});
             }
             case CrossV3: {
@@ -700,8 +833,8 @@ class Lowerer {
                     Val r;
                     if (block_op(U_CROSS3, 3, a, &b, 0, r)) return push(VtVec3, r) ? PROG_OK : PROG_STACK;
                 }
-                auto A = [&](int k) { return P(a.reg[k]); };
-                auto B = [&](int k) { return P(b.reg[k]); };
+                auto A = [&](int k) { return P(a.loc[k]); };
+                auto B = [&](int k) { return P(b.loc[k]); };
                 return produce(VtVec3,
                                {mi(U_MULSUB, {A(1), B(2), A(2), B(1)}), mi(U_MULSUB, {A(2), B(0), A(0), B(2)}),
                                 mi(U_MULSUB, {A(0), B(1), A(1), B(0)})},
@@ -712,7 +845,7 @@ class Lowerer {
                 Val c1 = pop(VtVec2), c0 = pop(VtVec2);
                 Val r;
                 r.n = 4;
-                r.reg[0] = c0.reg[0]; r.reg[1] = c0.reg[1]; r.reg[2] = c1.reg[0]; r.reg[3] = c1.reg[1];
+                r.loc[0] = c0.loc[0]; r.loc[1] = c0.loc[1]; r.loc[2] = c1.loc[0]; r.loc[3] = c1.loc[1];
                 return push(VtMat2, r) ? PROG_OK : PROG_STACK;
             }
             case MakeMat3: {
@@ -720,7 +853,7 @@ class Lowerer {
                 Val c2 = pop(VtVec3), c1 = pop(VtVec3), c0 = pop(VtVec3);
                 Val r;
                 r.n = 9;
-                for (int k = 0; k < 3; ++k) { r.reg[k] = c0.reg[k]; r.reg[3 + k] = c1.reg[k]; r.reg[6 + k] = c2.reg[k]; }
+                for (int k = 0; k < 3; ++k) { r.loc[k] = c0.loc[k]; r.loc[3 + k] = c1.loc[k]; r.loc[6 + k] = c2.loc[k]; }
                 return push(VtMat3, r) ? PROG_OK : PROG_STACK;
             }
             case MulM2: {  // out[r+2c] = a[r]*b[2c] + a[r+2]*b[2c+1]
@@ -733,32 +866,32 @@ class Lowerer {
                 std::vector<MI> seq;
                 for (int c = 0; c < 2; ++c)
                     for (int r = 0; r < 2; ++r)
-                        seq.push_back(mi(U_MULADD, {P(a.reg[r]), P(b.reg[2 * c]), P(a.reg[r + 2]), P(b.reg[2 * c + 1])}));
+                        seq.push_back(mi(U_MULADD, {P(a.loc[r]), P(b.loc[2 * c]), P(a.loc[r + 2]), P(b.loc[2 * c + 1])}));
                 return produce(VtMat2, seq, regs_of({&a, &b}), {V(0), V(1), V(2), V(3)});
             }
             case MulM2V2: {
                 if (!can_pop(VtMat2) || !can_pop(VtVec2)) return PROG_STACK;
                 Val m = pop(VtMat2), v = pop(VtVec2);
                 return produce(VtVec2,
-                               {mi(U_MULADD, {P(m.reg[0]), P(v.reg[0]), P(m.reg[2]), P(v.reg[1])}),
-                                mi(U_MULADD, {P(m.reg[1]), P(v.reg[0]), P(m.reg[3]), P(v.reg[1])})},
+                               {mi(U_MULADD, {P(m.loc[0]), P(v.loc[0]), P(m.loc[2]), P(v.loc[1])}),
+                                mi(U_MULADD, {P(m.loc[1]), P(v.loc[0]), P(m.loc[3]), P(v.loc[1])})},
                                regs_of({&m, &v}), {V(0), V(1)});
             }
             case DetM2: {
                 if (!can_pop(VtMat2)) return PROG_STACK;
                 Val m = pop(VtMat2);
-                return produce(VtFloat, {mi(U_MULSUB, {P(m.reg[0]), P(m.reg[3]), P(m.reg[1]), P(m.reg[2])})},
+                return produce(VtFloat, {mi(U_MULSUB, {P(m.loc[0]), P(m.loc[3]), P(m.loc[1]), P(m.loc[2])})},
                                regs_of({&m}), {V(0)});
             }
             case TraceM2: {
                 if (!can_pop(VtMat2)) return PROG_STACK;
                 Val m = pop(VtMat2);
-                return produce(VtFloat, {mi(U_ADD, {P(m.reg[0]), P(m.reg[3])})}, regs_of({&m}), {V(0)});
+                return produce(VtFloat, {mi(U_ADD, {P(m.loc[0]), P(m.loc[3])})}, regs_of({&m}), {V(0)});
             }
             case TransposeM2: {
                 if (!can_pop(VtMat2)) return PROG_STACK;
                 Val m = pop(VtMat2);
-                std::swap(m.reg[1], m.reg[2]);
+                std::swap(m.loc[1], m.loc[2]);
                 return push(VtMat2, m) ? PROG_OK : PROG_STACK;
             }
             case InverseM2: {
@@ -770,7 +903,7 @@ class Lowerer {
                     Val r;
                     if (block_op(U_INVM2, 4, m, nullptr, 0, r)) return push(VtMat2, r) ? PROG_OK : PROG_STACK;
                 }
-                auto M = [&](int k) { return P(m.reg[k]); };
+                auto M = [&](int k) { return P(m.loc[k]); };
                 // det, inv_d, then blended entries (linalg/eval.rs:316-329)
                 return produce(VtMat2,
                                {mi(U_MULSUB, {M(0), M(3), M(1), M(2)}),          // v0 det
@@ -789,8 +922,8 @@ class Lowerer {
                 for (int c = 0; c < 3; ++c)
                     for (int r = 0; r < 3; ++r) {
                         int t = (int)seq.size();
-                        seq.push_back(mi(U_MULADD, {P(a.reg[r]), P(b.reg[3 * c]), P(a.reg[r + 3]), P(b.reg[3 * c + 1])}));
-                        seq.push_back(mi(U_ADDMUL, {V(t), P(a.reg[r + 6]), P(b.reg[3 * c + 2])}));
+                        seq.push_back(mi(U_MULADD, {P(a.loc[r]), P(b.loc[3 * c]), P(a.loc[r + 3]), P(b.loc[3 * c + 1])}));
+                        seq.push_back(mi(U_ADDMUL, {V(t), P(a.loc[r + 6]), P(b.loc[3 * c + 2])}));
                         outs.push_back(V(t + 1));
                     }
                 return produce(VtMat3, seq, regs_of({&a, &b}), outs);
@@ -806,8 +939,8 @@ class Lowerer {
                 std::vector<Src> outs;
                 for (int r = 0; r < 3; ++r) {
                     int t = (int)seq.size();
-                    seq.push_back(mi(U_MULADD, {P(m.reg[r]), P(v.reg[0]), P(m.reg[r + 3]), P(v.reg[1])}));
-                    seq.push_back(mi(U_ADDMUL, {V(t), P(m.reg[r + 6]), P(v.reg[2])}));
+                    seq.push_back(mi(U_MULADD, {P(m.loc[r]), P(v.loc[0]), P(m.loc[r + 3]), P(v.loc[1])}));
+                    seq.push_back(mi(U_ADDMUL, {V(t), P(m.loc[r + 6]), P(v.loc[2])}));
                     outs.push_back(V(t + 1));
                 }
                 return produce(VtVec3, seq, regs_of({&m, &v}), outs);
@@ -815,7 +948,7 @@ class Lowerer {
             case DetM3: {
                 if (!can_pop(VtMat3)) return PROG_STACK;
                 Val m = pop(VtMat3);
-                auto M = [&](int k) { return P(m.reg[k]); };
+                auto M = [&](int k) { return P(m.loc[k]); };
                 return produce(VtFloat,
                                {mi(U_MULSUB, {M(4), M(8), M(5), M(7)}),  // t1
                                 mi(U_MULSUB, {M(1), M(8), M(2), M(7)}),  // t2
@@ -827,14 +960,14 @@ class Lowerer {
             case TraceM3: {
                 if (!can_pop(VtMat3)) return PROG_STACK;
                 Val m = pop(VtMat3);
-                return produce(VtFloat, {mi(U_ADD3, {P(m.reg[0]), P(m.reg[4]), P(m.reg[8])})}, regs_of({&m}), {V(0)});
+                return produce(VtFloat, {mi(U_ADD3, {P(m.loc[0]), P(m.loc[4]), P(m.loc[8])})}, regs_of({&m}), {V(0)});
             }
             case TransposeM3: {
                 if (!can_pop(VtMat3)) return PROG_STACK;
                 Val m = pop(VtMat3);
                 Val t = m;
                 static const int perm[9] = {0, 3, 6, 1, 4, 7, 2, 5, 8};
-                for (int k = 0; k < 9; ++k) t.reg[k] = m.reg[perm[k]];
+                for (int k = 0; k < 9; ++k) t.loc[k] = m.loc[perm[k]];
                 return push(VtMat3, t) ? PROG_OK : PROG_STACK;
             }
             case InverseM3: {
@@ -842,7 +975,7 @@ class Lowerer {
                 if (!can_pop(VtMat3)) return PROG_STACK;
                 if (dual_) return PROG_OK;
                 Val m = pop(VtMat3);
-                auto M = [&](int k) { return P(m.reg[k]); };
+                auto M = [&](int k) { return P(m.loc[k]); };
                 // det (same sequence as DetM3), inv_d, adjugate * inv_d blended (linalg/eval.rs:476-506)
                 std::vector<MI> seq = {
                     mi(U_MULSUB, {M(4), M(8), M(5), M(7)}),  // 0 t1 = adj0 numerator

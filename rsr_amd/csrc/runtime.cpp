@@ -252,6 +252,7 @@ extern "C" int rsr_context_destroy(rsr_context* ctx) {
     ctx->scratch_gstate.release();
     ctx->scratch_out.release();
     ctx->scratch_out2.release();
+    ctx->scratch_gspill.release();
     ctx->dev_counters.release();
     rsr::trim_pool(ctx);
     if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
@@ -635,6 +636,7 @@ int rsr::get_lowering(rsr_programs* p, int F, bool dual, Lowering** out) {
     rsr_context* ctx = p->ctx;
     auto L = std::make_unique<Lowering>();
     L->status.resize(p->n);
+    L->big.assign(p->n, 0);
     L->nparams.resize(p->n);
     L->differentiable.resize(p->n);
     std::vector<UInstr> code;
@@ -674,7 +676,13 @@ int rsr::get_lowering(rsr_programs* p, int F, bool dual, Lowering** out) {
             cost[i] = (uint32_t)lw.code.size();
             L->total_uops += lw.code.size();
             L->max_params = std::max<int>(L->max_params, L->nparams[i]);
-            L->spill_slots = std::max<int>(L->spill_slots, (int)lw.spill_slots);
+            L->spill_slots = std::max<int>(L->spill_slots, std::min<int>((int)lw.spill_slots, kMaxSpillSlots));
+            if ((int)lw.spill_slots > kMaxSpillSlots) {
+                descs[i].out_reg |= kBigProg;
+                L->big[i] = 1;
+                L->n_big++;
+                L->gslots = std::max<int>(L->gslots, (int)lw.spill_slots - kMaxSpillSlots);
+            }
         } else {
             descs[i].code_len = 0;
             descs[i].out_reg = 0;
@@ -682,10 +690,14 @@ int rsr::get_lowering(rsr_programs* p, int F, bool dual, Lowering** out) {
         }
     }
     code.push_back(UInstr{0u, 0u, 0u, 0u});  // prefetch padding
-    // scheduling order: most expensive first, so groups are balanced and the tail is short
+    // scheduling order: most expensive first, so groups are balanced and the tail is short;
+    // the big programs last (their own launches take the last n_big positions)
     L->order.resize(p->n);
     std::iota(L->order.begin(), L->order.end(), 0);
-    std::stable_sort(L->order.begin(), L->order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::stable_sort(L->order.begin(), L->order.end(), [&](int a, int b) {
+        if (L->big[a] != L->big[b]) return L->big[a] < L->big[b];
+        return cost[a] > cost[b];
+    });
     L->active_host = active;
     L->desc = descs;
     int rc;
@@ -709,6 +721,8 @@ DevPrograms rsr::dev_view(const rsr_programs* p, const Lowering* L) {
     v.order = (const int32_t*)L->dorder.ptr;
     v.n_progs = p->n;
     v.spill_slots = L->spill_slots;
+    v.n_big = L->n_big;
+    v.gslots = L->gslots;
     return v;
 }
 
@@ -766,7 +780,16 @@ extern "C" int rsr_eval_points(rsr_context* ctx, rsr_programs* p, int first, int
         if (L->status[i] != PROG_OK) return set_error(RSR_E_PROGRAM, "program %d rejected (%d)", i, L->status[i]);
     const size_t n = (size_t)count * ds->count8;
     if ((rc = ensure_scratch(ctx, ctx->scratch_out, n * sizeof(float)))) return rc;
-    HIP_TRY(launch_eval_points(dev_view(p, L), dev_view(ds), first, count, (float*)ctx->scratch_out.ptr, ctx->stream),
+    bool any_big = false;
+    for (int i = first; i < first + count; ++i) any_big = any_big || L->big[i];
+    BigSpill big;
+    if (any_big) {
+        big.floats = big_points_floats(L->gslots, 1);
+        if ((rc = ensure_scratch(ctx, ctx->scratch_gspill, big.floats * sizeof(float)))) return rc;
+        big.ptr = (float*)ctx->scratch_gspill.ptr;
+    }
+    HIP_TRY(launch_eval_points(dev_view(p, L), dev_view(ds), first, count, any_big, (float*)ctx->scratch_out.ptr,
+                               ctx->stream, big),
             "eval_points launch");
     HIP_TRY(hipMemcpyAsync(out, ctx->scratch_out.ptr, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
             "points download");
@@ -799,7 +822,14 @@ extern "C" int rsr_eval_points_dual(rsr_context* ctx, rsr_programs* p, int prog,
     const size_t n = (size_t)ds->count8;
     if ((rc = ensure_scratch(ctx, ctx->scratch_out, 2 * n * sizeof(float)))) return rc;
     float* dv = (float*)ctx->scratch_out.ptr;
-    HIP_TRY(launch_eval_points_dual(dev_view(p, L), dev_view(ds), prog, active_param, dv, dv + n, ctx->stream),
+    BigSpill big;
+    if (L->big[prog]) {
+        big.floats = big_points_floats(L->gslots, 2);
+        if ((rc = ensure_scratch(ctx, ctx->scratch_gspill, big.floats * sizeof(float)))) return rc;
+        big.ptr = (float*)ctx->scratch_gspill.ptr;
+    }
+    HIP_TRY(launch_eval_points_dual(dev_view(p, L), dev_view(ds), prog, active_param, L->big[prog] != 0, dv, dv + n,
+                                    ctx->stream, big),
             "eval_points_dual launch");
     HIP_TRY(hipMemcpyAsync(val_out, dv, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "download");
     HIP_TRY(hipMemcpyAsync(grad_out, dv + n, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "download");

@@ -38,9 +38,12 @@ struct Lowering {
     std::vector<int32_t> order;           // cost-sorted scheduling order
     std::vector<uint8_t> active_host;
     std::vector<ProgDesc> desc;           // host copy of the program descriptors
+    std::vector<uint8_t> big;             // spills beyond the LDS slots (kBigProg)
     int n_invalid = 0;
     int max_params = 0;
-    int spill_slots = 0;
+    int spill_slots = 0;                  // LDS spill slots (max over the batch, <= kMaxSpillSlots)
+    int n_big = 0;                        // big programs; the scheduling order puts them last
+    int gslots = 0;                       // device-memory spill slots of the big programs (max)
     uint64_t total_uops = 0;
     DevBuf dcode, ddesc, dorder, dactive, dnparams;
 };
@@ -79,6 +82,7 @@ struct rsr_context {
     int summation = 0;  // 0: pairwise + Kahan (fast), 1: reference order (bit-identical to the CPU evaluator)
     bool graphs = false;  // rsr_eval_mse replays a captured graph per (programs, dataset, settings)
     rsr::DevBuf scratch_partials, scratch_gpartials, scratch_state, scratch_gstate, scratch_out, scratch_out2, scratch_act;
+    rsr::DevBuf scratch_gspill;  // device spill areas of the big-program launches
     size_t arena_bytes = (size_t)4 << 30;  // summation partials / terms per launch round
     // per-job tick counters: device array and its pinned host mirror
     rsr::DevBuf dev_counters;
