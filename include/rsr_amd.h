@@ -171,6 +171,20 @@ RSR_API int rsr_programs_destroy(rsr_programs* progs);
  * RSR_E_PROGRAM after evaluating the others. */
 RSR_API int rsr_eval_mse(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, float* mse_out);
 
+/* compute_mse with no device allocation (no reference counterpart: the reference is
+ * in-process CPU code).  For callers that cannot allocate -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  The same evaluation as
+ * rsr_eval_mse (same summation and tuning, hence the same bits), enqueued on the context's
+ * stream without waiting: mse_out_device is a device pointer to n floats (NaN for programs
+ * the lowering rejected), complete after rsr_context_synchronize.  Its scratch (per-tile
+ * partials, or per-point terms in reference summation, and the spill areas of big programs)
+ * is one 64 MiB device array per GPU; larger evaluations fail with RSR_E_INVALID_ARG.  The
+ * batch must already be lowered for the dataset's feature count (rsr_programs_status or
+ * any evaluation), else RSR_E_INVALID_ARG.  Calls that overlap on one GPU race on the
+ * array: calls through one context are serialised; contexts sharing a GPU must not
+ * overlap these calls. */
+RSR_API int rsr_eval_mse_noalloc(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, float* mse_out_device);
+
 /* eval_simd over every 8-point batch (evaluator.rs:20-150): per-point values of
  * programs [first, first+count), out[(i-first) * num_batches*8 + point],
  * padding lanes included. */

@@ -21,6 +21,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librsr_amd.so")
 
 RSR_OK = 0
+RSR_E_INVALID_ARG = -1
 RSR_E_PROGRAM = -5
 
 _lib = None
@@ -302,6 +303,15 @@ def compute_mse(ctx: Context, progs: Programs, ds: Dataset, allow_rejected: bool
     out = np.zeros(max(progs.n, 1), np.float32)
     _check(lib().rsr_eval_mse(ctx.h, progs.h, ds.h, _ptr(out, _f32p)), (RSR_E_PROGRAM,) if allow_rejected else ())
     return out[: progs.n]
+
+
+def eval_mse_noalloc(ctx: Context, progs: Programs, ds: Dataset, mse_out_device: int, allow_rejected: bool = False):
+    """rsr_eval_mse_noalloc: compute_mse into a device buffer (address mse_out_device, n floats)
+    with no device allocation; asynchronous on the context's stream (ctx.synchronize())."""
+    fn = lib().rsr_eval_mse_noalloc
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    _check(fn(ctx.h, progs.h, ds.h, C.c_void_p(mse_out_device)), (RSR_E_PROGRAM,) if allow_rejected else ())
 
 
 def eval_points(ctx: Context, progs: Programs, ds: Dataset, first: int = 0, count: Optional[int] = None) -> np.ndarray:
