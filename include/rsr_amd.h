@@ -91,14 +91,14 @@ RSR_API int rsr_context_profile_read(rsr_context* ctx, double* kernel_ms, int64_
 /* Wait for all work queued on the context's stream. */
 RSR_API int rsr_context_synchronize(rsr_context* ctx);
 /* Summation of the per-point squared errors and gradient terms:
- *   RSR_SUM_KAHAN (default): per-tile pairwise sums combined with Kahan-compensated
- *     FP32: close to the exact sum (about 1e-6 relative to FP64), so it differs from the
- *     reference's plain f32 sums by their own rounding error, which grows with the
- *     number of points (about 1e-5 relative at 10^5 points);
- *   RSR_SUM_REFERENCE: the reference's order -- 8 lane accumulators fed point by point
- *     (evaluator.rs:159-170) and the f32x8 horizontal add -- bit-identical to the CPU
- *     evaluator.  MSE keeps the per-point terms in a device arena and folds them in
- *     order; gradients use one wave per (program, tangent pass). */
+ *   RSR_SUM_REFERENCE (default): the reference's order -- 8 lane accumulators fed point
+ *     by point (evaluator.rs:159-170) and the f32x8 horizontal add -- bit-identical to the
+ *     CPU evaluator.  MSE keeps the per-point terms in a device arena and folds them in
+ *     order; gradients use one wave per (program, tangent pass);
+ *   RSR_SUM_KAHAN: per-tile pairwise sums combined with Kahan-compensated FP32: close to
+ *     the exact sum (about 1e-6 relative to FP64), so it differs from the reference's
+ *     plain f32 sums by their own rounding error, which grows with the number of points
+ *     (up to 4e-5 relative at 10^5 points), and it is a few percent faster. */
 #define RSR_SUM_KAHAN 0
 #define RSR_SUM_REFERENCE 1
 RSR_API int rsr_context_set_summation(rsr_context* ctx, int mode);

@@ -151,7 +151,7 @@ class Context:
         return ms.value, n.value
 
     def set_summation(self, reference_order: bool):
-        """False: pairwise + Kahan (default); True: the reference's 8-lane order (bit-identical)."""
+        """True (default): the reference's 8-lane order (bit-identical); False: pairwise + Kahan."""
         _check(lib().rsr_context_set_summation(self.h, 1 if reference_order else 0))
 
     def set_graphs(self, enable: bool):

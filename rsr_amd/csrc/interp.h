@@ -151,12 +151,11 @@ __device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __re
                 fget<P(), PPL>(R) = t;
             }
         };
-        auto scalar_result = [&]() {
-            sfor<PPL>([&](auto P) {
-                A1[P()] = __builtin_nondeterministic_value(A1[P()]);
-                A2[P()] = __builtin_nondeterministic_value(A2[P()]);
-            });
-        };
+        // A one-component result leaves components 1 and 2 of the accumulator as they were:
+        // no consumer reads them, and keeping them costs no instruction (declaring them
+        // undefined made the compiler clear them with moves at every join, 5% of the
+        // evaluation kernel's time on the Lorentz workload).
+        auto scalar_result = [&]() {};
         if (kAcc ? !(u.a & (kFwd | kNone)) : true) {
             if (kAcc && (u.a & kVec3)) {
                 sfor<PPL>([&](auto P) {
@@ -493,8 +492,7 @@ __device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __re
 // and one indexed-read group per operand; the switch computes the new result into A;
 // the write-back after it stores A into the file unless kNoWb.  Results with four or
 // nine components write their components 1.. to the file inside the switch.  After a
-// one-component result A1 and A2 are dead (no consumer reads them), which the cases
-// state explicitly so that they hold no registers across the scalar cases.
+// one-component result A1 and A2 are dead (no consumer reads them).
 template <int PPL>
 __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restrict__ code, uint32_t len,
                                             const float* __restrict__ params, float* __restrict__ lds,
@@ -525,12 +523,11 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
         } else if (!(u.b & kNone)) {
             sfor<PPL>([&](auto P) { B0[P()] = fget<P(), PPL>(R)[b]; });
         }
-        auto scalar_result = [&]() {
-            sfor<PPL>([&](auto P) {
-                A1[P()] = __builtin_nondeterministic_value(A1[P()]);
-                A2[P()] = __builtin_nondeterministic_value(A2[P()]);
-            });
-        };
+        // A one-component result leaves components 1 and 2 of the accumulator as they were:
+        // no consumer reads them, and keeping them costs no instruction (declaring them
+        // undefined made the compiler clear them with moves at every join, 5% of the
+        // evaluation kernel's time on the Lorentz workload).
+        auto scalar_result = [&]() {};
         if (!(u.a & (kFwd | kNone))) {
             if (u.a & kVec3) {
                 sfor<PPL>([&](auto P) {

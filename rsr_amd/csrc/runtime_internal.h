@@ -79,7 +79,7 @@ struct rsr_context {
     std::recursive_mutex rmu;
     std::mutex mu;
     rsr::EvalTuning tuning;
-    int summation = 0;  // 0: pairwise + Kahan (fast), 1: reference order (bit-identical to the CPU evaluator)
+    int summation = 1;  // 1 (default): reference order (bit-identical to the CPU evaluator); 0: pairwise + Kahan
     bool graphs = false;  // rsr_eval_mse replays a captured graph per (programs, dataset, settings)
     rsr::DevBuf scratch_partials, scratch_gpartials, scratch_state, scratch_gstate, scratch_out, scratch_out2, scratch_act;
     rsr::DevBuf scratch_gspill;  // device spill areas of the big-program launches

@@ -16,5 +16,9 @@ def pytest_configure(config):
 def gpu_ctx():
     from rsr_amd import api
     ctx = api.Context(0)
+    # The library defaults to reference-order summation (pinned by
+    # test_gpu_eval.py::test_default_summation_is_reference_order); the shared context starts
+    # in Kahan mode, the ambient mode of the tests that do not pick one themselves.
+    ctx.set_summation(False)
     yield ctx
     ctx.close()
