@@ -135,7 +135,18 @@ class Context:
         return u.value, b.value
 
     def set_tuning(self, points_per_lane: int, programs_per_item: int):
+        """Fix the evaluation tuning; (0, 0) returns to the per-workload (autotuned) tuning."""
         _check(lib().rsr_context_set_tuning(self.h, points_per_lane, programs_per_item))
+
+    def autotune(self, progs: "Programs", ds: "Dataset"):
+        """rsr_autotune: time every tuning on this batch and dataset, keep the fastest for the
+        workload's shape; returns (points_per_lane, programs_per_item)."""
+        fn = lib().rsr_autotune
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        a, b = C.c_int(), C.c_int()
+        _check(fn(self.h, progs.h, ds.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def tuning(self):
         a, b = C.c_int(), C.c_int()

@@ -1124,8 +1124,11 @@ int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int
 // fits the workgroup LDS (per step, so a request's result does not depend on the others).
 int run_mse(rsr_context* ctx, std::vector<EvalStep>& steps) {
     std::vector<EvalStep> by_ppl[3];
-    for (const auto& s : steps) {
-        int ppl = ctx->tuning.ppl;
+    for (const auto& s0 : steps) {
+        const EvalTuning t = pick_tuning(ctx, s0);
+        EvalStep s = s0;
+        s.group = t.group;
+        int ppl = t.ppl;
         while (ppl > 1 && eval_mse_lds_bytes(s.P.spill_slots, ppl) > ctx->max_lds) ppl /= 2;
         if (eval_mse_lds_bytes(s.P.spill_slots, ppl) > ctx->max_lds)
             return set_error(RSR_E_UNSUPPORTED, "spill area of %zu bytes exceeds the workgroup LDS limit",
@@ -1215,7 +1218,14 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
     if (L->n_big > 0) return RSR_E_UNSUPPORTED;  // big programs run eagerly (context spill area)
     const int summ = ctx->summation;
     const bool terms = summ == RSR_SUM_REFERENCE;
-    EvalTuning t = ctx->tuning;
+    int n_ok = 0;
+    for (int i = 0; i < p->n; ++i) n_ok += L->status[i] == PROG_OK;
+    EvalTuning t;
+    {
+        EvalStep probe = mse_step(p, L, ds, nullptr, nullptr);
+        probe.P.n_progs = n_ok;
+        t = pick_tuning(ctx, probe);
+    }
     while (t.ppl > 1 && eval_mse_lds_bytes(L->spill_slots, t.ppl) > ctx->max_lds) t.ppl /= 2;
     if (eval_mse_lds_bytes(L->spill_slots, t.ppl) > ctx->max_lds)
         return set_error(RSR_E_UNSUPPORTED, "spill area exceeds the workgroup LDS limit");
@@ -1228,6 +1238,7 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
             if (L->status[i] == PROG_OK) order.push_back(i);
         EvalStep st = mse_step(p, L, ds, nullptr, nullptr);
         st.P.n_progs = (int)order.size();
+        st.group = t.group;
         const size_t need = mse_partials_floats(st, t.ppl, terms) * sizeof(float);
         if (need > ctx->arena_bytes) return RSR_E_UNSUPPORTED;
         auto ng = std::make_unique<EvalGraph>();
@@ -1274,6 +1285,62 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
         }
     }
     if (bad) return set_error(RSR_E_PROGRAM, "%d program(s) rejected by lowering", bad);
+    return RSR_OK;
+}
+
+// Points per lane x programs per work item, each candidate timed on the batch itself
+// (best of two evaluations after a warm-up, events on the context's stream).
+int autotune(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int* ppl_out, int* group_out) {
+    static const int kPpl[] = {1, 2, 4}, kGroup[] = {4, 8, 16, 32};
+    const bool saved_auto = ctx->auto_tune;
+    const EvalTuning saved = ctx->tuning;
+    std::vector<float> out(std::max(p->n, 1));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "autotune event");
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return set_error(RSR_E_HIP, "autotune event");
+    }
+    int rc = RSR_OK;
+    float best = -1.0f;
+    EvalTuning pick = saved;
+    int n_ok = 0;
+    ctx->auto_tune = false;
+    for (int ppl : kPpl)
+        for (int group : kGroup) {
+            ctx->tuning.ppl = ppl;
+            ctx->tuning.group = group;
+            float ms = -1.0f;
+            for (int rep = 0; rep < 3 && rc == RSR_OK; ++rep) {
+                std::unique_ptr<Job> job;
+                if ((rc = make_eval_job(ctx, p, ds, nullptr, out.data(), &job))) break;
+                (void)hipEventRecord(e0, ctx->stream);
+                rc = run_job(ctx, *job);
+                if (rc == RSR_E_PROGRAM) rc = RSR_OK;  // rejected programs do not change the timing
+                (void)hipEventRecord(e1, ctx->stream);
+                (void)hipEventSynchronize(e1);
+                float t = 0.0f;
+                (void)hipEventElapsedTime(&t, e0, e1);
+                if (rep > 0 && (ms < 0.0f || t < ms)) ms = t;  // the first run warms the caches
+            }
+            if (rc) break;
+            if (best < 0.0f || ms < best) {
+                best = ms;
+                pick.ppl = ppl;
+                pick.group = group;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->auto_tune = saved_auto;
+    ctx->tuning = saved;
+    if (rc) return rc;
+    Lowering* L;
+    if ((rc = get_lowering(p, ds->num_features, false, &L))) return rc;
+    for (int i = 0; i < p->n; ++i) n_ok += L->status[i] == PROG_OK;
+    ctx->tune_cache[tuning_key(ds->num_features, ds->count8, n_ok)] = pick;
+    if (ppl_out) *ppl_out = pick.ppl;
+    if (group_out) *group_out = pick.group;
     return RSR_OK;
 }
 

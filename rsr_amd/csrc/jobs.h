@@ -106,5 +106,8 @@ int run_job(rsr_context* ctx, Job& job);
 // compute_mse through a cached captured graph (caller holds ctx->mu); RSR_E_UNSUPPORTED
 // (nothing done) when the evaluation needs more than one launch round.
 int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, float* mse_out);
+// Time every tuning candidate on this batch and dataset and cache the fastest for their
+// workload shape (caller holds ctx->mu).
+int autotune(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int* ppl_out, int* group_out);
 
 }  // namespace rsr

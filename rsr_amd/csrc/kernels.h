@@ -79,6 +79,7 @@ struct EvalStep {
     int64_t tile_begin = 0, tile_count = -1;  // -1: every tile from tile_begin
     float* kstate = nullptr;                  // step_state_floats, needed unless first && last
     bool first = true, last = true;
+    int group = 0;                            // programs per work item (0: the launch's tuning)
 };
 
 // Floats of device memory that the big-program launches of these steps need (0: none).

@@ -721,7 +721,7 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
         for (int i = 0; i < cnt; ++i) {
             const EvalStep& st = steps[base + i];
             if (st.P.n_progs <= 0) continue;
-            SegDesc d = make_seg(st, tune.ppl, tune.group > 0 ? tune.group : 8);
+            SegDesc d = make_seg(st, tune.ppl, st.group > 0 ? st.group : tune.group > 0 ? tune.group : 8);
             if (d.n_tiles <= 0) continue;
             d.terms = terms ? 1 : 0;
             d.first_wave = waves;

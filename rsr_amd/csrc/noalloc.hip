@@ -71,16 +71,23 @@ extern "C" RSR_API int rsr_eval_mse_noalloc(rsr_context* ctx, rsr_programs* p, c
                          "with this dataset first",
                          ds->num_features);
     const Lowering* L = it->second.get();
-    EvalTuning t = ctx->tuning;
-    while (t.ppl > 1 && eval_mse_lds_bytes(L->spill_slots, t.ppl) > ctx->max_lds) t.ppl /= 2;
-    if (eval_mse_lds_bytes(L->spill_slots, t.ppl) > ctx->max_lds)
-        return set_error(RSR_E_UNSUPPORTED, "spill area exceeds the workgroup LDS limit");
     const bool terms = ctx->summation == RSR_SUM_REFERENCE;
     EvalStep st;
     st.P = dev_view(p, L);
     st.D = dev_view(ds);
     st.active = (const uint8_t*)L->dactive.ptr;
     st.mse_out = mse_out_device;
+    // the tuning rsr_eval_mse would take for this batch (so the bits agree)
+    EvalTuning t;
+    {
+        EvalStep probe = st;
+        probe.P.n_progs = p->n - L->n_invalid;
+        t = pick_tuning(ctx, probe);
+    }
+    st.group = t.group;
+    while (t.ppl > 1 && eval_mse_lds_bytes(L->spill_slots, t.ppl) > ctx->max_lds) t.ppl /= 2;
+    if (eval_mse_lds_bytes(L->spill_slots, t.ppl) > ctx->max_lds)
+        return set_error(RSR_E_UNSUPPORTED, "spill area exceeds the workgroup LDS limit");
     const size_t part = mse_partials_floats(st, t.ppl, terms);
     const size_t spill = big_spill_floats(&st, 1, t.ppl, false);
     const size_t cap = kScratchBytes / sizeof(float);

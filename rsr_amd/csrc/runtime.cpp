@@ -270,12 +270,51 @@ extern "C" int rsr_context_memory(const rsr_context* ctx, uint64_t* used_bytes, 
 
 extern "C" int rsr_context_set_tuning(rsr_context* ctx, int ppl, int group) {
     if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
+    if (ppl == 0 && group == 0) {  // back to per-workload tuning
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->auto_tune = true;
+        return RSR_OK;
+    }
     if (ppl != 1 && ppl != 2 && ppl != 4) return set_error(RSR_E_INVALID_ARG, "points_per_lane must be 1, 2 or 4");
     if (group < 1 || group > 4096) return set_error(RSR_E_INVALID_ARG, "programs_per_item out of range");
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->tuning.ppl = ppl;
     ctx->tuning.group = group;
+    ctx->auto_tune = false;
     return RSR_OK;
+}
+
+static int ceil_log2(int64_t v) {
+    int k = 0;
+    while (((int64_t)1 << k) < v) ++k;
+    return k;
+}
+
+uint64_t rsr::tuning_key(int num_features, int64_t count8, int n_progs) {
+    return ((uint64_t)num_features << 16) | ((uint64_t)ceil_log2(count8) << 8) | (uint64_t)ceil_log2(n_progs);
+}
+
+// Heuristic: the most points per lane (fewest waves per point, dispatch amortised over the
+// most points) that still gives the chip about four waves per SIMD (4,096 work items),
+// eight programs per item; small evaluations take one point per lane and four programs.
+static EvalTuning heuristic_tuning(int64_t count8, int n_progs) {
+    EvalTuning t;
+    for (int ppl = 4; ppl >= 1; ppl /= 2) {
+        const int64_t tiles = (count8 + 64 * ppl - 1) / (64 * ppl);
+        if (tiles * ((n_progs + 7) / 8) >= 4096 || ppl == 1) {
+            t.ppl = ppl;
+            t.group = (ppl == 1 && tiles * ((n_progs + 7) / 8) < 4096) ? 4 : 8;
+            return t;
+        }
+    }
+    return t;
+}
+
+EvalTuning rsr::pick_tuning(const rsr_context* ctx, const EvalStep& st) {
+    if (!ctx->auto_tune) return ctx->tuning;
+    auto it = ctx->tune_cache.find(tuning_key(st.D.num_features, st.D.count8, st.P.n_progs));
+    if (it != ctx->tune_cache.end()) return it->second;
+    return heuristic_tuning(st.D.count8, st.P.n_progs);
 }
 
 extern "C" int rsr_context_set_summation(rsr_context* ctx, int mode) {
@@ -314,8 +353,8 @@ extern "C" int rsr_context_set_partials_limit(rsr_context* ctx, uint64_t bytes) 
 
 extern "C" int rsr_context_get_tuning(const rsr_context* ctx, int* ppl, int* group) {
     if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
-    if (ppl) *ppl = ctx->tuning.ppl;
-    if (group) *group = ctx->tuning.group;
+    if (ppl) *ppl = ctx->auto_tune ? 0 : ctx->tuning.ppl;
+    if (group) *group = ctx->auto_tune ? 0 : ctx->tuning.group;
     return RSR_OK;
 }
 
@@ -764,6 +803,15 @@ extern "C" int rsr_eval_mse(rsr_context* ctx, rsr_programs* p, const rsr_dataset
     int rc = make_eval_job(ctx, p, ds, nullptr, mse_out, &job);
     if (rc) return rc;
     return run_job(ctx, *job);
+}
+
+extern "C" int rsr_autotune(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int* points_per_lane,
+                            int* programs_per_item) {
+    if (!ctx || !p || !ds) return set_error(RSR_E_INVALID_ARG, "null argument");
+    if (p->n == 0) return set_error(RSR_E_INVALID_ARG, "empty program batch");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    return autotune(ctx, p, ds, points_per_lane, programs_per_item);
 }
 
 extern "C" int rsr_eval_points(rsr_context* ctx, rsr_programs* p, int first, int count, const rsr_dataset* ds,

@@ -78,7 +78,9 @@ struct rsr_context {
     uint64_t pool_bytes = 0;
     std::recursive_mutex rmu;
     std::mutex mu;
-    rsr::EvalTuning tuning;
+    rsr::EvalTuning tuning;     // the fixed tuning (auto_tune off)
+    bool auto_tune = true;      // per-workload tuning: autotuned choice, else the heuristic
+    std::map<uint64_t, rsr::EvalTuning> tune_cache;  // rsr_autotune results by workload shape
     int summation = 1;  // 1 (default): reference order (bit-identical to the CPU evaluator); 0: pairwise + Kahan
     bool graphs = false;  // rsr_eval_mse replays a captured graph per (programs, dataset, settings)
     rsr::DevBuf scratch_partials, scratch_gpartials, scratch_state, scratch_gstate, scratch_out, scratch_out2, scratch_act;
@@ -134,6 +136,12 @@ struct rsr_programs {
 };
 
 namespace rsr {
+// The evaluation tuning (points per lane, programs per work item) for an MSE step: the
+// context's fixed tuning, or in auto mode the autotuned choice for the step's workload
+// shape (feature count, point count and program count, each in powers of two), else a
+// heuristic that fills the chip with the fewest waves per point.
+EvalTuning pick_tuning(const rsr_context* ctx, const EvalStep& st);
+uint64_t tuning_key(int num_features, int64_t count8, int n_progs);
 void trim_pool(rsr_context* c);
 int ensure_scratch(rsr_context* c, DevBuf& b, size_t n);
 DevDataset dev_view(const rsr_dataset* d);

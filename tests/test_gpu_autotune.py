@@ -1,0 +1,51 @@
+"""The library's autotuner (rsr_autotune) and its per-workload tuning mode.
+
+A fresh context is in the per-workload mode (get_tuning 0, 0).  rsr_autotune returns a
+valid choice and caches it for the workload shape: later evaluations in the per-workload
+mode -- direct calls and the no-allocation entry point -- run with that tuning, which is
+visible in Kahan summation (per-tile partials depend on the tile): their bits equal an
+evaluation at that tuning fixed by hand.  Reference-order results equal the oracle under
+any tuning."""
+import numpy as np
+import pytest
+
+import oracle as O
+from rsr_amd import api
+from rsr_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+
+def test_autotune_choice_is_used_by_per_workload_mode():
+    ctx = api.Context(0)
+    try:
+        assert ctx.tuning() == (0, 0)
+        X, y, t = W.data_c1(20000, seed=61)
+        progs = W.gen_c1(512, seed=62)
+        ds = api.Dataset.new(ctx, X, y, t)
+        P = api.Programs(ctx, progs)
+        print("autotune: 512 programs x 20000 points", flush=True)
+        ppl, group = ctx.autotune(P, ds)
+        assert ppl in (1, 2, 4) and group in (4, 8, 16, 32)
+        assert ctx.tuning() == (0, 0)
+        ctx.set_summation(False)
+        auto = api.compute_mse(ctx, P, ds)
+        ctx.set_tuning(ppl, group)
+        fixed = api.compute_mse(ctx, P, ds)
+        assert np.array_equal(auto.view(np.uint32), fixed.view(np.uint32))
+        # a different fixed tuning sums other tiles: it must not be what auto mode used
+        other = (1, 4) if ppl != 1 else (4, 8)
+        ctx.set_tuning(*other)
+        diff = api.compute_mse(ctx, P, ds)
+        assert not np.array_equal(diff.view(np.uint32), auto.view(np.uint32))
+        ctx.set_tuning(0, 0)
+        ctx.set_summation(True)
+        ref = api.compute_mse(ctx, P, ds)
+        ods = O.ODataset.new(X, y, 9)
+        want = np.array([O.compute_mse(p, ods) for p in progs[:64]], f32)
+        assert np.array_equal(ref[:64].view(np.uint32), want.view(np.uint32))
+        P.close()
+        ds.close()
+    finally:
+        ctx.close()
