@@ -44,9 +44,9 @@ namespace {
 
 uint64_t prog_nodes(const rsr_programs* p, int i) { return (uint64_t)(p->code_off[i + 1] - p->code_off[i]); }
 
-// Copy the device parameters back into the host constants (unflatten, program.rs:147-192).
-int download_params(rsr_programs* p) {
-    rsr_context* ctx = p->ctx;
+// Copy the device parameters back into the host constants (unflatten, program.rs:147-192),
+// on the stream of the context that ran the fit (a server lane's, or the batch's own).
+int download_params(rsr_context* ctx, rsr_programs* p) {
     HIP_TRY(hipMemcpyAsync(p->params.data(), p->dparams.ptr, p->params.size() * sizeof(float), hipMemcpyDeviceToHost,
                            ctx->stream),
             "params download");
@@ -300,7 +300,7 @@ class LbfgsJob final : public Job {
         std::vector<float> fit(prog.size());
         HIP_TRY(hipMemcpyAsync(fit.data(), d_fit, fit.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
                 "fitness");
-        int rc = download_params(p);  // synchronises the stream
+        int rc = download_params(ctx, p);  // synchronises the stream
         if (rc) return rc;
         for (size_t k = 0; k < prog.size(); ++k) fitness_out[prog[k]] = fit[k];
         return RSR_OK;
@@ -384,7 +384,7 @@ class CmaesJob final : public Job {
                 "fit");
         HIP_TRY(hipMemcpyAsync(its.data(), d_iters, its.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
                 "iters");
-        int rc = download_params(p);  // synchronises the stream
+        int rc = download_params(ctx, p);  // synchronises the stream
         if (rc) return rc;
         for (size_t k = 0; k < prog.size(); ++k) {
             fitness_out[prog[k]] = fit[k];
@@ -454,7 +454,7 @@ class NmJob final : public Job {
                 "fit");
         HIP_TRY(hipMemcpyAsync(its.data(), d_iters, its.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
                 "iters");
-        int rc = download_params(p);  // synchronises the stream
+        int rc = download_params(ctx, p);  // synchronises the stream
         if (rc) return rc;
         for (size_t k = 0; k < prog.size(); ++k) {
             fitness_out[prog[k]] = fit[k];
@@ -528,7 +528,7 @@ class OptimizeJob final : public Job {
                 f += s;
             }
         }
-        int rc = changed ? upload_params(p) : RSR_OK;
+        int rc = changed ? upload_params(p, ctx) : RSR_OK;
         if (rc) return rc;
         rc = make_eval_job(ctx, p, ds, chosen.data(), mse_tmp.data(), &eval);
         if (rc) return rc;
@@ -1339,6 +1339,7 @@ int autotune(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int* ppl_
     if ((rc = get_lowering(p, ds->num_features, false, &L))) return rc;
     for (int i = 0; i < p->n; ++i) n_ok += L->status[i] == PROG_OK;
     ctx->tune_cache[tuning_key(ds->num_features, ds->count8, n_ok)] = pick;
+    ctx->tune_version++;
     if (ppl_out) *ppl_out = pick.ppl;
     if (group_out) *group_out = pick.group;
     return RSR_OK;

@@ -273,6 +273,7 @@ extern "C" int rsr_context_set_tuning(rsr_context* ctx, int ppl, int group) {
     if (ppl == 0 && group == 0) {  // back to per-workload tuning
         std::lock_guard<std::mutex> g(ctx->mu);
         ctx->auto_tune = true;
+        ctx->tune_version++;
         return RSR_OK;
     }
     if (ppl != 1 && ppl != 2 && ppl != 4) return set_error(RSR_E_INVALID_ARG, "points_per_lane must be 1, 2 or 4");
@@ -281,6 +282,7 @@ extern "C" int rsr_context_set_tuning(rsr_context* ctx, int ppl, int group) {
     ctx->tuning.ppl = ppl;
     ctx->tuning.group = group;
     ctx->auto_tune = false;
+    ctx->tune_version++;
     return RSR_OK;
 }
 
@@ -644,8 +646,8 @@ extern "C" int rsr_programs_get_constants(const rsr_programs* p, float* out) {
     return RSR_OK;
 }
 
-int rsr::upload_params(rsr_programs* p) {
-    rsr_context* ctx = p->ctx;
+int rsr::upload_params(rsr_programs* p, rsr_context* on) {
+    rsr_context* ctx = on ? on : p->ctx;
     HIP_TRY(hipMemcpyAsync(p->dparams.ptr, p->params.data(), p->params.size() * sizeof(float), hipMemcpyHostToDevice,
                            ctx->stream),
             "params upload");

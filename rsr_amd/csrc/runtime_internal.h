@@ -81,6 +81,7 @@ struct rsr_context {
     rsr::EvalTuning tuning;     // the fixed tuning (auto_tune off)
     bool auto_tune = true;      // per-workload tuning: autotuned choice, else the heuristic
     std::map<uint64_t, rsr::EvalTuning> tune_cache;  // rsr_autotune results by workload shape
+    uint64_t tune_version = 0;  // bumped whenever the tuning settings or the cache change
     int summation = 1;  // 1 (default): reference order (bit-identical to the CPU evaluator); 0: pairwise + Kahan
     bool graphs = false;  // rsr_eval_mse replays a captured graph per (programs, dataset, settings)
     rsr::DevBuf scratch_partials, scratch_gpartials, scratch_state, scratch_gstate, scratch_out, scratch_out2, scratch_act;
@@ -148,5 +149,6 @@ DevDataset dev_view(const rsr_dataset* d);
 DevPrograms dev_view(const rsr_programs* p, const Lowering* L);
 RefProgram ref_program(const rsr_programs* p, int i);
 int get_lowering(rsr_programs* p, int F, bool dual, Lowering** out);
-int upload_params(rsr_programs* p);
+// on: the context whose stream carries the copy (default: the batch's own)
+int upload_params(rsr_programs* p, rsr_context* on = nullptr);
 }  // namespace rsr

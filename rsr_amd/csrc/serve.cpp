@@ -1,11 +1,15 @@
 // serve.cpp -- the request loop: admission by budget, step-level continuous batching.
 //
-// One device thread owns the loop.  Clients submit requests (any thread); the loop admits
-// them in arrival order as in-flight slots and device memory free up, builds each
-// admitted request's job (jobs.h), and advances every job by one tick per iteration:
-// all jobs' evaluation steps go into shared launches, then each job's optimizer update
-// runs, then one synchronisation.  A job that finishes is completed and its waiters are
-// woken; requests that arrive meanwhile join the next tick.
+// Clients submit requests (any thread).  The server runs lanes: each lane is an internal
+// context (own stream, scratch and tick counters) with its own loop thread.  A lane admits
+// queued requests in arrival order as in-flight slots and device memory free up -- large
+// requests to the large-request lanes, small ones to the others -- builds each admitted
+// request's job (jobs.h), and advances its jobs by one tick per iteration: all their
+// evaluation steps go into shared launches, then each job's optimizer update runs, then
+// one synchronisation of the lane's stream.  Lanes tick independently, so their launches
+// overlap on the device and a small request never waits for a large request's step.  A
+// job that finishes is completed and its waiters are woken; requests that arrive
+// meanwhile join a lane's next tick.
 #include <math.h>
 
 #include <chrono>
@@ -80,26 +84,55 @@ int make_job(rsr_context* ctx, Pending& r, std::unique_ptr<Job>* out) {
 
 }  // namespace
 
-struct rsr_server {
+// One lane: an internal context (its own stream, scratch arenas, tick counters and job
+// buffers) with its own loop thread and in-flight requests.  Lanes run their ticks
+// independently, so their launches overlap on the device and a small request never waits
+// for a large request's step.
+struct Lane {
     rsr_context* ctx = nullptr;
+    bool big = false;              // takes the large requests (and small ones when idle)
+    std::vector<Active> active;    // this lane's thread only
+    std::vector<uint8_t> is_big;   // per active request: estimated work above kBigWork
+    std::thread th;
+    uint64_t seen_version = ~0ull;  // the owner's tuning settings last copied
+};
+
+// A request is large when its evaluations are estimated above this many node-evals
+// (about a millisecond of device time).
+constexpr double kBigWork = 2e9;
+
+double request_work(const rsr_request& q) {
+    const rsr_programs* p = q.programs;
+    double w = (double)p->code_off[p->n] * (double)q.dataset->num_samples;
+    switch (q.kind) {
+        case RSR_REQ_EVAL: return w;
+        case RSR_REQ_GRAD: return 4.0 * w;
+        default: return 16.0 * w;  // fits: many evaluations of (part of) the batch
+    }
+}
+
+struct rsr_server {
+    rsr_context* ctx = nullptr;  // the owner: programs, datasets and settings
     int max_inflight = 64;
     int max_queue = 1024;
     uint64_t budget = 0;
 
-    std::mutex m;                 // guards everything below except `active` (loop thread only)
-    std::condition_variable cv;   // wakes the loop
+    std::mutex m;                 // guards everything below except each lane's `active`
+    std::condition_variable cv;   // wakes the lanes
     std::condition_variable done_cv;
     std::deque<Pending> queue;
     std::unordered_map<uint64_t, Result> results;
+    std::unordered_map<rsr_programs*, int> busy;  // batches with a request in flight
     uint64_t next_ticket = 1;
     bool stop = false;
     rsr_server_stats st{};
-
-    std::vector<Active> active;  // loop thread only
+    int inflight = 0;
+    double busy_since = 0.0;
     uint64_t inflight_bytes = 0;
-    std::thread th;
+    std::vector<std::unique_ptr<Lane>> lanes;
 
-    void complete(Active& a, int status, double t) {
+    void complete(Lane& L, size_t i, int status, double t) {
+        Active& a = L.active[i];
         std::lock_guard<std::mutex> g(m);
         Result& r = results[a.ticket];
         r.status = status;
@@ -113,9 +146,11 @@ struct rsr_server {
         else
             st.failed++;
         if (a.job) st.node_evals += a.job->node_evals;
-        st.inflight = active.size() > 0 ? active.size() - 1 : 0;
+        if (--busy[a.programs] == 0) busy.erase(a.programs);
         inflight_bytes -= a.bytes;
         st.inflight_bytes = inflight_bytes;
+        if (--inflight == 0) st.busy_seconds += t - busy_since;
+        st.inflight = (uint64_t)inflight;
     }
 
     void fail_pending(Pending& p, int status) {
@@ -128,54 +163,77 @@ struct rsr_server {
         st.failed++;
     }
 
-    // Admit queued requests in arrival order (skipping ones whose program batch is busy).
-    void admit() {
+    // Whether lane L takes request q now: large requests go to the big lanes, small ones to
+    // the small lanes or to a big lane with nothing large in flight.
+    bool takes(const Lane& L, bool big_req) const {
+        if (big_req) return L.big;
+        if (!L.big) return true;
+        for (uint8_t b : L.is_big)
+            if (b) return false;
+        return true;
+    }
+
+    // Admit queued requests into lane L in arrival order (skipping ones whose program batch
+    // is busy or that belong to another lane).
+    void admit(Lane& L) {
+        const int lane_cap = std::max(1, max_inflight / (int)lanes.size());
         for (;;) {
             Pending p;
+            bool big_req = false;
             {
                 std::lock_guard<std::mutex> g(m);
-                if ((int)active.size() >= max_inflight) return;
-                std::unordered_set<rsr_programs*> busy;
-                for (const auto& a : active) busy.insert(a.programs);
+                if (inflight >= max_inflight || (int)L.active.size() >= lane_cap) return;
                 auto it = queue.begin();
                 std::unordered_set<rsr_programs*> seen;
                 for (; it != queue.end(); ++it) {
                     rsr_programs* pr = it->req.programs;
-                    if (!busy.count(pr) && !seen.count(pr)) break;
+                    const bool blocked = busy.count(pr) || seen.count(pr);
                     seen.insert(pr);  // keep per-batch order: later requests on it wait too
+                    if (blocked) continue;
+                    const bool b = request_work(it->req) >= kBigWork;
+                    if (takes(L, b)) {
+                        big_req = b;
+                        break;
+                    }
                 }
                 if (it == queue.end()) return;
                 p = std::move(*it);
                 queue.erase(it);
                 st.queued = queue.size();
+                busy[p.req.programs]++;  // claimed while the job is built
             }
             std::unique_ptr<Job> job;
             int rc;
             {
-                std::lock_guard<std::mutex> g(ctx->mu);
+                std::lock_guard<std::mutex> g(ctx->mu);  // lowerings are the owner's
                 (void)hipSetDevice(ctx->device);
-                rc = make_job(ctx, p, &job);
+                rc = make_job(L.ctx, p, &job);
             }
             const size_t bytes = job ? job->device_bytes() : 0;
             const uint64_t limit = budget ? budget : ctx->budget;
-            if (rc == RSR_OK && !active.empty() && inflight_bytes + bytes > limit) rc = RSR_E_BUDGET;
-            if ((rc == RSR_E_BUDGET || rc == RSR_E_OOM) && !active.empty()) {
+            bool wait_for_memory = false;
+            {
+                std::lock_guard<std::mutex> g(m);
+                if (rc == RSR_OK && inflight > 0 && inflight_bytes + bytes > limit) rc = RSR_E_BUDGET;
+                if ((rc == RSR_E_BUDGET || rc == RSR_E_OOM) && inflight > 0) wait_for_memory = true;
+                if (rc != RSR_OK || wait_for_memory)
+                    if (--busy[p.req.programs] == 0) busy.erase(p.req.programs);
+            }
+            if (wait_for_memory) {
                 // wait for in-flight requests to free memory; retry from the head of the queue
-                {
-                    std::lock_guard<std::mutex> g(ctx->mu);
-                    job.reset();
-                }
+                job.reset();
                 std::lock_guard<std::mutex> g(m);
                 queue.push_front(std::move(p));
                 st.queued = queue.size();
                 return;
             }
-            if (rc == RSR_OK && bytes > limit) rc = set_error(RSR_E_BUDGET, "request needs %zu device bytes", bytes);
+            if (rc == RSR_OK && bytes > limit) {
+                rc = set_error(RSR_E_BUDGET, "request needs %zu device bytes", bytes);
+                std::lock_guard<std::mutex> g(m);
+                if (--busy[p.req.programs] == 0) busy.erase(p.req.programs);
+            }
             if (rc != RSR_OK) {
-                {
-                    std::lock_guard<std::mutex> g(ctx->mu);
-                    job.reset();
-                }
+                job.reset();
                 fail_pending(p, rc);
                 done_cv.notify_all();
                 continue;
@@ -187,62 +245,80 @@ struct rsr_server {
             a.bytes = bytes;
             a.t_submit = p.t_submit;
             a.t_admit = now_s();
-            inflight_bytes += bytes;
-            active.push_back(std::move(a));
+            L.active.push_back(std::move(a));
+            L.is_big.push_back(big_req ? 1 : 0);
             std::lock_guard<std::mutex> g(m);
-            st.inflight = active.size();
+            inflight_bytes += bytes;
+            if (inflight++ == 0) busy_since = now_s();
+            st.inflight = (uint64_t)inflight;
             st.inflight_bytes = inflight_bytes;
-            if (active.size() > st.max_inflight_seen) st.max_inflight_seen = active.size();
+            if ((uint64_t)inflight > st.max_inflight_seen) st.max_inflight_seen = (uint64_t)inflight;
         }
     }
 
-    void loop() {
+    // The owner's summation and tuning settings, copied into the lane when they changed.
+    void sync_settings(Lane& L) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        L.ctx->summation = ctx->summation;
+        if (L.seen_version != ctx->tune_version) {
+            L.ctx->auto_tune = ctx->auto_tune;
+            L.ctx->tuning = ctx->tuning;
+            L.ctx->tune_cache = ctx->tune_cache;
+            L.seen_version = ctx->tune_version;
+        }
+        L.ctx->arena_bytes = ctx->arena_bytes;
+    }
+
+    void loop(Lane& L) {
+        (void)hipSetDevice(L.ctx->device);
         for (;;) {
             {
                 std::unique_lock<std::mutex> lk(m);
-                cv.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
-                if (stop && queue.empty() && active.empty()) return;
+                cv.wait(lk, [&] { return stop || !queue.empty() || !L.active.empty(); });
+                if (stop && queue.empty() && L.active.empty()) return;
             }
-            admit();
-            if (active.empty()) {
-                // nothing admissible (every queued request waits on a busy batch): sleep briefly
+            admit(L);
+            if (L.active.empty()) {
+                // nothing this lane may admit (busy batches, another lane's class): sleep briefly
                 std::unique_lock<std::mutex> lk(m);
                 cv.wait_for(lk, std::chrono::milliseconds(1));
                 continue;
             }
-            const double t0 = now_s();
-            int rc = RSR_OK;
-            {
-                std::lock_guard<std::mutex> g(ctx->mu);
-                (void)hipSetDevice(ctx->device);
-                std::vector<Job*> jobs;
-                for (auto& a : active)
-                    if (!a.job->done) jobs.push_back(a.job.get());
-                rc = run_tick(ctx, jobs);
-                if (rc != RSR_OK) (void)hipStreamSynchronize(ctx->stream);
-                // complete finished jobs (every job fails with a failed tick)
-                for (size_t i = 0; i < active.size();) {
-                    Active& a = active[i];
-                    if (rc == RSR_OK && !a.job->done) {
-                        ++i;
-                        continue;
-                    }
-                    const int status = rc == RSR_OK ? a.job->finish() : rc;
-                    complete(a, status, now_s());
-                    a.job.reset();
-                    active.erase(active.begin() + (long)i);
+            sync_settings(L);
+            std::vector<Job*> jobs;
+            for (auto& a : L.active)
+                if (!a.job->done) jobs.push_back(a.job.get());
+            int rc = run_tick(L.ctx, jobs);
+            if (rc != RSR_OK) (void)hipStreamSynchronize(L.ctx->stream);
+            // complete finished jobs (every job fails with a failed tick)
+            bool any_done = false;
+            for (size_t i = 0; i < L.active.size();) {
+                Active& a = L.active[i];
+                if (rc == RSR_OK && !a.job->done) {
+                    ++i;
+                    continue;
                 }
+                const int status = rc == RSR_OK ? a.job->finish() : rc;
+                complete(L, i, status, now_s());
+                a.job.reset();
+                L.active.erase(L.active.begin() + (long)i);
+                L.is_big.erase(L.is_big.begin() + (long)i);
+                any_done = true;
             }
             {
                 std::lock_guard<std::mutex> g(m);
                 st.ticks++;
-                st.busy_seconds += now_s() - t0;
-                st.inflight = active.size();
             }
-            done_cv.notify_all();
+            if (any_done) {
+                done_cv.notify_all();
+                cv.notify_all();  // a freed batch or slot may let another lane admit
+            }
         }
     }
 };
+
+// Lanes of a server: two for large requests, two for small ones.
+constexpr int kLanes = 4;
 
 extern "C" int rsr_server_create(rsr_context* ctx, int max_inflight, uint64_t budget_bytes, int max_queue,
                                  rsr_server** out) {
@@ -254,11 +330,33 @@ extern "C" int rsr_server_create(rsr_context* ctx, int max_inflight, uint64_t bu
     s->max_inflight = max_inflight;
     s->max_queue = max_queue;
     s->budget = budget_bytes;
+    const int n_lanes = std::min(kLanes, max_inflight);
+    for (int i = 0; i < n_lanes; ++i) {
+        auto L = std::make_unique<Lane>();
+        int rc = rsr_context_create(ctx->device, ctx->budget, &L->ctx);
+        if (rc != RSR_OK) {
+            for (auto& l : s->lanes) (void)rsr_context_destroy(l->ctx);
+            return rc;
+        }
+        L->big = n_lanes == 1 || i < n_lanes / 2;
+        s->lanes.push_back(std::move(L));
+    }
     rsr_server* raw = s.get();
     try {
-        s->th = std::thread([raw] { raw->loop(); });
+        for (auto& L : s->lanes) {
+            Lane* lp = L.get();
+            L->th = std::thread([raw, lp] { raw->loop(*lp); });
+        }
     } catch (...) {
-        return set_error(RSR_E_UNSUPPORTED, "could not start the server thread");
+        {
+            std::lock_guard<std::mutex> g(s->m);
+            s->stop = true;
+        }
+        s->cv.notify_all();
+        for (auto& L : s->lanes)
+            if (L->th.joinable()) L->th.join();
+        for (auto& L : s->lanes) (void)rsr_context_destroy(L->ctx);
+        return set_error(RSR_E_UNSUPPORTED, "could not start the server threads");
     }
     ctx->live_servers++;
     *out = s.release();
@@ -291,7 +389,7 @@ extern "C" int rsr_server_submit(rsr_server* s, const rsr_request* req, uint64_t
         s->st.submitted++;
         s->st.queued = s->queue.size();
     }
-    s->cv.notify_one();
+    s->cv.notify_all();
     return RSR_OK;
 }
 
@@ -362,7 +460,9 @@ extern "C" int rsr_server_destroy(rsr_server* s) {
         s->stop = true;
     }
     s->cv.notify_all();
-    if (s->th.joinable()) s->th.join();
+    for (auto& L : s->lanes)
+        if (L->th.joinable()) L->th.join();
+    for (auto& L : s->lanes) (void)rsr_context_destroy(L->ctx);
     s->ctx->live_servers--;
     delete s;
     return RSR_OK;

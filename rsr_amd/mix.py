@@ -7,9 +7,14 @@ individuals).
 Jobs: Poisson arrivals (mean `rate` jobs/s); dataset sizes log-uniform in
 [400, 1,000,000] points; feature types and targets of the four workloads (scalar x;
 E, v, B Vec3; M Mat3 + v, u Vec3; A, B Mat2); populations uniform in [1,000, 16,384]
-programs drawn from per-type program pools.  A generation is one MSE request over the
-whole population followed by one fit request over its best `fit_frac` (L-BFGS, 15
-iterations, for differentiable programs; CMA-ES for programs with matrix inverses).
+programs drawn from per-type program pools.  Device-resident data is admitted under a
+200 GB HBM budget.  A generation is one MSE request over the whole population followed by
+one constant-fit request (Individual::optimize_constants, individual.rs:65-75, with the
+reference's default opt_iterations = 100, search/config.rs:50: L-BFGS 15 iterations for
+differentiable programs, CMA-ES max(100 / 16, 2) = 6 iterations for programs with vector or
+matrix constants, Nelder-Mead 100 iterations otherwise; then the 0.05 snap) over the best
+`fit_frac` of the population -- the individuals an island fits: those under its best
+fitness plus a 1% random pass (opt_prob = 0.01, config.rs:49).
 """
 from __future__ import annotations
 
@@ -87,9 +92,13 @@ def _nodes(progs, n):
     return int(sum(len(p.code) for p in progs)) * int(n)
 
 
-def run_served(ctx, jobs, generations=3, fit_frac=0.01, min_fit=8, max_inflight=64, cmaes_iters=6):
-    """Every job is a client thread submitting to one server as it arrives."""
-    srv = api.Server(ctx, max_inflight=max_inflight)
+HBM_BUDGET = 200 * 10**9  # the job mix's device admission budget
+
+
+def run_served(ctx, jobs, generations=3, fit_frac=0.01, min_fit=8, max_inflight=64, opt_iters=100, arrivals=True):
+    """Every job is a client thread submitting to one server as it arrives (arrivals=False:
+    all at once, the server at saturation)."""
+    srv = api.Server(ctx, max_inflight=max_inflight, budget_bytes=HBM_BUDGET)
     for jb in jobs:
         jb.latencies, jb.error = [], ""
     t_start = time.perf_counter()
@@ -98,18 +107,14 @@ def run_served(ctx, jobs, generations=3, fit_frac=0.01, min_fit=8, max_inflight=
         try:
             P = api.Programs(ctx, jb.programs)
             P.status(jb.ds)  # lowered ahead of the first generation
-            delay = jb.arrival - (time.perf_counter() - t_start)
+            delay = jb.arrival - (time.perf_counter() - t_start) if arrivals else 0.0
             if delay > 0:
                 time.sleep(delay)
             for g in range(generations):
                 t0 = time.perf_counter()
                 mse = np.array(srv.submit_eval(P, jb.ds).wait())
                 m = _fit_mask(mse, fit_frac, min_fit)
-                if jb.kind == 3:
-                    srv.submit_cmaes(P, jb.ds, iterations=cmaes_iters, lam=16, mu=8, seed=1000 * jb.index + g,
-                                     mask=m).wait()
-                else:
-                    srv.submit_lbfgs(P, jb.ds, iterations=15, history=6, mask=m).wait()
+                srv.submit_optimize(P, jb.ds, iterations=opt_iters, seed=1000 * jb.index + g, mask=m).wait()
                 jb.latencies.append(time.perf_counter() - t0)
             P.close()
         except Exception as e:  # reported in the summary
@@ -126,8 +131,9 @@ def run_served(ctx, jobs, generations=3, fit_frac=0.01, min_fit=8, max_inflight=
     return summarize(jobs, wall, st, generations)
 
 
-def run_sequential(ctx, jobs, generations=3, fit_frac=0.01, min_fit=8, cmaes_iters=6):
-    """The same requests one at a time through the direct entry points (no server)."""
+def run_sequential(ctx, jobs, generations=3, fit_frac=0.01, min_fit=8, opt_iters=100):
+    """The same requests one at a time through the direct entry points (no server): the
+    plain evaluator, one population per launch."""
     for jb in jobs:
         jb.latencies, jb.error = [], ""
     t_start = time.perf_counter()
@@ -138,10 +144,7 @@ def run_sequential(ctx, jobs, generations=3, fit_frac=0.01, min_fit=8, cmaes_ite
             t0 = time.perf_counter()
             mse = api.compute_mse(ctx, P, jb.ds, allow_rejected=True)
             m = _fit_mask(mse, fit_frac, min_fit)
-            if jb.kind == 3:
-                api.fit_cmaes(ctx, P, jb.ds, iterations=cmaes_iters, lam=16, mu=8, seed=1000 * jb.index + g, mask=m)
-            else:
-                api.fit_lbfgs(ctx, P, jb.ds, iterations=15, history=6, mask=m)
+            api.optimize_constants(ctx, P, jb.ds, iterations=opt_iters, seed=1000 * jb.index + g, mask=m)
             jb.latencies.append(time.perf_counter() - t0)
         P.close()
     wall = time.perf_counter() - t_start

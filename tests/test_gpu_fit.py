@@ -131,6 +131,34 @@ def test_cmaes_philox_improves_and_is_reproducible(gpu_ctx):
     assert np.median(fit1[fin] / before[fin]) < 1.0
 
 
+def test_cmaes_config_lambda32_mu16_30_iterations_bit_exact(gpu_ctx):
+    """The Mat2 config's CMA-ES setting -- lambda = 32, mu = 16, 30 iterations -- on its
+    programs, with injected normals: fitness, iterations run and fitted constants equal the
+    oracle's run_cma_es (cmaes.rs:19-192) bit for bit, in reference-order summation."""
+    X, y, t = W.data_c3(5003, seed=1003)
+    progs = W.gen_c3(16, seed=2003)
+    rng = np.random.default_rng(38)
+    iters, lam, mu = 30, 32, 16
+    normals = rng.standard_normal((iters, len(progs), lam, 32)).astype(f32)
+    gpu_ctx.set_summation(True)
+    try:
+        ds = api.Dataset.new(gpu_ctx, X, y, t)
+        ods = O.ODataset.new(X, y, 8)
+        P = api.Programs(gpu_ctx, progs)
+        assert (P.status(ds) == 0).all()
+        fit, its = api.fit_cmaes(gpu_ctx, P, ds, iters, lam, mu, normals=normals)
+        vals = P.get_constants()
+    finally:
+        gpu_ctx.set_summation(False)
+    off = 0
+    for i, p in enumerate(progs):
+        nv = sum(len(c.vals) for c in p.constants)
+        rf, rc, rits = O.run_cmaes(p, ods, iters, lam, mu, np.ascontiguousarray(normals[:, i]))
+        assert fit[i] == rf and its[i] == rits, (i, fit[i], rf, its[i], rits)
+        assert np.array_equal(vals[off:off + nv], rc[:nv]), i
+        off += nv
+
+
 def test_cmaes_normal_stream_matches_box_muller():
     # host restatement: Philox4x32-10 -> 24-bit uniforms -> Box-Muller with the device ln / cos
     import struct
