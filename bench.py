@@ -333,13 +333,20 @@ def main():
         jobs = mix.make_jobs(ctx, pools, n_jobs=args.serve_jobs, rate=args.serve_rate)
         serving = mix.run_served(ctx, jobs, generations=args.serve_gens)
         log(f"[bench] served: p50 {serving['p50_gen_latency_ms']:.1f} ms, p99 {serving['p99_gen_latency_ms']:.1f} ms")
+        log("[bench] the same requests submitted at once (the server at saturation)")
+        sat = mix.run_served(ctx, jobs, generations=args.serve_gens, arrivals=False)
+        serving["saturation"] = {k: sat[k] for k in ("wall_s", "p50_gen_latency_ms", "p99_gen_latency_ms",
+                                                     "eval_node_evals_per_s", "node_evals_per_s")}
         if not args.no_serve_sequential:
-            log("[bench] same requests one at a time (direct calls, no server)")
+            log("[bench] same requests one at a time (direct calls, no server: one population per launch)")
             seq = mix.run_sequential(ctx, jobs, generations=args.serve_gens)
             serving["sequential_direct"] = {k: seq[k] for k in ("wall_s", "p50_gen_latency_ms",
                                                                  "p99_gen_latency_ms", "eval_node_evals_per_s")}
+            # equal work: the same request stream, served at saturation vs one at a time
+            serving["saturation_over_sequential"] = sat["eval_node_evals_per_s"] / seq["eval_node_evals_per_s"]
         if plain:
             serving["served_over_plain"] = serving["eval_node_evals_per_s"] / plain["value"]
+            serving["saturation_over_plain"] = sat["eval_node_evals_per_s"] / plain["value"]
             if serving.get("node_evals_per_busy_s"):
                 serving["served_busy_over_plain"] = serving["node_evals_per_busy_s"] / plain["value"]
         for jb in jobs:

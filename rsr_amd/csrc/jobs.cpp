@@ -1061,16 +1061,28 @@ int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int
         if ((rc = ensure_scratch(ctx, ctx->scratch_gspill, big.floats * sizeof(float)))) return rc;
         big.ptr = (float*)ctx->scratch_gspill.ptr;
     }
-    if (ref && grad) {  // one wave per (program, tangent pass) walks the tiles in order
-        HIP_TRY(launch_grad_steps(steps.data(), (int)steps.size(), true, ctx->stream, ctx->next_timer(), big),
-                "reference-order gradient launch");
-        return RSR_OK;
+    if (ref && grad) {
+        // Reference order: one wave per (program, tangent pass) walks the tiles in order when
+        // the walks fill the chip (no memory traffic); the other steps write per-point terms
+        // tile-parallel and fold them in the same order (the same bits), in rounds below.
+        std::vector<EvalStep> walk, rest;
+        for (const auto& st : steps) {
+            const int64_t passes = (int64_t)st.P.n_progs * ((std::max(st.max_params, 1) + 3) / 4);
+            (passes >= 4096 ? walk : rest).push_back(st);
+        }
+        if (!walk.empty())
+            HIP_TRY(launch_grad_steps(walk.data(), (int)walk.size(), kGradWalk, ctx->stream, ctx->next_timer(), big),
+                    "reference-order gradient launch");
+        if (rest.empty()) return RSR_OK;
+        steps.swap(rest);
     }
-    const bool terms = ref;  // reference-order MSE: per-point terms, folded in order
+    const bool terms = ref;  // reference order: per-point terms, folded in order
     auto part_floats = [&](const EvalStep& st) {
-        return grad ? grad_partials_floats(st) : mse_partials_floats(st, ppl, terms);
+        return grad ? (terms ? grad_terms_floats(st) : grad_partials_floats(st)) : mse_partials_floats(st, ppl, terms);
     };
-    auto state_floats = [&](const EvalStep& st) { return grad ? grad_state_floats(st) : mse_state_floats(st, terms); };
+    auto state_floats = [&](const EvalStep& st) {
+        return grad ? (terms ? grad_terms_state_floats(st) : grad_state_floats(st)) : mse_state_floats(st, terms);
+    };
     size_t total = 0;
     std::vector<int64_t> tiles(steps.size());
     for (size_t i = 0; i < steps.size(); ++i) {
@@ -1113,7 +1125,8 @@ int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int
             s.partials = arena;
             arena += part_floats(s);
         }
-        HIP_TRY(grad ? launch_grad_steps(round.data(), (int)round.size(), false, ctx->stream, ctx->next_timer(), big)
+        HIP_TRY(grad ? launch_grad_steps(round.data(), (int)round.size(), terms ? kGradTerms : kGradKahan, ctx->stream,
+                                         ctx->next_timer(), big)
                      : launch_mse_steps(round.data(), (int)round.size(), t, terms, ctx->stream, ctx->next_timer(), big),
                 "evaluation launch");
     }

@@ -103,8 +103,11 @@ struct SegDesc {
     float* kstate;
     int64_t npad, count8, tile0, n_tiles, first_wave, first_thread;
     int64_t first_big;  // first work item of the segment in the big-program launch
+    int64_t first_fin;  // first thread of the segment in the gradient-terms finalize launch
+    float* gsum;        // gradient terms: the folded sum of every (position, slot) row
     int n_progs, group, n_groups, F, n_pass, slots, first, last, terms, tile_pts;
     int n_small;        // scheduling positions [0, n_small) run in the main launch
+    int fold_rows;      // rows per scheduling position in the fold (1: MSE; slots: gradients)
     float n_samples;
 };
 constexpr int kMaxSegs = 16;
@@ -123,6 +126,10 @@ int grad_slots(const EvalStep& st);
 // MSE partials: one per (tile, program), or with terms one per (point, program).
 size_t mse_partials_floats(const EvalStep& st, int ppl, bool terms);
 size_t grad_partials_floats(const EvalStep& st);
+// Reference-order gradients from per-point terms: one row per (program, slot) over the step's
+// tile range, plus the folded sum of every row.
+size_t grad_terms_floats(const EvalStep& st);
+size_t grad_terms_state_floats(const EvalStep& st);
 size_t mse_state_floats(const EvalStep& st, bool terms);
 size_t grad_state_floats(const EvalStep& st);
 
@@ -133,8 +140,14 @@ size_t grad_state_floats(const EvalStep& st);
 hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune, bool terms, hipStream_t s,
                             const KernelTimer* timer = nullptr, BigSpill big = {});
 // MSE + gradient (compute_mse_with_gradient, evaluator.rs:177-224) of every active
-// program with parameters, from the dual lowering.
-hipError_t launch_grad_steps(const EvalStep* steps, int n, bool reference_order, hipStream_t s,
+// program with parameters, from the dual lowering.  Summation:
+//   kGradKahan: per-tile partials, Kahan-compensated finalize;
+//   kGradWalk:  reference order, one wave per (program, tangent pass) walking the tiles;
+//   kGradTerms: reference order, per-point terms written by the tile-parallel kernel and
+//               folded in the reference's order (bit-identical to kGradWalk; for steps with
+//               too few programs to fill the chip one wave each).
+enum GradSum { kGradKahan = 0, kGradWalk = 1, kGradTerms = 2 };
+hipError_t launch_grad_steps(const EvalStep* steps, int n, GradSum mode, hipStream_t s,
                              const KernelTimer* timer = nullptr, BigSpill big = {});
 
 // Per-point values of programs [first, first+count) (eval_simd over every batch);

@@ -258,30 +258,35 @@ __global__ __launch_bounds__(256) void k_fold_terms(const SegTable T) {
     const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
     const int si = find_seg_first_thread(T, wave);
     const SegDesc& S = T.s[si];
-    const int64_t sp0 = (wave - S.first_thread) * 8;
-    if (sp0 >= S.n_progs) return;
+    // position q = sp * fold_rows + slot: one row of terms (MSE: slot 0 only)
+    const int64_t n_pos = (int64_t)S.n_progs * S.fold_rows;
+    const int64_t q0 = (wave - S.first_thread) * 8;
+    if (q0 >= n_pos) return;
     if (S.live != nullptr && *S.live == 0) return;
     const int j = lane >> 3, l = lane & 7;
-    const int64_t sp = sp0 + j;
-    const bool on = sp < S.n_progs && (S.active == nullptr || S.active[S.order[sp]]);
+    const int64_t q = q0 + j;
+    const int64_t sp = q / S.fold_rows;
+    const int slot = (int)(q - sp * S.fold_rows);
+    bool on = q < n_pos && (S.active == nullptr || S.active[S.order[sp]]);
+    if (on && S.fold_rows > 1 && slot > 0) on = slot - 1 < (int)S.nparams[S.order[sp]];
     const int64_t p0 = S.tile0 * S.tile_pts;
     const int64_t p1 = min(p0 + S.n_tiles * S.tile_pts, S.count8);  // whole 8-point batches
-    const float* row = S.partials + (on ? sp : 0) * S.n_tiles * S.tile_pts - p0 + l;
-    float acc = (on && !S.first) ? S.kstate[sp * 8 + l] : 0.0f;
+    const float* row = S.partials + (on ? q : 0) * S.n_tiles * S.tile_pts - p0 + l;
+    float acc = (on && !S.first) ? S.kstate[q * 8 + l] : 0.0f;
     if (on) {
         constexpr int kAhead = 32;
-        int64_t q = p0;
-        for (; q + 8 * kAhead <= p1; q += 8 * kAhead) {
+        int64_t p = p0;
+        for (; p + 8 * kAhead <= p1; p += 8 * kAhead) {
             float v[kAhead];
 #pragma unroll
-            for (int k = 0; k < kAhead; ++k) v[k] = row[q + 8 * k];
+            for (int k = 0; k < kAhead; ++k) v[k] = row[p + 8 * k];
 #pragma unroll
             for (int k = 0; k < kAhead; ++k) acc = acc + v[k];
         }
-        for (; q < p1; q += 8) acc = acc + row[q];
+        for (; p < p1; p += 8) acc = acc + row[p];
     }
     if (!S.last) {
-        if (on) S.kstate[sp * 8 + l] = acc;
+        if (on) S.kstate[q * 8 + l] = acc;
         return;
     }
     // f32x8::reduce_add within each group of eight lanes: (lo4 + hi4), then ((0 + 2) + (1 + 3))
@@ -290,8 +295,43 @@ __global__ __launch_bounds__(256) void k_fold_terms(const SegTable T) {
     const float a3 = __shfl(acc, g + 3, 64), a4 = __shfl(acc, g + 4, 64), a5 = __shfl(acc, g + 5, 64);
     const float a6 = __shfl(acc, g + 6, 64), a7 = __shfl(acc, g + 7, 64);
     const float s0 = a0 + a4, s1 = a1 + a5, s2 = a2 + a6, s3 = a3 + a7;
-    const float mse = ((s0 + s2) + (s1 + s3)) / S.n_samples;
-    if (on && l == 0) S.mse_out[S.order[sp]] = __builtin_isfinite(mse) ? mse : FLT_MAX;
+    const float sum = (s0 + s2) + (s1 + s3);
+    if (!on || l != 0) return;
+    if (S.fold_rows > 1) {  // gradient rows: the finalize kernel divides and checks
+        S.gsum[q] = sum;
+        return;
+    }
+    const float mse = sum / S.n_samples;
+    S.mse_out[S.order[sp]] = __builtin_isfinite(mse) ? mse : FLT_MAX;
+}
+
+// Gradient terms: one thread per (position, slot); slot 0 = MSE, 1 + k = gradient k, each the
+// folded row sum divided by N; the gradient is reported only for a finite MSE
+// (evaluator.rs:214-216), as k_eval_grad_exact does.
+__global__ void k_finalize_grad_terms(const SegTable T) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t0 = (int64_t)uni(blockIdx.x) * blockDim.x + ((int64_t)uni(threadIdx.x >> 6) << 6);
+    int si = 0;
+#pragma unroll
+    for (int s = 1; s < kMaxSegs; ++s) si = (s < T.n && t0 >= T.s[s].first_fin) ? s : si;
+    const SegDesc& S = T.s[si];
+    const int64_t q = t - S.first_fin;
+    if (q >= (int64_t)S.n_progs * S.slots || !S.last) return;
+    if (S.live != nullptr && *S.live == 0) return;
+    const int sp = (int)(q / S.slots);
+    const int slot = (int)(q - (int64_t)sp * S.slots);
+    const int prog = S.order[sp];
+    if (S.active != nullptr && !S.active[prog]) return;
+    const int np = S.nparams[prog];
+    if (np == 0) return;
+    const float mse = S.gsum[(int64_t)sp * S.slots] / S.n_samples;
+    const bool fin = __builtin_isfinite(mse);
+    if (slot == 0) {
+        S.mse_out[prog] = fin ? mse : FLT_MAX;
+    } else {
+        const int k = slot - 1;
+        S.grad_out[(int64_t)prog * 32 + k] = (fin && k < np) ? S.gsum[q] / S.n_samples : 0.0f;
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -338,13 +378,25 @@ __global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_s
                            nullptr, lane);
             const v32f vfile = G.v;
             const float diff = vfile[o] - S.target[pt];
+            const float two_diff = 2.0f * diff;
+            const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
+            if (S.terms) {
+                // reference order: per-point terms, row (sp, slot), folded in order later
+                float* rows = S.partials + (int64_t)sp * S.slots * S.n_tiles * 64 + lt * 64 + lane;
+                const int64_t rstride = S.n_tiles * 64;
+                if (pass == 0) __builtin_nontemporal_store(diff * diff, rows);
+#pragma unroll
+                for (int j = 0; j < kTan; ++j) {
+                    const int k = (int)k0 + j;
+                    if (k < np) __builtin_nontemporal_store(two_diff * gf[j][o], rows + (1 + k) * rstride);
+                }
+                continue;
+            }
             if (pass == 0) {
                 const float sq = diff * diff;
                 const float s = wave_sum(valid ? sq : 0.0f);
                 if (lane == 0) dst[0] = s;
             }
-            const float two_diff = 2.0f * diff;
-            const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
 #pragma unroll
             for (int j = 0; j < kTan; ++j) {
                 const int k = (int)k0 + j;
@@ -580,13 +632,24 @@ __global__ __launch_bounds__(256) void k_eval_grad_big(const SegTable T, int spi
                            gsp, lane);
             const v32f vfile = G.v;
             const float diff = vfile[o] - S.target[pt];
+            const float two_diff = 2.0f * diff;
+            const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
+            if (S.terms) {
+                float* rows = S.partials + (int64_t)sp * S.slots * S.n_tiles * 64 + lt * 64 + lane;
+                const int64_t rstride = S.n_tiles * 64;
+                if (pass == 0) rows[0] = diff * diff;
+#pragma unroll
+                for (int q = 0; q < kTan; ++q) {
+                    const int k = (int)k0 + q;
+                    if (k < np) rows[(1 + k) * rstride] = two_diff * gf[q][o];
+                }
+                continue;
+            }
             if (pass == 0) {
                 const float sq = diff * diff;
                 const float sum = wave_sum(valid ? sq : 0.0f);
                 if (lane == 0) dst[0] = sum;
             }
-            const float two_diff = 2.0f * diff;
-            const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
 #pragma unroll
             for (int q = 0; q < kTan; ++q) {
                 const int k = (int)k0 + q;
@@ -640,6 +703,11 @@ size_t grad_partials_floats(const EvalStep& st) {
     return (size_t)step_tiles(st, 1) * st.P.n_progs * grad_slots(st);
 }
 size_t mse_state_floats(const EvalStep& st, bool terms) { return (size_t)(terms ? 8 : 2) * st.P.n_progs; }
+size_t grad_terms_floats(const EvalStep& st) {
+    const size_t rows = (size_t)st.P.n_progs * grad_slots(st);
+    return (size_t)step_tiles(st, 1) * 64 * rows + rows;
+}
+size_t grad_terms_state_floats(const EvalStep& st) { return (size_t)8 * st.P.n_progs * grad_slots(st); }
 size_t grad_state_floats(const EvalStep& st) { return (size_t)4 * st.P.n_progs * grad_slots(st); }
 
 static SegDesc make_seg(const EvalStep& st, int ppl, int group) {
@@ -672,6 +740,7 @@ static SegDesc make_seg(const EvalStep& st, int ppl, int group) {
     d.F = st.D.num_features;
     d.n_pass = st.max_params > 0 ? (st.max_params + kTan - 1) / kTan : 1;
     d.n_samples = (float)st.D.num_samples;
+    d.fold_rows = 1;
     return d;
 }
 
@@ -771,25 +840,36 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
     return hipSuccess;
 }
 
-hipError_t launch_grad_steps(const EvalStep* steps, int n, bool reference_order, hipStream_t s,
-                             const KernelTimer* timer, BigSpill big) {
+hipError_t launch_grad_steps(const EvalStep* steps, int n, GradSum mode, hipStream_t s, const KernelTimer* timer,
+                             BigSpill big) {
+    const bool walk = mode == kGradWalk, terms = mode == kGradTerms;
     for (int base = 0; base < n; base += kMaxSegs) {
         const int cnt = min(kMaxSegs, n - base);
         SegTable T{};
-        int64_t waves = 0, threads = 0, big_items = 0;
+        int64_t waves = 0, threads = 0, big_items = 0, fin = 0;
         int spill = 0, gslots = 0;
         for (int i = 0; i < cnt; ++i) {
             const EvalStep& st = steps[base + i];
             if (st.max_params <= 0 || st.P.n_progs <= 0) continue;
-            SegDesc d = make_seg(st, 1, reference_order ? 1 : grad_group(step_tiles(st, 1), st.P.n_progs));
+            SegDesc d = make_seg(st, 1, walk ? 1 : grad_group(step_tiles(st, 1), st.P.n_progs));
+            if (d.n_tiles <= 0) continue;
+            d.terms = terms ? 1 : 0;
             d.first_wave = waves;
             d.first_thread = threads;
             d.first_big = big_items;
+            d.first_fin = fin;
             const int nb = d.n_progs - d.n_small;
-            waves += reference_order ? (int64_t)d.n_small * d.n_pass : d.n_tiles * d.n_groups;
-            big_items += reference_order ? (int64_t)nb * d.n_pass : d.n_tiles * nb;
+            waves += walk ? (int64_t)d.n_small * d.n_pass : d.n_tiles * d.n_groups;
+            big_items += walk ? (int64_t)nb * d.n_pass : d.n_tiles * nb;
             if (nb > 0) gslots = max(gslots, st.P.gslots);
-            threads += ((int64_t)d.n_progs * d.slots + 63) / 64 * 64;  // whole waves per segment
+            if (terms) {
+                d.fold_rows = d.slots;
+                d.gsum = st.partials + (size_t)d.n_tiles * 64 * d.n_progs * d.slots;
+                threads += ((int64_t)d.n_progs * d.slots + 7) / 8;                // fold waves
+                fin += ((int64_t)d.n_progs * d.slots + 63) / 64 * 64;               // finalize threads
+            } else {
+                threads += ((int64_t)d.n_progs * d.slots + 63) / 64 * 64;  // whole waves per segment
+            }
             spill = max(spill, lds_slots(st.P.spill_slots));
             T.s[T.n++] = d;
         }
@@ -799,7 +879,7 @@ hipError_t launch_grad_steps(const EvalStep* steps, int n, bool reference_order,
         const size_t lds = eval_grad_lds_bytes(spill);
         if (timer && base == 0) (void)hipEventRecord(timer->begin, s);
         if (waves > 0) {
-            if (reference_order)
+            if (walk)
                 hipLaunchKernelGGL(k_eval_grad_exact, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
             else
                 hipLaunchKernelGGL(k_eval_grad, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
@@ -807,14 +887,24 @@ hipError_t launch_grad_steps(const EvalStep* steps, int n, bool reference_order,
             if (e != hipSuccess) return e;
         }
         if (big_items > 0) {
-            const hipError_t e =
-                reference_order ? launch_big(k_eval_grad_exact_big, T, big_items, gslots, 1 + kTan, spill, big, s)
-                                : launch_big(k_eval_grad_big, T, big_items, gslots, 1 + kTan, spill, big, s);
+            const hipError_t e = walk ? launch_big(k_eval_grad_exact_big, T, big_items, gslots, 1 + kTan, spill, big, s)
+                                      : launch_big(k_eval_grad_big, T, big_items, gslots, 1 + kTan, spill, big, s);
             if (e != hipSuccess) return e;
         }
         if (timer && base + cnt >= n) (void)hipEventRecord(timer->end, s);
-        if (!reference_order) {
-            const int tb = 256;
+        const int tb = 256;
+        if (terms) {
+            hipLaunchKernelGGL(k_fold_terms, dim3((unsigned)((threads + kWaves - 1) / kWaves)), dim3(64 * kWaves), 0,
+                               s, T);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            bool last = false;
+            for (int i = 0; i < T.n; ++i) last = last || T.s[i].last;
+            if (last) {
+                hipLaunchKernelGGL(k_finalize_grad_terms, dim3((unsigned)((fin + tb - 1) / tb)), dim3(tb), 0, s, T);
+                if ((e = hipGetLastError()) != hipSuccess) return e;
+            }
+        } else if (!walk) {
             hipLaunchKernelGGL(k_finalize_grad, dim3((unsigned)((threads + tb - 1) / tb)), dim3(tb), 0, s, T);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;

@@ -362,3 +362,30 @@ def test_default_summation_is_reference_order():
         ds.close()
     finally:
         ctx.close()
+
+
+def test_reference_gradients_walk_and_terms_agree(gpu_ctx):
+    """Reference-order gradients take one wave per (program, pass) walking the tiles when a
+    step has at least 4,096 such walks, else tile-parallel per-point terms folded in the same
+    order: a batch of 4,200 programs (walks) and its first 160 programs alone (terms) give the
+    same bits, equal to the oracle on sampled programs."""
+    X, y, t = W.data_c2(1500, seed=95)
+    progs = W.gen_c2(4200, seed=96)
+    ds, ods = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, progs)
+    ok = P.status(ds) == 0
+    progs = [p for p, g in zip(progs, ok) if g]
+    P.close()
+    assert len(progs) >= 4096
+    Pw = api.Programs(gpu_ctx, progs)
+    Pt = api.Programs(gpu_ctx, progs[:160])
+    gpu_ctx.set_summation(True)
+    try:
+        mw, gw = api.compute_mse_with_gradient(gpu_ctx, Pw, ds)
+        mt, gt = api.compute_mse_with_gradient(gpu_ctx, Pt, ds)
+    finally:
+        gpu_ctx.set_summation(False)
+    assert same_bits(mw[:160], mt)[0] and same_bits(gw[:160], gt)[0]
+    for i in np.random.default_rng(5).choice(len(progs), 12, replace=False):
+        om, og = O.compute_mse_with_gradient(progs[i], ods)
+        assert same_bits(mw[i], om)[0] and same_bits(gw[i], og)[0], i
