@@ -344,6 +344,10 @@ def main():
                                                                  "p99_gen_latency_ms", "eval_node_evals_per_s")}
             # equal work: the same request stream, served at saturation vs one at a time
             serving["saturation_over_sequential"] = sat["eval_node_evals_per_s"] / seq["eval_node_evals_per_s"]
+            log("[bench] same arrivals, plain evaluator: one request at a time in arrival order")
+            fifo = mix.run_fifo(ctx, jobs, generations=args.serve_gens)
+            serving["plain_fifo"] = {k: fifo[k] for k in ("wall_s", "p50_gen_latency_ms", "p99_gen_latency_ms",
+                                                          "eval_node_evals_per_s")}
         if plain:
             serving["served_over_plain"] = serving["eval_node_evals_per_s"] / plain["value"]
             serving["saturation_over_plain"] = sat["eval_node_evals_per_s"] / plain["value"]

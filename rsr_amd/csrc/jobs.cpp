@@ -230,13 +230,11 @@ class LbfgsJob final : public Job {
     int32_t *order_p = nullptr, *order_d = nullptr;  // scheduling orders over the fitted programs
     int big_p = 0, big_d = 0;
     float* fitness_out = nullptr;
-    int iters = 0, it = 0, ls_round = 0, working = 0;
+    int iters = 0, it = 0, ls_round = 0;
     Stage stage = INIT;
     uint64_t mean_nodes = 0;
 
     int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>& grad) override {
-        const uint64_t n_eval = stage == INIT ? prog.size() : (uint64_t)working;
-        node_evals += n_eval * mean_nodes * (uint64_t)ds->num_samples;
         if (stage == SEARCH) {
             EvalStep st = mse_step(p, Lp, ds, act_ls, d_mse);
             set_order(st, order_p, (int)prog.size(), big_p);
@@ -253,7 +251,8 @@ class LbfgsJob final : public Job {
         switch (stage) {
             case INIT:
                 HIP_TRY(lbfgs_begin(st, params, d_mse, d_grad, ctx->stream), "lbfgs_begin");
-                if (iters > 0) HIP_TRY(lbfgs_direction(st, params, act_ls, d_counters, ctx->stream), "lbfgs_direction");
+                if (iters > 0)
+                    HIP_TRY(lbfgs_direction(st, params, act_ls, d_counters + 1, ctx->stream), "lbfgs_direction");
                 break;
             case SEARCH:
                 HIP_TRY(lbfgs_ls_check(st, params, d_mse, act_ls, act_grad, d_counters, ctx->stream), "ls_check");
@@ -261,16 +260,21 @@ class LbfgsJob final : public Job {
             case GRAD:
                 HIP_TRY(lbfgs_update(st, d_grad, act_grad, ctx->stream), "lbfgs_update");
                 if (it + 1 < iters)
-                    HIP_TRY(lbfgs_direction(st, params, act_ls, d_counters, ctx->stream), "lbfgs_direction");
+                    HIP_TRY(lbfgs_direction(st, params, act_ls, d_counters + 1, ctx->stream), "lbfgs_direction");
                 break;
         }
         return RSR_OK;
     }
-    int advance(const int* c) override {
+    // A fixed schedule, decided on the device: every iteration runs 12 line-search rounds
+    // (the reference's Armijo cap, lbfgs.rs:113-125) and then the gradient of the moved
+    // individuals.  An individual that accepted earlier, or converged (|g| < 1e-5), is
+    // inactive in the remaining rounds and iterations -- its kernels skip it -- so the results
+    // are those of stopping early, without a host synchronisation per round.
+    bool needs_counters() const override { return false; }
+    int advance(const int*) override {
         switch (stage) {
             case INIT:
-                working = c[0];
-                if (iters == 0 || working == 0) {
+                if (iters == 0) {
                     done = true;
                 } else {
                     stage = SEARCH;
@@ -278,15 +282,12 @@ class LbfgsJob final : public Job {
                 }
                 break;
             case SEARCH:
-                ++ls_round;
-                if (c[0] == 0 || ls_round == 12) stage = GRAD;  // Armijo backtracking, at most 12 evaluations
+                if (++ls_round == 12) stage = GRAD;
                 break;
             case GRAD:
-                ++it;
-                if (it >= iters || c[0] == 0) {
+                if (++it >= iters) {
                     done = true;
                 } else {
-                    working = c[0];
                     stage = SEARCH;
                     ls_round = 0;
                 }
@@ -298,11 +299,18 @@ class LbfgsJob final : public Job {
         if (prog.empty()) return RSR_OK;
         HIP_TRY(lbfgs_finish(st, (float*)p->dparams.ptr, d_fit, ctx->stream), "lbfgs_finish");
         std::vector<float> fit(prog.size());
+        int cnt[2] = {0, 0};
         HIP_TRY(hipMemcpyAsync(fit.data(), d_fit, fit.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
                 "fitness");
+        HIP_TRY(hipMemcpyAsync(cnt, d_counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream), "counters");
         int rc = download_params(ctx, p);  // synchronises the stream
         if (rc) return rc;
         for (size_t k = 0; k < prog.size(); ++k) fitness_out[prog[k]] = fit[k];
+        // evaluations run: gradients of all at the start and of every working individual per
+        // iteration (counter 1), line-search MSEs of the working individuals (counter 1) and of
+        // those still searching after a round (counter 0)
+        const uint64_t evals = prog.size() + 2 * (uint64_t)(uint32_t)cnt[1] + (uint64_t)(uint32_t)cnt[0];
+        node_evals += evals * mean_nodes * (uint64_t)ds->num_samples;
         return RSR_OK;
     }
     size_t device_bytes() const override { return jb.bytes(); }
@@ -334,7 +342,7 @@ class CmaesJob final : public Job {
     DevPrograms CP{};
     float* fitness_out = nullptr;
     int32_t* iters_out = nullptr;
-    int iters = 0, it = 0, working = 0;
+    int iters = 0, it = 0;
     Stage stage = INIT;
     uint64_t mean_nodes = 0;
 
@@ -343,7 +351,6 @@ class CmaesJob final : public Job {
             EvalStep st = mse_step(p, Lp, ds, act_fit, d_mse0);
             set_order(st, order_fit, (int)prog.size(), big_fit);
             mse.push_back(st);
-            node_evals += prog.size() * mean_nodes * (uint64_t)ds->num_samples;
             return RSR_OK;
         }
         HIP_TRY(cmaes_sample(st, d_cparams, it, ctx->stream), "cmaes_sample");
@@ -353,7 +360,6 @@ class CmaesJob final : public Job {
         s.active = st.cand_active;
         s.mse_out = d_cmse;
         mse.push_back(s);
-        node_evals += (uint64_t)working * st.lambda * mean_nodes * (uint64_t)ds->num_samples;
         return RSR_OK;
     }
     int post() override {
@@ -363,16 +369,17 @@ class CmaesJob final : public Job {
             HIP_TRY(cmaes_update(st, d_cmse, d_counters, ctx->stream), "cmaes_update");
         return RSR_OK;
     }
-    int advance(const int* c) override {
+    // A fixed schedule: `iters` generations; an individual that converged (best MSE below
+    // 1e-8, cmaes.rs:140-147) has no active candidates and its update is skipped, so the
+    // results are those of stopping early, without a host synchronisation per generation.
+    bool needs_counters() const override { return false; }
+    int advance(const int*) override {
         if (stage == INIT) {
-            working = (int)prog.size();
             stage = RUN;
             if (iters == 0) done = true;
             return RSR_OK;
         }
-        ++it;
-        working = c[0];
-        if (it >= iters || working == 0) done = true;  // every individual has converged
+        if (++it >= iters) done = true;
         return RSR_OK;
     }
     int finish() override {
@@ -380,16 +387,23 @@ class CmaesJob final : public Job {
         HIP_TRY(cmaes_finish(st, (float*)p->dparams.ptr, d_fit, d_iters, ctx->stream), "cmaes_finish");
         std::vector<float> fit(prog.size());
         std::vector<int> its(prog.size());
+        int cnt[2] = {0, 0};
         HIP_TRY(hipMemcpyAsync(fit.data(), d_fit, fit.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
                 "fit");
         HIP_TRY(hipMemcpyAsync(its.data(), d_iters, its.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
                 "iters");
+        HIP_TRY(hipMemcpyAsync(cnt, d_counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream), "counters");
         int rc = download_params(ctx, p);  // synchronises the stream
         if (rc) return rc;
         for (size_t k = 0; k < prog.size(); ++k) {
             fitness_out[prog[k]] = fit[k];
             if (iters_out) iters_out[prog[k]] = its[k];
         }
+        // evaluations run: the parents, then lambda candidates of every individual still
+        // working in each generation (counter 0: the working count after each update)
+        const uint64_t n = prog.size();
+        const uint64_t gens = std::min<uint64_t>(n * (uint64_t)iters, n + (uint64_t)(uint32_t)cnt[0]);
+        node_evals += (n + gens * (uint64_t)st.lambda) * mean_nodes * (uint64_t)ds->num_samples;
         return RSR_OK;
     }
     size_t device_bytes() const override { return jb.bytes(); }
@@ -417,16 +431,10 @@ class NmJob final : public Job {
     float* fitness_out = nullptr;
     int32_t* iters_out = nullptr;
     bool started = false;
-    int working = 0;
     uint64_t mean_nodes = 0, mean_dim = 0;
 
     int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>&) override {
-        if (!started) {
-            HIP_TRY(nm_begin(st, (const float*)p->dparams.ptr, d_cparams, ctx->stream), "nm_begin");
-            node_evals += prog.size() * (mean_dim + 1) * mean_nodes * (uint64_t)ds->num_samples;
-        } else {
-            node_evals += (uint64_t)working * mean_nodes * (uint64_t)ds->num_samples;
-        }
+        if (!started) HIP_TRY(nm_begin(st, (const float*)p->dparams.ptr, d_cparams, ctx->stream), "nm_begin");
         EvalStep s;
         s.P = CP;
         s.D = dev_view(ds);
@@ -439,10 +447,13 @@ class NmJob final : public Job {
         HIP_TRY(nm_step(st, d_cparams, d_cmse, d_counters, ctx->stream), "nm_step");
         return RSR_OK;
     }
+    // The counter (individuals that stepped) is read every eighth tick: when none stepped in
+    // the ticks since the last read, every individual is done (done is absorbing, and a done
+    // individual's step is skipped, so the extra ticks change nothing).
+    bool needs_counters() const override { return ticks % 8 == 7; }
     int advance(const int* c) override {
         started = true;
-        working = c[0];
-        if (working == 0) done = true;
+        if (c && c[0] == 0) done = true;
         return RSR_OK;
     }
     int finish() override {
@@ -450,16 +461,21 @@ class NmJob final : public Job {
         HIP_TRY(nm_finish(st, (float*)p->dparams.ptr, d_fit, d_iters, ctx->stream), "nm_finish");
         std::vector<float> fit(prog.size());
         std::vector<int> its(prog.size());
+        int cnt[2] = {0, 0};
         HIP_TRY(hipMemcpyAsync(fit.data(), d_fit, fit.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
                 "fit");
         HIP_TRY(hipMemcpyAsync(its.data(), d_iters, its.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
                 "iters");
+        HIP_TRY(hipMemcpyAsync(cnt, d_counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream), "counters");
         int rc = download_params(ctx, p);  // synchronises the stream
         if (rc) return rc;
         for (size_t k = 0; k < prog.size(); ++k) {
             fitness_out[prog[k]] = fit[k];
             if (iters_out) iters_out[prog[k]] = its[k];
         }
+        // evaluations run: the initial simplex, then one step per working individual per tick
+        const uint64_t evals = prog.size() * (mean_dim + 1) + (uint64_t)(uint32_t)cnt[0];
+        node_evals += evals * mean_nodes * (uint64_t)ds->num_samples;
         return RSR_OK;
     }
     size_t device_bytes() const override { return jb.bytes(); }
@@ -673,7 +689,7 @@ int make_lbfgs_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
         (rc = sc.raw((size_t)n * tb, &st.ddg)))
         return rc;
     if ((rc = sc.get((size_t)n, &st.hist)) || (rc = sc.get((size_t)n, &st.head)) || (rc = sc.get((size_t)n, &st.ls)) ||
-        (rc = sc.get((size_t)n, &st.flags)))
+        (rc = sc.get((size_t)n, &st.flags)) || (rc = sc.get(2, &j->d_counters)))
         return rc;
     std::vector<uint8_t> fitmask(p->n, 0);
     for (int i : j->prog) fitmask[i] = 1;
@@ -824,7 +840,7 @@ int make_cmaes_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
     int32_t* d_order;
     if ((rc = sc.put(hdesc, &d_desc)) || (rc = sc.put(corder, &d_order)) ||
         (rc = sc.get((size_t)n * lambda, &j->d_cmse)) || (rc = sc.get((size_t)n, &j->d_fit)) ||
-        (rc = sc.get((size_t)n, &j->d_iters)))
+        (rc = sc.get((size_t)n, &j->d_iters)) || (rc = sc.get(2, &j->d_counters)))
         return rc;
     j->CP.code = (const UInstr*)Lp->dcode.ptr;
     j->CP.progs = d_desc;
@@ -918,7 +934,8 @@ int make_nm_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int it
         (rc = sc.get((size_t)kMaxP * n, &st.refl)) || (rc = sc.get((size_t)kMaxP * n, &st.trial)) ||
         (rc = sc.get((size_t)n, &st.rf)) || (rc = sc.get((size_t)n, &st.rm)) || (rc = sc.get((size_t)n, &st.lim)) ||
         (rc = sc.get((size_t)kNmSlots * n, &st.idx)) || (rc = sc.get((size_t)n, &st.phase)) ||
-        (rc = sc.get((size_t)n, &st.iter)) || (rc = sc.get((size_t)n * kNmSlots, &st.cand_active)))
+        (rc = sc.get((size_t)n, &st.iter)) || (rc = sc.get((size_t)n * kNmSlots, &st.cand_active)) ||
+        (rc = sc.get(2, &j->d_counters)))
         return rc;
     std::vector<ProgDesc> hdesc((size_t)n * kNmSlots);
     for (int k = 0; k < n; ++k)
@@ -1022,27 +1039,6 @@ int make_optimize_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, 
 // ticks
 // ---------------------------------------------------------------------------------
 namespace {
-
-int ensure_counters(rsr_context* ctx, size_t n_ints) {
-    if (ctx->host_counters_cap < n_ints) {
-        if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
-        ctx->host_counters = nullptr;
-        ctx->host_counters_cap = 0;
-        const size_t cap = std::max<size_t>(n_ints, 64);
-        HIP_TRY(hipHostMalloc((void**)&ctx->host_counters, cap * sizeof(int), hipHostMallocDefault), "counters");
-        ctx->host_counters_cap = cap;
-    }
-    const size_t before = ctx->dev_counters.bytes;
-    const int rc = ensure_scratch(ctx, ctx->dev_counters, n_ints * sizeof(int));
-    if (rc) return rc;
-    if (ctx->dev_counters.bytes != before || ctx->counters_dirty) {
-        // a new buffer, or a tick that did not read its counters back: start from zero
-        HIP_TRY(hipMemsetAsync(ctx->dev_counters.ptr, 0, ctx->dev_counters.bytes, ctx->stream), "counter clear");
-        ctx->counters_seen.assign(ctx->dev_counters.bytes / sizeof(int), 0u);
-        ctx->counters_dirty = false;
-    }
-    return RSR_OK;
-}
 
 // Run steps of one kind (MSE or gradient) at one points-per-lane.  The per-tile Kahan
 // partials (or, for reference-order MSE, the per-point terms) live in a shared arena of at
@@ -1175,43 +1171,52 @@ int run_tick(rsr_context* ctx, const std::vector<Job*>& top) {
         }
         return RSR_OK;
     }
-    bool counters = false;
-    for (Job* j : jobs) counters = counters || j->needs_counters();
     int rc;
-    int* dc = nullptr;
-    if (counters) {
-        if ((rc = ensure_counters(ctx, 2 * nj))) return rc;
-        dc = (int*)ctx->dev_counters.ptr;
-        ctx->counters_dirty = true;  // until this tick's increases are read back
-    }
     std::vector<EvalStep> mse, grad;
-    for (size_t k = 0; k < nj; ++k) {
-        jobs[k]->d_counters = dc ? dc + 2 * k : nullptr;
+    for (size_t k = 0; k < nj; ++k)
         if ((rc = jobs[k]->plan(mse, grad))) return rc;
-    }
     if (!mse.empty() && (rc = run_mse(ctx, mse))) return rc;
     if (!grad.empty() && (rc = run_grad(ctx, grad))) return rc;
     for (Job* j : jobs)
         if ((rc = j->post())) return rc;
-    // without counters the next stream synchronisation is the job's own finish()
-    int* hc = nullptr;
-    std::vector<int> delta;
-    if (counters) {
-        int* raw = ctx->host_counters;
-        HIP_TRY(hipMemcpyAsync(raw, dc, 2 * nj * sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "counter download");
-        HIP_TRY(hipStreamSynchronize(ctx->stream), "tick");
-        delta.resize(2 * nj);
-        for (size_t i = 0; i < 2 * nj; ++i) {  // this tick's increase of each slot
-            const uint32_t now = (uint32_t)raw[i];
-            delta[i] = (int)(now - ctx->counters_seen[i]);
-            ctx->counters_seen[i] = now;
+    // counters of the jobs that decide on them this tick: one copy each, one synchronisation;
+    // without any, the next tick is enqueued right away (the next synchronisation is a job's
+    // finish() or a later tick's counters)
+    std::vector<size_t> want;
+    std::vector<uint8_t> reads(nj, 0);
+    for (size_t k = 0; k < nj; ++k)
+        if (jobs[k]->needs_counters() && jobs[k]->d_counters) {
+            want.push_back(k);
+            reads[k] = 1;
         }
-        ctx->counters_dirty = false;
-        hc = delta.data();
+    std::vector<int> delta(2 * nj, 0);
+    if (!want.empty()) {
+        if (ctx->host_counters_cap < 2 * want.size()) {
+            if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
+            ctx->host_counters = nullptr;
+            ctx->host_counters_cap = 0;
+            const size_t cap = std::max<size_t>(2 * want.size(), 64);
+            HIP_TRY(hipHostMalloc((void**)&ctx->host_counters, cap * sizeof(int), hipHostMallocDefault), "counters");
+            ctx->host_counters_cap = cap;
+        }
+        int* raw = ctx->host_counters;
+        for (size_t w = 0; w < want.size(); ++w)
+            HIP_TRY(hipMemcpyAsync(raw + 2 * w, jobs[want[w]]->d_counters, 2 * sizeof(int), hipMemcpyDeviceToHost,
+                                   ctx->stream),
+                    "counter download");
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "tick");
+        for (size_t w = 0; w < want.size(); ++w) {
+            Job* j = jobs[want[w]];
+            for (int c = 0; c < 2; ++c) {  // this tick's increase of each counter
+                const uint32_t now = (uint32_t)raw[2 * w + c];
+                delta[2 * want[w] + c] = (int)(now - j->counters_seen[c]);
+                j->counters_seen[c] = now;
+            }
+        }
     }
     for (size_t k = 0; k < nj; ++k) {
         jobs[k]->ticks++;
-        if ((rc = jobs[k]->advance(hc ? hc + 2 * k : nullptr))) return rc;
+        if ((rc = jobs[k]->advance(reads[k] ? delta.data() + 2 * k : nullptr))) return rc;
     }
     for (Job* j : top) {
         if (std::find(jobs.begin(), jobs.end(), j) == jobs.end()) j->ticks++;

@@ -5,10 +5,12 @@
 //   1. every job enqueues its pre-step kernels and describes its evaluation step(s);
 //   2. all MSE steps of all jobs run as merged launches (segment tables), then all
 //      gradient steps;
-//   3. every job enqueues its post-step kernels (optimizer updates), which write two
-//      counters per job;
-//   4. the counters come back in one copy and one stream synchronisation;
-//   5. every job advances its state from its counters.
+//   3. every job enqueues its post-step kernels (optimizer updates), which add to the
+//      job's two device counters;
+//   4. the counters of the jobs that decide on them this tick come back in one
+//      synchronisation (L-BFGS and CMA-ES follow fixed schedules decided on the device and
+//      need none: their ticks are enqueued back to back);
+//   5. every job advances its state.
 // The direct C-ABI calls run one job until it is done; the server runs the ticks of
 // every admitted request together (continuous batching at the step level).
 #pragma once
@@ -61,7 +63,8 @@ class Job {
     virtual int finish() = 0;
     // Device bytes the job holds (admission accounting).
     virtual size_t device_bytes() const = 0;
-    // Whether advance() reads the tick's counters (single-step jobs do not).
+    // Whether this tick must read the job's counters back (a host synchronisation) for
+    // advance(); when false, advance() gets null and the next tick is enqueued at once.
     virtual bool needs_counters() const { return true; }
     // The jobs a tick advances on behalf of this one (a composite job lists its parts).
     virtual void leaves(std::vector<Job*>& out) {
@@ -70,7 +73,10 @@ class Job {
     // Called after every tick (a composite job moves between its stages here).
     virtual int refresh() { return RSR_OK; }
     bool done = false;
-    int* d_counters = nullptr;  // two device ints owned by the tick driver
+    // Two device ints of the job's own, zero at creation: its kernels add to them, and they
+    // only ever grow; a read takes the increase since the previous read (counters_seen).
+    int* d_counters = nullptr;
+    uint32_t counters_seen[2] = {0u, 0u};
     uint64_t node_evals = 0;    // node-evals launched so far (statistics; fits estimate from active counts)
     int ticks = 0;
 };

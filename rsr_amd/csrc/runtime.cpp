@@ -253,7 +253,6 @@ extern "C" int rsr_context_destroy(rsr_context* ctx) {
     ctx->scratch_out.release();
     ctx->scratch_out2.release();
     ctx->scratch_gspill.release();
-    ctx->dev_counters.release();
     rsr::trim_pool(ctx);
     if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
     (void)hipStreamDestroy(ctx->stream);

@@ -87,16 +87,11 @@ struct rsr_context {
     rsr::DevBuf scratch_partials, scratch_gpartials, scratch_state, scratch_gstate, scratch_out, scratch_out2, scratch_act;
     rsr::DevBuf scratch_gspill;  // device spill areas of the big-program launches
     size_t arena_bytes = (size_t)4 << 30;  // summation partials / terms per launch round
-    // per-job tick counters: device array and its pinned host mirror
-    rsr::DevBuf dev_counters;
+    // pinned host mirror of the jobs' counters a tick reads back (each job owns its device pair)
     int* host_counters = nullptr;
     size_t host_counters_cap = 0;
-    // The device counters only ever grow (kernels atomically add to them); a tick reads
-    // each slot's increase since the last read instead of clearing the slots first.
-    std::vector<uint32_t> counters_seen;
     std::atomic<int> live_servers{0};  // a context is not destroyed under a running server
     std::atomic<int> live_objects{0};  // ... nor while datasets or program batches use it
-    bool counters_dirty = true;  // cleared on the device before the next use
     // profiling: events around every main evaluation kernel, accumulated on read
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;

@@ -151,6 +151,68 @@ def run_sequential(ctx, jobs, generations=3, fit_frac=0.01, min_fit=8, opt_iters
     return summarize(jobs, wall, None, generations)
 
 
+def run_fifo(ctx, jobs, generations=3, fit_frac=0.01, min_fit=8, opt_iters=100):
+    """The plain evaluator serving the same arrivals: every job is a client thread as in
+    run_served, but one worker runs the generations' requests one at a time, in arrival
+    order, through the direct entry points (one population per launch, no batching); the
+    latency counts the wait in that queue, as the served latency counts the server's."""
+    import queue as _q
+    work = _q.Queue()
+    for jb in jobs:
+        jb.latencies, jb.error = [], ""
+    t_start = time.perf_counter()
+
+    def worker():
+        while True:
+            item = work.get()
+            if item is None:
+                return
+            fn, done = item
+            try:
+                done.append(fn())
+            except Exception as e:  # handed to the waiting client
+                done.append(e)
+
+    def call(fn):
+        done = []
+        work.put((fn, done))
+        while not done:
+            time.sleep(0.0002)
+        if isinstance(done[0], Exception):
+            raise done[0]
+        return done[0]
+
+    def client(jb: MixJob):
+        try:
+            P = api.Programs(ctx, jb.programs)
+            P.status(jb.ds)
+            delay = jb.arrival - (time.perf_counter() - t_start)
+            if delay > 0:
+                time.sleep(delay)
+            for g in range(generations):
+                t0 = time.perf_counter()
+                mse = call(lambda: api.compute_mse(ctx, P, jb.ds, allow_rejected=True))
+                m = _fit_mask(mse, fit_frac, min_fit)
+                call(lambda: api.optimize_constants(ctx, P, jb.ds, iterations=opt_iters, seed=1000 * jb.index + g,
+                                                    mask=m))
+                jb.latencies.append(time.perf_counter() - t0)
+            P.close()
+        except Exception as e:  # reported in the summary
+            jb.error = repr(e)
+
+    wt = threading.Thread(target=worker)
+    wt.start()
+    threads = [threading.Thread(target=client, args=(jb,)) for jb in jobs]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    work.put(None)
+    wt.join()
+    wall = time.perf_counter() - t_start
+    return summarize(jobs, wall, None, generations)
+
+
 def summarize(jobs, wall, stats, generations):
     lat = np.array([x for jb in jobs for x in jb.latencies], np.float64)
     eval_nodes = sum(_nodes(jb.programs, jb.n_points) for jb in jobs if jb.latencies) * generations
