@@ -269,7 +269,7 @@ def main():
         if graph:  # the kernel timer does not see graph replays; the kernels are the eager ones
             kern_ms, launches = runs[base]["kernel_ms"], runs[base]["launches"]
         runs[m] = {"sec": sec, "kernel_ms": kern_ms, "launches": launches, "mse": mse}
-    ctx.set_summation(False)
+    ctx.set_summation(True)  # back to the library default (reference order)
     ctx.set_graphs(False)
 
     # The same evaluation from host buffers: dataset and program upload (with the programs'
@@ -297,7 +297,7 @@ def main():
             ds2.close()
             parts = (t3 - t0, t1 - t0, t2 - t1, t3 - t2)
             best = parts if best is None or parts[0] < best[0] else best
-        ctx.set_summation(False)
+        ctx.set_summation(True)  # back to the library default (reference order)
         from_host = {"ms": best[0] * 1e3, "node_evals_per_s": ne / best[0],
                      "dataset_upload_ms": best[1] * 1e3, "program_upload_ms": best[2] * 1e3,
                      "lowering_and_evaluation_ms": best[3] * 1e3, "summation": fh_mode,
@@ -353,6 +353,7 @@ def main():
             serving["saturation_over_plain"] = sat["eval_node_evals_per_s"] / plain["value"]
             if serving.get("node_evals_per_busy_s"):
                 serving["served_busy_over_plain"] = serving["node_evals_per_busy_s"] / plain["value"]
+        mix.release(jobs)
         for jb in jobs:
             jb.ds.close()
 

@@ -323,6 +323,7 @@ extern "C" int rsr_context_set_summation(rsr_context* ctx, int mode) {
     if (mode != RSR_SUM_KAHAN && mode != RSR_SUM_REFERENCE) return set_error(RSR_E_INVALID_ARG, "bad summation mode");
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->summation = mode;
+    ctx->tune_version++;
     return RSR_OK;
 }
 
@@ -349,6 +350,7 @@ extern "C" int rsr_context_set_partials_limit(rsr_context* ctx, uint64_t bytes) 
     if (bytes < 4096) return set_error(RSR_E_INVALID_ARG, "partials limit below 4096 bytes");
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->arena_bytes = (size_t)bytes;
+    ctx->tune_version++;
     return RSR_OK;
 }
 

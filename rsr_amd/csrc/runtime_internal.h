@@ -81,7 +81,9 @@ struct rsr_context {
     rsr::EvalTuning tuning;     // the fixed tuning (auto_tune off)
     bool auto_tune = true;      // per-workload tuning: autotuned choice, else the heuristic
     std::map<uint64_t, rsr::EvalTuning> tune_cache;  // rsr_autotune results by workload shape
-    uint64_t tune_version = 0;  // bumped whenever the tuning settings or the cache change
+    // bumped whenever the summation, tuning, tuning cache or partials limit change (server
+    // lanes copy the settings only then)
+    std::atomic<uint64_t> tune_version{0};
     int summation = 1;  // 1 (default): reference order (bit-identical to the CPU evaluator); 0: pairwise + Kahan
     bool graphs = false;  // rsr_eval_mse replays a captured graph per (programs, dataset, settings)
     rsr::DevBuf scratch_partials, scratch_gpartials, scratch_state, scratch_gstate, scratch_out, scratch_out2, scratch_act;

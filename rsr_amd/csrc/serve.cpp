@@ -258,15 +258,14 @@ struct rsr_server {
 
     // The owner's summation and tuning settings, copied into the lane when they changed.
     void sync_settings(Lane& L) {
+        if (L.seen_version == ctx->tune_version.load()) return;  // unchanged: no lock taken
         std::lock_guard<std::mutex> g(ctx->mu);
         L.ctx->summation = ctx->summation;
-        if (L.seen_version != ctx->tune_version) {
-            L.ctx->auto_tune = ctx->auto_tune;
-            L.ctx->tuning = ctx->tuning;
-            L.ctx->tune_cache = ctx->tune_cache;
-            L.seen_version = ctx->tune_version;
-        }
+        L.ctx->auto_tune = ctx->auto_tune;
+        L.ctx->tuning = ctx->tuning;
+        L.ctx->tune_cache = ctx->tune_cache;
         L.ctx->arena_bytes = ctx->arena_bytes;
+        L.seen_version = ctx->tune_version.load();
     }
 
     void loop(Lane& L) {
