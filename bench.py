@@ -198,6 +198,24 @@ def run_configs(ctx, log):
     return out
 
 
+def host_cpu_info():
+    """The host CPU as /proc/cpuinfo and the scheduler describe it."""
+    info = {"nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0))}
+    try:
+        fields = {}
+        for line in open("/proc/cpuinfo"):
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                fields.setdefault(k, v)
+        info["host_cpu"] = fields.get("model name")
+        info["cpu_family_model_stepping"] = "/".join(fields.get(k, "?") for k in ("cpu family", "model", "stepping"))
+        info["cpu_mhz"] = fields.get("cpu MHz")
+        info["threads_per_core"] = int(fields.get("siblings", 1)) // max(1, int(fields.get("cpu cores", 1)))
+    except (OSError, ValueError):
+        pass
+    return info
+
+
 def load_traffic():
     try:
         with open(TRAFFIC_FILE) as f:
@@ -362,7 +380,9 @@ def main():
     if not args.no_cpu:
         import oracle as O
         ods = O.ODataset.new(X, y, 9)
-        threads = min(os.cpu_count() or 1, 16)
+        # one worker per core of this process's CPU share (the affinity mask), at most 16:
+        # the CPU share that comes with one GPU of the node
+        threads = max(1, min(len(os.sched_getaffinity(0)), 16))
         sample = progs[:args.cpu_programs]
         log(f"[bench] cpu baseline: oracle, {len(sample)} programs x 100000 points, {threads} threads")
         cref, cdt = cpu_run(O.compute_mse, sample, ods, threads)
@@ -376,11 +396,7 @@ def main():
         ne1 = node_evals(one, len(y))
         cpu["single_thread"] = {"value": ne1 / dt1, "cores": 1,
                                 "sample": f"first {len(one)} programs x 100000 points ({dt1:.1f} s wall)"}
-        try:
-            cpu["host_cpu"] = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
-                                   if l.startswith("model name"))
-        except (OSError, StopIteration):
-            pass
+        cpu.update(host_cpu_info())
         k = min(args.fp64_programs, len(sample))
         log(f"[bench] fp64 oracle on {k} programs")
         idx = np.random.default_rng(0).choice(len(sample), k, replace=False)
@@ -431,7 +447,8 @@ def main():
         "fit_workloads": per_config,
         "serving": serving,
         "batched_over_plain": value / plain["value"] if plain else None,
-        "roofline": {"bound": "mfma", "kind": "fp32 vector ALU (no MFMA: not a contraction)",
+        "roofline": {"bound": "valu", "kind": "fp32 vector ALU issue (no MFMA: not a contraction; HBM is "
+                                                "under 1% busy)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "frac_of_nofma_ceiling": achieved / PEAK_FP32_NOFMA_TOPS,

@@ -1041,7 +1041,7 @@ int make_optimize_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, 
 namespace {
 
 // Run steps of one kind (MSE or gradient) at one points-per-lane.  The per-tile Kahan
-// partials (or, for reference-order MSE, the per-point terms) live in a shared arena of at
+// partials (or, in reference order, the per-point terms) live in a shared arena of at
 // most ctx->arena_bytes; when the steps need more, every step is split into the same
 // number of tile ranges, evaluated round by round with the summation state carried
 // between rounds (the sums are the same as unsplit).
@@ -1056,21 +1056,6 @@ int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int
     if (big.floats) {
         if ((rc = ensure_scratch(ctx, ctx->scratch_gspill, big.floats * sizeof(float)))) return rc;
         big.ptr = (float*)ctx->scratch_gspill.ptr;
-    }
-    if (ref && grad) {
-        // Reference order: one wave per (program, tangent pass) walks the tiles in order when
-        // the walks fill the chip (no memory traffic); the other steps write per-point terms
-        // tile-parallel and fold them in the same order (the same bits), in rounds below.
-        std::vector<EvalStep> walk, rest;
-        for (const auto& st : steps) {
-            const int64_t passes = (int64_t)st.P.n_progs * ((std::max(st.max_params, 1) + 3) / 4);
-            (passes >= 4096 ? walk : rest).push_back(st);
-        }
-        if (!walk.empty())
-            HIP_TRY(launch_grad_steps(walk.data(), (int)walk.size(), kGradWalk, ctx->stream, ctx->next_timer(), big),
-                    "reference-order gradient launch");
-        if (rest.empty()) return RSR_OK;
-        steps.swap(rest);
     }
     const bool terms = ref;  // reference order: per-point terms, folded in order
     auto part_floats = [&](const EvalStep& st) {
@@ -1121,7 +1106,7 @@ int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int
             s.partials = arena;
             arena += part_floats(s);
         }
-        HIP_TRY(grad ? launch_grad_steps(round.data(), (int)round.size(), terms ? kGradTerms : kGradKahan, ctx->stream,
+        HIP_TRY(grad ? launch_grad_steps(round.data(), (int)round.size(), terms, ctx->stream,
                                          ctx->next_timer(), big)
                      : launch_mse_steps(round.data(), (int)round.size(), t, terms, ctx->stream, ctx->next_timer(), big),
                 "evaluation launch");

@@ -105,7 +105,7 @@ struct SegDesc {
     int64_t first_big;  // first work item of the segment in the big-program launch
     int64_t first_fin;  // first thread of the segment in the gradient-terms finalize launch
     float* gsum;        // gradient terms: the folded sum of every (position, slot) row
-    int n_progs, group, n_groups, F, n_pass, slots, first, last, terms, tile_pts;
+    int n_progs, group, n_groups, F, slots, first, last, terms, tile_pts;
     int n_small;        // scheduling positions [0, n_small) run in the main launch
     int fold_rows;      // rows per scheduling position in the fold (1: MSE; slots: gradients)
     float n_samples;
@@ -140,14 +140,10 @@ size_t grad_state_floats(const EvalStep& st);
 hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune, bool terms, hipStream_t s,
                             const KernelTimer* timer = nullptr, BigSpill big = {});
 // MSE + gradient (compute_mse_with_gradient, evaluator.rs:177-224) of every active
-// program with parameters, from the dual lowering.  Summation:
-//   kGradKahan: per-tile partials, Kahan-compensated finalize;
-//   kGradWalk:  reference order, one wave per (program, tangent pass) walking the tiles;
-//   kGradTerms: reference order, per-point terms written by the tile-parallel kernel and
-//               folded in the reference's order (bit-identical to kGradWalk; for steps with
-//               too few programs to fill the chip one wave each).
-enum GradSum { kGradKahan = 0, kGradWalk = 1, kGradTerms = 2 };
-hipError_t launch_grad_steps(const EvalStep* steps, int n, GradSum mode, hipStream_t s,
+// program with parameters, from the dual lowering.  terms = false: per-tile partials,
+// Kahan-compensated finalize; terms = true: per-point terms written by the tile-parallel
+// kernel and folded in the reference's order (bit-identical to the CPU evaluator).
+hipError_t launch_grad_steps(const EvalStep* steps, int n, bool terms, hipStream_t s,
                              const KernelTimer* timer = nullptr, BigSpill big = {});
 
 // Per-point values of programs [first, first+count) (eval_simd over every batch);

@@ -16,8 +16,8 @@
 // partial sum; the finalize kernel adds the partials of each program in tile order with
 // Kahan-compensated FP32, so results are bitwise reproducible run to run.
 //
-// Reference-order summation: one wave per (program[, pass]) walks the tiles in order
-// and folds each 8-point batch into 8 lane accumulators (see fold_batches).
+// Reference-order summation: the same tile-parallel kernels write every per-point term
+// (squared error, or gradient term) and k_fold_terms adds them in the reference's order.
 #include <float.h>
 #include <algorithm>
 #include "interp.h"
@@ -124,19 +124,6 @@ __device__ __forceinline__ float reduce_lanes8(float acc) {
     const float a4 = __shfl(acc, 4, 64), a5 = __shfl(acc, 5, 64), a6 = __shfl(acc, 6, 64), a7 = __shfl(acc, 7, 64);
     const float s0 = a0 + a4, s1 = a1 + a5, s2 = a2 + a6, s3 = a3 + a7;
     return (s0 + s2) + (s1 + s3);
-}
-
-// The reference accumulates per-point terms in 8 lanes, lane l receiving points l, l+8,
-// l+16, ... in order (evaluator.rs:159-170).  Inside a 64-point tile the eight 8-point
-// batches are folded into the 8 accumulators (lanes 0..7) one batch after the other,
-// which reproduces that serial chain exactly when the tiles are visited in order.
-__device__ __forceinline__ float fold_batches(float acc, float term, int lane) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const float v = __shfl(term, (lane & 7) + 8 * k, 64);
-        acc = acc + v;
-    }
-    return acc;
 }
 
 // ---------------------------------------------------------------------------------
@@ -247,56 +234,83 @@ __global__ void k_finalize_mse(const SegTable T) {
 // ---------------------------------------------------------------------------------
 // MSE, reference-order summation from per-point terms
 // ---------------------------------------------------------------------------------
-// Eight scheduling positions per wave, eight lanes each: lane 8j + l is the reference's
-// lane accumulator l of position j, adding the terms of points l, l+8, l+16, ... of the
-// launch's range in order (evaluator.rs:159-170), starting from the state carried from
-// earlier ranges.  Every lane runs its own serial chain (no shuffles); the loads are
-// issued 32 batches ahead of the adds.  The last range reduces each position's eight
-// accumulators in f32x8::reduce_add order.  (first_thread counts fold waves here.)
-__global__ __launch_bounds__(256) void k_fold_terms(const SegTable T) {
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
+// One wave per position q = sp * fold_rows + slot (one row of terms; MSE: slot 0 only).
+// Lanes 0..7 are the reference's eight lane accumulators: lane l adds the terms of points
+// l, l+8, l+16, ... of the launch's range in order (evaluator.rs:159-170), starting from the
+// state carried from earlier ranges.  The serial chains cannot be split, so the wave keeps
+// the memory busy instead: the whole wave loads the row in chunks of kFoldChunk terms with
+// coalesced 16-byte loads, transposes each chunk through LDS so that chain l finds its terms
+// contiguous, and issues the next chunk's loads before folding the current one.  The last
+// range reduces the eight accumulators in f32x8::reduce_add order.  (first_thread counts fold
+// waves here, one per position.)
+static constexpr int kFoldChunk = 2048;                 // terms per chunk: 8 float4 per lane
+static constexpr int kFoldRow = kFoldChunk / 8 + 4;     // LDS row of one chain (+4: banks)
+__global__ __launch_bounds__(64) void k_fold_terms(const SegTable T) {
+    __shared__ __align__(16) float bt[8 * kFoldRow];
+    const int lane = threadIdx.x;
+    const int64_t wave = uni(blockIdx.x);
     const int si = find_seg_first_thread(T, wave);
     const SegDesc& S = T.s[si];
-    // position q = sp * fold_rows + slot: one row of terms (MSE: slot 0 only)
     const int64_t n_pos = (int64_t)S.n_progs * S.fold_rows;
-    const int64_t q0 = (wave - S.first_thread) * 8;
-    if (q0 >= n_pos) return;
+    const int64_t q = wave - S.first_thread;
+    if (q >= n_pos) return;
     if (S.live != nullptr && *S.live == 0) return;
-    const int j = lane >> 3, l = lane & 7;
-    const int64_t q = q0 + j;
     const int64_t sp = q / S.fold_rows;
     const int slot = (int)(q - sp * S.fold_rows);
-    bool on = q < n_pos && (S.active == nullptr || S.active[S.order[sp]]);
-    if (on && S.fold_rows > 1 && slot > 0) on = slot - 1 < (int)S.nparams[S.order[sp]];
+    if (S.active != nullptr && !S.active[S.order[sp]]) return;
+    if (S.fold_rows > 1 && slot > 0 && slot - 1 >= (int)S.nparams[S.order[sp]]) return;
+    const int64_t row_len = S.n_tiles * S.tile_pts;  // a multiple of 64: whole float4s
     const int64_t p0 = S.tile0 * S.tile_pts;
-    const int64_t p1 = min(p0 + S.n_tiles * S.tile_pts, S.count8);  // whole 8-point batches
-    const float* row = S.partials + (on ? q : 0) * S.n_tiles * S.tile_pts - p0 + l;
-    float acc = (on && !S.first) ? S.kstate[q * 8 + l] : 0.0f;
-    if (on) {
-        constexpr int kAhead = 32;
-        int64_t p = p0;
-        for (; p + 8 * kAhead <= p1; p += 8 * kAhead) {
-            float v[kAhead];
+    const int64_t len = min(p0 + row_len, S.count8) - p0;  // whole 8-point batches
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4* row = reinterpret_cast<const f4*>(S.partials + q * row_len);
+    const int l = lane & 7;
+    float acc = (!S.first && lane < 8) ? S.kstate[q * 8 + l] : 0.0f;
+    const int64_t n_chunks = (len + kFoldChunk - 1) / kFoldChunk;
+    f4 r[8];
+    auto load = [&](int64_t c) {
 #pragma unroll
-            for (int k = 0; k < kAhead; ++k) v[k] = row[p + 8 * k];
-#pragma unroll
-            for (int k = 0; k < kAhead; ++k) acc = acc + v[k];
+        for (int u = 0; u < 8; ++u) {
+            const int64_t i4 = c * (kFoldChunk / 4) + u * 64 + lane;
+            r[u] = i4 * 4 < len ? __builtin_nontemporal_load(&row[i4]) : (f4)(0.0f);
         }
-        for (; p < p1; p += 8) acc = acc + row[p];
+    };
+    if (n_chunks > 0) load(0);
+    // this lane's float4 u holds chunk points 256u + 4 lane + e (e = 0..3): chain
+    // 4 (lane & 1) + e, position 32u + (lane >> 1) along it
+    float* const wr = bt + 4 * (lane & 1) * kFoldRow + (lane >> 1);
+    const float* const rd = bt + l * kFoldRow;
+    for (int64_t c = 0; c < n_chunks; ++c) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            wr[0 * kFoldRow + 32 * u] = r[u].x;
+            wr[1 * kFoldRow + 32 * u] = r[u].y;
+            wr[2 * kFoldRow + 32 * u] = r[u].z;
+            wr[3 * kFoldRow + 32 * u] = r[u].w;
+        }
+        __syncthreads();
+        if (c + 1 < n_chunks) load(c + 1);  // in flight while this chunk is folded
+        const int kmax = (int)(min<int64_t>(kFoldChunk, len - c * kFoldChunk) >> 3);  // terms per chain
+        if (lane < 8) {
+            int k = 0;
+            for (; k + 16 <= kmax; k += 16) {
+                f4 v[4];
+#pragma unroll
+                for (int h = 0; h < 4; ++h) v[h] = *reinterpret_cast<const f4*>(rd + k + 4 * h);
+#pragma unroll
+                for (int h = 0; h < 4; ++h) acc = (((acc + v[h].x) + v[h].y) + v[h].z) + v[h].w;
+            }
+            for (; k < kmax; ++k) acc = acc + rd[k];
+        }
+        __syncthreads();  // the chunk's reads are done before the next chunk is stored
     }
     if (!S.last) {
-        if (on) S.kstate[q * 8 + l] = acc;
+        if (lane < 8) S.kstate[q * 8 + l] = acc;
         return;
     }
-    // f32x8::reduce_add within each group of eight lanes: (lo4 + hi4), then ((0 + 2) + (1 + 3))
-    const int g = lane & ~7;
-    const float a0 = __shfl(acc, g + 0, 64), a1 = __shfl(acc, g + 1, 64), a2 = __shfl(acc, g + 2, 64);
-    const float a3 = __shfl(acc, g + 3, 64), a4 = __shfl(acc, g + 4, 64), a5 = __shfl(acc, g + 5, 64);
-    const float a6 = __shfl(acc, g + 6, 64), a7 = __shfl(acc, g + 7, 64);
-    const float s0 = a0 + a4, s1 = a1 + a5, s2 = a2 + a6, s3 = a3 + a7;
-    const float sum = (s0 + s2) + (s1 + s3);
-    if (!on || l != 0) return;
+    // f32x8::reduce_add: (lo4 + hi4), then ((0 + 2) + (1 + 3))
+    const float sum = reduce_lanes8(acc);
+    if (lane != 0) return;
     if (S.fold_rows > 1) {  // gradient rows: the finalize kernel divides and checks
         S.gsum[q] = sum;
         return;
@@ -307,7 +321,7 @@ __global__ __launch_bounds__(256) void k_fold_terms(const SegTable T) {
 
 // Gradient terms: one thread per (position, slot); slot 0 = MSE, 1 + k = gradient k, each the
 // folded row sum divided by N; the gradient is reported only for a finite MSE
-// (evaluator.rs:214-216), as k_eval_grad_exact does.
+// (evaluator.rs:214-216).
 __global__ void k_finalize_grad_terms(const SegTable T) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t t0 = (int64_t)uni(blockIdx.x) * blockDim.x + ((int64_t)uni(threadIdx.x >> 6) << 6);
@@ -474,62 +488,6 @@ __global__ void k_finalize_grad(const SegTable T) {
     }
 }
 
-// Reference-order gradient sums of one (program, pass): the wave walks the tiles in order.
-__device__ __forceinline__ void grad_exact_one(const SegDesc& S, int sp, int pass, float* lds, float* gsp, int lane) {
-    const int prog = (int)uni((uint32_t)S.order[sp]);
-    const int np = (int)uni(S.nparams[prog]);
-    if ((S.active != nullptr && !S.active[prog]) || np == 0 || pass * kTan >= np) return;
-    const ProgDesc pd = S.progs[prog];
-    const uint32_t o = uni(pd.out_reg) & 31u;
-    const uint32_t k0 = (uint32_t)(pass * kTan);
-    float acc = 0.0f, gacc[kTan] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int64_t tile = 0; tile < S.n_tiles; ++tile) {
-        const int64_t base = tile * 64;
-        Files<1> R1;
-        R1.a = (v32f)(0.0f);
-        load_features<1>(R1, S.feat, S.F, S.npad, base, lane);
-        DualFiles<kTan> G;
-        G.v = R1.a;
-        G.a = G.b = G.c = G.d = (v32f)(0.0f);
-        run_dual<kTan>(G, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), k0, lds, gsp,
-                       lane);
-        const int64_t pt = base + lane;
-        const bool valid = pt < S.count8;
-        const v32f vf = G.v;
-        const float diff = vf[o] - S.target[pt];
-        acc = fold_batches(acc, valid ? diff * diff : 0.0f, lane);
-        const float two_diff = 2.0f * diff;
-        const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
-#pragma unroll
-        for (int j = 0; j < kTan; ++j) gacc[j] = fold_batches(gacc[j], valid ? two_diff * gf[j][o] : 0.0f, lane);
-    }
-    const float mse = reduce_lanes8(acc) / S.n_samples;
-    const bool fin = __builtin_isfinite(mse);
-    if (pass == 0 && lane == 0) S.mse_out[prog] = fin ? mse : FLT_MAX;
-#pragma unroll
-    for (int j = 0; j < kTan; ++j) {
-        const int k = (int)k0 + j;
-        const float g = reduce_lanes8(gacc[j]) / S.n_samples;
-        if (lane == 0 && k < 32) S.grad_out[(int64_t)prog * 32 + k] = (fin && k < np) ? g : 0.0f;
-    }
-}
-
-// One wave per (program, pass) of the small positions.
-__global__ __launch_bounds__(256) void k_eval_grad_exact(const SegTable T, int spill_slots) {
-    extern __shared__ float smem[];
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
-    const int si = find_seg_wave(T, wave);
-    const SegDesc& S = T.s[si];
-    const int64_t w = wave - S.first_wave;
-    const int sp = (int)(w / S.n_pass);
-    const int pass = (int)(w - (int64_t)sp * S.n_pass);
-    if (sp >= S.n_small) return;
-    if (S.live != nullptr && *S.live == 0) return;
-    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * (1 + kTan) * 64;
-    grad_exact_one(S, sp, pass, lds, nullptr, lane);
-}
-
 // ---------------------------------------------------------------------------------
 // Big programs (spill slots in device memory): bounded launches, grid-stride over work
 // items; wave w owns the device area gspill + w * gwave_floats for its whole life, so no
@@ -662,25 +620,6 @@ __global__ __launch_bounds__(256) void k_eval_grad_big(const SegTable T, int spi
     }
 }
 
-// Gradients, reference order: item = (big position j, pass), walking the tiles.
-__global__ __launch_bounds__(256) void k_eval_grad_exact_big(const SegTable T, int spill_slots, int64_t n_items,
-                                                             float* __restrict__ gspill, int64_t gwave_floats) {
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)uni(blockIdx.x) * kWaves + uni(threadIdx.x >> 6);
-    const int64_t n_waves = (int64_t)uni(gridDim.x) * kWaves;
-    float* lds = gspill + wave * gwave_floats;  // every spill slot in device memory
-    float* gsp = lds + kMaxSpillSlots * (1 + kTan) * 64;
-    for (int64_t it = wave; it < n_items; it += n_waves) {
-        const int si = find_seg_big(T, it);
-        const SegDesc& S = T.s[si];
-        const int64_t w = it - S.first_big;
-        const int j = (int)(w / S.n_pass);
-        const int pass = (int)(w - (int64_t)j * S.n_pass);
-        if (S.live != nullptr && *S.live == 0) continue;
-        grad_exact_one(S, S.n_small + j, pass, lds, gsp, lane);
-    }
-}
-
 // ---------------------------------------------------------------------------------
 // host side: segment tables and launches
 // ---------------------------------------------------------------------------------
@@ -738,7 +677,6 @@ static SegDesc make_seg(const EvalStep& st, int ppl, int group) {
     d.group = group;
     d.n_groups = (d.n_small + group - 1) / group;
     d.F = st.D.num_features;
-    d.n_pass = st.max_params > 0 ? (st.max_params + kTan - 1) / kTan : 1;
     d.n_samples = (float)st.D.num_samples;
     d.fold_rows = 1;
     return d;
@@ -800,7 +738,7 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
             big_items += d.n_tiles * (d.n_progs - d.n_small);
             if (d.n_progs > d.n_small) gslots = max(gslots, st.P.gslots);
             // finalize threads (whole waves per segment), or fold waves
-            threads += terms ? ((int64_t)d.n_progs + 7) / 8 : ((int64_t)d.n_progs + 63) / 64 * 64;
+            threads += terms ? (int64_t)d.n_progs : ((int64_t)d.n_progs + 63) / 64 * 64;
             spill = max(spill, lds_slots(st.P.spill_slots));
             T.s[T.n++] = d;
         }
@@ -829,10 +767,10 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
         }
         if (timer && base + cnt >= n) (void)hipEventRecord(timer->end, s);
         const int tb = 256;
-        if (terms)
-            hipLaunchKernelGGL(k_fold_terms, dim3((unsigned)((threads + kWaves - 1) / kWaves)), dim3(64 * kWaves), 0,
-                               s, T);
-        else
+        if (terms) {
+            if (threads > INT32_MAX) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(k_fold_terms, dim3((unsigned)threads), dim3(64), 0, s, T);
+        } else
             hipLaunchKernelGGL(k_finalize_mse, dim3((unsigned)((threads + tb - 1) / tb)), dim3(tb), 0, s, T);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -840,9 +778,8 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
     return hipSuccess;
 }
 
-hipError_t launch_grad_steps(const EvalStep* steps, int n, GradSum mode, hipStream_t s, const KernelTimer* timer,
+hipError_t launch_grad_steps(const EvalStep* steps, int n, bool terms, hipStream_t s, const KernelTimer* timer,
                              BigSpill big) {
-    const bool walk = mode == kGradWalk, terms = mode == kGradTerms;
     for (int base = 0; base < n; base += kMaxSegs) {
         const int cnt = min(kMaxSegs, n - base);
         SegTable T{};
@@ -851,7 +788,7 @@ hipError_t launch_grad_steps(const EvalStep* steps, int n, GradSum mode, hipStre
         for (int i = 0; i < cnt; ++i) {
             const EvalStep& st = steps[base + i];
             if (st.max_params <= 0 || st.P.n_progs <= 0) continue;
-            SegDesc d = make_seg(st, 1, walk ? 1 : grad_group(step_tiles(st, 1), st.P.n_progs));
+            SegDesc d = make_seg(st, 1, grad_group(step_tiles(st, 1), st.P.n_progs));
             if (d.n_tiles <= 0) continue;
             d.terms = terms ? 1 : 0;
             d.first_wave = waves;
@@ -859,13 +796,13 @@ hipError_t launch_grad_steps(const EvalStep* steps, int n, GradSum mode, hipStre
             d.first_big = big_items;
             d.first_fin = fin;
             const int nb = d.n_progs - d.n_small;
-            waves += walk ? (int64_t)d.n_small * d.n_pass : d.n_tiles * d.n_groups;
-            big_items += walk ? (int64_t)nb * d.n_pass : d.n_tiles * nb;
+            waves += d.n_tiles * d.n_groups;
+            big_items += d.n_tiles * nb;
             if (nb > 0) gslots = max(gslots, st.P.gslots);
             if (terms) {
                 d.fold_rows = d.slots;
                 d.gsum = st.partials + (size_t)d.n_tiles * 64 * d.n_progs * d.slots;
-                threads += ((int64_t)d.n_progs * d.slots + 7) / 8;                // fold waves
+                threads += (int64_t)d.n_progs * d.slots;                           // fold waves
                 fin += ((int64_t)d.n_progs * d.slots + 63) / 64 * 64;               // finalize threads
             } else {
                 threads += ((int64_t)d.n_progs * d.slots + 63) / 64 * 64;  // whole waves per segment
@@ -879,23 +816,19 @@ hipError_t launch_grad_steps(const EvalStep* steps, int n, GradSum mode, hipStre
         const size_t lds = eval_grad_lds_bytes(spill);
         if (timer && base == 0) (void)hipEventRecord(timer->begin, s);
         if (waves > 0) {
-            if (walk)
-                hipLaunchKernelGGL(k_eval_grad_exact, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
-            else
-                hipLaunchKernelGGL(k_eval_grad, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
+            hipLaunchKernelGGL(k_eval_grad, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
         if (big_items > 0) {
-            const hipError_t e = walk ? launch_big(k_eval_grad_exact_big, T, big_items, gslots, 1 + kTan, spill, big, s)
-                                      : launch_big(k_eval_grad_big, T, big_items, gslots, 1 + kTan, spill, big, s);
+            const hipError_t e = launch_big(k_eval_grad_big, T, big_items, gslots, 1 + kTan, spill, big, s);
             if (e != hipSuccess) return e;
         }
         if (timer && base + cnt >= n) (void)hipEventRecord(timer->end, s);
         const int tb = 256;
         if (terms) {
-            hipLaunchKernelGGL(k_fold_terms, dim3((unsigned)((threads + kWaves - 1) / kWaves)), dim3(64 * kWaves), 0,
-                               s, T);
+            if (threads > INT32_MAX) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(k_fold_terms, dim3((unsigned)threads), dim3(64), 0, s, T);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
             bool last = false;
@@ -904,7 +837,7 @@ hipError_t launch_grad_steps(const EvalStep* steps, int n, GradSum mode, hipStre
                 hipLaunchKernelGGL(k_finalize_grad_terms, dim3((unsigned)((fin + tb - 1) / tb)), dim3(tb), 0, s, T);
                 if ((e = hipGetLastError()) != hipSuccess) return e;
             }
-        } else if (!walk) {
+        } else {
             hipLaunchKernelGGL(k_finalize_grad, dim3((unsigned)((threads + tb - 1) / tb)), dim3(tb), 0, s, T);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;

@@ -364,11 +364,10 @@ def test_default_summation_is_reference_order():
         ctx.close()
 
 
-def test_reference_gradients_walk_and_terms_agree(gpu_ctx):
-    """Reference-order gradients take one wave per (program, pass) walking the tiles when a
-    step has at least 4,096 such walks, else tile-parallel per-point terms folded in the same
-    order: a batch of 4,200 programs (walks) and its first 160 programs alone (terms) give the
-    same bits, equal to the oracle on sampled programs."""
+def test_reference_gradients_large_batch_and_subset_agree(gpu_ctx):
+    """Reference-order gradients of a 4,200-program batch (per-point terms of every program
+    and tangent slot, folded per row) and of its first 160 programs alone give the same bits,
+    equal to the oracle on sampled programs."""
     X, y, t = W.data_c2(1500, seed=95)
     progs = W.gen_c2(4200, seed=96)
     ds, ods = make(gpu_ctx, X, y, t)
@@ -389,3 +388,29 @@ def test_reference_gradients_walk_and_terms_agree(gpu_ctx):
     for i in np.random.default_rng(5).choice(len(progs), 12, replace=False):
         om, og = O.compute_mse_with_gradient(progs[i], ods)
         assert same_bits(mw[i], om)[0] and same_bits(gw[i], og)[0], i
+
+
+@pytest.mark.parametrize("limit", [None, 700 * 1024])
+def test_reference_order_few_programs_many_points(gpu_ctx, limit):
+    """Few programs over many points: each program's eight reference accumulators run
+    over 37,500 terms in chunks of 2,048, and with a small partials arena over tile ranges
+    that cut through the chunks, with the accumulators carried between ranges.  MSEs and
+    gradients equal the oracle's bit for bit."""
+    X, y, t = W.data_c0(300_003, seed=61)
+    progs = W.gen_c0(6, seed=62)
+    ds, ods = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, progs)
+    gpu_ctx.set_summation(True)
+    if limit:
+        gpu_ctx.set_partials_limit(limit)
+    try:
+        mse = api.compute_mse(gpu_ctx, P, ds)
+        m2, g2 = api.compute_mse_with_gradient(gpu_ctx, P, ds)
+    finally:
+        gpu_ctx.set_partials_limit(4 << 30)
+        gpu_ctx.set_summation(False)
+    for i, p in enumerate(progs):
+        assert mse[i] == O.compute_mse(p, ods), i
+        rm, rg = O.compute_mse_with_gradient(p, ods)
+        assert m2[i] == rm, (i, m2[i], rm)
+        assert same_bits(g2[i], rg)[0], i
