@@ -322,6 +322,107 @@ RSR_API int rsr_assign_rank_and_crowding(int n, const float* fitness, const uint
 RSR_API int rsr_tournament(int n, const uint32_t* rank, const float* crowding, int k, const uint32_t* draws,
                            uint32_t* winner_out);
 
+/* ---- device-side variation (operators/{generator,mutation,crossover}.rs) ----
+ * One offspring per task, all produced in one launch (one GPU thread each) from parents
+ * given in the rsr_programs_create layout.  Each task draws from its own Xoshiro256++
+ * stream keyed by `seed`, consuming draws in the order of the reference operator:
+ *   GENERATE   generate_random_ast(type, depth) (generator.rs:9-148), depth <= 5
+ *   CROSSOVER  crossover(a, b, max_size) (crossover.rs:7-65)
+ *   POINT      point_mutation(a) (mutation.rs:11-57)
+ *   CONSTANT   constant_perturbation(a) (mutation.rs:59-116)
+ *   SUBTREE    subtree_mutation(a, max_size, depth) (mutation.rs:118-160)
+ *   MUTATE     one of POINT / CONSTANT / SUBTREE drawn uniformly first (island.rs:156-179)
+ * The offspring are compiled programs (Program::from_nodes order of constants), before
+ * simplify_ast; at most 384 nodes each.  kind_out (optional): the operator applied. */
+#define RSR_VARY_GENERATE 0
+#define RSR_VARY_CROSSOVER 1
+#define RSR_VARY_POINT 2
+#define RSR_VARY_CONSTANT 3
+#define RSR_VARY_SUBTREE 4
+#define RSR_VARY_MUTATE 5
+typedef struct rsr_vary_task {
+    uint8_t kind;     /* RSR_VARY_* */
+    uint8_t type;     /* GENERATE: the root's ValueType */
+    uint8_t depth;    /* GENERATE: max depth; SUBTREE / MUTATE: mutation_max_depth */
+    uint8_t reserved;
+    int32_t a, b;     /* parent indices (b: CROSSOVER only) */
+    int32_t max_size; /* CROSSOVER / SUBTREE / MUTATE: max_tree_size */
+    uint64_t seed;    /* the task's stream */
+} rsr_vary_task;
+/* The variation streams on the host: raw Xoshiro256++ outputs from a state; the state
+ * SplitMix64 seeding (rand_xoshiro seed_from_u64) gives a seed; n samples of the stream
+ * seeded with `seed`: kind 0 f32 in [0, 1), 1 bool (0 / 1), 2 uniform integer below
+ * `bound`, 3 f32 uniform in [a, b). */
+RSR_API int rsr_xoshiro256pp(const uint64_t* state, int n, uint64_t* out);
+RSR_API int rsr_xoshiro_seed(uint64_t seed, uint64_t* state_out);
+RSR_API int rsr_vary_samples(uint64_t seed, int kind, int n, float a, float b, uint32_t bound, float* out);
+RSR_API int rsr_vary(rsr_context* ctx, int n_tasks, const rsr_vary_task* tasks, int n_parents, const uint8_t* ops,
+                     const uint16_t* args, const int64_t* code_offsets, const uint8_t* const_types,
+                     const int64_t* const_offsets, const float* const_values, const int64_t* value_offsets,
+                     int n_vars, const uint8_t* var_types, const uint8_t* var_indices, int n_allowed,
+                     const uint8_t* allowed_ops, int64_t code_capacity, int64_t const_capacity,
+                     int64_t value_capacity, uint8_t* ops_out, uint16_t* args_out, int64_t* code_offsets_out,
+                     uint8_t* const_types_out, int64_t* const_offsets_out, float* const_values_out,
+                     int64_t* value_offsets_out, uint8_t* kind_out);
+
+/* ---- the search loop (src/engine/search/{island,engine}.rs) as a client of the evaluator ----
+ * Engine::new / Engine::run with the islands of Island::new / step_generation /
+ * fill_next_generation / evaluate_buffer / nuke, migration and the halls of fame.  One
+ * generation of every island runs as merged device calls (variation launches, batched
+ * evaluations, rsr_islands_evaluate).  Each island's sequential decisions (crossover or
+ * mutation, tournaments, the evaluation step's draws) come from its Xoshiro256++ stream
+ * seeded with seed + island (reference: 42 + i); the variation operators draw from
+ * per-offspring streams (see rsr_vary).  The dataset must outlive the engine. */
+typedef struct rsr_engine rsr_engine;
+typedef struct rsr_engine_config {  /* Config (config.rs:11-66); defaults from rsr_engine_default_config */
+    int32_t num_islands, island_size, max_generations;
+    float crossover_rate;
+    int32_t tournament_size, migration_interval;
+    float parsimony_penalty, opt_prob;
+    int32_t opt_iterations, final_opt_iterations, stagnation_threshold;
+    float target_mse, min_improvement, random_injection_rate;
+    int32_t min_random_injection, max_tree_size, mutation_max_depth, mutation_cycles, mini_batch_size;
+    uint64_t seed;
+} rsr_engine_config;
+typedef struct rsr_engine_info {
+    int64_t generation;    /* generations run */
+    int64_t nukes;         /* island nukes so far */
+    int32_t best_island;   /* get_global_best's island */
+    float best_fitness;    /* its best individual's fitness */
+    float best_mse;        /* that individual's MSE on the dataset after the last generation */
+    int32_t target_reached, finished, hof_size;
+} rsr_engine_info;
+RSR_API void rsr_engine_default_config(rsr_engine_config* cfg);
+/* variables: (ValueType, feature index) pairs, the registry of the reference; allowed_ops:
+ * the operators (RefOp codes) the search may use (Strategy::get_allowed_operators). */
+RSR_API int rsr_engine_create(rsr_context* ctx, const rsr_engine_config* cfg, const rsr_dataset* dataset,
+                              int n_vars, const uint8_t* var_types, const uint8_t* var_indices, int n_allowed,
+                              const uint8_t* allowed_ops, rsr_engine** out);
+/* One iteration of Engine::run's loop: every island's step_generation, the halls of fame
+ * merged, the global best rescored on the dataset (target check), migration every
+ * migration_interval generations. */
+RSR_API int rsr_engine_step(rsr_engine* e, int* target_reached);
+/* Engine::run: steps until max_generations or the target, then the global best fitted
+ * with final_opt_iterations, snapped, simplified and rescored as island 0's best. */
+RSR_API int rsr_engine_run(rsr_engine* e, int* generations_out);
+RSR_API int rsr_engine_get_info(const rsr_engine* e, rsr_engine_info* info);
+/* Programs of the engine in the rsr_programs_create layout, with one value and one count
+ * each: POPULATION (island: fitness, age), ISLAND_BEST (island: fitness, age), GLOBAL_BEST
+ * (fitness, age), HALL_OF_FAME (global: MSE, complexity), ISLAND_HOF (island: MSE,
+ * complexity), PARETO (get_pareto_front, engine.rs:131-155: MSE, complexity). */
+#define RSR_ENGINE_POPULATION 0
+#define RSR_ENGINE_ISLAND_BEST 1
+#define RSR_ENGINE_GLOBAL_BEST 2
+#define RSR_ENGINE_HALL_OF_FAME 3
+#define RSR_ENGINE_ISLAND_HOF 4
+#define RSR_ENGINE_PARETO 5
+RSR_API int rsr_engine_programs(const rsr_engine* e, int which, int island, int64_t program_capacity,
+                                int64_t code_capacity, int64_t const_capacity, int64_t value_capacity, int* n_out,
+                                uint8_t* ops_out, uint16_t* args_out, int64_t* code_offsets_out,
+                                uint8_t* const_types_out, int64_t* const_offsets_out, float* const_values_out,
+                                int64_t* value_offsets_out, float* value_out, int64_t* count_out);
+RSR_API int rsr_engine_destroy(rsr_engine* e);
+
 /* ---------------------------------------------------------------------------------
  * Serving (the "async submit/poll" of the boundary; the reference's callers are rayon
  * workers calling the evaluator in-process, engine.rs:52, island.rs:150-238).

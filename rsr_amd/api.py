@@ -831,3 +831,162 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---- device-side variation and the search loop ----------------------------------
+VARY_GENERATE, VARY_CROSSOVER, VARY_POINT, VARY_CONSTANT, VARY_SUBTREE, VARY_MUTATE = range(6)
+
+
+class VaryTask(C.Structure):
+    _fields_ = [("kind", C.c_uint8), ("type", C.c_uint8), ("depth", C.c_uint8), ("reserved", C.c_uint8),
+                ("a", C.c_int32), ("b", C.c_int32), ("max_size", C.c_int32), ("seed", C.c_uint64)]
+
+
+def vary(ctx: Context, tasks: Sequence[tuple], parents: Sequence[Program], variables: Sequence[tuple],
+         allowed_ops: Sequence[int]):
+    """rsr_vary: one offspring per task on the device.
+
+    tasks: (kind, seed, a=-1, b=-1, type=0, depth=0, max_size=32) tuples; variables: (ValueType,
+    feature index) pairs; allowed_ops: RefOp codes.  Returns (offspring, applied kinds)."""
+    l = lib()
+    fn = l.rsr_vary
+    fn.restype = C.c_int
+    fn.argtypes = [_P, C.c_int, C.POINTER(VaryTask), C.c_int, _u8p, _u16p, _i64p, _u8p, _i64p, _f32p, _i64p,
+                   C.c_int, _u8p, _u8p, C.c_int, _u8p, C.c_int64, C.c_int64, C.c_int64, _u8p, _u16p, _i64p, _u8p,
+                   _i64p, _f32p, _i64p, _u8p]
+    n = len(tasks)
+    arr = (VaryTask * max(n, 1))()
+    for i, t in enumerate(tasks):
+        kind, seed = int(t[0]), int(t[1])
+        a = int(t[2]) if len(t) > 2 else -1
+        b = int(t[3]) if len(t) > 3 else -1
+        vt = int(t[4]) if len(t) > 4 else 0
+        depth = int(t[5]) if len(t) > 5 else 0
+        ms = int(t[6]) if len(t) > 6 else 32
+        arr[i] = VaryTask(kind, vt, depth, 0, a, b, ms, seed)
+    ops, args, co, ct, cof, cv, vo = pack_programs(list(parents))
+    vt_ = np.array([v[0] for v in variables] or [0], np.uint8)
+    vi_ = np.array([v[1] for v in variables] or [0], np.uint8)
+    al = np.array(list(allowed_ops) or [0], np.uint8)
+    cap = 384 * max(n, 1)
+    ops_o, args_o = np.zeros(cap, np.uint8), np.zeros(cap, np.uint16)
+    ct_o, cv_o = np.zeros(cap, np.uint8), np.zeros(9 * cap, np.float32)
+    co_o, cof_o, vo_o = np.zeros(n + 1, np.int64), np.zeros(n + 1, np.int64), np.zeros(n + 1, np.int64)
+    kinds = np.zeros(max(n, 1), np.uint8)
+    pad = lambda x, t: x if x.size else np.zeros(1, t)
+    _check(fn(ctx.h, n, arr, len(parents), _ptr(pad(ops, np.uint8), _u8p), _ptr(pad(args, np.uint16), _u16p),
+              _ptr(co, _i64p), _ptr(pad(ct, np.uint8), _u8p), _ptr(cof, _i64p), _ptr(pad(cv, np.float32), _f32p),
+              _ptr(vo, _i64p), len(variables), _ptr(vt_, _u8p), _ptr(vi_, _u8p), len(allowed_ops), _ptr(al, _u8p),
+              cap, cap, 9 * cap, _ptr(ops_o, _u8p), _ptr(args_o, _u16p), _ptr(co_o, _i64p), _ptr(ct_o, _u8p),
+              _ptr(cof_o, _i64p), _ptr(cv_o, _f32p), _ptr(vo_o, _i64p), _ptr(kinds, _u8p)))
+    return unpack_programs(ops_o, args_o, co_o, ct_o, cof_o, cv_o, vo_o), kinds[:n]
+
+
+class EngineConfig(C.Structure):
+    _fields_ = [("num_islands", C.c_int32), ("island_size", C.c_int32), ("max_generations", C.c_int32),
+                ("crossover_rate", C.c_float), ("tournament_size", C.c_int32), ("migration_interval", C.c_int32),
+                ("parsimony_penalty", C.c_float), ("opt_prob", C.c_float), ("opt_iterations", C.c_int32),
+                ("final_opt_iterations", C.c_int32), ("stagnation_threshold", C.c_int32),
+                ("target_mse", C.c_float), ("min_improvement", C.c_float),
+                ("random_injection_rate", C.c_float), ("min_random_injection", C.c_int32),
+                ("max_tree_size", C.c_int32), ("mutation_max_depth", C.c_int32), ("mutation_cycles", C.c_int32),
+                ("mini_batch_size", C.c_int32), ("seed", C.c_uint64)]
+
+
+class EngineInfo(C.Structure):
+    _fields_ = [("generation", C.c_int64), ("nukes", C.c_int64), ("best_island", C.c_int32),
+                ("best_fitness", C.c_float), ("best_mse", C.c_float), ("target_reached", C.c_int32),
+                ("finished", C.c_int32), ("hof_size", C.c_int32)]
+
+
+ENGINE_POPULATION, ENGINE_ISLAND_BEST, ENGINE_GLOBAL_BEST, ENGINE_HALL_OF_FAME, ENGINE_ISLAND_HOF, ENGINE_PARETO = \
+    range(6)
+
+
+def engine_default_config() -> EngineConfig:
+    l = lib()
+    l.rsr_engine_default_config.restype = None
+    l.rsr_engine_default_config.argtypes = [C.POINTER(EngineConfig)]
+    c = EngineConfig()
+    l.rsr_engine_default_config(C.byref(c))
+    return c
+
+
+class Engine:
+    """The search loop (rsr_engine_*): Engine::new / run over the given dataset."""
+
+    def __init__(self, ctx: Context, dataset: Dataset, variables: Sequence[tuple], allowed_ops: Sequence[int],
+                 config: Optional[EngineConfig] = None, **overrides):
+        l = lib()
+        self._l = l
+        cfg = config if config is not None else engine_default_config()
+        for k, v in overrides.items():
+            setattr(cfg, k, v)
+        self.config = cfg
+        l.rsr_engine_create.restype = C.c_int
+        l.rsr_engine_create.argtypes = [_P, C.POINTER(EngineConfig), _P, C.c_int, _u8p, _u8p, C.c_int, _u8p,
+                                        C.POINTER(_P)]
+        for name, args in (("rsr_engine_step", [_P, C.POINTER(C.c_int)]),
+                           ("rsr_engine_run", [_P, C.POINTER(C.c_int)]),
+                           ("rsr_engine_get_info", [_P, C.POINTER(EngineInfo)]),
+                           ("rsr_engine_destroy", [_P]),
+                           ("rsr_engine_programs", [_P, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                                                    C.POINTER(C.c_int), _u8p, _u16p, _i64p, _u8p, _i64p, _f32p,
+                                                    _i64p, _f32p, _i64p])):
+            f = getattr(l, name)
+            f.restype = C.c_int
+            f.argtypes = args
+        vt_ = np.array([v[0] for v in variables] or [0], np.uint8)
+        vi_ = np.array([v[1] for v in variables] or [0], np.uint8)
+        al = np.array(list(allowed_ops) or [0], np.uint8)
+        h = _P()
+        _check(l.rsr_engine_create(ctx.h, C.byref(cfg), dataset.h, len(variables), _ptr(vt_, _u8p), _ptr(vi_, _u8p),
+                                   len(allowed_ops), _ptr(al, _u8p), C.byref(h)))
+        self.h = h
+        self.dataset = dataset  # kept alive with the engine
+
+    def step(self) -> bool:
+        t = C.c_int(0)
+        _check(self._l.rsr_engine_step(self.h, C.byref(t)))
+        return bool(t.value)
+
+    def run(self) -> int:
+        g = C.c_int(0)
+        _check(self._l.rsr_engine_run(self.h, C.byref(g)))
+        return g.value
+
+    def info(self) -> EngineInfo:
+        i = EngineInfo()
+        _check(self._l.rsr_engine_get_info(self.h, C.byref(i)))
+        return i
+
+    def programs(self, which: int, island: int = 0):
+        """(programs, values, counts) of rsr_engine_programs."""
+        n = C.c_int(0)
+        fn = self._l.rsr_engine_programs
+        nul = lambda t: C.cast(None, t)
+        rc = fn(self.h, which, island, 0, 0, 0, 0, C.byref(n), nul(_u8p), nul(_u16p), nul(_i64p), nul(_u8p),
+                nul(_i64p), nul(_f32p), nul(_i64p), nul(_f32p), nul(_i64p))
+        if rc not in (0, RSR_E_INVALID_ARG):
+            _check(rc)
+        cnt = n.value
+        cap = 384 * max(cnt, 1)
+        ops_o, args_o = np.zeros(cap, np.uint8), np.zeros(cap, np.uint16)
+        ct_o, cv_o = np.zeros(cap, np.uint8), np.zeros(9 * cap, np.float32)
+        co_o, cof_o, vo_o = np.zeros(cnt + 1, np.int64), np.zeros(cnt + 1, np.int64), np.zeros(cnt + 1, np.int64)
+        val, num = np.zeros(max(cnt, 1), np.float32), np.zeros(max(cnt, 1), np.int64)
+        _check(fn(self.h, which, island, cnt, cap, cap, 9 * cap, C.byref(n), _ptr(ops_o, _u8p), _ptr(args_o, _u16p),
+                  _ptr(co_o, _i64p), _ptr(ct_o, _u8p), _ptr(cof_o, _i64p), _ptr(cv_o, _f32p), _ptr(vo_o, _i64p),
+                  _ptr(val, _f32p), _ptr(num, _i64p)))
+        return unpack_programs(ops_o, args_o, co_o, ct_o, cof_o, cv_o, vo_o), val[:cnt], num[:cnt]
+
+    def close(self):
+        if self.h:
+            self._l.rsr_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

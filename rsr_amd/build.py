@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librsr_amd.so")
-SOURCES = ["kernels.hip", "lower.cpp", "runtime.cpp", "hostcheck.hip", "optim.hip", "jobs.cpp", "fit.cpp", "serve.cpp", "select.cpp", "simplify.cpp", "island.cpp", "ingest.cpp", "noalloc.hip"]
+SOURCES = ["kernels.hip", "lower.cpp", "runtime.cpp", "hostcheck.hip", "optim.hip", "jobs.cpp", "fit.cpp", "serve.cpp", "select.cpp", "simplify.cpp", "island.cpp", "ingest.cpp", "noalloc.hip", "variation.hip", "engine.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [

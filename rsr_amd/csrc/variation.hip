@@ -1,0 +1,308 @@
+// variation.hip -- device-side variation: the search loop's generator, mutations and
+// crossover (operators/{generator,mutation,crossover}.rs) run on the GPU, one thread per
+// offspring, over parents resident in device memory.  Each offspring draws from its own
+// Xoshiro256++ stream keyed by its task's seed, so every offspring of a generation is
+// produced at once and the result does not depend on the launch shape.
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/rsr_amd.h"
+#include "runtime_internal.h"
+#include "variation.h"
+#include "variation_host.h"
+
+#define HIP_TRY(expr, what)                               \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return hip_error(e_, what); \
+    } while (0)
+
+namespace rsr {
+namespace {
+
+struct VarySet {  // device views of one launch
+    const rsr_vary_task* tasks;
+    int n;
+    // parents (rsr_programs_create layout)
+    const uint8_t* ops;
+    const uint16_t* args;
+    const int64_t* co;
+    const uint8_t* ct;
+    const int64_t* cof;
+    const float* cv;
+    const int64_t* vo;
+    // registry and operators
+    const uint8_t* var_type;
+    const uint8_t* var_idx;
+    int n_vars;
+    const uint8_t* allowed;
+    int n_allowed;
+    // fixed-capacity output slots
+    uint8_t* o_ops;
+    uint16_t* o_args;
+    uint8_t* o_ct;
+    float* o_cv;
+    int32_t* o_meta;  // [n][4]: len, constants, payload floats, status
+};
+
+__device__ VProg parent_view(const VarySet& S, int i) {
+    VProg p;
+    const int64_t c0 = S.co[i], k0 = S.cof[i];
+    p.op = S.ops + c0;
+    p.arg = S.args + c0;
+    p.ctype = S.ct + k0;
+    p.cval = S.cv + S.vo[i];
+    p.len = (int)(S.co[i + 1] - c0);
+    p.nconst = (int)(S.cof[i + 1] - k0);
+    return p;
+}
+
+__global__ __launch_bounds__(64) void k_vary(const VarySet S) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.n) return;
+    const rsr_vary_task task = S.tasks[t];
+    VRng r = vrng_seed(task.seed);
+    VOut o;
+    o.op = S.o_ops + (int64_t)t * kVMaxNodes;
+    o.arg = S.o_args + (int64_t)t * kVMaxNodes;
+    o.ctype = S.o_ct + (int64_t)t * kVMaxNodes;
+    o.cval = S.o_cv + (int64_t)t * kVMaxVals;
+    o.len = o.nconst = o.nval = 0;
+    o.overflow = false;
+    VEnv env;
+    env.var_type = S.var_type;
+    env.var_idx = S.var_idx;
+    env.n_vars = S.n_vars;
+    env.ops = S.allowed;
+    env.n_ops = S.n_allowed;
+    int kind = task.kind;
+    if (kind == RSR_VARY_MUTATE) {  // island.rs:156-179: point, constant or subtree, uniformly
+        const uint32_t m = vrng_below(r, 3);
+        kind = m == 0 ? RSR_VARY_POINT : m == 1 ? RSR_VARY_CONSTANT : RSR_VARY_SUBTREE;
+    }
+    switch (kind) {
+        case RSR_VARY_GENERATE: v_generate(o, r, env, task.type, task.depth); break;
+        case RSR_VARY_CROSSOVER:
+            v_crossover(o, r, parent_view(S, task.a), parent_view(S, task.b), task.max_size);
+            break;
+        case RSR_VARY_POINT: v_point_mutation(o, r, env, parent_view(S, task.a)); break;
+        case RSR_VARY_CONSTANT: v_constant_perturbation(o, r, parent_view(S, task.a)); break;
+        default: v_subtree_mutation(o, r, env, parent_view(S, task.a), task.max_size, task.depth); break;
+    }
+    int32_t* meta = S.o_meta + 4 * (int64_t)t;
+    meta[0] = o.len;
+    meta[1] = o.nconst;
+    meta[2] = o.nval;
+    meta[3] = o.overflow ? 1 : (kind << 8);
+}
+
+bool needs_parent(int kind) { return kind != RSR_VARY_GENERATE; }
+
+}  // namespace
+
+int vary(rsr_context* ctx, const std::vector<rsr_vary_task>& tasks, const Wire& parents, const VaryEnv& env,
+         Wire& out, std::vector<uint8_t>* applied) {
+    out.clear();
+    const int n = (int)tasks.size();
+    if (n == 0) return RSR_OK;
+    const int np = parents.n();
+    for (int t = 0; t < n; ++t) {
+        const rsr_vary_task& k = tasks[t];
+        if (k.kind > RSR_VARY_MUTATE) return set_error(RSR_E_INVALID_ARG, "task %d: unknown kind %d", t, k.kind);
+        if (needs_parent(k.kind) && (k.a < 0 || k.a >= np)) return set_error(RSR_E_INVALID_ARG, "task %d: parent a", t);
+        if (k.kind == RSR_VARY_CROSSOVER && (k.b < 0 || k.b >= np))
+            return set_error(RSR_E_INVALID_ARG, "task %d: parent b", t);
+        if ((k.kind == RSR_VARY_GENERATE || k.kind == RSR_VARY_SUBTREE || k.kind == RSR_VARY_MUTATE) &&
+            (k.depth < 0 || k.depth > 5))
+            return set_error(RSR_E_INVALID_ARG, "task %d: depth %d outside [0, 5]", t, k.depth);
+        if (k.kind == RSR_VARY_GENERATE && (k.type == VtBool || k.type == VtInt || k.type > VtMat3))
+            return set_error(RSR_E_INVALID_ARG, "task %d: target type %d", t, k.type);
+    }
+    // the kernel walks parents by their LoadConst nodes and payload sizes: every parent must
+    // be well formed (opcodes, one constant per LoadConst node, payloads of its types)
+    for (int i = 0; i < np; ++i) {
+        if (parents.len(i) > kVMaxNodes)
+            return set_error(RSR_E_INVALID_ARG, "parent %d has %lld nodes, above %d", i, (long long)parents.len(i),
+                             kVMaxNodes);
+        int64_t nk = 0, nv = 0;
+        for (int64_t k = parents.co[i]; k < parents.co[i + 1]; ++k) {
+            if (parents.ops[k] > NegF) return set_error(RSR_E_INVALID_ARG, "parent %d: opcode %d", i, parents.ops[k]);
+            nk += v_is_const(parents.ops[k]);
+        }
+        if (nk != parents.cof[i + 1] - parents.cof[i])
+            return set_error(RSR_E_INVALID_ARG, "parent %d: %lld constant nodes, %lld constants", i, (long long)nk,
+                             (long long)(parents.cof[i + 1] - parents.cof[i]));
+        for (int64_t k = parents.cof[i]; k < parents.cof[i + 1]; ++k) {
+            if (parents.ct[k] > VtMat3) return set_error(RSR_E_INVALID_ARG, "parent %d: constant type", i);
+            nv += v_payload(parents.ct[k]);
+        }
+        if (nv != parents.vo[i + 1] - parents.vo[i]) return set_error(RSR_E_INVALID_ARG, "parent %d: payload size", i);
+    }
+    for (size_t i = 0; i < env.ops.size(); ++i)
+        if (env.ops[i] < AddF || env.ops[i] >= NegF)
+            return set_error(RSR_E_INVALID_ARG, "allowed operator %d is not an operator", env.ops[i]);
+    for (size_t i = 0; i < env.var_type.size(); ++i)
+        if (env.var_type[i] > VtMat3) return set_error(RSR_E_INVALID_ARG, "variable %zu: type", i);
+
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(e, "hipSetDevice");
+    // one staging buffer: inputs, then the output slots
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t b_tasks = al(n * sizeof(rsr_vary_task)), b_ops = al(parents.ops.size() + 1),
+                 b_args = al(2 * parents.args.size() + 2), b_co = al(8 * parents.co.size()),
+                 b_ct = al(parents.ct.size() + 1), b_cof = al(8 * parents.cof.size()),
+                 b_cv = al(4 * parents.cv.size() + 4), b_vo = al(8 * parents.vo.size()),
+                 b_vt = al(env.var_type.size() + 1), b_vi = al(env.var_idx.size() + 1), b_al = al(env.ops.size() + 1),
+                 b_oops = al((size_t)n * kVMaxNodes), b_oargs = al((size_t)n * kVMaxNodes * 2),
+                 b_oct = al((size_t)n * kVMaxNodes), b_ocv = al((size_t)n * kVMaxVals * 4), b_meta = al((size_t)n * 16);
+    const size_t in_bytes = b_tasks + b_ops + b_args + b_co + b_ct + b_cof + b_cv + b_vo + b_vt + b_vi + b_al;
+    const size_t total = in_bytes + b_oops + b_oargs + b_oct + b_ocv + b_meta;
+    DevBuf buf;
+    int rc = buf.alloc(ctx, total);
+    if (rc) return rc;
+    std::vector<char> host(in_bytes, 0);
+    size_t off = 0;
+    auto put = [&](const void* src, size_t bytes, size_t span) {
+        const size_t at = off;
+        if (bytes) memcpy(host.data() + at, src, bytes);
+        off += span;
+        return (char*)buf.ptr + at;
+    };
+    VarySet S;
+    S.tasks = (const rsr_vary_task*)put(tasks.data(), n * sizeof(rsr_vary_task), b_tasks);
+    S.n = n;
+    S.ops = (const uint8_t*)put(parents.ops.data(), parents.ops.size(), b_ops);
+    S.args = (const uint16_t*)put(parents.args.data(), 2 * parents.args.size(), b_args);
+    S.co = (const int64_t*)put(parents.co.data(), 8 * parents.co.size(), b_co);
+    S.ct = (const uint8_t*)put(parents.ct.data(), parents.ct.size(), b_ct);
+    S.cof = (const int64_t*)put(parents.cof.data(), 8 * parents.cof.size(), b_cof);
+    S.cv = (const float*)put(parents.cv.data(), 4 * parents.cv.size(), b_cv);
+    S.vo = (const int64_t*)put(parents.vo.data(), 8 * parents.vo.size(), b_vo);
+    S.var_type = (const uint8_t*)put(env.var_type.data(), env.var_type.size(), b_vt);
+    S.var_idx = (const uint8_t*)put(env.var_idx.data(), env.var_idx.size(), b_vi);
+    S.n_vars = (int)env.var_type.size();
+    S.allowed = (const uint8_t*)put(env.ops.data(), env.ops.size(), b_al);
+    S.n_allowed = (int)env.ops.size();
+    char* o = (char*)buf.ptr + in_bytes;
+    S.o_ops = (uint8_t*)o;
+    S.o_args = (uint16_t*)(o + b_oops);
+    S.o_ct = (uint8_t*)(o + b_oops + b_oargs);
+    S.o_cv = (float*)(o + b_oops + b_oargs + b_oct);
+    S.o_meta = (int32_t*)(o + b_oops + b_oargs + b_oct + b_ocv);
+    HIP_TRY(hipMemcpyAsync(buf.ptr, host.data(), in_bytes, hipMemcpyHostToDevice, ctx->stream), "variation upload");
+    hipLaunchKernelGGL(k_vary, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, ctx->stream, S);
+    HIP_TRY(hipGetLastError(), "variation launch");
+    std::vector<int32_t> meta((size_t)n * 4);
+    HIP_TRY(hipMemcpyAsync(meta.data(), S.o_meta, meta.size() * 4, hipMemcpyDeviceToHost, ctx->stream),
+            "variation meta");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "variation");
+    // the slots' used prefixes (one copy each of the code, constants and payload areas)
+    std::vector<uint8_t> ops((size_t)n * kVMaxNodes), ct((size_t)n * kVMaxNodes);
+    std::vector<uint16_t> args((size_t)n * kVMaxNodes);
+    std::vector<float> cv((size_t)n * kVMaxVals);
+    HIP_TRY(hipMemcpyAsync(ops.data(), S.o_ops, ops.size(), hipMemcpyDeviceToHost, ctx->stream), "variation code");
+    HIP_TRY(hipMemcpyAsync(args.data(), S.o_args, 2 * args.size(), hipMemcpyDeviceToHost, ctx->stream),
+            "variation operands");
+    HIP_TRY(hipMemcpyAsync(ct.data(), S.o_ct, ct.size(), hipMemcpyDeviceToHost, ctx->stream), "variation types");
+    HIP_TRY(hipMemcpyAsync(cv.data(), S.o_cv, 4 * cv.size(), hipMemcpyDeviceToHost, ctx->stream), "variation values");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "variation download");
+    if (applied) applied->assign((size_t)n, 0);
+    for (int t = 0; t < n; ++t) {
+        const int32_t* m = meta.data() + 4 * (size_t)t;
+        if (m[3] & 1) return set_error(RSR_E_UNSUPPORTED, "task %d: offspring above %d nodes", t, kVMaxNodes);
+        const size_t a = (size_t)t * kVMaxNodes, v = (size_t)t * kVMaxVals;
+        out.ops.insert(out.ops.end(), ops.begin() + a, ops.begin() + a + m[0]);
+        out.args.insert(out.args.end(), args.begin() + a, args.begin() + a + m[0]);
+        out.ct.insert(out.ct.end(), ct.begin() + a, ct.begin() + a + m[1]);
+        out.cv.insert(out.cv.end(), cv.begin() + v, cv.begin() + v + m[2]);
+        out.co.push_back((int64_t)out.ops.size());
+        out.cof.push_back((int64_t)out.ct.size());
+        out.vo.push_back((int64_t)out.cv.size());
+        if (applied) (*applied)[t] = (uint8_t)(m[3] >> 8);
+    }
+    return RSR_OK;
+}
+
+}  // namespace rsr
+
+using namespace rsr;
+
+extern "C" RSR_API int rsr_vary(rsr_context* ctx, int n_tasks, const rsr_vary_task* tasks, int n_parents,
+                                const uint8_t* ops, const uint16_t* args, const int64_t* code_offsets,
+                                const uint8_t* const_types, const int64_t* const_offsets, const float* const_values,
+                                const int64_t* value_offsets, int n_vars, const uint8_t* var_types,
+                                const uint8_t* var_indices, int n_allowed, const uint8_t* allowed_ops,
+                                int64_t code_capacity, int64_t const_capacity, int64_t value_capacity,
+                                uint8_t* ops_out, uint16_t* args_out, int64_t* code_offsets_out,
+                                uint8_t* const_types_out, int64_t* const_offsets_out, float* const_values_out,
+                                int64_t* value_offsets_out, uint8_t* kind_out) {
+    if (!ctx || n_tasks < 0 || n_parents < 0 || n_vars < 0 || n_allowed < 0 || (n_tasks && !tasks) ||
+        (n_parents && (!code_offsets || !const_offsets || !value_offsets)) || (n_vars && (!var_types || !var_indices)) ||
+        (n_allowed && !allowed_ops) || !code_offsets_out || !const_offsets_out || !value_offsets_out)
+        return set_error(RSR_E_INVALID_ARG, "null argument");
+    Wire parents;
+    for (int i = 0; i < n_parents; ++i) parents.add(ops, args, code_offsets, const_types, const_offsets, const_values,
+                                                     value_offsets, i);
+    VaryEnv env;
+    env.var_type.assign(var_types, var_types + n_vars);
+    env.var_idx.assign(var_indices, var_indices + n_vars);
+    env.ops.assign(allowed_ops, allowed_ops + n_allowed);
+    Wire out;
+    std::vector<uint8_t> kinds;
+    int rc = vary(ctx, std::vector<rsr_vary_task>(tasks, tasks + n_tasks), parents, env, out, &kinds);
+    if (rc) return rc;
+    if ((int64_t)out.ops.size() > code_capacity || (int64_t)out.ct.size() > const_capacity ||
+        (int64_t)out.cv.size() > value_capacity)
+        return set_error(RSR_E_INVALID_ARG, "outputs need %zu code, %zu constant and %zu value entries",
+                         out.ops.size(), out.ct.size(), out.cv.size());
+    if (!out.ops.empty() && (!ops_out || !args_out)) return set_error(RSR_E_INVALID_ARG, "null code output");
+    if (!out.ct.empty() && !const_types_out) return set_error(RSR_E_INVALID_ARG, "null constant output");
+    if (!out.cv.empty() && !const_values_out) return set_error(RSR_E_INVALID_ARG, "null value output");
+    if (!out.ops.empty()) {
+        memcpy(ops_out, out.ops.data(), out.ops.size());
+        memcpy(args_out, out.args.data(), 2 * out.args.size());
+    }
+    if (!out.ct.empty()) memcpy(const_types_out, out.ct.data(), out.ct.size());
+    if (!out.cv.empty()) memcpy(const_values_out, out.cv.data(), 4 * out.cv.size());
+    memcpy(code_offsets_out, out.co.data(), 8 * out.co.size());
+    memcpy(const_offsets_out, out.cof.data(), 8 * out.cof.size());
+    memcpy(value_offsets_out, out.vo.data(), 8 * out.vo.size());
+    if (kind_out) memcpy(kind_out, kinds.data(), kinds.size());
+    return RSR_OK;
+}
+
+// The variation streams on the host (the kernel's own functions): raw Xoshiro256++ outputs
+// from a given state, the state seed_from_u64 makes of a seed, and the samplers.
+extern "C" RSR_API int rsr_xoshiro256pp(const uint64_t* state, int n, uint64_t* out) {
+    if (!state || n < 0 || (n && !out)) return set_error(RSR_E_INVALID_ARG, "null argument");
+    VRng r;
+    for (int i = 0; i < 4; ++i) r.s[i] = state[i];
+    for (int i = 0; i < n; ++i) out[i] = vrng_next(r);
+    return RSR_OK;
+}
+
+extern "C" RSR_API int rsr_xoshiro_seed(uint64_t seed, uint64_t* state_out) {
+    if (!state_out) return set_error(RSR_E_INVALID_ARG, "null argument");
+    const VRng r = vrng_seed(seed);
+    for (int i = 0; i < 4; ++i) state_out[i] = r.s[i];
+    return RSR_OK;
+}
+
+extern "C" RSR_API int rsr_vary_samples(uint64_t seed, int kind, int n, float a, float b, uint32_t bound,
+                                        float* out) {
+    if (n < 0 || (n && !out) || kind < 0 || kind > 3 || (kind == 2 && bound == 0) || (kind == 3 && !(a < b)))
+        return set_error(RSR_E_INVALID_ARG, "invalid sampler arguments");
+    VRng r = vrng_seed(seed);
+    for (int i = 0; i < n; ++i) {
+        switch (kind) {
+            case 0: out[i] = vrng_f32(r); break;
+            case 1: out[i] = vrng_bool(r) ? 1.0f : 0.0f; break;
+            case 2: out[i] = (float)vrng_below(r, bound); break;
+            default: out[i] = vrng_range(r, a, b); break;
+        }
+    }
+    return RSR_OK;
+}
