@@ -240,6 +240,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving job mix")
     ap.add_argument("--no-graphs", action="store_true", help="skip the graph-replay modes")
+    ap.add_argument("--launch", choices=["persistent", "oneshot"], default="oneshot",
+                    help="launch form of the evaluation kernel (the same bits either way)")
     ap.add_argument("--no-configs", action="store_true", help="skip the matrix / fitting workloads")
     ap.add_argument("--serve-jobs", type=int, default=64)
     ap.add_argument("--serve-gens", type=int, default=3)
@@ -259,6 +261,7 @@ def main():
     X, y, types = W.data_c1(100_000, seed=1001)
     progs = W.gen_c1(8192, seed=2001)
     ctx = api.Context(0)
+    ctx.set_launch_mode(api.LAUNCH_ONESHOT if args.launch == "oneshot" else api.LAUNCH_PERSISTENT)
     ds = api.Dataset.new(ctx, X, y, types)
     P = api.Programs(ctx, progs)
     st = P.status(ds)
@@ -438,6 +441,7 @@ def main():
         "config": {"workload": "Lorentz vector eval: 8192 programs (5-128 ops, {+,-,*,dot,cross,norm,scale,"
                                "sin,sqrt}) x 100000 points (E, v, B ~ N(0,1))",
                    "programs": len(progs), "points": int(len(y)), "tuning": tuning, "summation": head,
+                   "launch": args.launch,
                    "rejected_programs": int((st != 0).sum())},
         "modes": {m: {"value": ne / v["sec"], "ms_per_step": v["sec"] * 1e3, "kernel_ms": v["kernel_ms"]}
                   for m, v in runs.items()},

@@ -102,6 +102,19 @@ RSR_API int rsr_context_synchronize(rsr_context* ctx);
 #define RSR_SUM_KAHAN 0
 #define RSR_SUM_REFERENCE 1
 RSR_API int rsr_context_set_summation(rsr_context* ctx, int mode);
+
+/* The launch form of the MSE evaluation kernel (no reference counterpart: the reference's
+ * evaluator is a CPU loop, evaluator.rs:153-174).
+ *   RSR_LAUNCH_ONESHOT (default): one workgroup per four work items (tile x program group,
+ *     over every merged request), dealt to the CUs by the hardware dispatcher;
+ *   RSR_LAUNCH_PERSISTENT: a launch with more work items than the device holds at once runs
+ *     as a grid of the workgroups that fit, draining the items through a device work queue
+ *     with one counter per XCD (a smaller launch stays one-shot).  Measured on one MI355X it
+ *     is within 1% of one-shot on the Lorentz workload and slower on the serving mix.
+ * Both give the same bits (every work item writes its own partial sums or terms). */
+#define RSR_LAUNCH_ONESHOT 0
+#define RSR_LAUNCH_PERSISTENT 1
+RSR_API int rsr_context_set_launch_mode(rsr_context* ctx, int mode);
 /* Graph mode (default off): rsr_eval_mse captures the evaluation of a program batch over a
  * dataset (its launches and the copy of the results) into a hipGraph on first use and
  * replays it on later calls with the same dataset, summation and tuning; new constants

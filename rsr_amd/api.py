@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(_HERE, "librsr_amd.so")
 
 RSR_OK = 0
 RSR_E_INVALID_ARG = -1
+LAUNCH_ONESHOT, LAUNCH_PERSISTENT = 0, 1
 RSR_E_PROGRAM = -5
 
 _lib = None
@@ -164,6 +165,12 @@ class Context:
     def set_summation(self, reference_order: bool):
         """True (default): the reference's 8-lane order (bit-identical); False: pairwise + Kahan."""
         _check(lib().rsr_context_set_summation(self.h, 1 if reference_order else 0))
+
+    def set_launch_mode(self, mode: int):
+        """LAUNCH_ONESHOT (default): one workgroup per four work items; LAUNCH_PERSISTENT:
+        launches with more work than the device holds at once run as a resident grid draining
+        a per-XCD device work queue (smaller ones one-shot).  The same bits either way."""
+        _check(lib().rsr_context_set_launch_mode(self.h, int(mode)))
 
     def set_graphs(self, enable: bool):
         """compute_mse replays a captured hipGraph per (programs, dataset, settings)."""

@@ -1108,7 +1108,8 @@ int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int
         }
         HIP_TRY(grad ? launch_grad_steps(round.data(), (int)round.size(), terms, ctx->stream,
                                          ctx->next_timer(), big)
-                     : launch_mse_steps(round.data(), (int)round.size(), t, terms, ctx->stream, ctx->next_timer(), big),
+                     : launch_mse_steps(round.data(), (int)round.size(), t, terms, ctx->stream, ctx->next_timer(), big,
+                                        ctx->queue_ptr()),
                 "evaluation launch");
     }
     return RSR_OK;
@@ -1234,7 +1235,9 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
         return set_error(RSR_E_UNSUPPORTED, "spill area exceeds the workgroup LDS limit");
     EvalGraph* g = nullptr;
     for (auto& e : p->graphs)
-        if (e->ds_id == ds->id && e->summation == summ && e->ppl == t.ppl && e->group == t.group) g = e.get();
+        if (e->ds_id == ds->id && e->summation == summ && e->ppl == t.ppl && e->group == t.group &&
+            e->launch_mode == ctx->launch_mode)
+            g = e.get();
     if (!g) {
         std::vector<int32_t> order;
         for (int32_t i : L->order)
@@ -1249,6 +1252,7 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
         ng->summation = summ;
         ng->ppl = t.ppl;
         ng->group = t.group;
+        ng->launch_mode = ctx->launch_mode;
         if ((rc = ng->d_mse.alloc(ctx, std::max<size_t>(p->n, 1) * sizeof(float))) ||
             (rc = ng->d_order.upload(ctx, order.data(), std::max<size_t>(order.size(), 1) * sizeof(int32_t))) ||
             (rc = ng->arena.alloc(ctx, std::max<size_t>(need, 16))))
@@ -1261,7 +1265,7 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
         st.mse_out = (float*)ng->d_mse.ptr;
         st.partials = (float*)ng->arena.ptr;
         HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed), "graph capture");
-        hipError_t e = launch_mse_steps(&st, 1, t, terms, ctx->stream, nullptr);
+        hipError_t e = launch_mse_steps(&st, 1, t, terms, ctx->stream, nullptr, BigSpill{}, ctx->queue_ptr());
         if (e == hipSuccess)
             e = hipMemcpyAsync(ng->host, ng->d_mse.ptr, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost,
                                ctx->stream);

@@ -137,8 +137,11 @@ size_t grad_state_floats(const EvalStep& st);
 // merged launch per kMaxSegs steps.  terms = false: per-tile partials summed with
 // Kahan-compensated FP32 by the finalize kernel.  terms = true: per-point terms folded in
 // the reference's order (bit-identical to the CPU evaluator).
+// queue: kQueueInts zero ints of device memory (the device work queue of the persistent
+// evaluation launch; left zero); null: one-shot launches.
+constexpr int kQueueInts = 16;
 hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune, bool terms, hipStream_t s,
-                            const KernelTimer* timer = nullptr, BigSpill big = {});
+                            const KernelTimer* timer = nullptr, BigSpill big = {}, int* queue = nullptr);
 // MSE + gradient (compute_mse_with_gradient, evaluator.rs:177-224) of every active
 // program with parameters, from the dual lowering.  terms = false: per-tile partials,
 // Kahan-compensated finalize; terms = true: per-point terms written by the tile-parallel

@@ -20,6 +20,9 @@
 // (squared error, or gradient term) and k_fold_terms adds them in the reference's order.
 #include <float.h>
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include "interp.h"
 #include "kernels.h"
 
@@ -129,14 +132,9 @@ __device__ __forceinline__ float reduce_lanes8(float acc) {
 // ---------------------------------------------------------------------------------
 // MSE, Kahan summation
 // ---------------------------------------------------------------------------------
-// At four points per lane the register file alone is 128 VGPRs; the occupancy hint
-// makes the allocator fit the kernel in 168 (three waves per SIMD).
+// One work item (wave index `wave` of the launch): a tile of points x a group of programs.
 template <int PPL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_eval_mse(const SegTable T,
-                                                                                           int spill_slots) {
-    extern __shared__ float smem[];
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = xcd_block() * kWaves + uni(threadIdx.x >> 6);
+__device__ __forceinline__ void mse_item(const SegTable& T, int64_t wave, float* lds, int lane) {
     const int si = find_seg_wave(T, wave);
     const SegDesc& S = T.s[si];
     const int64_t w = wave - S.first_wave;
@@ -151,7 +149,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         for (int sp = first; sp < last; ++sp) any = any || S.active[uni((uint32_t)S.order[sp])] != 0;
         if (!any) return;  // the finalize and fold kernels never read inactive programs' partials
     }
-    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * PPL * 64;
     const int64_t base = (S.tile0 + lt) * (64 * PPL);
 
     Files<PPL> R;
@@ -193,6 +190,55 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             s = wave_sum(s);
         }
         if (!S.terms && lane == 0) S.partials[lt * S.n_progs + sp] = s;
+    }
+}
+
+// At four points per lane the register file alone is 128 VGPRs; the occupancy hint
+// makes the allocator fit the kernel in 168 (three waves per SIMD).
+template <int PPL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_eval_mse(const SegTable T,
+                                                                                           int spill_slots) {
+    extern __shared__ float smem[];
+    float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * PPL * 64;
+    mse_item<PPL>(T, xcd_block() * kWaves + uni(threadIdx.x >> 6), lds, threadIdx.x & 63);
+}
+
+// The persistent form: a resident grid drains the launch's work items through a device work
+// queue, one counter per XCD.  XCD x owns the same contiguous range of items that
+// xcd_block() gives it in the one-shot launch (a run of tiles with their program groups, so
+// its L2 keeps that part of the dataset and the groups' code); a wave takes the next item of
+// its own XCD's range (workgroup b runs on XCD b % 8) and, once that range is drained, of the
+// other XCDs' ranges in turn.  Every take is one atomic increment and the ranges are finite,
+// so every wave exits.  The queue is zero at launch and the launch leaves it zero: the last
+// wave to finish (queue[kXcds] counts finished waves) clears it.
+template <int PPL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_eval_mse_q(
+    const SegTable T, int spill_slots, int64_t n_waves, int* __restrict__ queue) {
+    extern __shared__ float smem[];
+    const int64_t G = (n_waves + kWaves - 1) / kWaves;  // blocks of the one-shot launch
+    const int64_t q = G / kXcds, r = G % kXcds;
+    uint32_t k = 0, x = uni(blockIdx.x) % kXcds;
+    for (;;) {
+        // the next item: a take from queue x, moving on to the next XCD's range when drained
+        const int64_t w0 = ((int64_t)x * q + min<int64_t>(x, r)) * kWaves;
+        const int64_t nw = min<int64_t>((q + (x < r ? 1 : 0)) * kWaves, n_waves - w0);
+        int t = 0;
+        if ((threadIdx.x & 63) == 0) t = atomicAdd(&queue[x], 1);
+        t = __builtin_amdgcn_readfirstlane(t);
+        if ((int64_t)t >= nw) {
+            if (++k == kXcds) break;
+            x = (x + 1) % kXcds;
+            continue;
+        }
+        float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * PPL * 64;
+        mse_item<PPL>(T, w0 + t, lds, threadIdx.x & 63);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const int launched = (int)(uni(gridDim.x) * kWaves);
+        if (atomicAdd(&queue[kXcds], 1) == launched - 1) {  // every other wave is past its takes
+#pragma unroll
+            for (uint32_t y = 0; y <= kXcds; ++y) (void)atomicExch(&queue[y], 0);
+        }
     }
 }
 
@@ -718,8 +764,44 @@ static hipError_t launch_big(K kernel, const SegTable& T, int64_t items, int gsl
     return hipGetLastError();
 }
 
+// Workgroups of `kernel` that fit the device at once (every CU at its occupancy for this
+// block size and dynamic LDS), cached per (kernel, LDS bytes, device).
+static int64_t resident_blocks(const void* kernel, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, size_t, int>, int64_t> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> g(mu);
+    const auto key = std::make_tuple(kernel, lds, dev);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * kWaves, lds) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    const int64_t n = (int64_t)std::max(per_cu, 1) * std::max(cus, 1);
+    cache[key] = n;
+    return n;
+}
+
+template <int PPL>
+static hipError_t launch_eval_mse(const SegTable& T, int64_t waves, int spill, size_t lds, int* queue,
+                                  hipStream_t s) {
+    const int64_t blocks = (waves + kWaves - 1) / kWaves;
+    // the queue pays off only when the work exceeds what the device holds at once: a smaller
+    // launch is one wave per item in either form, and its waves would only add the takes
+    const int64_t res = queue ? resident_blocks((const void*)k_eval_mse_q<PPL>, lds) : 0;
+    if (queue == nullptr || res <= 0 || blocks <= res) {
+        hipLaunchKernelGGL(k_eval_mse<PPL>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
+        return hipGetLastError();
+    }
+    if (waves / kXcds >= INT32_MAX) return hipErrorInvalidValue;  // per-XCD counters are 32-bit
+    hipLaunchKernelGGL(k_eval_mse_q<PPL>, dim3((unsigned)res), dim3(64 * kWaves), lds, s, T, spill, waves, queue);
+    return hipGetLastError();
+}
+
 hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune, bool terms, hipStream_t s,
-                            const KernelTimer* timer, BigSpill big) {
+                            const KernelTimer* timer, BigSpill big, int* queue) {
     for (int base = 0; base < n; base += kMaxSegs) {
         const int cnt = min(kMaxSegs, n - base);
         SegTable T{};
@@ -748,12 +830,12 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
         if (timer && base == 0) (void)hipEventRecord(timer->begin, s);
         const size_t lds = eval_mse_lds_bytes(spill, tune.ppl);
         if (waves > 0) {
+            hipError_t e;
             switch (tune.ppl) {
-                case 1: hipLaunchKernelGGL(k_eval_mse<1>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill); break;
-                case 4: hipLaunchKernelGGL(k_eval_mse<4>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill); break;
-                default: hipLaunchKernelGGL(k_eval_mse<2>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill); break;
+                case 1: e = launch_eval_mse<1>(T, waves, spill, lds, queue, s); break;
+                case 4: e = launch_eval_mse<4>(T, waves, spill, lds, queue, s); break;
+                default: e = launch_eval_mse<2>(T, waves, spill, lds, queue, s); break;
             }
-            hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
         if (big_items > 0) {

@@ -2,14 +2,15 @@
 // allocation at all, for callers that cannot allocate (code under HIP graph capture, tight
 // serving loops, harnesses that forbid it).
 //
-// The evaluation's scratch -- the per-tile Kahan partials (or, in reference summation, the
-// per-point terms) and the device spill areas of big programs -- lives in one file-scope
+// The evaluation's scratch -- the persistent launch's work queue, the per-tile Kahan partials
+// (or, in reference summation, the per-point terms) and the device spill areas of big
+// programs -- lives in one file-scope
 // device array of kScratchBytes per GPU, held until the process ends.  Host code never
 // passes the array's name to a kernel or a copy: its per-device address comes from
 // hipGetSymbolAddress, taken once per device on the entry point's first call there.
-// Every call on one GPU shares that array, so calls that overlap on one GPU race: calls
-// through one context are serialised by the context (one lock, one stream); calls through
-// two contexts on the same GPU must not overlap.
+// Every call on one GPU shares that array (its work queue included), so calls that overlap
+// on one GPU race: calls through one context are serialised by the context (one lock, one
+// stream); calls through two contexts on the same GPU must not overlap.
 #include <math.h>
 #include <mutex>
 
@@ -90,23 +91,27 @@ extern "C" RSR_API int rsr_eval_mse_noalloc(rsr_context* ctx, rsr_programs* p, c
         return set_error(RSR_E_UNSUPPORTED, "spill area exceeds the workgroup LDS limit");
     const size_t part = mse_partials_floats(st, t.ppl, terms);
     const size_t spill = big_spill_floats(&st, 1, t.ppl, false);
-    const size_t cap = kScratchBytes / sizeof(float);
-    const size_t part_aligned = (part + 63) / 64 * 64;  // the spill areas start on 256 bytes
+    // the array: the work queue (kQueueInts ints, zero at load and after every launch; 256
+    // bytes), the partials or terms, then the spill areas (256-byte aligned)
+    const size_t head = 64;
+    const size_t cap = kScratchBytes / sizeof(float) - head;
+    const size_t part_aligned = (part + 63) / 64 * 64;
     if (part_aligned + spill > cap)
         return set_error(RSR_E_INVALID_ARG,
                          "the evaluation needs %zu bytes of scratch, above the %zu bytes of rsr_eval_mse_noalloc; "
                          "use rsr_eval_mse",
-                         (part_aligned + spill) * sizeof(float), kScratchBytes);
-    st.partials = base;
+                         (head + part_aligned + spill) * sizeof(float), kScratchBytes);
+    int* queue = ctx->launch_mode == RSR_LAUNCH_PERSISTENT ? (int*)base : nullptr;
+    st.partials = base + head;
     BigSpill big;
     if (spill) {
-        big.ptr = base + part_aligned;
+        big.ptr = base + head + part_aligned;
         big.floats = cap - part_aligned;
     }
     hipLaunchKernelGGL(k_fill_rejected, dim3((unsigned)((p->n + 255) / 256)), dim3(256), 0, ctx->stream,
                        mse_out_device, st.active, p->n);
     if ((e = hipGetLastError()) != hipSuccess) return hip_error(e, "rejected-program fill");
-    if ((e = launch_mse_steps(&st, 1, t, terms, ctx->stream, nullptr, big)) != hipSuccess)
+    if ((e = launch_mse_steps(&st, 1, t, terms, ctx->stream, nullptr, big, queue)) != hipSuccess)
         return hip_error(e, "no-alloc evaluation launch");
     if (L->n_invalid) return set_error(RSR_E_PROGRAM, "%d program(s) rejected by lowering", L->n_invalid);
     return RSR_OK;

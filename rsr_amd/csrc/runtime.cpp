@@ -174,6 +174,10 @@ extern "C" int rsr_context_create(int device_ordinal, uint64_t hbm_budget_bytes,
         c->num_cus = prop.multiProcessorCount;
         c->max_lds = prop.sharedMemPerBlock;
     }
+    int rc = c->queue.alloc(c.get(), rsr::kQueueInts * sizeof(int));
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(c->queue.ptr, 0, rsr::kQueueInts * sizeof(int), c->stream), "work queue");
+    HIP_TRY(hipStreamSynchronize(c->stream), "work queue");
     *out = c.release();
     return RSR_OK;
 }
@@ -253,6 +257,7 @@ extern "C" int rsr_context_destroy(rsr_context* ctx) {
     ctx->scratch_out.release();
     ctx->scratch_out2.release();
     ctx->scratch_gspill.release();
+    ctx->queue.release();
     rsr::trim_pool(ctx);
     if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
     (void)hipStreamDestroy(ctx->stream);
@@ -323,6 +328,16 @@ extern "C" int rsr_context_set_summation(rsr_context* ctx, int mode) {
     if (mode != RSR_SUM_KAHAN && mode != RSR_SUM_REFERENCE) return set_error(RSR_E_INVALID_ARG, "bad summation mode");
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->summation = mode;
+    ctx->tune_version++;
+    return RSR_OK;
+}
+
+extern "C" int rsr_context_set_launch_mode(rsr_context* ctx, int mode) {
+    if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
+    if (mode != RSR_LAUNCH_ONESHOT && mode != RSR_LAUNCH_PERSISTENT)
+        return set_error(RSR_E_INVALID_ARG, "bad launch mode");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->launch_mode = mode;
     ctx->tune_version++;
     return RSR_OK;
 }

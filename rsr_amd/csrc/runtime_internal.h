@@ -56,7 +56,7 @@ uint64_t next_object_id();  // process-unique, never reused
 // owns every buffer its nodes touch, so later allocations cannot invalidate it.
 struct EvalGraph {
     uint64_t ds_id = 0;
-    int summation = 0, ppl = 0, group = 0;
+    int summation = 0, ppl = 0, group = 0, launch_mode = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     DevBuf d_mse, d_order, arena;
@@ -89,6 +89,11 @@ struct rsr_context {
     rsr::DevBuf scratch_partials, scratch_gpartials, scratch_state, scratch_gstate, scratch_out, scratch_out2, scratch_act;
     rsr::DevBuf scratch_gspill;  // device spill areas of the big-program launches
     size_t arena_bytes = (size_t)4 << 30;  // summation partials / terms per launch round
+    // The device work queue of the persistent evaluation launch (zero between launches) and
+    // the launch form the evaluations use.
+    rsr::DevBuf queue;
+    int launch_mode = 0;  // RSR_LAUNCH_ONESHOT
+    int* queue_ptr() const { return launch_mode == 1 ? (int*)queue.ptr : nullptr; }
     // pinned host mirror of the jobs' counters a tick reads back (each job owns its device pair)
     int* host_counters = nullptr;
     size_t host_counters_cap = 0;

@@ -265,6 +265,7 @@ struct rsr_server {
         L.ctx->tuning = ctx->tuning;
         L.ctx->tune_cache = ctx->tune_cache;
         L.ctx->arena_bytes = ctx->arena_bytes;
+        L.ctx->launch_mode = ctx->launch_mode;
         L.seen_version = ctx->tune_version.load();
     }
 

@@ -414,3 +414,32 @@ def test_reference_order_few_programs_many_points(gpu_ctx, limit):
         rm, rg = O.compute_mse_with_gradient(p, ods)
         assert m2[i] == rm, (i, m2[i], rm)
         assert same_bits(g2[i], rg)[0], i
+
+
+@pytest.mark.parametrize("reference", [False, True])
+def test_persistent_and_oneshot_launches_agree(gpu_ctx, reference):
+    """The persistent evaluation launch (a resident grid draining a per-XCD device work queue;
+    every launch here has more work items than the device holds at once) and the one-shot
+    launch give the same bits at every points-per-lane; the queue is left empty, so launches
+    after one another (and the other form after it) agree too."""
+    X, y, t = W.data_c1(20_011, seed=81)
+    progs = W.gen_c1(1500, seed=82)
+    ds, _ = make(gpu_ctx, X, y, t)
+    P = api.Programs(gpu_ctx, progs)
+    gpu_ctx.set_summation(reference)
+    try:
+        outs = []
+        for ppl in (1, 2, 4):
+            gpu_ctx.set_tuning(ppl, 8)
+            for mode in (api.LAUNCH_PERSISTENT, api.LAUNCH_ONESHOT, api.LAUNCH_PERSISTENT):
+                gpu_ctx.set_launch_mode(mode)
+                outs.append((ppl, api.compute_mse(gpu_ctx, P, ds)))
+    finally:
+        gpu_ctx.set_launch_mode(api.LAUNCH_ONESHOT)
+        gpu_ctx.set_tuning(2, 8)
+        gpu_ctx.set_summation(False)
+    for ppl in (1, 2, 4):
+        same = [o for p, o in outs if p == ppl]
+        assert all(np.array_equal(same[0], o) for o in same[1:]), ppl
+    if reference:
+        assert all(np.array_equal(outs[0][1], o) for _, o in outs)
