@@ -1,7 +1,8 @@
 """Selection parity on device fitnesses (operators/selection.rs:13-134, island.rs:241-255).
 
-Populations of the scalar and the Lorentz workloads are scored on the GPU in the library's
-default (Kahan-compensated) summation and on the oracle in the reference's f32 order; both
+Populations of the scalar and the Lorentz workloads are scored on the GPU with
+Kahan-compensated summation (the mode whose MSEs differ from the CPU evaluator's in their
+last bits; reference order is bit-identical) and on the oracle in the reference's f32 order; both
 go through fitness = mse + complexity * parsimony, NSGA-II ranks and crowding distances on
 islands of 25 with random ages, and the same tournament draws (k = 2, config.rs:46).
 Wherever the fitness gaps inside an island exceed 1e-5 relative (so both sides see the same
