@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 
 from rsr_amd import api  # noqa: E402
 from rsr_amd import workloads as W  # noqa: E402
-from rsr_amd.isa import alg_flops, node_evals, pack_programs  # noqa: E402
+from rsr_amd.isa import Op, VT, alg_flops, node_evals, pack_programs  # noqa: E402
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 peak
 PEAK_FP32_NOFMA_TOPS = 39.3  # 256 CUs x 64 lanes x 2.4 GHz, one non-fused op per lane per clock
@@ -194,6 +194,49 @@ def run_configs(ctx, log):
                            "batched_ms": t_all * 1e3, "one_call_per_island_ms": t_per * 1e3,
                            "fitted": int(sum(int(o.sum()) for o in res["optimized"]))}
     mini.close()
+    # the whole search loop at the reference's default Config (config.rs:41-66): 24 islands x
+    # 25, mini-batch 64, basic operators, on the 400-point scalar workload
+    gens = 20
+    log(f"[bench] search loop: Engine::run's generations, 24 islands x 25, {gens} generations")
+    ops = [int(o) for o in (Op.AddF, Op.SubF, Op.MulF, Op.DivF, Op.SinF, Op.CosF, Op.ExpF, Op.SqrF, Op.LnF,
+                            Op.SqrtF)]
+    E = api.Engine(ctx, ds, [(VT.Float, 0)], ops)
+    E.step()  # the first generation scores the random initial islands
+    t0 = time.perf_counter()
+    for _ in range(gens):
+        E.step()
+    te = time.perf_counter() - t0
+    info = E.info()
+    E.close()
+    # Engine::run from a fresh engine: generations until the best MSE reaches target_mse
+    # (1e-7), then the final fit (final_opt_iterations 4000)
+    E = api.Engine(ctx, ds, [(VT.Float, 0)], ops)
+    t0 = time.perf_counter()
+    run_gens = E.run()
+    tr = time.perf_counter() - t0
+    rinfo = E.info()
+    E.close()
+    out["search_loop"] = {"islands": 24, "island_size": 25, "points": 400, "generations": gens,
+                          "ms_per_generation": te / gens * 1e3, "generations_per_s": gens / te,
+                          "best_mse": float(info.best_mse), "nukes": int(info.nukes),
+                          "run_to_target": {"target_mse": 1e-7, "reached": bool(rinfo.target_reached),
+                                            "generations": int(run_gens), "seconds": tr,
+                                            "final_fitness": float(rinfo.best_fitness)}}
+    # device-side variation: 16,384 offspring (every operator) from a 1,024-program population
+    parents = W.gen_c0(1024, seed=2200)
+    rng = np.random.default_rng(11)
+    kinds = [api.VARY_GENERATE, api.VARY_CROSSOVER, api.VARY_MUTATE, api.VARY_SUBTREE]
+    tasks = [(kinds[i % 4], 10_000 + i, int(rng.integers(1024)), int(rng.integers(1024)), 0, 4 if i % 4 != 0 else 5,
+              32) for i in range(16384)]
+    api.vary(ctx, tasks[:64], parents, [(VT.Float, 0)], ops)
+    t0 = time.perf_counter()
+    offspring, _ = api.vary(ctx, tasks, parents, [(VT.Float, 0)], ops)
+    tv = time.perf_counter() - t0
+    # the time of one rsr_vary call from host arrays to host arrays (upload, one launch, the
+    # download and compaction, and the Python packing of parents and offspring)
+    out["variation"] = {"offspring": len(tasks), "parents": len(parents), "ms": tv * 1e3,
+                        "offspring_per_s": len(tasks) / tv,
+                        "mean_nodes": float(np.mean([len(p.code) for p in offspring]))}
     ds.close()
     return out
 

@@ -1212,8 +1212,10 @@ int run_tick(rsr_context* ctx, const std::vector<Job*>& top) {
 }
 
 // compute_mse of every program through a captured graph (built on first use for this
-// dataset and these settings, replayed afterwards).  Returns RSR_E_UNSUPPORTED without
-// side effects when the evaluation does not fit one launch round; the caller then runs it
+// dataset and these settings, replayed afterwards; at most eight per batch, the least
+// recently replayed dropped first, and those of destroyed datasets dropped on the next
+// call).  Returns RSR_E_UNSUPPORTED without results when the evaluation does not fit one
+// launch round or the graph's buffers do not fit the budget; the caller then runs it
 // eagerly.
 int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, float* out) {
     Lowering* L;
@@ -1233,6 +1235,11 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
     while (t.ppl > 1 && eval_mse_lds_bytes(L->spill_slots, t.ppl) > ctx->max_lds) t.ppl /= 2;
     if (eval_mse_lds_bytes(L->spill_slots, t.ppl) > ctx->max_lds)
         return set_error(RSR_E_UNSUPPORTED, "spill area exceeds the workgroup LDS limit");
+    // drop the graphs whose dataset is gone (their nodes would read freed memory)
+    auto& graphs = p->graphs;
+    graphs.erase(std::remove_if(graphs.begin(), graphs.end(),
+                                [&](const std::unique_ptr<EvalGraph>& e) { return !ctx->live_datasets.count(e->ds_id); }),
+                 graphs.end());
     EvalGraph* g = nullptr;
     for (auto& e : p->graphs)
         if (e->ds_id == ds->id && e->summation == summ && e->ppl == t.ppl && e->group == t.group &&
@@ -1247,18 +1254,40 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
         st.group = t.group;
         const size_t need = mse_partials_floats(st, t.ppl, terms) * sizeof(float);
         if (need > ctx->arena_bytes) return RSR_E_UNSUPPORTED;
+        // at most kMaxGraphs per batch: the least recently replayed one makes room
+        constexpr size_t kMaxGraphs = 8;
+        if (graphs.size() >= kMaxGraphs) {
+            auto lru = std::min_element(graphs.begin(), graphs.end(), [](const auto& a, const auto& b) {
+                return a->last_use < b->last_use;
+            });
+            graphs.erase(lru);
+        }
         auto ng = std::make_unique<EvalGraph>();
         ng->ds_id = ds->id;
         ng->summation = summ;
         ng->ppl = t.ppl;
         ng->group = t.group;
         ng->launch_mode = ctx->launch_mode;
-        if ((rc = ng->d_mse.alloc(ctx, std::max<size_t>(p->n, 1) * sizeof(float))) ||
-            (rc = ng->d_order.upload(ctx, order.data(), std::max<size_t>(order.size(), 1) * sizeof(int32_t))) ||
-            (rc = ng->arena.alloc(ctx, std::max<size_t>(need, 16))))
-            return rc;
-        HIP_TRY(hipHostMalloc((void**)&ng->host, std::max<size_t>(p->n, 1) * sizeof(float), hipHostMallocDefault),
-                "graph output");
+        // the graph's own buffers; when they do not fit, the batch's other graphs are
+        // released and the allocation tried once more, else the call runs eagerly
+        auto setup = [&]() {
+            int r;
+            if ((r = ng->d_mse.alloc(ctx, std::max<size_t>(p->n, 1) * sizeof(float))) ||
+                (r = ng->d_order.upload(ctx, order.data(), std::max<size_t>(order.size(), 1) * sizeof(int32_t))) ||
+                (r = ng->arena.alloc(ctx, std::max<size_t>(need, 16))))
+                return r;
+            if (!ng->host &&
+                hipHostMalloc((void**)&ng->host, std::max<size_t>(p->n, 1) * sizeof(float), hipHostMallocDefault) !=
+                    hipSuccess) {
+                ng->host = nullptr;
+                return set_error(RSR_E_HIP, "graph output");
+            }
+            return RSR_OK;
+        };
+        if (setup() != RSR_OK) {
+            graphs.clear();
+            if (setup() != RSR_OK) return RSR_E_UNSUPPORTED;
+        }
         HIP_TRY(hipMemsetAsync(ng->d_mse.ptr, 0, (size_t)p->n * sizeof(float), ctx->stream), "graph output clear");
         HIP_TRY(hipStreamSynchronize(ctx->stream), "graph set-up");  // uploads done before capture
         st.P.order = (const int32_t*)ng->d_order.ptr;
@@ -1280,6 +1309,7 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
         g = ng.get();
         p->graphs.push_back(std::move(ng));
     }
+    g->last_use = ++ctx->graph_clock;
     HIP_TRY(hipGraphLaunch(g->exec, ctx->stream), "graph launch");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "graph replay");
     int bad = 0;

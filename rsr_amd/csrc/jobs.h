@@ -110,7 +110,8 @@ int run_tick(rsr_context* ctx, const std::vector<Job*>& jobs);
 // Run one job to completion and finish it (caller holds ctx->mu).
 int run_job(rsr_context* ctx, Job& job);
 // compute_mse through a cached captured graph (caller holds ctx->mu); RSR_E_UNSUPPORTED
-// (nothing done) when the evaluation needs more than one launch round.
+// (nothing done) when the evaluation needs more than one launch round or the graph's
+// buffers do not fit the budget, and the caller then runs it eagerly.
 int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, float* mse_out);
 // Time every tuning candidate on this batch and dataset and cache the fastest for their
 // workload shape (caller holds ctx->mu).

@@ -461,6 +461,7 @@ extern "C" int rsr_dataset_create(rsr_context* ctx, const float* rows, const flo
     for (int64_t i = 0; i < num_samples; ++i) d->target[i] = (y[i] - d->tmean) / d->tstd;
     int rc = dataset_upload(ctx, d);
     if (rc) return rc;
+    ctx->live_datasets.insert(d->id);
     ctx->live_objects++;
     *out = d.release();
     return RSR_OK;
@@ -491,6 +492,7 @@ extern "C" int rsr_dataset_create_aosoa(rsr_context* ctx, const float* feature_f
         }
     int rc = dataset_upload(ctx, d);
     if (rc) return rc;
+    ctx->live_datasets.insert(d->id);
     ctx->live_objects++;
     *out = d.release();
     return RSR_OK;
@@ -530,6 +532,7 @@ extern "C" int rsr_dataset_subset(rsr_context* ctx, const rsr_dataset* src, int6
     }
     int rc = dataset_upload(ctx, d);
     if (rc) return rc;
+    ctx->live_datasets.insert(d->id);
     ctx->live_objects++;
     *out = d.release();
     return RSR_OK;
@@ -562,6 +565,7 @@ extern "C" int rsr_dataset_destroy(rsr_dataset* ds) {
     std::lock_guard<std::mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    ctx->live_datasets.erase(ds->id);
     delete ds;
     ctx->live_objects--;
     return RSR_OK;

@@ -7,6 +7,7 @@
 #include <memory>
 #include <mutex>
 #include <atomic>
+#include <unordered_set>
 #include <vector>
 
 #include "kernels.h"
@@ -52,10 +53,14 @@ int set_error(int code, const char* fmt, ...);
 int hip_error(hipError_t e, const char* what);
 uint64_t next_object_id();  // process-unique, never reused
 
-// A captured evaluation of one program batch over one dataset (graph mode): the graph
-// owns every buffer its nodes touch, so later allocations cannot invalidate it.
+// A captured evaluation of one program batch over one dataset (graph mode).  The graph owns
+// its outputs, scheduling order and summation arena, so the context's later allocations
+// cannot invalidate them.  The buffers of the dataset, of the batch's lowering and of its
+// parameters belong to those objects: the batch's buffers live as long as the batch (its
+// parameters are rewritten in place), and a graph is dropped once its dataset is destroyed.
 struct EvalGraph {
     uint64_t ds_id = 0;
+    uint64_t last_use = 0;  // replay order, for the per-batch bound
     int summation = 0, ppl = 0, group = 0, launch_mode = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -98,6 +103,8 @@ struct rsr_context {
     int* host_counters = nullptr;
     size_t host_counters_cap = 0;
     std::atomic<int> live_servers{0};  // a context is not destroyed under a running server
+    std::unordered_set<uint64_t> live_datasets;  // ids of the context's datasets (guarded by mu)
+    uint64_t graph_clock = 0;                     // graph replays, for the graphs' last_use
     std::atomic<int> live_objects{0};  // ... nor while datasets or program batches use it
     // profiling: events around every main evaluation kernel, accumulated on read
     bool profiling = false;

@@ -47,3 +47,33 @@ def test_graph_replays_match_eager(gpu_ctx, reference, gen, data, npts):
         assert np.array_equal(_bits(g2), _bits(eager2))
     finally:
         gpu_ctx.set_summation(False)
+
+
+def test_graph_cache_bound_and_destroyed_datasets(gpu_ctx):
+    """A batch keeps at most eight graphs (the least recently replayed dropped first) and
+    drops the graphs of destroyed datasets: replays over eleven datasets, some destroyed
+    between calls and new ones made in their place, all equal the eager results."""
+    progs = W.gen_c0(200, seed=95)
+    P = api.Programs(gpu_ctx, progs)
+    sets = {}
+    for k in range(11):
+        X, y, t = W.data_c0(300 + 37 * k, seed=100 + k)
+        sets[k] = api.Dataset.new(gpu_ctx, X, y, t)
+    try:
+        for rnd in range(3):
+            for k in sorted(sets):
+                eager = api.compute_mse(gpu_ctx, P, sets[k], allow_rejected=True)
+                gpu_ctx.set_graphs(True)
+                try:
+                    g = api.compute_mse(gpu_ctx, P, sets[k], allow_rejected=True)
+                finally:
+                    gpu_ctx.set_graphs(False)
+                assert np.array_equal(_bits(g), _bits(eager)), (rnd, k)
+            for k in list(sets)[:3]:  # destroyed; new datasets take their place
+                sets.pop(k).close()
+                X, y, t = W.data_c0(500 + k + 50 * rnd, seed=200 + k + 10 * rnd)
+                sets[100 * (rnd + 1) + k] = api.Dataset.new(gpu_ctx, X, y, t)
+    finally:
+        for ds in sets.values():
+            ds.close()
+        P.close()
