@@ -209,8 +209,9 @@ def run_configs(ctx, log):
     info = E.info()
     E.close()
     # Engine::run from a fresh engine: generations until the best MSE reaches target_mse
-    # (1e-7), then the final fit (final_opt_iterations 4000)
-    E = api.Engine(ctx, ds, [(VT.Float, 0)], ops)
+    # (1e-7; at most 500 generations here, to bound the bench), then the final fit
+    # (final_opt_iterations 4000)
+    E = api.Engine(ctx, ds, [(VT.Float, 0)], ops, max_generations=500)
     t0 = time.perf_counter()
     run_gens = E.run()
     tr = time.perf_counter() - t0
