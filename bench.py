@@ -217,7 +217,34 @@ def run_configs(ctx, log):
     tr = time.perf_counter() - t0
     rinfo = E.info()
     E.close()
+    # independent jobs as clients of one server: 4 engines (seeds 1-4) x 10 generations on 4
+    # threads at once, against the same engines one after another with direct calls
+    n_eng, g_eng = 4, 10
+
+    def engine_run(seed, srv=None):
+        Ej = api.Engine(ctx, ds, [(VT.Float, 0)], ops, seed=seed)
+        if srv is not None:
+            Ej.set_server(srv)
+        for _ in range(g_eng):
+            Ej.step()
+        Ej.close()
+
+    t0 = time.perf_counter()
+    for j in range(n_eng):
+        engine_run(1 + j)
+    t_seq = time.perf_counter() - t0
+    srv = api.Server(ctx, max_inflight=64)
+    th = [threading.Thread(target=engine_run, args=(1 + j, srv)) for j in range(n_eng)]
+    t0 = time.perf_counter()
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    t_srv = time.perf_counter() - t0
+    srv.close()
     out["search_loop"] = {"islands": 24, "island_size": 25, "points": 400, "generations": gens,
+                          "served_engines": {"engines": n_eng, "generations": g_eng, "served_s": t_srv,
+                                             "one_after_another_s": t_seq, "speedup": t_seq / t_srv},
                           "ms_per_generation": te / gens * 1e3, "generations_per_s": gens / te,
                           "best_mse": float(info.best_mse), "nukes": int(info.nukes),
                           "run_to_target": {"target_mse": 1e-7, "reached": bool(rinfo.target_reached),

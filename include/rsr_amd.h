@@ -496,6 +496,14 @@ RSR_API int rsr_server_get_stats(rsr_server* s, rsr_server_stats* out);
 /* Stop after every queued and in-flight request has completed. */
 RSR_API int rsr_server_destroy(rsr_server* s);
 
+/* The search loop as a client of the serving loop (no reference counterpart for the
+ * sharing: the reference's islands call the evaluator in-process, engine.rs:52): the
+ * engine's evaluations and constant fits become requests to `server` (NULL: direct calls
+ * again), batched with those of every other client of it, for instance other engines on
+ * other threads; its variation stays a direct device launch.  The server's context must be
+ * the engine's.  The results are the same bits as direct calls. */
+RSR_API int rsr_engine_set_server(rsr_engine* e, rsr_server* server);
+
 /* Lowering self-check (test hook, no reference counterpart, no GPU): lower one
  * program exactly as the evaluation entry points do and run the emitted micro-code
  * on the host over n points (feat[f * n + i]).  Returns the RSR_PROG_* status. */

@@ -952,6 +952,13 @@ class Engine:
         self.h = h
         self.dataset = dataset  # kept alive with the engine
 
+    def set_server(self, server=None):
+        """Route the evaluations and fits through `server` (None: direct calls)."""
+        self._l.rsr_engine_set_server.restype = C.c_int
+        self._l.rsr_engine_set_server.argtypes = [_P, _P]
+        _check(self._l.rsr_engine_set_server(self.h, server.h if server is not None else None))
+        self.server = server  # kept alive with the engine
+
     def step(self) -> bool:
         t = C.c_int(0)
         _check(self._l.rsr_engine_step(self.h, C.byref(t)))

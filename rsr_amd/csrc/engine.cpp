@@ -32,6 +32,7 @@
 
 #include "../../include/rsr_amd.h"
 #include "isa.h"
+#include "backend.h"
 #include "runtime_internal.h"
 #include "variation.h"
 #include "variation_host.h"
@@ -97,13 +98,13 @@ struct Batch {
 };
 
 // compute_mse of every program (NaN for programs the lowering rejects)
-int eval(rsr_context* ctx, const Wire& w, const rsr_dataset* ds, std::vector<float>& out) {
+int eval(const Backend& B, const Wire& w, const rsr_dataset* ds, std::vector<float>& out) {
     out.assign((size_t)w.n(), NAN);
     if (w.n() == 0) return RSR_OK;
     Batch b;
-    int rc = w.create(ctx, &b.p);
+    int rc = w.create(B.ctx, &b.p);
     if (rc) return rc;
-    rc = rsr_eval_mse(ctx, b.p, ds, out.data());
+    rc = B.eval(b.p, ds, out.data());
     return rc == RSR_E_PROGRAM ? RSR_OK : rc;
 }
 
@@ -169,6 +170,7 @@ struct Island {
 
 struct rsr_engine {
     rsr_context* ctx = nullptr;
+    rsr::Backend B;  // evaluations and fits: direct on ctx, or through a server
     rsr_engine_config cfg{};
     const rsr_dataset* ds = nullptr;
     rsr_dataset* mini = nullptr;
@@ -358,7 +360,7 @@ int step_islands(rsr_engine* E) {
                     idx.push_back(k);
                 }
             std::vector<float> m;
-            if ((rc = eval(ctx, unscored, E->mini, m))) return rc;
+            if ((rc = eval(E->B, unscored, E->mini, m))) return rc;
             for (size_t q = 0; q < idx.size(); ++q) cur[idx[q]] = m[q];
         }
         for (int cyc = 0; cyc < C.mutation_cycles && !at.empty(); ++cyc) {
@@ -383,7 +385,7 @@ int step_islands(rsr_engine* E) {
                     idx.push_back(k);
                 }
             std::vector<float> m;
-            if ((rc = eval(ctx, ok, E->mini, m))) return rc;
+            if ((rc = eval(E->B, ok, E->mini, m))) return rc;
             std::vector<uint8_t> take(at.size(), 0);
             for (size_t q = 0; q < idx.size(); ++q)
                 if (m[q] < cur[idx[q]]) {
@@ -418,7 +420,7 @@ int step_islands(rsr_engine* E) {
             bests.add(E->isl[g].best, 0);
         }
         std::vector<float> best_mini;
-        if ((rc = eval(ctx, bests, E->mini, best_mini))) return rc;
+        if ((rc = eval(E->B, bests, E->mini, best_mini))) return rc;
         std::vector<float> best_fit(G), draws;
         std::vector<int64_t> doff{0};
         std::vector<VRng> saved(G);
@@ -439,7 +441,7 @@ int step_islands(rsr_engine* E) {
         const uint8_t u8 = 0;
         const uint16_t u16 = 0;
         const float f0 = 0.0f;
-        rc = rsr_islands_evaluate(ctx, E->ds, E->mini, G, off.data(), all.ops.empty() ? &u8 : all.ops.data(),
+        rc = islands_evaluate(E->B, E->ds, E->mini, G, off.data(), all.ops.empty() ? &u8 : all.ops.data(),
                                   all.args.empty() ? &u16 : all.args.data(), all.co.data(),
                                   all.ct.empty() ? &u8 : all.ct.data(), all.cof.data(),
                                   all.cv.empty() ? &f0 : all.cv.data(), all.vo.data(), best_fit.data(),
@@ -493,7 +495,7 @@ int step_islands(rsr_engine* E) {
         Wire made;
         if ((rc = generate(E, who, gen, made))) return rc;
         std::vector<float> m;
-        if ((rc = eval(ctx, made, E->ds, m))) return rc;
+        if ((rc = eval(E->B, made, E->ds, m))) return rc;
         int k = 0;
         for (int g : nuke) {
             Island& I = E->isl[g];
@@ -525,9 +527,9 @@ int engine_step(rsr_engine* E) {
     if (rc) return rc;
     for (auto& I : E->isl)  // the islands' halls of fame into the global one
         for (auto& kv : I.hof.m) E->ghof.offer(kv.first, kv.second.first, kv.second.second, 0);
-    const Island& B = E->isl[island_of_global_best(E)];
+    const Island& Bst = E->isl[island_of_global_best(E)];
     std::vector<float> m;
-    if ((rc = eval(E->ctx, B.best, E->ds, m))) return rc;
+    if ((rc = eval(E->B, Bst.best, E->ds, m))) return rc;
     E->pure_mse = m[0];
     const int64_t generation = E->generation++;
     if (E->pure_mse <= E->cfg.target_mse) {
@@ -570,10 +572,7 @@ int engine_finish(rsr_engine* E) {
     int rc = best.create(E->ctx, &b.p);
     if (rc) return rc;
     float mse = NAN;
-    uint8_t method = 0;
-    const uint8_t on = 1;
-    rc = rsr_optimize_constants(E->ctx, b.p, E->ds, E->cfg.final_opt_iterations, E->cfg.seed, 0.05f, &on, &mse,
-                                &method);
+    rc = E->B.optimize(b.p, E->ds, E->cfg.final_opt_iterations, E->cfg.seed, 0.05f, &mse);
     if (rc && rc != RSR_E_PROGRAM) return rc;
     int64_t nv = 0;
     int nprog = 0;
@@ -584,7 +583,7 @@ int engine_finish(rsr_engine* E) {
     Wire simp;
     if ((rc = simplify(best, simp))) return rc;
     std::vector<float> m;
-    if ((rc = eval(E->ctx, simp, E->ds, m))) return rc;
+    if ((rc = eval(E->B, simp, E->ds, m))) return rc;
     Island& I0 = E->isl[0];
     I0.best = std::move(simp);
     I0.best_fit = penalised(m[0], complexity_of(I0.best, 0), E->cfg.parsimony_penalty);
@@ -673,6 +672,7 @@ extern "C" int rsr_engine_create(rsr_context* ctx, const rsr_engine_config* cfg,
     (void)float_root;
     auto E = std::make_unique<rsr_engine>();
     E->ctx = ctx;
+    E->B.ctx = ctx;
     E->cfg = c;
     E->ds = dataset;
     E->env.var_type.assign(var_types, var_types + n_vars);
@@ -700,6 +700,12 @@ extern "C" int rsr_engine_create(rsr_context* ctx, const rsr_engine_config* cfg,
         I.best.add(I.pop, 0);
     }
     *out = E.release();
+    return RSR_OK;
+}
+
+extern "C" int rsr_engine_set_server(rsr_engine* e, rsr_server* server) {
+    if (!e) return set_error(RSR_E_INVALID_ARG, "null engine");
+    e->B.srv = server;
     return RSR_OK;
 }
 

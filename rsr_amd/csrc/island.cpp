@@ -27,6 +27,7 @@
 #include "../../include/rsr_amd.h"
 #include "isa.h"
 #include "runtime_internal.h"
+#include "backend.h"
 #include "wire.h"
 
 using namespace rsr;
@@ -40,20 +41,55 @@ struct Batch {  // an rsr_programs handle destroyed with the scope
     }
 };
 
-// rsr_eval_mse that tolerates rejected programs (their MSE is NaN)
-int eval(rsr_context* ctx, const Wire& w, const rsr_dataset* ds, std::vector<float>& out) {
+// compute_mse that tolerates rejected programs (their MSE is NaN)
+int eval(const Backend& B, const Wire& w, const rsr_dataset* ds, std::vector<float>& out) {
     out.assign((size_t)w.n(), NAN);
     if (w.n() == 0) return RSR_OK;
     Batch b;
-    int rc = w.create(ctx, &b.p);
+    int rc = w.create(B.ctx, &b.p);
     if (rc) return rc;
-    rc = rsr_eval_mse(ctx, b.p, ds, out.data());
+    rc = B.eval(b.p, ds, out.data());
     return rc == RSR_E_PROGRAM ? RSR_OK : rc;
+}
+
+// A request through the server, waited for.
+int served(rsr_server* srv, const rsr_request& q) {
+    uint64_t ticket = 0;
+    int rc = rsr_server_submit(srv, &q, &ticket);
+    if (rc) return rc;
+    int status = RSR_OK;
+    rc = rsr_server_wait(srv, ticket, -1.0, &status);
+    (void)rsr_server_release(srv, ticket);
+    return rc ? rc : status;
 }
 
 }  // namespace
 
-extern "C" int rsr_islands_evaluate(rsr_context* ctx, const rsr_dataset* dataset, const rsr_dataset* mini_batch,
+int rsr::Backend::eval(rsr_programs* p, const rsr_dataset* ds, float* out) const {
+    if (!srv) return rsr_eval_mse(ctx, p, ds, out);
+    rsr_request q{};
+    q.kind = RSR_REQ_EVAL;
+    q.programs = p;
+    q.dataset = ds;
+    q.out = out;
+    return served(srv, q);
+}
+
+int rsr::Backend::optimize(rsr_programs* p, const rsr_dataset* ds, int iterations, uint64_t seed, float threshold,
+                           float* mse_out) const {
+    if (!srv || threshold != 0.05f)
+        return rsr_optimize_constants(ctx, p, ds, iterations, seed, threshold, nullptr, mse_out, nullptr);
+    rsr_request q{};
+    q.kind = RSR_REQ_OPTIMIZE;
+    q.programs = p;
+    q.dataset = ds;
+    q.iterations = iterations;
+    q.seed = seed;
+    q.out = mse_out;
+    return served(srv, q);
+}
+
+int rsr::islands_evaluate(const Backend& B, const rsr_dataset* dataset, const rsr_dataset* mini_batch,
                                     int n_islands, const int64_t* island_offsets, const uint8_t* ops,
                                     const uint16_t* args, const int64_t* code_offsets, const uint8_t* const_types,
                                     const int64_t* const_offsets, const float* const_values,
@@ -66,6 +102,7 @@ extern "C" int rsr_islands_evaluate(rsr_context* ctx, const rsr_dataset* dataset
                                     float* const_values_out, int64_t* value_offsets_out, float* fitness_out,
                                     float* mse_out, uint8_t* optimized_out, int32_t* best_index_out,
                                     int64_t* draws_used_out) {
+    rsr_context* ctx = B.ctx;
     if (!ctx || !dataset || !mini_batch || n_islands < 0 || !island_offsets || !code_offsets || !const_offsets ||
         !value_offsets || !best_fitness || !best_mini_mse || !draw_offsets || !code_offsets_out ||
         !const_offsets_out || !value_offsets_out || !best_index_out || !draws_used_out)
@@ -89,7 +126,7 @@ extern "C" int rsr_islands_evaluate(rsr_context* ctx, const rsr_dataset* dataset
 
     // 1. mini-batch MSEs
     std::vector<float> mini;
-    int rc = eval(ctx, buf, mini_batch, mini);
+    int rc = eval(B, buf, mini_batch, mini);
     if (rc) return rc;
 
     // 2. draws (in the reference's order), promising individuals, full MSEs
@@ -125,7 +162,7 @@ extern "C" int rsr_islands_evaluate(rsr_context* ctx, const rsr_dataset* dataset
             prom_idx.push_back(i);
         }
     std::vector<float> full_p;
-    if ((rc = eval(ctx, prom, dataset, full_p))) return rc;
+    if ((rc = eval(B, prom, dataset, full_p))) return rc;
     std::vector<float> full((size_t)n, NAN);
     for (size_t k = 0; k < prom_idx.size(); ++k) full[prom_idx[k]] = full_p[k];
 
@@ -144,7 +181,7 @@ extern "C" int rsr_islands_evaluate(rsr_context* ctx, const rsr_dataset* dataset
         Batch b;
         if ((rc = cand.create(ctx, &b.p))) return rc;
         std::vector<float> unused((size_t)cand.n());
-        rc = rsr_optimize_constants(ctx, b.p, dataset, opt_iterations, seed, threshold, nullptr, unused.data(), nullptr);
+        rc = B.optimize(b.p, dataset, opt_iterations, seed, threshold, unused.data());
         if (rc && rc != RSR_E_PROGRAM) return rc;
         Wire snapped = cand;
         if ((rc = rsr_programs_get_constants(b.p, snapped.cv.data()))) return rc;
@@ -170,7 +207,7 @@ extern "C" int rsr_islands_evaluate(rsr_context* ctx, const rsr_dataset* dataset
         fitted.args.resize((size_t)fitted.co.back());
         fitted.ct.resize((size_t)fitted.cof.back());
         fitted.cv.resize((size_t)fitted.vo.back());
-        if ((rc = eval(ctx, fitted, dataset, fitted_mse))) return rc;
+        if ((rc = eval(B, fitted, dataset, fitted_mse))) return rc;
     }
     std::vector<int> cand_of((size_t)n, -1);
     for (size_t k = 0; k < cand_idx.size(); ++k) cand_of[cand_idx[k]] = (int)k;
@@ -239,4 +276,22 @@ extern "C" int rsr_islands_evaluate(rsr_context* ctx, const rsr_dataset* dataset
         value_offsets_out[i + 1] = nv;
     }
     return RSR_OK;
+}
+
+extern "C" int rsr_islands_evaluate(rsr_context* ctx, const rsr_dataset* dataset, const rsr_dataset* mini_batch,
+                                    int n_islands, const int64_t* island_offsets, const uint8_t* ops,
+                                    const uint16_t* args, const int64_t* code_offsets, const uint8_t* const_types,
+                                    const int64_t* const_offsets, const float* const_values,
+                                    const int64_t* value_offsets, const float* best_fitness,
+                                    const float* best_mini_mse, const float* draws, const int64_t* draw_offsets,
+                                    float opt_prob, int opt_iterations, float parsimony, float threshold,
+                                    uint64_t seed, int64_t code_capacity, int64_t const_capacity,
+                                    int64_t value_capacity, uint8_t* ops_out, uint16_t* args_out,
+                                    int64_t* code_offsets_out, uint8_t* const_types_out, int64_t* const_offsets_out,
+                                    float* const_values_out, int64_t* value_offsets_out, float* fitness_out,
+                                    float* mse_out, uint8_t* optimized_out, int32_t* best_index_out,
+                                    int64_t* draws_used_out) {
+    Backend B;
+    B.ctx = ctx;
+    return islands_evaluate(B, dataset, mini_batch, n_islands, island_offsets, ops, args, code_offsets, const_types, const_offsets, const_values, value_offsets, best_fitness, best_mini_mse, draws, draw_offsets, opt_prob, opt_iterations, parsimony, threshold, seed, code_capacity, const_capacity, value_capacity, ops_out, args_out, code_offsets_out, const_types_out, const_offsets_out, const_values_out, value_offsets_out, fitness_out, mse_out, optimized_out, best_index_out, draws_used_out);
 }

@@ -173,3 +173,49 @@ def test_run_improves_the_square_target(gpu_ctx, square):
         assert last <= first and last < 0.1 * np.var(W.data_c0(400, seed=71)[1])
     finally:
         E.close()
+
+
+def test_engines_served_together_equal_alone(gpu_ctx, square):
+    """The search loop as a client of the serving loop: two engines (different seeds) stepping
+    at once on two threads, their evaluations and fits batched by one server, give the same
+    populations, fitnesses and bests bit for bit as each engine alone with direct calls."""
+    import threading
+
+    gpu_ctx.set_summation(True)
+    try:
+        alone = []
+        for seed in (5, 6):
+            E = api.Engine(gpu_ctx, square, VARS, BASIC, **small(seed=seed, max_generations=5))
+            for _ in range(5):
+                E.step()
+            alone.append(snapshot(E))
+            E.close()
+        srv = api.Server(gpu_ctx, max_inflight=16)
+        engines = [api.Engine(gpu_ctx, square, VARS, BASIC, **small(seed=seed, max_generations=5)) for seed in (5, 6)]
+        errors = []
+
+        def run(E):
+            try:
+                E.set_server(srv)
+                for _ in range(5):
+                    E.step()
+            except Exception as e:  # reported below
+                errors.append(repr(e))
+
+        threads = [threading.Thread(target=run, args=(E,)) for E in engines]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        assert not errors, errors
+        assert srv.stats()["completed"] > 0
+        for E, (pops, bests) in zip(engines, alone):
+            p2, b2 = snapshot(E)
+            for g in range(E.config.num_islands):
+                assert all(PC.same(x, y) for x, y in zip(pops[g][0], p2[g][0]))
+                assert np.array_equal(pops[g][1], p2[g][1]) and np.array_equal(pops[g][2], p2[g][2])
+                assert PC.same(bests[g][0][0], b2[g][0][0]) and bests[g][1][0] == b2[g][1][0]
+            E.close()
+        srv.close()
+    finally:
+        gpu_ctx.set_summation(False)
