@@ -39,12 +39,18 @@ struct VarySet {  // device views of one launch
     int n_vars;
     const uint8_t* allowed;
     int n_allowed;
-    // fixed-capacity output slots
+    // fixed-capacity output slots (one per task, worked in place)
     uint8_t* o_ops;
     uint16_t* o_args;
     uint8_t* o_ct;
     float* o_cv;
-    int32_t* o_meta;  // [n][4]: len, constants, payload floats, status
+    int32_t* o_meta;  // [n][8]: len, constants, payload floats, status, offsets of the packed copy
+    // the packed copy: each task reserves its exact extent (bump: nodes, constants, values)
+    uint8_t* p_ops;
+    uint16_t* p_args;
+    uint8_t* p_ct;
+    float* p_cv;
+    int* bump;
 };
 
 __device__ VProg parent_view(const VarySet& S, int i) {
@@ -91,11 +97,24 @@ __global__ __launch_bounds__(64) void k_vary(const VarySet S) {
         case RSR_VARY_CONSTANT: v_constant_perturbation(o, r, parent_view(S, task.a)); break;
         default: v_subtree_mutation(o, r, env, parent_view(S, task.a), task.max_size, task.depth); break;
     }
-    int32_t* meta = S.o_meta + 4 * (int64_t)t;
+    int32_t* meta = S.o_meta + 8 * (int64_t)t;
     meta[0] = o.len;
     meta[1] = o.nconst;
     meta[2] = o.nval;
     meta[3] = o.overflow ? 1 : (kind << 8);
+    if (o.overflow) return;
+    // pack: the host copies back only the used extents, and reassembles in task order
+    const int at_n = atomicAdd(&S.bump[0], o.len), at_k = atomicAdd(&S.bump[1], o.nconst),
+              at_v = atomicAdd(&S.bump[2], o.nval);
+    meta[4] = at_n;
+    meta[5] = at_k;
+    meta[6] = at_v;
+    for (int i = 0; i < o.len; ++i) {
+        S.p_ops[at_n + i] = o.op[i];
+        S.p_args[at_n + i] = o.arg[i];
+    }
+    for (int i = 0; i < o.nconst; ++i) S.p_ct[at_k + i] = o.ctype[i];
+    for (int i = 0; i < o.nval; ++i) S.p_cv[at_v + i] = o.cval[i];
 }
 
 bool needs_parent(int kind) { return kind != RSR_VARY_GENERATE; }
@@ -107,6 +126,7 @@ int vary(rsr_context* ctx, const std::vector<rsr_vary_task>& tasks, const Wire& 
     out.clear();
     const int n = (int)tasks.size();
     if (n == 0) return RSR_OK;
+    if (n > 65536) return set_error(RSR_E_INVALID_ARG, "%d tasks: at most 65536 per call", n);
     const int np = parents.n();
     for (int t = 0; t < n; ++t) {
         const rsr_vary_task& k = tasks[t];
@@ -149,17 +169,19 @@ int vary(rsr_context* ctx, const std::vector<rsr_vary_task>& tasks, const Wire& 
     std::lock_guard<std::mutex> g(ctx->mu);
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_error(e, "hipSetDevice");
-    // one staging buffer: inputs, then the output slots
+    // one device buffer: the inputs, the output slots, their packed copy and the metadata
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t b_tasks = al(n * sizeof(rsr_vary_task)), b_ops = al(parents.ops.size() + 1),
                  b_args = al(2 * parents.args.size() + 2), b_co = al(8 * parents.co.size()),
                  b_ct = al(parents.ct.size() + 1), b_cof = al(8 * parents.cof.size()),
                  b_cv = al(4 * parents.cv.size() + 4), b_vo = al(8 * parents.vo.size()),
                  b_vt = al(env.var_type.size() + 1), b_vi = al(env.var_idx.size() + 1), b_al = al(env.ops.size() + 1),
-                 b_oops = al((size_t)n * kVMaxNodes), b_oargs = al((size_t)n * kVMaxNodes * 2),
-                 b_oct = al((size_t)n * kVMaxNodes), b_ocv = al((size_t)n * kVMaxVals * 4), b_meta = al((size_t)n * 16);
-    const size_t in_bytes = b_tasks + b_ops + b_args + b_co + b_ct + b_cof + b_cv + b_vo + b_vt + b_vi + b_al;
-    const size_t total = in_bytes + b_oops + b_oargs + b_oct + b_ocv + b_meta;
+                 b_bump = al(3 * sizeof(int)), b_oops = al((size_t)n * kVMaxNodes),
+                 b_oargs = al((size_t)n * kVMaxNodes * 2), b_oct = al((size_t)n * kVMaxNodes),
+                 b_ocv = al((size_t)n * kVMaxVals * 4), b_meta = al((size_t)n * 32);
+    const size_t in_bytes = b_tasks + b_ops + b_args + b_co + b_ct + b_cof + b_cv + b_vo + b_vt + b_vi + b_al + b_bump;
+    const size_t slots = b_oops + b_oargs + b_oct + b_ocv;
+    const size_t total = in_bytes + 2 * slots + b_meta;  // the slots, then the packed copy
     DevBuf buf;
     int rc = buf.alloc(ctx, total);
     if (rc) return rc;
@@ -186,38 +208,48 @@ int vary(rsr_context* ctx, const std::vector<rsr_vary_task>& tasks, const Wire& 
     S.n_vars = (int)env.var_type.size();
     S.allowed = (const uint8_t*)put(env.ops.data(), env.ops.size(), b_al);
     S.n_allowed = (int)env.ops.size();
+    S.bump = (int*)put(nullptr, 0, b_bump);  // zero (the staging buffer is zero-filled)
     char* o = (char*)buf.ptr + in_bytes;
     S.o_ops = (uint8_t*)o;
     S.o_args = (uint16_t*)(o + b_oops);
     S.o_ct = (uint8_t*)(o + b_oops + b_oargs);
     S.o_cv = (float*)(o + b_oops + b_oargs + b_oct);
-    S.o_meta = (int32_t*)(o + b_oops + b_oargs + b_oct + b_ocv);
+    char* pk = o + slots;
+    S.p_ops = (uint8_t*)pk;
+    S.p_args = (uint16_t*)(pk + b_oops);
+    S.p_ct = (uint8_t*)(pk + b_oops + b_oargs);
+    S.p_cv = (float*)(pk + b_oops + b_oargs + b_oct);
+    S.o_meta = (int32_t*)(pk + slots);
     HIP_TRY(hipMemcpyAsync(buf.ptr, host.data(), in_bytes, hipMemcpyHostToDevice, ctx->stream), "variation upload");
     hipLaunchKernelGGL(k_vary, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, ctx->stream, S);
     HIP_TRY(hipGetLastError(), "variation launch");
-    std::vector<int32_t> meta((size_t)n * 4);
+    std::vector<int32_t> meta((size_t)n * 8);
+    int used[3] = {0, 0, 0};
     HIP_TRY(hipMemcpyAsync(meta.data(), S.o_meta, meta.size() * 4, hipMemcpyDeviceToHost, ctx->stream),
             "variation meta");
+    HIP_TRY(hipMemcpyAsync(used, S.bump, sizeof(used), hipMemcpyDeviceToHost, ctx->stream), "variation extents");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "variation");
-    // the slots' used prefixes (one copy each of the code, constants and payload areas)
-    std::vector<uint8_t> ops((size_t)n * kVMaxNodes), ct((size_t)n * kVMaxNodes);
-    std::vector<uint16_t> args((size_t)n * kVMaxNodes);
-    std::vector<float> cv((size_t)n * kVMaxVals);
-    HIP_TRY(hipMemcpyAsync(ops.data(), S.o_ops, ops.size(), hipMemcpyDeviceToHost, ctx->stream), "variation code");
-    HIP_TRY(hipMemcpyAsync(args.data(), S.o_args, 2 * args.size(), hipMemcpyDeviceToHost, ctx->stream),
+    for (int t = 0; t < n; ++t)
+        if (meta[8 * (size_t)t + 3] & 1)
+            return set_error(RSR_E_UNSUPPORTED, "task %d: offspring above %d nodes", t, kVMaxNodes);
+    // the packed extents only
+    std::vector<uint8_t> ops((size_t)used[0] + 1), ct((size_t)used[1] + 1);
+    std::vector<uint16_t> args((size_t)used[0] + 1);
+    std::vector<float> cv((size_t)used[2] + 1);
+    HIP_TRY(hipMemcpyAsync(ops.data(), S.p_ops, used[0], hipMemcpyDeviceToHost, ctx->stream), "variation code");
+    HIP_TRY(hipMemcpyAsync(args.data(), S.p_args, 2 * (size_t)used[0], hipMemcpyDeviceToHost, ctx->stream),
             "variation operands");
-    HIP_TRY(hipMemcpyAsync(ct.data(), S.o_ct, ct.size(), hipMemcpyDeviceToHost, ctx->stream), "variation types");
-    HIP_TRY(hipMemcpyAsync(cv.data(), S.o_cv, 4 * cv.size(), hipMemcpyDeviceToHost, ctx->stream), "variation values");
+    HIP_TRY(hipMemcpyAsync(ct.data(), S.p_ct, used[1], hipMemcpyDeviceToHost, ctx->stream), "variation types");
+    HIP_TRY(hipMemcpyAsync(cv.data(), S.p_cv, 4 * (size_t)used[2], hipMemcpyDeviceToHost, ctx->stream),
+            "variation values");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "variation download");
     if (applied) applied->assign((size_t)n, 0);
-    for (int t = 0; t < n; ++t) {
-        const int32_t* m = meta.data() + 4 * (size_t)t;
-        if (m[3] & 1) return set_error(RSR_E_UNSUPPORTED, "task %d: offspring above %d nodes", t, kVMaxNodes);
-        const size_t a = (size_t)t * kVMaxNodes, v = (size_t)t * kVMaxVals;
-        out.ops.insert(out.ops.end(), ops.begin() + a, ops.begin() + a + m[0]);
-        out.args.insert(out.args.end(), args.begin() + a, args.begin() + a + m[0]);
-        out.ct.insert(out.ct.end(), ct.begin() + a, ct.begin() + a + m[1]);
-        out.cv.insert(out.cv.end(), cv.begin() + v, cv.begin() + v + m[2]);
+    for (int t = 0; t < n; ++t) {  // in task order, whatever order the tasks packed in
+        const int32_t* m = meta.data() + 8 * (size_t)t;
+        out.ops.insert(out.ops.end(), ops.begin() + m[4], ops.begin() + m[4] + m[0]);
+        out.args.insert(out.args.end(), args.begin() + m[4], args.begin() + m[4] + m[0]);
+        out.ct.insert(out.ct.end(), ct.begin() + m[5], ct.begin() + m[5] + m[1]);
+        out.cv.insert(out.cv.end(), cv.begin() + m[6], cv.begin() + m[6] + m[2]);
         out.co.push_back((int64_t)out.ops.size());
         out.cof.push_back((int64_t)out.ct.size());
         out.vo.push_back((int64_t)out.cv.size());
