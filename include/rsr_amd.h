@@ -126,7 +126,7 @@ RSR_API int rsr_context_set_graphs(rsr_context* ctx, int enable);
  * split into tile ranges with the summation state carried between them, which gives the
  * same results.  Default 4 GiB. */
 RSR_API int rsr_context_set_partials_limit(rsr_context* ctx, uint64_t bytes);
-/* Evaluation tuning: points per lane (1, 2, 4: the wave's tile is 64 of them) and programs
+/* Evaluation tuning: points per lane (1 to 4: the wave's tile is 64 of them) and programs
  * per work item (the programs that share one tile's features in registers).  By default
  * (or after set_tuning(ctx, 0, 0)) each evaluation takes the tuning that rsr_autotune
  * found for its workload shape -- feature count, point count and program count, each
@@ -136,10 +136,10 @@ RSR_API int rsr_context_set_partials_limit(rsr_context* ctx, uint64_t bytes);
  * counterpart (tuning of the device launches). */
 RSR_API int rsr_context_set_tuning(rsr_context* ctx, int points_per_lane, int programs_per_item);
 RSR_API int rsr_context_get_tuning(const rsr_context* ctx, int* points_per_lane, int* programs_per_item);
-/* The autotuner: times every tuning (points per lane 1, 2, 4 x programs per work item 4, 8,
+/* The autotuner: times every tuning (points per lane 1 to 4 x programs per work item 4, 8,
  * 16, 32) on one MSE evaluation of this batch over this dataset and keeps the fastest for
  * the workload's shape, where direct calls, the server and the fitters find it; returns
- * the choice.  Costs about 36 evaluations.  No reference counterpart. */
+ * the choice.  Costs about 48 evaluations.  No reference counterpart. */
 RSR_API int rsr_autotune(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, int* points_per_lane,
                          int* programs_per_item);
 

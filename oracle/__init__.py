@@ -50,6 +50,7 @@ SIGS = {
     "orc_tournament": (C.c_int, [C.c_int, _i32p, _u32p, _f32p]),
     "orc_fitness": (C.c_float, [C.c_float, C.c_int, C.c_float]),
     "orc_cmaes_constants": (None, [C.c_int, _f32p, _f32p, _f32p]),
+    "orc_ulp_sweep": (C.c_int, [C.c_int, C.c_float, C.c_float, C.c_uint32, C.POINTER(C.c_double)]),
 }
 
 
@@ -260,3 +261,16 @@ def expf(x: float) -> float:
 
 def logf(x: float) -> float:
     return lib().orc_logf(x)
+
+
+SWEEP_FNS = {"sin": 0, "cos": 1, "exp": 2, "ln": 3}
+
+
+def ulp_sweep(fn: str, lo: float, hi: float, stride: int):
+    """Every stride-th float of [lo, hi] through the oracle's sin/cos/exp/ln against FP64
+    libm: (max ulp error where the result is a normal float away from zeros, max absolute
+    error elsewhere, the argument of the max ulp error, points swept)."""
+    out = (C.c_double * 4)()
+    rc = lib().orc_ulp_sweep(SWEEP_FNS[fn], lo, hi, stride, out)
+    assert rc == 0
+    return out[0], out[1], out[2], int(out[3])

@@ -31,10 +31,11 @@
  *   - NSGA-II rank / crowding / tournament .. src/engine/search/operators/selection.rs:5-134
  *
  * The transcendental functions sin/cos/exp/ln live in the third-party crate
- * `wide 1.1.1` (Cargo.lock:506-513) whose source is absent.  Here they are the
- * Cephes single-precision algorithms (S. L. Moshier, Cephes Math Library:
- * sinf.c, cosf.c, expf.c, logf.c: Cody-Waite reduction + minimax
- * polynomials).  They are pinned against FP64 libm by ulp-error tests, not
+ * `wide 1.1.1` (Cargo.lock:506-513) whose source is absent.  Here they are
+ * Cody-Waite reductions (with fused multiply-adds for sin/cos/exp) and the Cephes
+ * single-precision minimax polynomials (S. L. Moshier, Cephes Math Library:
+ * sinf.c, cosf.c, expf.c, logf.c), the device library's own sequences restated
+ * operation for operation.  They are pinned against FP64 libm by ulp-error tests, not
  * against wide (parity unpinned for these four functions).  The final
  * horizontal add of the 8 MSE lanes (`f32x8::reduce_add`) is likewise inside
  * wide; it is restated as (lo4 + hi4) followed by ((0+2)+(1+3)) (unpinned).
@@ -88,94 +89,71 @@ static int vt_size(int vt) {
 }
 
 /* =====================================================================
- * Transcendentals: Cephes single-precision algorithms, no FMA.
+ * Transcendentals.  The reference's come from the crate `wide 1.1.1`, whose source is not
+ * here, so the bits are the device library's own definition, restated operation for
+ * operation: sin / cos / exp reduce with fused multiply-adds (Cody-Waite: pi/2 in three
+ * float parts, ln 2 in two) and evaluate the Cephes single-precision minimax polynomials
+ * (S. L. Moshier, Cephes Math Library, sinf.c / cosf.c / expf.c) in Horner form with fused
+ * multiply-adds; exp scales by ldexpf.  ln is Cephes logf with plain mul/add.  fmaf,
+ * rintf and ldexpf are correctly rounded, as the device's v_fma_f32 / v_rndne_f32 /
+ * v_ldexp_f32 are.  Accuracy (at most 2 ulp against FP64) is pinned by the test-suite.
  * ===================================================================== */
 ORC_API float orc_sinf(float xx);
 ORC_API float orc_cosf(float xx);
 ORC_API float orc_expf(float xx);
 ORC_API float orc_logf(float xx);
 
-static const float FOPI = 1.27323954473516f;      /* 4/pi */
-static const float DP1 = 0.78515625f;
-static const float DP2 = 2.4187564849853515625e-4f;
-static const float DP3 = 3.77489497744594108e-8f;
+static uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
-static float sin_poly(float x, float z) {
-    float y = ((-1.9515295891E-4f * z + 8.3321608736E-3f) * z - 1.6666654611E-1f) * z * x;
-    return y + x;
-}
-static float cos_poly(float z) {
-    float y = 2.443315711809948E-005f;
-    y = y * z + -1.388731625493765E-003f;
-    y = y * z + 4.166664568298827E-002f;
-    y = y * (z * z);
-    y = y - 0.5f * z;
-    return y + 1.0f;
-}
-
-/* Octant reduction shared by sin and cos (Cephes sinf.c / cosf.c). */
-static void reduce_octant(float x, unsigned *jout, float *rout) {
-    unsigned j = (unsigned)(FOPI * x);
-    float y = (float)j;
-    if (j & 1u) { j += 1u; y += 1.0f; }
-    j &= 7u;
-    float r = ((x - y * DP1) - y * DP2) - y * DP3;
-    *jout = j;
-    *rout = r;
+/* |x| = k*pi/2 + r, k = rint(|x|*2/pi); s ~ sin(r), c ~ cos(r); n = k (quadrant bits). */
+static void sincos_reduced(float ax, float *s, float *c, uint32_t *n) {
+    float k = rintf(ax * 0.636619746685028076171875f);
+    float r = fmaf(-k, 1.57079637050628662109375f, ax);
+    r = fmaf(-k, -4.37113882867379300296e-08f, r);
+    r = fmaf(-k, -1.71512451000588190000e-15f, r);
+    float z = r * r;
+    float p = fmaf(z, -1.9515295891E-4f, 8.3321608736E-3f);
+    p = fmaf(z, p, -1.6666654611E-1f);
+    *s = fmaf(r * z, p, r);
+    float q = fmaf(z, 2.443315711809948E-5f, -1.388731625493765E-3f);
+    q = fmaf(z, q, 4.166664568298827E-2f);
+    *c = fmaf(z * z, q, fmaf(-0.5f, z, 1.0f));
+    *n = (uint32_t)(int)fminf(k, 16777216.0f);
 }
 
 ORC_API float orc_sinf(float xx) {
-    if (isnan(xx)) return xx;
-    if (isinf(xx)) return NAN;
-    int sign = 1;
-    float x = xx;
-    if (xx < 0.0f) { sign = -1; x = -xx; }
-    if (x > 16777215.0f) return 0.0f * xx; /* total loss of significance (Cephes T24M1) */
-    unsigned j; float r;
-    reduce_octant(x, &j, &r);
-    if (j > 3u) { sign = -sign; j -= 4u; }
-    float z = r * r;
-    float y = (j == 1u || j == 2u) ? cos_poly(z) : sin_poly(r, z);
-    return sign < 0 ? -y : y;
+    float ax = fabsf(xx);
+    if (!(ax <= 16777215.0f)) return 0.0f * xx; /* NaN for inf/NaN; +-0 past 2^24 - 1 (Cephes T24M1) */
+    float s, c; uint32_t n;
+    sincos_reduced(ax, &s, &c, &n);
+    float y = (n & 1u) ? c : s;
+    return u2f(f2u(y) ^ (((n << 30) ^ f2u(xx)) & 0x80000000u));
 }
 
 ORC_API float orc_cosf(float xx) {
-    if (isnan(xx)) return xx;
-    if (isinf(xx)) return NAN;
-    float x = xx < 0.0f ? -xx : xx;
-    if (x > 16777215.0f) return 1.0f;
-    int sign = 1;
-    unsigned j; float r;
-    reduce_octant(x, &j, &r);
-    if (j > 3u) { j -= 4u; sign = -sign; }
-    if (j > 1u) sign = -sign;
-    float z = r * r;
-    float y = (j == 1u || j == 2u) ? sin_poly(r, z) : cos_poly(z);
-    return sign < 0 ? -y : y;
-}
-
-/* 2^n for n in [-149, 127] built from bits (no libm). */
-static float pow2i(int n) {
-    if (n >= -126) { uint32_t b = (uint32_t)(n + 127) << 23; float f; memcpy(&f, &b, 4); return f; }
-    uint32_t b = 1u << (n + 149); float f; memcpy(&f, &b, 4); return f;
+    float ax = fabsf(xx);
+    if (isnan(ax) || isinf(ax)) return NAN;
+    if (ax > 16777215.0f) return 1.0f;
+    float s, c; uint32_t n;
+    sincos_reduced(ax, &s, &c, &n);
+    float y = (n & 1u) ? s : c;
+    return u2f(f2u(y) ^ (((n + 1u) << 30) & 0x80000000u));
 }
 
 ORC_API float orc_expf(float xx) {
     if (isnan(xx)) return xx;
-    if (xx > 88.72283905206835f) return INFINITY;
-    if (xx < -103.278929903431851103f) return 0.0f;
-    float x = xx;
-    float z = floorf(1.44269504088896341f * x + 0.5f);
-    x = x - z * 0.693359375f;
-    x = x - z * -2.12194440e-4f;
-    int n = (int)z;
-    z = x * x;
-    z = (((((1.9875691500E-4f * x + 1.3981999507E-3f) * x + 8.3334519073E-3f) * x
-           + 4.1665795894E-2f) * x + 1.6666665459E-1f) * x + 5.0000001201E-1f) * z + x + 1.0f;
-    /* ldexp in two exact steps so that subnormal results round once */
-    if (n > 127) { z = z * pow2i(127); n -= 127; }
-    if (n < -126) { z = z * pow2i(-126); n += 126; }
-    return z * pow2i(n);
+    float x = fminf(fmaxf(xx, -104.0f), 89.0f); /* past these, ldexpf gives 0 or inf */
+    float k = rintf(x * 1.44269502162933349609375f);
+    float r = fmaf(-k, 0.693147182464599609375f, x);
+    r = fmaf(-k, -1.904654212125933566e-09f, r);
+    float p = fmaf(1.9875691500E-4f, r, 1.3981999507E-3f);
+    p = fmaf(p, r, 8.3334519073E-3f);
+    p = fmaf(p, r, 4.1665795894E-2f);
+    p = fmaf(p, r, 1.6666665459E-1f);
+    p = fmaf(p, r, 5.0000001201E-1f);
+    float y = fmaf(r * r, p, r) + 1.0f;
+    return ldexpf(y, (int)k);
 }
 
 ORC_API float orc_logf(float xx) {
@@ -197,6 +175,60 @@ ORC_API float orc_logf(float xx) {
     z = x + y;
     if (e) z = z + 0.693359375f * fe;
     return z;
+}
+
+/* Accuracy sweep (test support): every stride-th float of [lo, hi] (finite, lo <= hi, both of
+ * one sign or lo = -hi mirrored) through orc_sinf (0) / orc_cosf (1) / orc_expf (2) /
+ * orc_logf (3) against the FP64 libm value.  out[0] = largest error in ulps of the float
+ * nearest the exact value where that is a normal float (sin/cos: where |f(x)| >= 1e-3;
+ * ln: where |ln x| >= 1e-6), out[1] = largest absolute error elsewhere, out[2] = the
+ * argument of out[0], out[3] = points swept.  Floats are walked by their bit patterns over
+ * [0, hi] and mirrored to -x when lo < 0. */
+static double ulp_at(double t) {
+    float tf = (float)fabs(t);
+    if (tf < FLT_MIN) return (double)1.40129846e-45f;
+    if (isinf(tf)) tf = FLT_MAX;
+    return (double)(nextafterf(tf, INFINITY) - tf);
+}
+ORC_API int orc_ulp_sweep(int which, float lo, float hi, uint32_t stride, double *out) {
+    if (!(lo <= hi) || stride == 0 || which < 0 || which > 3) return -1;
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    const float amax = fmaxf(fabsf(lo), fabsf(hi));
+    const float amin = (lo <= 0.0f && hi >= 0.0f) ? 0.0f : fminf(fabsf(lo), fabsf(hi));
+    uint32_t u0, u1;
+    memcpy(&u0, &amin, 4);
+    memcpy(&u1, &amax, 4);
+    for (uint64_t u = u0; u <= u1; u += stride) {
+        uint32_t b = (uint32_t)u;
+        float a;
+        memcpy(&a, &b, 4);
+        for (int sg = 0; sg < 2; ++sg) {
+            const float x = sg ? -a : a;
+            if (x < lo || x > hi || (sg && a == 0.0f)) continue;
+            float got;
+            double t;
+            switch (which) {
+                case 0: got = orc_sinf(x); t = sin((double)x); break;
+                case 1: got = orc_cosf(x); t = cos((double)x); break;
+                case 2: got = orc_expf(x); t = exp((double)x); break;
+                default: got = orc_logf(x); t = log((double)x); break;
+            }
+            out[3] += 1.0;
+            int rel;
+            if (which <= 1) rel = fabs(t) >= 1e-3;
+            else if (which == 2) rel = t >= (double)FLT_MIN && t <= (double)FLT_MAX;
+            else rel = fabs(t) >= 1e-6 && isfinite(t);
+            if (which == 2 && t > (double)FLT_MAX) { if (!isinf(got)) out[1] = INFINITY; continue; }
+            const double e = fabs((double)got - t);
+            if (rel) {
+                const double eu = e / ulp_at(t);
+                if (eu > out[0]) { out[0] = eu; out[2] = x; }
+            } else if (e > out[1]) {
+                out[1] = e;
+            }
+        }
+    }
+    return 0;
 }
 
 /* =====================================================================

@@ -46,6 +46,7 @@ __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_re
 template <int N> struct Files;
 template <> struct Files<1> { v32f a; };
 template <> struct Files<2> { v32f a, b; };
+template <> struct Files<3> { v32f a, b, c; };
 template <> struct Files<4> { v32f a, b, c, d; };
 
 // Value file plus T tangent files in one aggregate (two separate aggregates passed by
@@ -68,6 +69,42 @@ __device__ __forceinline__ void sfor(Fn&& fn) {
     [&]<int... I>(std::integer_sequence<int, I...>) {
         (fn(std::integral_constant<int, I>{}), ...);
     }(std::make_integer_sequence<int, N>{});
+}
+
+// Correctly rounded f32 square root: the same bits as __builtin_sqrtf.  The compiler's
+// expansion of that (correctly rounded mode) is the hardware estimate followed by the
+// choice between it and its neighbours one ulp either side by the signs of their exact
+// residuals x - s' * s; around it, it scales inputs below 2^-96 (where the residuals would
+// underflow) and returns +-0 and +inf unchanged, six more VALU operations on every input.
+// sqrt_core is the choice alone.  It gives the expansion's bits for every x with
+// |x| >= 2^-96: +inf, NaN and negative inputs come out of the estimate as they do from the
+// full sequence (a negative input's NaN passes the scaling multiply unchanged).  sqrt_n
+// takes it when no value of the wave is below 2^-96 in magnitude (one compare per value
+// and one uniform branch), and the full sequence otherwise.
+__device__ __forceinline__ float sqrt_core(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const uint32_t sb = __builtin_bit_cast(uint32_t, s);
+    const float sd = __builtin_bit_cast(float, sb - 1u), su = __builtin_bit_cast(float, sb + 1u);
+    const float rd = __builtin_fmaf(-sd, s, x), ru = __builtin_fmaf(-su, s, x);
+    const float r = rd <= 0.0f ? sd : s;
+    return ru > 0.0f ? su : r;
+}
+
+template <int N>
+__device__ __forceinline__ void sqrt_n(float (&v)[N]) {
+    uint64_t slow = 0;
+    sfor<N>([&](auto P) { slow |= __builtin_amdgcn_ballot_w64(__builtin_fabsf(v[P()]) < 0x1p-96f); });
+    if (slow == 0) {
+        sfor<N>([&](auto P) { v[P()] = sqrt_core(v[P()]); });
+    } else {
+        sfor<N>([&](auto P) { v[P()] = __builtin_sqrtf(v[P()]); });
+    }
+}
+
+__device__ __forceinline__ float sqrt1(float x) {
+    float v[1] = {x};
+    sqrt_n<1>(v);
+    return v[0];
 }
 
 // The micro-op at code[i], in SGPRs (one 16-byte scalar load).
@@ -189,17 +226,24 @@ __device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __re
             RSR_CASE(U_MUL, A * B)
             RSR_CASE(U_DIV, A / B)
             RSR_CASE(U_SQR, A * A)
-            RSR_CASE(U_SQRT, __builtin_sqrtf(A))
             RSR_CASE(U_SIN, dm_sin(A))
             RSR_CASE(U_COS, dm_cos(A))
             RSR_CASE(U_EXP, dm_exp(A))
             RSR_CASE(U_LN, dm_ln(A))
             RSR_CASE(U_NEG, -A)
-            RSR_CASE(U_SQRTACC, __builtin_sqrtf(A + (B * B)))
             RSR_CASE(U_RCPSAFE, 1.0f / ((__builtin_fabsf(A) < 1e-9f) ? 1.0f : A))
             RSR_CASE(U_RSUB, B - A)
             RSR_CASE(U_RDIV, B / A)
 #undef RSR_CASE
+            case U_SQRT:
+                sqrt_n<PPL>(A0);
+                scalar_result();
+                break;
+            case U_SQRTACC:
+                sfor<PPL>([&](auto P) { A0[P()] = A0[P()] + (B0[P()] * B0[P()]); });
+                sqrt_n<PPL>(A0);
+                scalar_result();
+                break;
             // point-invariant operand: one evaluation, copied to the lane's other points
 #define RSR_UCASE(OPC, EXPR)                                  \
     case OPC: {                                               \
@@ -213,7 +257,7 @@ __device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __re
             RSR_UCASE(U_COSU, dm_cos(A))
             RSR_UCASE(U_EXPU, dm_exp(A))
             RSR_UCASE(U_LNU, dm_ln(A))
-            RSR_UCASE(U_SQRTU, __builtin_sqrtf(A))
+            RSR_UCASE(U_SQRTU, sqrt1(A))
 #undef RSR_UCASE
             // unary op of a constant: one evaluation, copied to every point of the lane
 #define RSR_UKCASE(OPC, EXPR)                                 \
@@ -228,7 +272,7 @@ __device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __re
             RSR_UKCASE(U_COSK, dm_cos(A))
             RSR_UKCASE(U_EXPK, dm_exp(A))
             RSR_UKCASE(U_LNK, dm_ln(A))
-            RSR_UKCASE(U_SQRTK, __builtin_sqrtf(A))
+            RSR_UKCASE(U_SQRTK, sqrt1(A))
             RSR_UKCASE(U_NEGK, -A)
             RSR_UKCASE(U_SQRK, A * A)
 #undef RSR_UKCASE
@@ -450,8 +494,9 @@ __device__ __forceinline__ void run_program_p1(Files<PPL>& R, const UInstr* __re
             case U_NORM3:
                 sfor<PPL>([&](auto P) {
                     const float x0 = A0[P()], x1 = a1(P), x2 = a2(P);
-                    A0[P()] = __builtin_sqrtf(((x0 * x0) + (x1 * x1)) + (x2 * x2));
+                    A0[P()] = ((x0 * x0) + (x1 * x1)) + (x2 * x2);
                 });
+                sqrt_n<PPL>(A0);
                 scalar_result();
                 break;
             default:
@@ -561,17 +606,24 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             RSR_CASE(U_MUL, A * B)
             RSR_CASE(U_DIV, A / B)
             RSR_CASE(U_SQR, A * A)
-            RSR_CASE(U_SQRT, __builtin_sqrtf(A))
             RSR_CASE(U_SIN, dm_sin(A))
             RSR_CASE(U_COS, dm_cos(A))
             RSR_CASE(U_EXP, dm_exp(A))
             RSR_CASE(U_LN, dm_ln(A))
             RSR_CASE(U_NEG, -A)
-            RSR_CASE(U_SQRTACC, __builtin_sqrtf(A + (B * B)))
             RSR_CASE(U_RCPSAFE, 1.0f / ((__builtin_fabsf(A) < 1e-9f) ? 1.0f : A))
             RSR_CASE(U_RSUB, B - A)
             RSR_CASE(U_RDIV, B / A)
 #undef RSR_CASE
+            case U_SQRT:
+                sqrt_n<PPL>(A0);
+                scalar_result();
+                break;
+            case U_SQRTACC:
+                sfor<PPL>([&](auto P) { A0[P()] = A0[P()] + (B0[P()] * B0[P()]); });
+                sqrt_n<PPL>(A0);
+                scalar_result();
+                break;
             // point-invariant operand: one evaluation, copied to the lane's other points
 #define RSR_UCASE(OPC, EXPR)                                  \
     case OPC: {                                               \
@@ -585,7 +637,7 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             RSR_UCASE(U_COSU, dm_cos(A))
             RSR_UCASE(U_EXPU, dm_exp(A))
             RSR_UCASE(U_LNU, dm_ln(A))
-            RSR_UCASE(U_SQRTU, __builtin_sqrtf(A))
+            RSR_UCASE(U_SQRTU, sqrt1(A))
 #undef RSR_UCASE
             // unary op of a constant: one evaluation, copied to every point of the lane
 #define RSR_UKCASE(OPC, EXPR)                                 \
@@ -600,7 +652,7 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             RSR_UKCASE(U_COSK, dm_cos(A))
             RSR_UKCASE(U_EXPK, dm_exp(A))
             RSR_UKCASE(U_LNK, dm_ln(A))
-            RSR_UKCASE(U_SQRTK, __builtin_sqrtf(A))
+            RSR_UKCASE(U_SQRTK, sqrt1(A))
             RSR_UKCASE(U_NEGK, -A)
             RSR_UKCASE(U_SQRK, A * A)
 #undef RSR_UKCASE
@@ -837,8 +889,9 @@ __device__ __forceinline__ void run_program(Files<PPL>& R, const UInstr* __restr
             case U_NORM3:
                 sfor<PPL>([&](auto P) {
                     const float a0 = A0[P()], a1 = A1[P()], a2 = A2[P()];
-                    A0[P()] = __builtin_sqrtf(((a0 * a0) + (a1 * a1)) + (a2 * a2));
+                    A0[P()] = ((a0 * a0) + (a1 * a1)) + (a2 * a2);
                 });
+                sqrt_n<PPL>(A0);
                 scalar_result();
                 break;
             default:
@@ -957,21 +1010,22 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                 break;
             }
             case U_SQRT: {
-                const float sq = __builtin_sqrtf(va);
+                const float sq = sqrt1(va);
                 rv = sq;
                 const float s2 = 2.0f * sq;
                 RSR_TAN(rg[j] = ga[j] / s2);
                 break;
             }
             case U_SIN: {
-                rv = dm_sin(va);
-                const float cs = dm_cos(va);
+                float cs;
+                dm_sincos(va, rv, cs);
                 RSR_TAN(rg[j] = ga[j] * cs);
                 break;
             }
             case U_COS: {
-                rv = dm_cos(va);
-                const float ns = -dm_sin(va);
+                float sn;
+                dm_sincos(va, sn, rv);
+                const float ns = -sn;
                 RSR_TAN(rg[j] = ga[j] * ns);
                 break;
             }
@@ -1010,7 +1064,7 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                 break;
             }
             case U_SQRTACC: {
-                const float sq = __builtin_sqrtf(va + (vb * vb));
+                const float sq = sqrt1(va + (vb * vb));
                 rv = sq;
                 const float s2 = 2.0f * sq;
                 RSR_TAN(rg[j] = (ga[j] + ((vb * gb[j]) + (gb[j] * vb))) / s2);
@@ -1063,7 +1117,7 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
                 __builtin_assume(a < 30u);
                 const v32f Vv = G.v;
                 const float a1 = Vv[a + 1], a2 = Vv[a + 2];
-                const float sq = __builtin_sqrtf(((va * va) + (a1 * a1)) + (a2 * a2));
+                const float sq = sqrt1(((va * va) + (a1 * a1)) + (a2 * a2));
                 rv = sq;
                 const float s2 = 2.0f * sq;
                 RSR_TAN(rg[j] = ((((va * ga[j]) + (ga[j] * va)) + ((a1 * Gj[a + 1]) + (Gj[a + 1] * a1))) +

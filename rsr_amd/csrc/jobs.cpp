@@ -1118,7 +1118,7 @@ int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int
 // MSE steps: each step at the tuning's points per lane, lowered until its own spill area
 // fits the workgroup LDS (per step, so a request's result does not depend on the others).
 int run_mse(rsr_context* ctx, std::vector<EvalStep>& steps) {
-    std::vector<EvalStep> by_ppl[3];
+    std::vector<EvalStep> by_ppl[5];  // by points per lane (1 to 4)
     for (const auto& s0 : steps) {
         const EvalTuning t = pick_tuning(ctx, s0);
         EvalStep s = s0;
@@ -1128,11 +1128,11 @@ int run_mse(rsr_context* ctx, std::vector<EvalStep>& steps) {
         if (eval_mse_lds_bytes(s.P.spill_slots, ppl) > ctx->max_lds)
             return set_error(RSR_E_UNSUPPORTED, "spill area of %zu bytes exceeds the workgroup LDS limit",
                              eval_mse_lds_bytes(s.P.spill_slots, ppl));
-        by_ppl[ppl == 4 ? 2 : ppl - 1].push_back(s);
+        by_ppl[ppl].push_back(s);
     }
     int rc;
-    for (int k = 2; k >= 0; --k)
-        if (!by_ppl[k].empty() && (rc = launch_rounds(ctx, by_ppl[k], false, k == 2 ? 4 : k + 1))) return rc;
+    for (int k = 4; k >= 1; --k)
+        if (!by_ppl[k].empty() && (rc = launch_rounds(ctx, by_ppl[k], false, k))) return rc;
     return RSR_OK;
 }
 
@@ -1328,7 +1328,7 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
 // Points per lane x programs per work item, each candidate timed on the batch itself
 // (best of two evaluations after a warm-up, events on the context's stream).
 int autotune(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int* ppl_out, int* group_out) {
-    static const int kPpl[] = {1, 2, 4}, kGroup[] = {4, 8, 16, 32};
+    static const int kPpl[] = {1, 2, 3, 4}, kGroup[] = {4, 8, 16, 32};
     const bool saved_auto = ctx->auto_tune;
     const EvalTuning saved = ctx->tuning;
     std::vector<float> out(std::max(p->n, 1));

@@ -50,7 +50,7 @@ inline size_t big_points_floats(int gslots, int channels) {
 
 
 struct EvalTuning {
-    int ppl = 2;        // points per lane: 1, 2 or 4
+    int ppl = 2;        // points per lane: 1 to 4
     int group = 8;      // programs per work item
 };
 

@@ -193,10 +193,13 @@ __device__ __forceinline__ void mse_item(const SegTable& T, int64_t wave, float*
     }
 }
 
-// At four points per lane the register file alone is 128 VGPRs; the occupancy hint
-// makes the allocator fit the kernel in 168 (three waves per SIMD).
+// At four points per lane the register files alone are 128 VGPRs; the occupancy hint
+// makes the allocator fit the kernel in 168 (three waves per SIMD).  At three points per
+// lane (96 VGPRs of files) it fits in 128: four waves per SIMD.
 template <int PPL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_eval_mse(const SegTable T,
+constexpr int kEvalWaves = PPL == 3 ? 4 : 3;
+template <int PPL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kEvalWaves<PPL>))) void k_eval_mse(const SegTable T,
                                                                                            int spill_slots) {
     extern __shared__ float smem[];
     float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * PPL * 64;
@@ -211,6 +214,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // other XCDs' ranges in turn.  Every take is one atomic increment and the ranges are finite,
 // so every wave exits.  The queue is zero at launch and the launch leaves it zero: the last
 // wave to finish (queue[kXcds] counts finished waves) clears it.
+// (The work loop's extra live state does not fit 128 VGPRs at three points per lane: three
+// waves per SIMD for every form.)
 template <int PPL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_eval_mse_q(
     const SegTable T, int spill_slots, int64_t n_waves, int* __restrict__ queue) {
@@ -833,6 +838,7 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
             hipError_t e;
             switch (tune.ppl) {
                 case 1: e = launch_eval_mse<1>(T, waves, spill, lds, queue, s); break;
+                case 3: e = launch_eval_mse<3>(T, waves, spill, lds, queue, s); break;
                 case 4: e = launch_eval_mse<4>(T, waves, spill, lds, queue, s); break;
                 default: e = launch_eval_mse<2>(T, waves, spill, lds, queue, s); break;
             }
@@ -842,6 +848,7 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
             hipError_t e;
             switch (tune.ppl) {
                 case 1: e = launch_big(k_eval_mse_big<1>, T, big_items, gslots, 1, spill, big, s); break;
+                case 3: e = launch_big(k_eval_mse_big<3>, T, big_items, gslots, 3, spill, big, s); break;
                 case 4: e = launch_big(k_eval_mse_big<4>, T, big_items, gslots, 4, spill, big, s); break;
                 default: e = launch_big(k_eval_mse_big<2>, T, big_items, gslots, 2, spill, big, s); break;
             }

@@ -280,7 +280,7 @@ extern "C" int rsr_context_set_tuning(rsr_context* ctx, int ppl, int group) {
         ctx->tune_version++;
         return RSR_OK;
     }
-    if (ppl != 1 && ppl != 2 && ppl != 4) return set_error(RSR_E_INVALID_ARG, "points_per_lane must be 1, 2 or 4");
+    if (ppl < 1 || ppl > 4) return set_error(RSR_E_INVALID_ARG, "points_per_lane must be 1, 2, 3 or 4");
     if (group < 1 || group > 4096) return set_error(RSR_E_INVALID_ARG, "programs_per_item out of range");
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->tuning.ppl = ppl;
@@ -381,7 +381,8 @@ extern "C" int rsr_context_get_tuning(const rsr_context* ctx, int* ppl, int* gro
 // ---------------------------------------------------------------------------------
 static int dataset_upload(rsr_context* ctx, std::unique_ptr<rsr_dataset>& d) {
     d->ctx = ctx;
-    const int64_t npad = std::max<int64_t>(256, (d->count8 + 255) / 256 * 256);
+    // a multiple of every tile size (64 x 1, 2, 3 or 4 points): whole tiles stay in bounds
+    const int64_t npad = std::max<int64_t>(768, (d->count8 + 767) / 768 * 768);
     d->npad = npad;
     // host SoA padded to npad with zeros (tile padding is masked in the kernels)
     std::vector<float> f((size_t)std::max(d->num_features, 1) * npad, 0.0f), t((size_t)npad, 0.0f);

@@ -47,7 +47,7 @@ def c2_case():
     return progs, [20, 21, 22]
 
 
-@pytest.mark.parametrize("ppl", [1, 2, 4])
+@pytest.mark.parametrize("ppl", [1, 2, 3, 4])
 def test_refused_c2_programs_now_evaluate_bit_identical(gpu_ctx, c2_case, ppl):
     progs, big_idx = c2_case
     assert all(_is_big(progs[i], 15) for i in big_idx)

@@ -45,7 +45,7 @@ def check_mse(gpu, ref, tol=1e-5):
     return ok
 
 
-@pytest.mark.parametrize("ppl", [1, 2, 4])
+@pytest.mark.parametrize("ppl", [1, 2, 3, 4])
 def test_known_answers_exactly_zero(gpu_ctx, ppl):
     gpu_ctx.set_tuning(ppl, 8)
     X, y, t = W.data_c1(4000, seed=7)
@@ -106,7 +106,7 @@ def test_c0_population(gpu_ctx):
     assert np.median(rel_err(mse[:100][fin], truth[fin])) < 1e-5
 
 
-@pytest.mark.parametrize("ppl", [1, 2, 4])
+@pytest.mark.parametrize("ppl", [1, 2, 3, 4])
 def test_c1_population(gpu_ctx, ppl):
     X, y, t = W.data_c1(3001, seed=1001)
     progs = W.gen_c1(256, seed=2001)
@@ -126,7 +126,7 @@ def test_all_tunings_bitwise_identical_mse(gpu_ctx):
     ds, _ = make(gpu_ctx, X, y, t)
     P = api.Programs(gpu_ctx, progs)
     outs = []
-    for ppl in (1, 2, 4):
+    for ppl in (1, 2, 3, 4):
         for g in (1, 8, 32):
             gpu_ctx.set_tuning(ppl, g)
             outs.append(api.compute_mse(gpu_ctx, P, ds))
@@ -282,7 +282,7 @@ def test_full_size_c1_properties(gpu_ctx):
     assert np.array_equal(mse2, kahan[perm])
 
 
-@pytest.mark.parametrize("ppl", [1, 2, 4])
+@pytest.mark.parametrize("ppl", [1, 2, 3, 4])
 @pytest.mark.parametrize("gen,data,npts", [(W.gen_c0, W.data_c0, 403), (W.gen_c1, W.data_c1, 1001),
                                             (W.gen_c2, W.data_c2, 500), (W.gen_c3, W.data_c3, 777)])
 def test_reference_order_summation_bit_identical(gpu_ctx, gen, data, npts, ppl):
@@ -429,7 +429,7 @@ def test_persistent_and_oneshot_launches_agree(gpu_ctx, reference):
     gpu_ctx.set_summation(reference)
     try:
         outs = []
-        for ppl in (1, 2, 4):
+        for ppl in (1, 2, 3, 4):
             gpu_ctx.set_tuning(ppl, 8)
             for mode in (api.LAUNCH_PERSISTENT, api.LAUNCH_ONESHOT, api.LAUNCH_PERSISTENT):
                 gpu_ctx.set_launch_mode(mode)
@@ -438,7 +438,7 @@ def test_persistent_and_oneshot_launches_agree(gpu_ctx, reference):
         gpu_ctx.set_launch_mode(api.LAUNCH_ONESHOT)
         gpu_ctx.set_tuning(2, 8)
         gpu_ctx.set_summation(False)
-    for ppl in (1, 2, 4):
+    for ppl in (1, 2, 3, 4):
         same = [o for p, o in outs if p == ppl]
         assert all(np.array_equal(same[0], o) for o in same[1:]), ppl
     if reference:
