@@ -3,7 +3,8 @@
 // Clients submit requests (any thread).  The server runs lanes: each lane is an internal
 // context (own stream, scratch and tick counters) with its own loop thread.  A lane admits
 // queued requests in arrival order as in-flight slots and device memory free up -- large
-// requests to the large-request lanes, small ones to the others -- builds each admitted
+// requests to the large-request lanes, small ones to the others, never both kinds to one
+// lane -- builds each admitted
 // request's job (jobs.h), and advances its jobs by one tick per iteration: all their
 // evaluation steps go into shared launches, then each job's optimizer update runs, then
 // one synchronisation of the lane's stream.  Lanes tick independently, so their launches
@@ -90,9 +91,8 @@ int make_job(rsr_context* ctx, Pending& r, std::unique_ptr<Job>* out) {
 // for a large request's step.
 struct Lane {
     rsr_context* ctx = nullptr;
-    bool big = false;              // takes the large requests (and small ones when idle)
+    bool big = false;              // takes the large requests (else the small ones)
     std::vector<Active> active;    // this lane's thread only
-    std::vector<uint8_t> is_big;   // per active request: estimated work above kBigWork
     std::thread th;
     uint64_t seen_version = ~0ull;  // the owner's tuning settings last copied
 };
@@ -163,15 +163,12 @@ struct rsr_server {
         st.failed++;
     }
 
-    // Whether lane L takes request q now: large requests go to the big lanes, small ones to
-    // the small lanes or to a big lane with nothing large in flight.
-    bool takes(const Lane& L, bool big_req) const {
-        if (big_req) return L.big;
-        if (!L.big) return true;
-        for (uint8_t b : L.is_big)
-            if (b) return false;
-        return true;
-    }
+    // Whether lane L takes request q: large requests go to the big lanes, small ones to the
+    // small lanes, never both kinds to one lane.  A lane advances its requests' ticks
+    // together, so a small request in a lane with a large one would take a tick per large
+    // step; with the classes apart, a small request shares the device with large ones but
+    // never waits for their steps.  (A server of one lane takes everything.)
+    bool takes(const Lane& L, bool big_req) const { return lanes.size() == 1 || big_req == L.big; }
 
     // Admit queued requests into lane L in arrival order (skipping ones whose program batch
     // is busy or that belong to another lane).
@@ -179,7 +176,6 @@ struct rsr_server {
         const int lane_cap = std::max(1, max_inflight / (int)lanes.size());
         for (;;) {
             Pending p;
-            bool big_req = false;
             {
                 std::lock_guard<std::mutex> g(m);
                 if (inflight >= max_inflight || (int)L.active.size() >= lane_cap) return;
@@ -191,10 +187,7 @@ struct rsr_server {
                     seen.insert(pr);  // keep per-batch order: later requests on it wait too
                     if (blocked) continue;
                     const bool b = request_work(it->req) >= kBigWork;
-                    if (takes(L, b)) {
-                        big_req = b;
-                        break;
-                    }
+                    if (takes(L, b)) break;
                 }
                 if (it == queue.end()) return;
                 p = std::move(*it);
@@ -246,7 +239,6 @@ struct rsr_server {
             a.t_submit = p.t_submit;
             a.t_admit = now_s();
             L.active.push_back(std::move(a));
-            L.is_big.push_back(big_req ? 1 : 0);
             std::lock_guard<std::mutex> g(m);
             inflight_bytes += bytes;
             if (inflight++ == 0) busy_since = now_s();
@@ -302,7 +294,6 @@ struct rsr_server {
                 complete(L, i, status, now_s());
                 a.job.reset();
                 L.active.erase(L.active.begin() + (long)i);
-                L.is_big.erase(L.is_big.begin() + (long)i);
                 any_done = true;
             }
             {
