@@ -440,6 +440,9 @@ def main():
             fifo = mix.run_fifo(ctx, jobs, generations=args.serve_gens)
             serving["plain_fifo"] = {k: fifo[k] for k in ("wall_s", "p50_gen_latency_ms", "p99_gen_latency_ms",
                                                           "eval_node_evals_per_s")}
+        log("[bench] small requests: 256 populations of 1000 programs over 400-4000 points, "
+            "one at a time vs all served at once")
+        serving["small_requests"] = mix.small_requests(ctx, pools)
         if plain:
             serving["served_over_plain"] = serving["eval_node_evals_per_s"] / plain["value"]
             serving["saturation_over_plain"] = sat["eval_node_evals_per_s"] / plain["value"]

@@ -130,16 +130,17 @@ RSR_API int rsr_context_set_partials_limit(rsr_context* ctx, uint64_t bytes);
  * per work item (the programs that share one tile's features in registers).  By default
  * (or after set_tuning(ctx, 0, 0)) each evaluation takes the tuning that rsr_autotune
  * found for its workload shape -- feature count, point count and program count, each
- * rounded to a power of two -- or, before any, a heuristic (the most points per lane that
- * still gives the chip about four waves per SIMD).  A nonzero setting fixes the tuning for
+ * rounded to a power of two -- or, before any, a heuristic (four points per lane, two for
+ * small evaluations, three for Mat3-sized feature sets; as many programs per work item as
+ * keep 65,536 work items).  A nonzero setting fixes the tuning for
  * every evaluation; get_tuning returns 0, 0 in the per-workload mode.  No reference
  * counterpart (tuning of the device launches). */
 RSR_API int rsr_context_set_tuning(rsr_context* ctx, int points_per_lane, int programs_per_item);
 RSR_API int rsr_context_get_tuning(const rsr_context* ctx, int* points_per_lane, int* programs_per_item);
-/* The autotuner: times every tuning (points per lane 1 to 4 x programs per work item 4, 8,
- * 16, 32) on one MSE evaluation of this batch over this dataset and keeps the fastest for
+/* The autotuner: times every tuning (points per lane 1 to 4 x programs per work item 1, 2,
+ * 4, 8, 16, 32) on one MSE evaluation of this batch over this dataset and keeps the fastest for
  * the workload's shape, where direct calls, the server and the fitters find it; returns
- * the choice.  Costs about 48 evaluations.  No reference counterpart. */
+ * the choice.  Costs about 72 evaluations.  No reference counterpart. */
 RSR_API int rsr_autotune(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, int* points_per_lane,
                          int* programs_per_item);
 

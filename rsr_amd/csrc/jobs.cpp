@@ -126,6 +126,7 @@ class EvalJob final : public Job {
     std::vector<uint8_t> on;
     int n_bad = 0;
     uint64_t nodes = 0;
+    const float* staged = nullptr;  // results in pinned host memory (two-phase finish)
 
     int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>&) override {
         EvalStep st = mse_step(p, L, ds, d_act, d_mse);
@@ -139,10 +140,21 @@ class EvalJob final : public Job {
         return RSR_OK;
     }
     bool needs_counters() const override { return false; }
-    int finish() override {
-        HIP_TRY(hipMemcpyAsync(out, d_mse, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+    size_t stage_bytes() const override { return (size_t)p->n * sizeof(float); }
+    int stage(char* host) override {
+        HIP_TRY(hipMemcpyAsync(host, d_mse, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
                 "mse download");
-        HIP_TRY(hipStreamSynchronize(ctx->stream), "mse download");
+        staged = (const float*)host;
+        return RSR_OK;
+    }
+    int finish() override {
+        if (staged) {
+            memcpy(out, staged, (size_t)p->n * sizeof(float));
+        } else {
+            HIP_TRY(hipMemcpyAsync(out, d_mse, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+                    "mse download");
+            HIP_TRY(hipStreamSynchronize(ctx->stream), "mse download");
+        }
         for (int i = 0; i < p->n; ++i)
             if (!on[i]) out[i] = NAN;
         if (n_bad) return set_error(RSR_E_PROGRAM, "%d program(s) rejected by lowering", n_bad);
@@ -586,7 +598,9 @@ int make_eval_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, cons
         if (!j->on[i]) all = false;
         if (j->on[i]) j->nodes += prog_nodes(p, i);
     }
-    if ((rc = j->jb.get((size_t)p->n, &j->d_mse))) return rc;
+    // not cleared: the evaluation writes the MSE of every evaluated program, and finish()
+    // sets the others
+    if ((rc = j->jb.raw((size_t)p->n * sizeof(float), (void**)&j->d_mse, false))) return rc;
     if (!all && ((rc = j->jb.put(j->on, &j->d_act)) || (rc = subset_order(j->jb, j->L, j->on, &j->d_order, &j->n_order, &j->n_order_big))))
         return rc;
     *out = std::move(j);
@@ -1328,7 +1342,7 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
 // Points per lane x programs per work item, each candidate timed on the batch itself
 // (best of two evaluations after a warm-up, events on the context's stream).
 int autotune(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int* ppl_out, int* group_out) {
-    static const int kPpl[] = {1, 2, 3, 4}, kGroup[] = {4, 8, 16, 32};
+    static const int kPpl[] = {1, 2, 3, 4}, kGroup[] = {1, 2, 4, 8, 16, 32};
     const bool saved_auto = ctx->auto_tune;
     const EvalTuning saved = ctx->tuning;
     std::vector<float> out(std::max(p->n, 1));

@@ -61,6 +61,12 @@ class Job {
     virtual int advance(const int* counters) = 0;
     // Copy results to the caller's buffers (stream is synchronised afterwards).
     virtual int finish() = 0;
+    // Two-phase finish for many jobs at once: stage_bytes() host bytes of results, which
+    // stage() enqueues into pinned host memory without synchronising; after the caller has
+    // synchronised the stream, finish() copies them out with no synchronisation of its own.
+    // A job that returns 0 finishes in one phase.
+    virtual size_t stage_bytes() const { return 0; }
+    virtual int stage(char*) { return RSR_OK; }
     // Device bytes the job holds (admission accounting).
     virtual size_t device_bytes() const = 0;
     // Whether this tick must read the job's counters back (a host synchronisation) for

@@ -260,6 +260,7 @@ extern "C" int rsr_context_destroy(rsr_context* ctx) {
     ctx->queue.release();
     rsr::trim_pool(ctx);
     if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
+    if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RSR_OK;
@@ -300,19 +301,24 @@ uint64_t rsr::tuning_key(int num_features, int64_t count8, int n_progs) {
     return ((uint64_t)num_features << 16) | ((uint64_t)ceil_log2(count8) << 8) | (uint64_t)ceil_log2(n_progs);
 }
 
-// Heuristic: the most points per lane (fewest waves per point, dispatch amortised over the
-// most points) that still gives the chip about four waves per SIMD (4,096 work items),
-// eight programs per item; small evaluations take one point per lane and four programs.
-static EvalTuning heuristic_tuning(int64_t count8, int n_progs) {
+// Heuristic, fitted to timings of every tuning on MI355X over the four workload types
+// (400 to 10^6 points, 1,000 to 16,384 programs): four points per lane (the dispatch
+// amortised over the most points) when that still gives 8,000 (tile, program) work items,
+// else two; three instead of four for 12 or more feature components (Mat3 workloads: less
+// register pressure, 8% faster there); then the most programs per work item (1 to 32) that
+// keeps 65,536 work items, else one.  Small evaluations thus get many one-program items
+// (up to 3x faster than eight-program items there), large ones 8 to 32 programs per item.
+static EvalTuning heuristic_tuning(int num_features, int64_t count8, int n_progs) {
+    auto tiles = [&](int ppl) { return (count8 + 64 * ppl - 1) / (64 * ppl); };
     EvalTuning t;
-    for (int ppl = 4; ppl >= 1; ppl /= 2) {
-        const int64_t tiles = (count8 + 64 * ppl - 1) / (64 * ppl);
-        if (tiles * ((n_progs + 7) / 8) >= 4096 || ppl == 1) {
-            t.ppl = ppl;
-            t.group = (ppl == 1 && tiles * ((n_progs + 7) / 8) < 4096) ? 4 : 8;
-            return t;
+    const int64_t np = std::max(n_progs, 1);
+    t.ppl = tiles(4) * np >= 8000 ? (num_features >= 12 ? 3 : 4) : 2;
+    t.group = 1;
+    for (int g = 32; g >= 2; g /= 2)
+        if (tiles(t.ppl) * ((np + g - 1) / g) >= 65536) {
+            t.group = g;
+            break;
         }
-    }
     return t;
 }
 
@@ -320,7 +326,7 @@ EvalTuning rsr::pick_tuning(const rsr_context* ctx, const EvalStep& st) {
     if (!ctx->auto_tune) return ctx->tuning;
     auto it = ctx->tune_cache.find(tuning_key(st.D.num_features, st.D.count8, st.P.n_progs));
     if (it != ctx->tune_cache.end()) return it->second;
-    return heuristic_tuning(st.D.count8, st.P.n_progs);
+    return heuristic_tuning(st.D.num_features, st.D.count8, st.P.n_progs);
 }
 
 extern "C" int rsr_context_set_summation(rsr_context* ctx, int mode) {

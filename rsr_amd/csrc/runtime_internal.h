@@ -102,6 +102,10 @@ struct rsr_context {
     // pinned host mirror of the jobs' counters a tick reads back (each job owns its device pair)
     int* host_counters = nullptr;
     size_t host_counters_cap = 0;
+    // pinned host staging for the results of the jobs a server lane finishes in one tick
+    // (downloaded together, one synchronisation)
+    char* host_stage = nullptr;
+    size_t host_stage_cap = 0;
     std::atomic<int> live_servers{0};  // a context is not destroyed under a running server
     std::unordered_set<uint64_t> live_datasets;  // ids of the context's datasets (guarded by mu)
     uint64_t graph_clock = 0;                     // graph replays, for the graphs' last_use

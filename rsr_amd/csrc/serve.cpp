@@ -91,8 +91,9 @@ int make_job(rsr_context* ctx, Pending& r, std::unique_ptr<Job>* out) {
 // for a large request's step.
 struct Lane {
     rsr_context* ctx = nullptr;
-    bool big = false;              // takes the large requests (else the small ones)
+    bool big = false;              // takes the large requests (and small ones while none waits)
     std::vector<Active> active;    // this lane's thread only
+    std::vector<uint8_t> is_big;   // per active request: estimated work above kBigWork
     std::thread th;
     uint64_t seen_version = ~0ull;  // the owner's tuning settings last copied
 };
@@ -163,12 +164,22 @@ struct rsr_server {
         st.failed++;
     }
 
-    // Whether lane L takes request q: large requests go to the big lanes, small ones to the
-    // small lanes, never both kinds to one lane.  A lane advances its requests' ticks
-    // together, so a small request in a lane with a large one would take a tick per large
-    // step; with the classes apart, a small request shares the device with large ones but
-    // never waits for their steps.  (A server of one lane takes everything.)
-    bool takes(const Lane& L, bool big_req) const { return lanes.size() == 1 || big_req == L.big; }
+    // Whether lane L takes request q: never large and small requests in one lane.  A lane
+    // advances its requests' ticks together, so a small request in a lane with a large one
+    // would take a tick per large step; with the classes apart, a small request shares the
+    // device with large ones but never waits for their steps.  Large requests go to the big
+    // lanes, to one that holds no small request; small ones to the small lanes, and to a
+    // big lane that holds no large request while no large request waits (so the big lanes
+    // help with a stream of small requests, and a large one waits at most for the small
+    // ones a big lane holds to finish).  A server of one lane takes everything.
+    bool takes(const Lane& L, bool big_req, bool large_waiting) const {
+        if (lanes.size() == 1) return true;
+        bool holds_big = false, holds_small = false;
+        for (uint8_t b : L.is_big) (b ? holds_big : holds_small) = true;
+        if (big_req) return L.big && !holds_small;
+        if (!L.big) return true;
+        return !holds_big && !large_waiting;
+    }
 
     // Admit queued requests into lane L in arrival order (skipping ones whose program batch
     // is busy or that belong to another lane).
@@ -176,9 +187,12 @@ struct rsr_server {
         const int lane_cap = std::max(1, max_inflight / (int)lanes.size());
         for (;;) {
             Pending p;
+            bool big_req = false;
             {
                 std::lock_guard<std::mutex> g(m);
                 if (inflight >= max_inflight || (int)L.active.size() >= lane_cap) return;
+                bool large_waiting = false;
+                for (const Pending& q : queue) large_waiting = large_waiting || request_work(q.req) >= kBigWork;
                 auto it = queue.begin();
                 std::unordered_set<rsr_programs*> seen;
                 for (; it != queue.end(); ++it) {
@@ -187,7 +201,10 @@ struct rsr_server {
                     seen.insert(pr);  // keep per-batch order: later requests on it wait too
                     if (blocked) continue;
                     const bool b = request_work(it->req) >= kBigWork;
-                    if (takes(L, b)) break;
+                    if (takes(L, b, large_waiting)) {
+                        big_req = b;
+                        break;
+                    }
                 }
                 if (it == queue.end()) return;
                 p = std::move(*it);
@@ -239,6 +256,7 @@ struct rsr_server {
             a.t_submit = p.t_submit;
             a.t_admit = now_s();
             L.active.push_back(std::move(a));
+            L.is_big.push_back(big_req ? 1 : 0);
             std::lock_guard<std::mutex> g(m);
             inflight_bytes += bytes;
             if (inflight++ == 0) busy_since = now_s();
@@ -261,6 +279,41 @@ struct rsr_server {
         L.seen_version = ctx->tune_version.load();
     }
 
+    // The results of the requests that finished this tick, downloaded together into the
+    // lane's pinned staging memory with one synchronisation (their finish() then copies
+    // them out without one of its own).  Requests whose jobs do not stage finish as before.
+    int stage_results(Lane& L) {
+        auto round_up = [](size_t b) { return (b + 255) / 256 * 256; };
+        size_t need = 0;
+        for (auto& a : L.active)
+            if (a.job->done) need += round_up(a.job->stage_bytes());
+        if (need == 0) return RSR_OK;
+        rsr_context* c = L.ctx;
+        if (c->host_stage_cap < need) {  // nothing is in flight into the old block: the last tick synchronised
+            if (c->host_stage) (void)hipHostFree(c->host_stage);
+            c->host_stage = nullptr;
+            c->host_stage_cap = 0;
+            const size_t cap = std::max<size_t>(round_up(need) * 2, (size_t)1 << 20);
+            if (hipHostMalloc((void**)&c->host_stage, cap, hipHostMallocDefault) != hipSuccess) {
+                c->host_stage = nullptr;
+                return RSR_OK;  // no staging memory: every job finishes in one phase
+            }
+            c->host_stage_cap = cap;
+        }
+        char* at = c->host_stage;
+        for (auto& a : L.active) {
+            if (!a.job->done) continue;
+            const size_t b = a.job->stage_bytes();
+            if (b == 0) continue;
+            const int rc = a.job->stage(at);
+            if (rc != RSR_OK) return rc;
+            at += round_up(b);
+        }
+        const hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return set_error(RSR_E_HIP, "result download: %s", hipGetErrorString(e));
+        return RSR_OK;
+    }
+
     void loop(Lane& L) {
         (void)hipSetDevice(L.ctx->device);
         for (;;) {
@@ -281,6 +334,7 @@ struct rsr_server {
             for (auto& a : L.active)
                 if (!a.job->done) jobs.push_back(a.job.get());
             int rc = run_tick(L.ctx, jobs);
+            if (rc == RSR_OK) rc = stage_results(L);
             if (rc != RSR_OK) (void)hipStreamSynchronize(L.ctx->stream);
             // complete finished jobs (every job fails with a failed tick)
             bool any_done = false;
@@ -294,6 +348,7 @@ struct rsr_server {
                 complete(L, i, status, now_s());
                 a.job.reset();
                 L.active.erase(L.active.begin() + (long)i);
+                L.is_big.erase(L.is_big.begin() + (long)i);
                 any_done = true;
             }
             {

@@ -230,6 +230,56 @@ def run_fifo(ctx, jobs, generations=3, fit_frac=0.01, min_fit=8, opt_iters=100):
     return summarize(jobs, wall, None, generations)
 
 
+def small_requests(ctx, pools, n_req=256, seed=4200, n_range=(400, 4000), pop=1000, reps=3):
+    """Many small evaluation requests, the traffic of islands and small regression jobs:
+    n_req populations of `pop` programs, each over its own dataset of log-uniform size in
+    n_range, every workload type.  plain: one request at a time through rsr_eval_mse (one
+    population per launch); served: all submitted to one server at once, their steps merged
+    into shared launches.  Same work, the library's default summation; returns the best of
+    `reps` timings of each and whether every served MSE equals its direct one."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for j in range(n_req):
+        k = int(rng.integers(0, 4))
+        n = int(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        X, y, t = DATA[k](n, seed=6000 + j)
+        ds = api.Dataset.new(ctx, X, y, t)
+        progs = [pools[k][i] for i in rng.integers(0, len(pools[k]), pop)]
+        P = api.Programs(ctx, progs)
+        ok = P.status(ds) == 0
+        P.close()
+        progs = [q for q, good in zip(progs, ok) if good]
+        reqs.append((ds, api.Programs(ctx, progs), _nodes(progs, n)))
+    nodes = sum(r[2] for r in reqs)
+    direct = [api.compute_mse(ctx, P, ds) for ds, P, _ in reqs]  # warm-up: lowerings, tunings
+    best_plain = best_served = None
+    same = True
+    srv = api.Server(ctx, max_inflight=n_req, budget_bytes=HBM_BUDGET)
+    try:
+        for _ in range(reps):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for ds, P, _ in reqs:
+                api.compute_mse(ctx, P, ds)
+            dt = time.perf_counter() - t0
+            best_plain = dt if best_plain is None else min(best_plain, dt)
+            t0 = time.perf_counter()
+            tickets = [srv.submit_eval(P, ds) for ds, P, _ in reqs]
+            outs = [np.array(tk.wait()) for tk in tickets]
+            dt = time.perf_counter() - t0
+            best_served = dt if best_served is None else min(best_served, dt)
+            same = same and all(np.array_equal(o.view(np.uint32), d.view(np.uint32)) for o, d in zip(outs, direct))
+    finally:
+        srv.close()
+        for ds, P, _ in reqs:
+            P.close()
+            ds.close()
+    return {"requests": n_req, "programs_per_request": pop, "points": list(n_range), "node_evals": nodes,
+            "plain_s": best_plain, "served_s": best_served,
+            "plain_node_evals_per_s": nodes / best_plain, "served_node_evals_per_s": nodes / best_served,
+            "served_over_plain": best_plain / best_served, "served_equals_direct": bool(same)}
+
+
 def summarize(jobs, wall, stats, generations):
     lat = np.array([x for jb in jobs for x in jb.latencies], np.float64)
     eval_nodes = sum(_nodes(jb.programs, jb.n_points) for jb in jobs if jb.latencies) * generations

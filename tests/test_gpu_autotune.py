@@ -27,7 +27,7 @@ def test_autotune_choice_is_used_by_per_workload_mode():
         P = api.Programs(ctx, progs)
         print("autotune: 512 programs x 20000 points", flush=True)
         ppl, group = ctx.autotune(P, ds)
-        assert ppl in (1, 2, 3, 4) and group in (4, 8, 16, 32)
+        assert ppl in (1, 2, 3, 4) and group in (1, 2, 4, 8, 16, 32)
         assert ctx.tuning() == (0, 0)
         ctx.set_summation(False)
         auto = api.compute_mse(ctx, P, ds)

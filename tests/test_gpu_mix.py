@@ -88,3 +88,17 @@ def test_reduced_mix_served_equals_direct_and_oracle(gpu_ctx):
         gpu_ctx.set_summation(False)
         for job in jobs:
             job[3].close()
+
+
+def test_small_requests_served_together_equal_direct(gpu_ctx):
+    """Many small evaluation requests (every workload type, 400 to 4,000 points) submitted
+    at once: the server merges their steps into shared launches, and every request's MSEs
+    equal its direct evaluation bit for bit."""
+    pools = [g(512, seed=4300 + k) for k, g in enumerate(mix.GEN)]
+    gpu_ctx.set_summation(True)
+    try:
+        r = mix.small_requests(gpu_ctx, pools, n_req=48, pop=200, reps=1)
+    finally:
+        gpu_ctx.set_summation(False)
+    print("small requests:", r, flush=True)
+    assert r["served_equals_direct"]
