@@ -3,6 +3,7 @@
 #include "jobs.h"
 
 #include <math.h>
+#include <float.h>
 #include <string.h>
 
 #include <algorithm>
@@ -155,8 +156,10 @@ class EvalJob final : public Job {
                     "mse download");
             HIP_TRY(hipStreamSynchronize(ctx->stream), "mse download");
         }
+        // over a dataset without points no tile runs: the reference's 0 / 0 is not finite,
+        // which it reports as f32::MAX (evaluator.rs:173)
         for (int i = 0; i < p->n; ++i)
-            if (!on[i]) out[i] = NAN;
+            out[i] = !on[i] ? NAN : ds->num_samples == 0 ? FLT_MAX : out[i];
         if (n_bad) return set_error(RSR_E_PROGRAM, "%d program(s) rejected by lowering", n_bad);
         return RSR_OK;
     }
@@ -211,8 +214,14 @@ class GradJob final : public Job {
                                ctx->stream),
                 "grad download");
         HIP_TRY(hipStreamSynchronize(ctx->stream), "grad download");
-        for (int i = 0; i < p->n; ++i)
-            if (!on[i]) mse_out[i] = NAN;
+        for (int i = 0; i < p->n; ++i) {
+            if (!on[i]) {
+                mse_out[i] = NAN;
+            } else if (ds->num_samples == 0) {  // (f32::MAX, 0) for a non-finite MSE (evaluator.rs:214-216)
+                mse_out[i] = FLT_MAX;
+                for (int k = 0; k < 32; ++k) grad_out[(size_t)i * 32 + k] = 0.0f;
+            }
+        }
         if (n_bad) return set_error(RSR_E_PROGRAM, "%d program(s) rejected by lowering", n_bad);
         return RSR_OK;
     }
@@ -655,6 +664,7 @@ int make_grad_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, cons
 
 int make_lbfgs_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, int history, int mode,
                    const uint8_t* mask, float* fitness_out, std::unique_ptr<Job>* out) {
+    if (ds->num_samples == 0) return set_error(RSR_E_INVALID_ARG, "the dataset has no points to fit on");
     if (history < 1 || history > 8 || iterations < 0)
         return set_error(RSR_E_INVALID_ARG, "history must be 1..8, iterations >= 0");
     if (mode != RSR_LBFGS_REFERENCE && mode != RSR_LBFGS_FP64) return set_error(RSR_E_INVALID_ARG, "bad mode");
@@ -731,6 +741,7 @@ int make_lbfgs_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
 int make_cmaes_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, int lambda, int mu,
                    uint64_t seed, const float* normals, const uint8_t* mask, float* fitness_out, int32_t* iters_out,
                    std::unique_ptr<Job>* out) {
+    if (ds->num_samples == 0) return set_error(RSR_E_INVALID_ARG, "the dataset has no points to fit on");
     if (lambda < 2 || lambda > kMaxLambda || mu < 1 || mu > lambda || iterations < 0)
         return set_error(RSR_E_INVALID_ARG, "lambda must be 2..%d and 1 <= mu <= lambda", kMaxLambda);
     auto j = std::make_unique<CmaesJob>(ctx);
@@ -880,6 +891,7 @@ int make_cmaes_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
 
 int make_nm_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, const uint8_t* mask,
                 float* fitness_out, int32_t* iters_out, std::unique_ptr<Job>* out) {
+    if (ds->num_samples == 0) return set_error(RSR_E_INVALID_ARG, "the dataset has no points to fit on");
     if (iterations < 0) return set_error(RSR_E_INVALID_ARG, "iterations must be >= 0");
     auto j = std::make_unique<NmJob>(ctx);
     j->p = p;
@@ -998,6 +1010,7 @@ int make_nm_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int it
 int make_optimize_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int iterations, uint64_t seed,
                       float threshold, const uint8_t* mask, float* mse_out, uint8_t* method_out,
                       std::unique_ptr<Job>* out) {
+    if (ds->num_samples == 0) return set_error(RSR_E_INVALID_ARG, "the dataset has no points to fit on");
     if (iterations < 0) return set_error(RSR_E_INVALID_ARG, "iterations must be >= 0");
     if (!(threshold >= 0.0f)) return set_error(RSR_E_INVALID_ARG, "threshold must be >= 0");
     auto j = std::make_unique<OptimizeJob>(ctx);
@@ -1246,6 +1259,7 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
     int rc = get_lowering(p, ds->num_features, false, &L);
     if (rc) return rc;
     if (L->n_big > 0) return RSR_E_UNSUPPORTED;  // big programs run eagerly (context spill area)
+    if (ds->num_samples == 0) return RSR_E_UNSUPPORTED;  // nothing to launch: the eager job fills f32::MAX
     const int summ = ctx->summation;
     const bool terms = summ == RSR_SUM_REFERENCE;
     int n_ok = 0;

@@ -11,6 +11,7 @@
 // Every call on one GPU shares that array (its work queue included), so calls that overlap
 // on one GPU race: calls through one context are serialised by the context (one lock, one
 // stream); calls through two contexts on the same GPU must not overlap.
+#include <float.h>
 #include <math.h>
 #include <mutex>
 
@@ -44,9 +45,12 @@ int scratch_for_current_device(int device, float** out) {
 }
 
 // NaN for every program the lowering rejected (the evaluation launch skips them).
-__global__ void k_fill_rejected(float* __restrict__ out, const uint8_t* __restrict__ active, int n) {
+// Rejected programs get NaN; over a dataset without points every evaluated program gets
+// f32::MAX (the reference's 0 / 0 is not finite, evaluator.rs:173).
+__global__ void k_fill_rejected(float* __restrict__ out, const uint8_t* __restrict__ active, int n, int empty) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && !active[i]) out[i] = __builtin_nanf("");
+    else if (i < n && empty) out[i] = FLT_MAX;
 }
 
 }  // namespace
@@ -109,7 +113,7 @@ extern "C" RSR_API int rsr_eval_mse_noalloc(rsr_context* ctx, rsr_programs* p, c
         big.floats = cap - part_aligned;
     }
     hipLaunchKernelGGL(k_fill_rejected, dim3((unsigned)((p->n + 255) / 256)), dim3(256), 0, ctx->stream,
-                       mse_out_device, st.active, p->n);
+                       mse_out_device, st.active, p->n, ds->num_samples == 0 ? 1 : 0);
     if ((e = hipGetLastError()) != hipSuccess) return hip_error(e, "rejected-program fill");
     if ((e = launch_mse_steps(&st, 1, t, terms, ctx->stream, nullptr, big, queue)) != hipSuccess)
         return hip_error(e, "no-alloc evaluation launch");

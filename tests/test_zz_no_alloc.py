@@ -105,3 +105,20 @@ def test_noalloc_refuses_what_does_not_fit(gpu_ctx):
     out.close()
     P.close()
     ds.close()
+
+
+def test_noalloc_dataset_without_points(gpu_ctx):
+    """Over a dataset without points every evaluated program reads f32::MAX (the reference's
+    0 / 0 is not finite), as from rsr_eval_mse; rejected programs read NaN."""
+    X, y, t = W.data_c0(16, seed=81)
+    progs = W.gen_c0(40, seed=82) + [Program([(Op.LoadVarF, 3), (Op.SinF, 0)], [])]
+    ds = api.Dataset.new(gpu_ctx, X[:0], y[:0], t)
+    P = api.Programs(gpu_ctx, progs)
+    try:
+        want = api.compute_mse(gpu_ctx, P, ds, allow_rejected=True)
+        got = _run_noalloc(gpu_ctx, P, ds, len(progs))
+        assert np.array_equal(_bits(got), _bits(want))
+        assert (want[:-1] == np.finfo(f32).max).all() and np.isnan(want[-1])
+    finally:
+        P.close()
+        ds.close()
