@@ -250,13 +250,23 @@ class LbfgsJob final : public Job {
     float *d_mse = nullptr, *d_grad = nullptr, *d_fit = nullptr;
     int32_t *order_p = nullptr, *order_d = nullptr;  // scheduling orders over the fitted programs
     int big_p = 0, big_d = 0;
+    DevPrograms CP{};           // one-round line search: the candidate batch (st.K per individual)
+    float* d_cmse = nullptr;
     float* fitness_out = nullptr;
     int iters = 0, it = 0, ls_round = 0;
     Stage stage = INIT;
     uint64_t mean_nodes = 0;
 
     int plan(std::vector<EvalStep>& mse, std::vector<EvalStep>& grad) override {
-        if (stage == SEARCH) {
+        if (stage == SEARCH && st.K) {
+            HIP_TRY(lbfgs_ls_fill(st, ctx->stream), "lbfgs_ls_fill");
+            EvalStep s;
+            s.P = CP;
+            s.D = dev_view(ds);
+            s.active = st.cand_active;
+            s.mse_out = d_cmse;
+            mse.push_back(s);
+        } else if (stage == SEARCH) {
             EvalStep st = mse_step(p, Lp, ds, act_ls, d_mse);
             set_order(st, order_p, (int)prog.size(), big_p);
             mse.push_back(st);
@@ -276,7 +286,10 @@ class LbfgsJob final : public Job {
                     HIP_TRY(lbfgs_direction(st, params, act_ls, d_counters + 1, ctx->stream), "lbfgs_direction");
                 break;
             case SEARCH:
-                HIP_TRY(lbfgs_ls_check(st, params, d_mse, act_ls, act_grad, d_counters, ctx->stream), "ls_check");
+                if (st.K)
+                    HIP_TRY(lbfgs_ls_pick(st, params, d_cmse, act_ls, act_grad, d_counters, ctx->stream), "ls_pick");
+                else
+                    HIP_TRY(lbfgs_ls_check(st, params, d_mse, act_ls, act_grad, d_counters, ctx->stream), "ls_check");
                 break;
             case GRAD:
                 HIP_TRY(lbfgs_update(st, d_grad, act_grad, ctx->stream), "lbfgs_update");
@@ -307,7 +320,7 @@ class LbfgsJob final : public Job {
                 }
                 break;
             case SEARCH:
-                if (++ls_round == 12 || (c && c[0] == 0)) stage = GRAD;
+                if (++ls_round == 12 || st.K || (c && c[0] == 0)) stage = GRAD;
                 break;
             case GRAD:
                 if (++it >= iters || (c && c[1] == 0)) {
@@ -734,6 +747,54 @@ int make_lbfgs_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
         (rc = subset_order(sc, j->Lp, fitmask, &j->order_p, &cnt, &j->big_p)) ||
         (rc = subset_order(sc, j->Ld, fitmask, &j->order_d, &cnt, &j->big_d)))
         return rc;
+    // Small fits search in one round: the 12 steps of every searching individual as one
+    // candidate batch (each candidate a copy of its individual's parameter block, sharing the
+    // micro-code, scheduled in the parents' cost order), then the first step that passes.
+    // Twelve times the line-search evaluations, in one launch instead of twelve rounds of
+    // three: below 1e8 node-evaluations per round the launches cost more than the work.
+    constexpr int kRounds = 12;
+    if (!j->adaptive && (double)n * (double)ds->num_samples * (double)j->mean_nodes * kRounds <= 1e8) {
+        const Lowering* Lp = j->Lp;
+        std::vector<int64_t> coff((size_t)n * kRounds);
+        std::vector<float> cparams;
+        std::vector<ProgDesc> hdesc((size_t)n * kRounds);
+        for (int k = 0; k < n; ++k) {
+            const int i = j->prog[k];
+            const float* pp = &p->params[p->param_off[i]];
+            const int pc = (int)(p->param_off[i + 1] - p->param_off[i]);
+            for (int r = 0; r < kRounds; ++r) {
+                coff[(size_t)k * kRounds + r] = (int64_t)cparams.size();
+                cparams.insert(cparams.end(), pp, pp + pc);
+                ProgDesc d = Lp->desc[i];
+                d.param_off = (uint32_t)coff[(size_t)k * kRounds + r];
+                hdesc[(size_t)k * kRounds + r] = d;
+            }
+        }
+        cparams.push_back(0.0f);
+        std::vector<int> rank(p->n);
+        for (int r = 0; r < p->n; ++r) rank[Lp->order[r]] = r;
+        std::vector<int32_t> corder((size_t)n * kRounds);
+        std::iota(corder.begin(), corder.end(), 0);
+        std::stable_sort(corder.begin(), corder.end(),
+                         [&](int a, int b) { return rank[j->prog[a / kRounds]] < rank[j->prog[b / kRounds]]; });
+        int64_t* d_coff;
+        ProgDesc* d_desc;
+        int32_t* d_corder;
+        if ((rc = sc.put(coff, &d_coff)) || (rc = sc.put(cparams, &st.cand_params)) ||
+            (rc = sc.put(hdesc, &d_desc)) || (rc = sc.put(corder, &d_corder)) ||
+            (rc = sc.get((size_t)n * kRounds, &st.cand_active)) || (rc = sc.get((size_t)n * kRounds, &j->d_cmse)))
+            return rc;
+        st.K = kRounds;
+        st.cand_off = d_coff;
+        j->CP.code = (const UInstr*)Lp->dcode.ptr;
+        j->CP.progs = d_desc;
+        j->CP.params = st.cand_params;
+        j->CP.order = d_corder;
+        j->CP.n_progs = n * kRounds;
+        j->CP.spill_slots = Lp->spill_slots;
+        j->CP.n_big = j->big_p * kRounds;  // the candidates of big parents sort last, as their parents
+        j->CP.gslots = Lp->gslots;
+    }
     *out = std::move(j);
     return RSR_OK;
 }

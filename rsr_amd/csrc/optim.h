@@ -20,6 +20,13 @@ struct LbfgsDev {
     float *x, *g, *p, *step, *cur_mse, *next_mse;
     void *S, *Y, *rho, *alpha, *ddg;  // float or double
     int *hist, *head, *ls, *flags;
+    // Line search in one round (K > 0): every step 2^-r, r < K, of a searching individual is
+    // evaluated at once as candidate k * K + r, with its own parameter block at
+    // cand_params + cand_off[k * K + r] (a copy of the individual's block)
+    int K = 0;
+    const int64_t* cand_off = nullptr;
+    float* cand_params = nullptr;
+    uint8_t* cand_active = nullptr;
 };
 
 hipError_t lbfgs_begin(const LbfgsDev& st, const float* params, const float* d_mse, const float* d_grad,
@@ -28,6 +35,12 @@ hipError_t lbfgs_direction(const LbfgsDev& st, float* params, uint8_t* active_ls
 hipError_t lbfgs_ls_check(const LbfgsDev& st, float* params, const float* d_mse, uint8_t* active_ls,
                           uint8_t* active_grad, int* counter, hipStream_t s);
 hipError_t lbfgs_update(const LbfgsDev& st, const float* d_grad, uint8_t* active_grad, hipStream_t s);
+// One-round line search: the K candidate points of every searching individual, then the
+// first that passes the Armijo test (lbfgs.rs:113-125), the last forced, as K sequential
+// rounds would take it.
+hipError_t lbfgs_ls_fill(const LbfgsDev& st, hipStream_t s);
+hipError_t lbfgs_ls_pick(const LbfgsDev& st, float* params, const float* cand_mse, uint8_t* active_ls,
+                         uint8_t* active_grad, int* counter, hipStream_t s);
 hipError_t lbfgs_finish(const LbfgsDev& st, float* params, float* fitness, hipStream_t s);
 
 // Diagonal CMA-ES state of n individuals with lambda candidates each.

@@ -202,6 +202,67 @@ __global__ void k_lbfgs_update(LbfgsDev st, const float* __restrict__ d_grad, ui
     st.cur_mse[i] = st.next_mse[i];
 }
 
+// One thread per (individual k, round r): candidate k * K + r is x + 2^-r p, the point that
+// round r of the sequential search evaluates (the step halves exactly from 1, and the point is
+// the same x + step * p expression), active while the individual searches.
+__global__ void k_lbfgs_ls_fill(LbfgsDev st) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)st.n * st.K) return;
+    const int i = (int)(t / st.K), r = (int)(t - (int64_t)i * st.K);
+    const bool on = (st.flags[i] & LB_SEARCH) != 0;
+    st.cand_active[t] = on ? 1 : 0;
+    if (!on) return;
+    float step = 1.0f;
+    for (int q = 0; q < r; ++q) step = step * 0.5f;
+    const int n = st.nparam[i];
+    const int N = st.n;
+    float* cp = st.cand_params + st.cand_off[t];
+    for (int j = 0; j < n; ++j) cp[j] = st.x[j * N + i] + step * st.p[j * N + i];
+}
+
+// One thread per individual: the rounds of lbfgs.rs:113-125 over the candidates' MSEs in
+// order; the accepted round's point, step and round count are what the sequential rounds
+// leave (k_lbfgs_ls_check), and counter gets the K - 1 evaluations beyond the one the
+// working count covers.
+template <typename T>
+__global__ void k_lbfgs_ls_pick(LbfgsDev st, float* __restrict__ params, const float* __restrict__ cand_mse,
+                                uint8_t* __restrict__ active_ls, uint8_t* __restrict__ active_grad,
+                                int* __restrict__ n_evals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st.n) return;
+    if (!(st.flags[i] & LB_SEARCH)) return;
+    const int prog = st.prog[i];
+    const int n = st.nparam[i];
+    const int N = st.n;
+    float step = 1.0f;
+    for (int r = 0; r < st.K; ++r) {
+        const int64_t c = (int64_t)i * st.K + r;
+        const float m = cand_mse[c];
+        bool accept;
+        if constexpr (sizeof(T) == 4) {
+            const float c1 = 1e-4f;
+            accept = m <= st.cur_mse[i] + c1 * step * ((float*)st.ddg)[i];
+        } else {
+            accept = (double)m <= (double)st.cur_mse[i] + 1e-4 * (double)step * ((double*)st.ddg)[i];
+        }
+        if (accept || r > 10) {  // K = 12: the last round is forced (lbfgs.rs:113-125)
+            st.next_mse[i] = m;
+            st.step[i] = step;
+            st.ls[i] = r;
+            st.flags[i] &= ~LB_SEARCH;
+            st.flags[i] |= LB_MOVED;
+            active_ls[prog] = 0;
+            active_grad[prog] = 1;
+            const float* cp = st.cand_params + st.cand_off[c];
+            float* pp = params + st.param_off[i];
+            for (int j = 0; j < n; ++j) pp[j] = cp[j];
+            break;
+        }
+        step = step * 0.5f;
+    }
+    atomicAdd(n_evals, st.K - 1);
+}
+
 __global__ void k_lbfgs_finish(LbfgsDev st, float* __restrict__ params, float* __restrict__ fitness) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= st.n) return;
@@ -233,6 +294,22 @@ hipError_t lbfgs_ls_check(const LbfgsDev& st, float* params, const float* d_mse,
                            counter);
     else
         hipLaunchKernelGGL(k_lbfgs_ls_check<float>, RSR_BLK(st.n), 0, s, st, params, d_mse, active_ls, active_grad,
+                           counter);
+    return hipGetLastError();
+}
+
+hipError_t lbfgs_ls_fill(const LbfgsDev& st, hipStream_t s) {
+    hipLaunchKernelGGL(k_lbfgs_ls_fill, RSR_BLK((int64_t)st.n * st.K), 0, s, st);
+    return hipGetLastError();
+}
+
+hipError_t lbfgs_ls_pick(const LbfgsDev& st, float* params, const float* cand_mse, uint8_t* active_ls,
+                         uint8_t* active_grad, int* counter, hipStream_t s) {
+    if (st.fp64)
+        hipLaunchKernelGGL(k_lbfgs_ls_pick<double>, RSR_BLK(st.n), 0, s, st, params, cand_mse, active_ls, active_grad,
+                           counter);
+    else
+        hipLaunchKernelGGL(k_lbfgs_ls_pick<float>, RSR_BLK(st.n), 0, s, st, params, cand_mse, active_ls, active_grad,
                            counter);
     return hipGetLastError();
 }

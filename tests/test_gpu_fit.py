@@ -51,6 +51,40 @@ def test_lbfgs_reference_order_bit_exact(gpu_ctx, fp64):
         off += nv
 
 
+@pytest.mark.parametrize("n_progs,n_points,schedule", [(64, 403, "one-round line search"),
+                                                       (512, 2000, "fixed schedule")])
+def test_lbfgs_schedules_bit_identical(gpu_ctx, n_progs, n_points, schedule):
+    """The L-BFGS schedules by fit size: small fits (below 1e8 node-evaluations for the 12
+    line-search steps of every individual) evaluate all 12 steps at once and take the first
+    that passes; mid-size fits run the 12 rounds back to back.  Sampled programs' fitness and
+    constants equal the oracle's bit for bit (reference order, FP64 mode)."""
+    X, y, t = W.data_c2(n_points, seed=41)
+    progs = W.gen_c2(n_progs, seed=42)
+    ds = api.Dataset.new(gpu_ctx, X, y, t)
+    ods = O.ODataset.new(X, y, 15)
+    P = api.Programs(gpu_ctx, progs)
+    ok = P.status(ds) == 0
+    work = sum(len(p.code) for p, g in zip(progs, ok) if g) * n_points
+    assert (work * 12 <= 1e8) == (schedule == "one-round line search") and work < 2e8
+    gpu_ctx.set_summation(True)
+    try:
+        fit = api.fit_lbfgs(gpu_ctx, P, ds, iterations=20, history=5, mode=api.LBFGS_FP64, mask=ok.astype(np.uint8))
+        vals = P.get_constants()
+    finally:
+        gpu_ctx.set_summation(False)
+    offs = np.cumsum([0] + [sum(len(c.vals) for c in p.constants) for p in progs])
+    sample = np.random.default_rng(43).choice(np.flatnonzero(ok), 12, replace=False)
+    for i in sample:
+        p = progs[i]
+        nv = offs[i + 1] - offs[i]
+        rf, rc = O.run_lbfgs(p, ods, iters=20, m=5, fp64=True)
+        assert (fit[i] == rf) or (np.isnan(fit[i]) and (rf is None or np.isnan(rf))), (i, fit[i], rf)
+        assert np.array_equal(vals[offs[i]:offs[i] + nv].view(np.uint32), rc[:nv].view(np.uint32)) or \
+            np.array_equal(vals[offs[i]:offs[i] + nv], rc[:nv], equal_nan=True), i
+    P.close()
+    ds.close()
+
+
 def test_lbfgs_large_fit_counter_driven_schedule_bit_identical(gpu_ctx):
     """A fit large enough (1,024 Mat3 programs x 20,000 points: over 2e8 node-evaluations per
     round) that the host reads the counters back each tick and ends line searches and the
