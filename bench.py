@@ -184,14 +184,22 @@ def run_configs(ctx, log):
     Pb.close()
     rng = np.random.default_rng(7)
     draws = [rng.random(100).astype(np.float32) for _ in range(24)]
-    step = lambda: api.islands_evaluate(ctx, ds, mini, buffers, best_fit, best_mini, draws)
+    # the library's step: programs in the wire format in and out, as a compiled caller holds
+    # them (the Python objects' packing and unpacking timed separately)
+    packed = pack_programs([p for b in buffers for p in b])
+    packed_g = [pack_programs(b) for b in buffers]
+    step = lambda: api.islands_evaluate(ctx, ds, mini, buffers, best_fit, best_mini, draws, packed=packed,
+                                        unpack=False)
     step()
-    res, t_all = _timed_best(ctx, step, reps=3)
+    res, t_all = _timed_best(ctx, step, reps=5)
     per_island = lambda: [api.islands_evaluate(ctx, ds, mini, [buffers[g]], best_fit[g:g + 1], best_mini[g:g + 1],
-                                               [draws[g]]) for g in range(24)]
+                                               [draws[g]], packed=packed_g[g], unpack=False) for g in range(24)]
     _, t_per = _timed_best(ctx, per_island, reps=3)
+    _, t_py = _timed_best(ctx, lambda: api.islands_evaluate(ctx, ds, mini, buffers, best_fit, best_mini, draws),
+                          reps=3)
     out["islands_step"] = {"islands": 24, "island_size": 25, "points": 400, "mini_batch": 64,
                            "batched_ms": t_all * 1e3, "one_call_per_island_ms": t_per * 1e3,
+                           "batched_with_python_objects_ms": t_py * 1e3,
                            "fitted": int(sum(int(o.sum()) for o in res["optimized"]))}
     mini.close()
     # the whole search loop at the reference's default Config (config.rs:41-66): 24 islands x

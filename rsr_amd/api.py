@@ -602,13 +602,17 @@ def simplify_programs(progs: Sequence[Program]) -> List[Program]:
 
 def islands_evaluate(ctx: Context, dataset: Dataset, mini_batch: Dataset, buffers: Sequence[Sequence[Program]],
                      best_fitness, best_mini_mse, draws: Sequence[np.ndarray], opt_prob: float = 0.01,
-                     opt_iterations: int = 100, parsimony: float = 5e-6, threshold: float = 0.05, seed: int = 0):
+                     opt_iterations: int = 100, parsimony: float = 5e-6, threshold: float = 0.05, seed: int = 0,
+                     packed=None, unpack: bool = True):
     """Island::evaluate_buffer of every island (rsr_islands_evaluate; defaults of config.rs:40-66).
 
     buffers[g]: island g's next-generation buffer; best_fitness[g] / best_mini_mse[g]: its
     best individual's fitness and mini-batch MSE; draws[g]: its uniform f32 stream.
     Returns dict(buffers, fitness, mse, optimized (per island lists / arrays),
-    best_index[g] (-1: unchanged), draws_used[g])."""
+    best_index[g] (-1: unchanged), draws_used[g]).  packed: pack_programs of the
+    concatenated buffers, when the caller has it already; unpack=False leaves the output
+    programs in the wire format (buffers: None, packed_out: the arrays), which is what a
+    compiled caller gets -- the Python objects cost more to build than the step itself."""
     l = lib()
     fn = l.rsr_islands_evaluate
     fn.restype = C.c_int
@@ -620,7 +624,7 @@ def islands_evaluate(ctx: Context, dataset: Dataset, mini_batch: Dataset, buffer
     n = len(flat)
     isl = np.zeros(G + 1, np.int64)
     isl[1:] = np.cumsum([len(b) for b in buffers])
-    ops, args, co, ct, cof, cv, vo = pack_programs(flat)
+    ops, args, co, ct, cof, cv, vo = packed if packed is not None else pack_programs(flat)
     dr = [np.ascontiguousarray(d, np.float32) for d in draws]
     doff = np.zeros(G + 1, np.int64)
     doff[1:] = np.cumsum([d.size for d in dr])
@@ -642,10 +646,15 @@ def islands_evaluate(ctx: Context, dataset: Dataset, mini_batch: Dataset, buffer
               parsimony, threshold, seed, cap, cap, 9 * cap, _ptr(ops_o, _u8p), _ptr(args_o, _u16p),
               _ptr(co_o, _i64p), _ptr(ct_o, _u8p), _ptr(cof_o, _i64p), _ptr(cv_o, _f32p), _ptr(vo_o, _i64p),
               _ptr(fit, _f32p), _ptr(mse, _f32p), _ptr(opt, _u8p), _ptr(besti, _i32p), _ptr(used, _i64p)))
-    progs = unpack_programs(ops_o, args_o, co_o, ct_o, cof_o, cv_o, vo_o)
     split = lambda a: [a[isl[g]:isl[g + 1]] for g in range(G)]
-    return {"buffers": split(progs), "fitness": split(fit[:n]), "mse": split(mse[:n]),
-            "optimized": split(opt[:n].astype(bool)), "best_index": besti[:G], "draws_used": used[:G]}
+    out = {"fitness": split(fit[:n]), "mse": split(mse[:n]), "optimized": split(opt[:n].astype(bool)),
+           "best_index": besti[:G], "draws_used": used[:G]}
+    if unpack:
+        out["buffers"] = split(unpack_programs(ops_o, args_o, co_o, ct_o, cof_o, cv_o, vo_o))
+    else:
+        out["buffers"] = None
+        out["packed_out"] = (ops_o, args_o, co_o, ct_o, cof_o, cv_o, vo_o)
+    return out
 
 
 def assign_rank_and_crowding(fitness_, age):
