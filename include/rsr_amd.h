@@ -130,9 +130,9 @@ RSR_API int rsr_context_set_partials_limit(rsr_context* ctx, uint64_t bytes);
  * per work item (the programs that share one tile's features in registers).  By default
  * (or after set_tuning(ctx, 0, 0)) each evaluation takes the tuning that rsr_autotune
  * found for its workload shape -- feature count, point count and program count, each
- * rounded to a power of two -- or, before any, a heuristic (four points per lane, two for
- * small evaluations, three for Mat3-sized feature sets; as many programs per work item as
- * keep 65,536 work items).  A nonzero setting fixes the tuning for
+ * rounded to a power of two -- or, before any, a heuristic (three points per lane and 8
+ * programs per work item for large evaluations; four, or two for small ones, with up to 4
+ * programs per item while that keeps 8,000 work items).  A nonzero setting fixes the tuning for
  * every evaluation; get_tuning returns 0, 0 in the per-workload mode.  No reference
  * counterpart (tuning of the device launches). */
 RSR_API int rsr_context_set_tuning(rsr_context* ctx, int points_per_lane, int programs_per_item);

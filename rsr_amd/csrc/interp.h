@@ -28,6 +28,19 @@ namespace rsr {
 // The buffer is only written between launches, never during one.
 typedef const __attribute__((address_space(4))) float* kptr;
 __device__ __forceinline__ float kld(const float* p, uint32_t i) { return ((kptr)p)[i]; }
+// The same for the other launch-constant tables a wave reads at uniform addresses (micro-
+// code, program descriptors, scheduling order, active flags): s_loads by construction,
+// not by the compiler's proof that nothing in the kernel writes them, which a wait or a
+// fence placed before the loads would defeat.
+template <class T>
+__device__ __forceinline__ T kget(const T* p, int64_t i) {
+    if constexpr (sizeof(T) == 16) {  // four-dword records (16-byte aligned arrays)
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        return __builtin_bit_cast(T, ((const __attribute__((address_space(4))) u4*)p)[i]);
+    } else {
+        return ((const __attribute__((address_space(4))) T*)p)[i];
+    }
+}
 
 typedef float v32f __attribute__((ext_vector_type(32)));
 
@@ -112,11 +125,12 @@ struct Uop {
     uint32_t w0, d, a, b;
 };
 __device__ __forceinline__ Uop load_uop(const UInstr* __restrict__ code, uint32_t i) {
+    const UInstr c = kget(code, i);
     Uop u;
-    u.w0 = uni(code[i].w0);
-    u.d = uni(code[i].w1);
-    u.a = uni(code[i].w2);
-    u.b = uni(code[i].w3);
+    u.w0 = uni(c.w0);
+    u.d = uni(c.w1);
+    u.a = uni(c.w2);
+    u.b = uni(c.w3);
     return u;
 }
 
@@ -1360,11 +1374,34 @@ __device__ __forceinline__ void run_dual(DualFiles<T>& G, const UInstr* __restri
 #undef RSR_TAN
 #undef RSR_TAN1
 
-// Sum over the 64 lanes of a wavefront in a fixed butterfly order (deterministic).
+// v from the lane that DPP control Ctrl names (rows of 16 lanes; lanes of the rows that
+// RowMask leaves out, and lanes whose source is out of range, read 0).  Every lane of the
+// wave is active.
+template <int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), Ctrl, RowMask, 0xf,
+                                                                 false));
+}
+
+__device__ __forceinline__ float lane_f(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+
+// Sum over the 64 lanes of a wavefront in a fixed order (deterministic), the same value in
+// every lane.  Four DPP steps leave every lane of a row of 16 holding the row's sum: the
+// partner lane of quad_perm [1,0,3,2], of quad_perm [2,3,0,1], of row_half_mirror (lane i
+// of 8 reads 7 - i) and of row_mirror (lane i of 16 reads 15 - i); a + b and b + a are the
+// same float, so the partners agree.  row_bcast:15 then adds row 0's sum into row 1 and
+// row 2's into row 3, row_bcast:31 adds row 1's (r0 + r1) into row 3: lane 63 holds
+// (r2 + r3) + (r0 + r1).  Called with the whole wave active.
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
+    v = v + dpp_f<0xB1>(v);        // quad_perm [1,0,3,2]
+    v = v + dpp_f<0x4E>(v);        // quad_perm [2,3,0,1]
+    v = v + dpp_f<0x141>(v);       // row_half_mirror
+    v = v + dpp_f<0x140>(v);       // row_mirror
+    v = v + dpp_f<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = v + dpp_f<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return lane_f(v, 63);
 }
 
 }  // namespace rsr

@@ -89,6 +89,11 @@ __device__ __forceinline__ void load_features(Files<PPL>& R, const float* __rest
             });
         }
     }
+    // vmcnt(0): every load has landed before the interpreter runs.  Its indexed moves read
+    // the register files through a run-time index that the compiler's wait insertion
+    // cannot resolve to one register, so loads still in flight would give every staging
+    // read inside the interpreter loop a wait of its own, executed on every micro-op.
+    __builtin_amdgcn_s_waitcnt(0x0F70);
 }
 
 // Segment owning a work index (first_wave / first_thread are increasing prefix sums).
@@ -123,8 +128,8 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
 
 // f32x8::reduce_add order: (lo4 + hi4), then ((0 + 2) + (1 + 3)); result in every lane.
 __device__ __forceinline__ float reduce_lanes8(float acc) {
-    const float a0 = __shfl(acc, 0, 64), a1 = __shfl(acc, 1, 64), a2 = __shfl(acc, 2, 64), a3 = __shfl(acc, 3, 64);
-    const float a4 = __shfl(acc, 4, 64), a5 = __shfl(acc, 5, 64), a6 = __shfl(acc, 6, 64), a7 = __shfl(acc, 7, 64);
+    const float a0 = lane_f(acc, 0), a1 = lane_f(acc, 1), a2 = lane_f(acc, 2), a3 = lane_f(acc, 3);
+    const float a4 = lane_f(acc, 4), a5 = lane_f(acc, 5), a6 = lane_f(acc, 6), a7 = lane_f(acc, 7);
     const float s0 = a0 + a4, s1 = a1 + a5, s2 = a2 + a6, s3 = a3 + a7;
     return (s0 + s2) + (s1 + s3);
 }
@@ -133,7 +138,10 @@ __device__ __forceinline__ float reduce_lanes8(float acc) {
 // MSE, Kahan summation
 // ---------------------------------------------------------------------------------
 // One work item (wave index `wave` of the launch): a tile of points x a group of programs.
-template <int PPL>
+// TERMS (the launch's summation mode, the same for every segment of a launch): per-point
+// terms for the fold kernel (reference order) instead of per-tile partials (Kahan); each
+// mode's kernel holds only its own epilogue.
+template <int PPL, bool TERMS>
 __device__ __forceinline__ void mse_item(const SegTable& T, int64_t wave, float* lds, int lane) {
     const int si = find_seg_wave(T, wave);
     const SegDesc& S = T.s[si];
@@ -146,7 +154,7 @@ __device__ __forceinline__ void mse_item(const SegTable& T, int64_t wave, float*
     const int last = min(first + S.group, S.n_small);
     if (S.active != nullptr) {  // nothing to evaluate in this group: skip the feature loads too
         bool any = false;
-        for (int sp = first; sp < last; ++sp) any = any || S.active[uni((uint32_t)S.order[sp])] != 0;
+        for (int sp = first; sp < last; ++sp) any = any || kget(S.active, uni((uint32_t)kget(S.order, sp))) != 0;
         if (!any) return;  // the finalize and fold kernels never read inactive programs' partials
     }
     const int64_t base = (S.tile0 + lt) * (64 * PPL);
@@ -158,10 +166,10 @@ __device__ __forceinline__ void mse_item(const SegTable& T, int64_t wave, float*
 #pragma unroll
     for (int p = 0; p < PPL; ++p) valid[p] = base + p * 64 + lane < S.count8;
     for (int sp = first; sp < last; ++sp) {
-        const int prog = (int)uni((uint32_t)S.order[sp]);
+        const int prog = (int)uni((uint32_t)kget(S.order, sp));
         float s = 0.0f;
-        if (S.active == nullptr || S.active[prog]) {
-            const ProgDesc pd = S.progs[prog];
+        if (S.active == nullptr || kget(S.active, prog)) {
+            const ProgDesc pd = kget(S.progs, prog);
             run_program<PPL>(R, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), lds,
                              nullptr, lane);
             const uint32_t o = uni(pd.out_reg) & 31u;
@@ -169,7 +177,7 @@ __device__ __forceinline__ void mse_item(const SegTable& T, int64_t wave, float*
             // through the interpreter loop
             float y[PPL];
             sfor<PPL>([&](auto P) { y[P()] = S.target[base + P() * 64 + lane]; });
-            if (S.terms) {
+            if constexpr (TERMS) {
                 // reference order: keep every per-point term; the fold kernel adds them.
                 // The terms (3.3 GB per launch on the Lorentz workload) are only read back by
                 // the fold kernel: non-temporal stores, measured 2% faster per step.
@@ -189,7 +197,7 @@ __device__ __forceinline__ void mse_item(const SegTable& T, int64_t wave, float*
             });
             s = wave_sum(s);
         }
-        if (!S.terms && lane == 0) S.partials[lt * S.n_progs + sp] = s;
+        if (!TERMS && lane == 0) S.partials[lt * S.n_progs + sp] = s;
     }
 }
 
@@ -198,12 +206,12 @@ __device__ __forceinline__ void mse_item(const SegTable& T, int64_t wave, float*
 // lane (96 VGPRs of files) it fits in 128: four waves per SIMD.
 template <int PPL>
 constexpr int kEvalWaves = PPL == 3 ? 4 : 3;
-template <int PPL>
+template <int PPL, bool TERMS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kEvalWaves<PPL>))) void k_eval_mse(const SegTable T,
                                                                                            int spill_slots) {
     extern __shared__ float smem[];
     float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * PPL * 64;
-    mse_item<PPL>(T, xcd_block() * kWaves + uni(threadIdx.x >> 6), lds, threadIdx.x & 63);
+    mse_item<PPL, TERMS>(T, xcd_block() * kWaves + uni(threadIdx.x >> 6), lds, threadIdx.x & 63);
 }
 
 // The persistent form: a resident grid drains the launch's work items through a device work
@@ -216,7 +224,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kEvalWaves<
 // wave to finish (queue[kXcds] counts finished waves) clears it.
 // (The work loop's extra live state does not fit 128 VGPRs at three points per lane: three
 // waves per SIMD for every form.)
-template <int PPL>
+template <int PPL, bool TERMS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_eval_mse_q(
     const SegTable T, int spill_slots, int64_t n_waves, int* __restrict__ queue) {
     extern __shared__ float smem[];
@@ -236,7 +244,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             continue;
         }
         float* lds = smem + (size_t)uni(threadIdx.x >> 6) * spill_slots * PPL * 64;
-        mse_item<PPL>(T, w0 + t, lds, threadIdx.x & 63);
+        mse_item<PPL, TERMS>(T, w0 + t, lds, threadIdx.x & 63);
     }
     if ((threadIdx.x & 63) == 0) {
         const int launched = (int)(uni(gridDim.x) * kWaves);
@@ -431,10 +439,10 @@ __global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_s
     G.v = R1.a;
     G.a = G.b = G.c = G.d = (v32f)(0.0f);
     for (int sp = first; sp < last; ++sp) {
-        const int prog = (int)uni((uint32_t)S.order[sp]);
-        const int np = (int)uni(S.nparams[prog]);
-        if ((S.active != nullptr && !S.active[prog]) || np == 0) continue;
-        const ProgDesc pd = S.progs[prog];
+        const int prog = (int)uni((uint32_t)kget(S.order, sp));
+        const int np = (int)uni(kget(S.nparams, prog));
+        if ((S.active != nullptr && !kget(S.active, prog)) || np == 0) continue;
+        const ProgDesc pd = kget(S.progs, prog);
         const uint32_t o = uni(pd.out_reg) & 31u;
         float* dst = S.partials + (lt * S.n_progs + sp) * S.slots;
         for (int pass = 0; pass * kTan < np; ++pass) {
@@ -571,15 +579,15 @@ __global__ __launch_bounds__(256) void k_eval_mse_big(const SegTable T, int spil
         const int64_t lt = w - (int64_t)j * S.n_tiles;
         const int sp = S.n_small + j;
         if (S.live != nullptr && *S.live == 0) continue;
-        const int prog = (int)uni((uint32_t)S.order[sp]);
-        const bool on = S.active == nullptr || S.active[prog];
+        const int prog = (int)uni((uint32_t)kget(S.order, sp));
+        const bool on = S.active == nullptr || kget(S.active, prog);
         const int64_t base = (S.tile0 + lt) * (64 * PPL);
         float s = 0.0f;
         if (on) {
             Files<PPL> R;
             sfor<PPL>([&](auto P) { fget<P(), PPL>(R) = (v32f)(0.0f); });
             load_features<PPL>(R, S.feat, S.F, S.npad, base, lane);
-            const ProgDesc pd = S.progs[prog];
+            const ProgDesc pd = kget(S.progs, prog);
             run_program<PPL>(R, S.code + uni(pd.code_off), uni(pd.code_len), S.params + uni(pd.param_off), lds, gsp,
                              lane);
             const uint32_t o = uni(pd.out_reg) & 31u;
@@ -620,9 +628,9 @@ __global__ __launch_bounds__(256) void k_eval_grad_big(const SegTable T, int spi
         const int64_t lt = w - (int64_t)j * S.n_tiles;
         const int sp = S.n_small + j;
         if (S.live != nullptr && *S.live == 0) continue;
-        const int prog = (int)uni((uint32_t)S.order[sp]);
-        const int np = (int)uni(S.nparams[prog]);
-        if ((S.active != nullptr && !S.active[prog]) || np == 0) continue;
+        const int prog = (int)uni((uint32_t)kget(S.order, sp));
+        const int np = (int)uni(kget(S.nparams, prog));
+        if ((S.active != nullptr && !kget(S.active, prog)) || np == 0) continue;
         const int64_t base = (S.tile0 + lt) * 64;
         const int64_t pt = base + lane;
         const bool valid = pt < S.count8;
@@ -632,7 +640,7 @@ __global__ __launch_bounds__(256) void k_eval_grad_big(const SegTable T, int spi
         DualFiles<kTan> G;
         G.v = R1.a;
         G.a = G.b = G.c = G.d = (v32f)(0.0f);
-        const ProgDesc pd = S.progs[prog];
+        const ProgDesc pd = kget(S.progs, prog);
         const uint32_t o = uni(pd.out_reg) & 31u;
         float* dst = S.partials + (lt * S.n_progs + sp) * S.slots;
         for (int pass = 0; pass * kTan < np; ++pass) {
@@ -789,20 +797,28 @@ static int64_t resident_blocks(const void* kernel, size_t lds) {
     return n;
 }
 
-template <int PPL>
-static hipError_t launch_eval_mse(const SegTable& T, int64_t waves, int spill, size_t lds, int* queue,
-                                  hipStream_t s) {
+template <int PPL, bool TERMS>
+static hipError_t launch_eval_mse_t(const SegTable& T, int64_t waves, int spill, size_t lds, int* queue,
+                                    hipStream_t s) {
     const int64_t blocks = (waves + kWaves - 1) / kWaves;
     // the queue pays off only when the work exceeds what the device holds at once: a smaller
     // launch is one wave per item in either form, and its waves would only add the takes
-    const int64_t res = queue ? resident_blocks((const void*)k_eval_mse_q<PPL>, lds) : 0;
+    const int64_t res = queue ? resident_blocks((const void*)k_eval_mse_q<PPL, TERMS>, lds) : 0;
     if (queue == nullptr || res <= 0 || blocks <= res) {
-        hipLaunchKernelGGL(k_eval_mse<PPL>, dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
+        hipLaunchKernelGGL((k_eval_mse<PPL, TERMS>), dim3((unsigned)blocks), dim3(64 * kWaves), lds, s, T, spill);
         return hipGetLastError();
     }
     if (waves / kXcds >= INT32_MAX) return hipErrorInvalidValue;  // per-XCD counters are 32-bit
-    hipLaunchKernelGGL(k_eval_mse_q<PPL>, dim3((unsigned)res), dim3(64 * kWaves), lds, s, T, spill, waves, queue);
+    hipLaunchKernelGGL((k_eval_mse_q<PPL, TERMS>), dim3((unsigned)res), dim3(64 * kWaves), lds, s, T, spill, waves,
+                       queue);
     return hipGetLastError();
+}
+
+template <int PPL>
+static hipError_t launch_eval_mse(const SegTable& T, bool terms, int64_t waves, int spill, size_t lds, int* queue,
+                                  hipStream_t s) {
+    return terms ? launch_eval_mse_t<PPL, true>(T, waves, spill, lds, queue, s)
+                 : launch_eval_mse_t<PPL, false>(T, waves, spill, lds, queue, s);
 }
 
 hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune, bool terms, hipStream_t s,
@@ -837,10 +853,10 @@ hipError_t launch_mse_steps(const EvalStep* steps, int n, const EvalTuning& tune
         if (waves > 0) {
             hipError_t e;
             switch (tune.ppl) {
-                case 1: e = launch_eval_mse<1>(T, waves, spill, lds, queue, s); break;
-                case 3: e = launch_eval_mse<3>(T, waves, spill, lds, queue, s); break;
-                case 4: e = launch_eval_mse<4>(T, waves, spill, lds, queue, s); break;
-                default: e = launch_eval_mse<2>(T, waves, spill, lds, queue, s); break;
+                case 1: e = launch_eval_mse<1>(T, terms, waves, spill, lds, queue, s); break;
+                case 3: e = launch_eval_mse<3>(T, terms, waves, spill, lds, queue, s); break;
+                case 4: e = launch_eval_mse<4>(T, terms, waves, spill, lds, queue, s); break;
+                default: e = launch_eval_mse<2>(T, terms, waves, spill, lds, queue, s); break;
             }
             if (e != hipSuccess) return e;
         }

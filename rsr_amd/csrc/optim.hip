@@ -245,7 +245,7 @@ __global__ void k_lbfgs_ls_pick(LbfgsDev st, float* __restrict__ params, const f
         } else {
             accept = (double)m <= (double)st.cur_mse[i] + 1e-4 * (double)step * ((double*)st.ddg)[i];
         }
-        if (accept || r > 10) {  // K = 12: the last round is forced (lbfgs.rs:113-125)
+        if (accept || r == st.K - 1) {  // K = 12: the last round is forced (lbfgs.rs:113-125)
             st.next_mse[i] = m;
             st.step[i] = step;
             st.ls[i] = r;

@@ -301,21 +301,29 @@ uint64_t rsr::tuning_key(int num_features, int64_t count8, int n_progs) {
     return ((uint64_t)num_features << 16) | ((uint64_t)ceil_log2(count8) << 8) | (uint64_t)ceil_log2(n_progs);
 }
 
-// Heuristic, fitted to timings of every tuning on MI355X over the four workload types
-// (400 to 10^6 points, 1,000 to 16,384 programs): four points per lane (the dispatch
-// amortised over the most points) when that still gives 8,000 (tile, program) work items,
-// else two; three instead of four for 12 or more feature components (Mat3 workloads: less
-// register pressure, 8% faster there); then the most programs per work item (1 to 32) that
-// keeps 65,536 work items, else one.  Small evaluations thus get many one-program items
-// (up to 3x faster than eight-program items there), large ones 8 to 32 programs per item.
+// Heuristic, fitted to timings of every tuning (points per lane 2 to 4 x programs per item
+// 1 to 32) on MI355X over the four workload types at 13 shapes (400 to 10^6 points, 1,000 to
+// 16,384 programs), in reference order; it is within 3% of the best tuning at each.
+//   - Large evaluations (200,000 or more (tile of 192 points, program) pairs): three points
+//     per lane (four waves per SIMD) and 8 programs per work item.
+//   - Mid-size ones (8,000 or more pairs of 256-point tiles): four points per lane and the
+//     most programs per item, up to 4, that still gives 8,000 work items (an item of 8
+//     programs was up to 37% slower there: too few items for the chip).
+//   - Smaller ones: two points per lane, the same rule for the programs per item.
 static EvalTuning heuristic_tuning(int num_features, int64_t count8, int n_progs) {
+    (void)num_features;
     auto tiles = [&](int ppl) { return (count8 + 64 * ppl - 1) / (64 * ppl); };
     EvalTuning t;
     const int64_t np = std::max(n_progs, 1);
-    t.ppl = tiles(4) * np >= 8000 ? (num_features >= 12 ? 3 : 4) : 2;
+    if (tiles(3) * np >= 200000) {
+        t.ppl = 3;
+        t.group = 8;
+        return t;
+    }
+    t.ppl = tiles(4) * np >= 8000 ? 4 : 2;
     t.group = 1;
-    for (int g = 32; g >= 2; g /= 2)
-        if (tiles(t.ppl) * ((np + g - 1) / g) >= 65536) {
+    for (int g = 4; g >= 2; g /= 2)
+        if (tiles(t.ppl) * ((np + g - 1) / g) >= 8000) {
             t.group = g;
             break;
         }

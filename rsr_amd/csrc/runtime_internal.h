@@ -153,7 +153,7 @@ namespace rsr {
 // The evaluation tuning (points per lane, programs per work item) for an MSE step: the
 // context's fixed tuning, or in auto mode the autotuned choice for the step's workload
 // shape (feature count, point count and program count, each in powers of two), else a
-// heuristic that fills the chip with the fewest waves per point.
+// heuristic fitted to a sweep of every tuning over the four workload types.
 EvalTuning pick_tuning(const rsr_context* ctx, const EvalStep& st);
 uint64_t tuning_key(int num_features, int64_t count8, int n_progs);
 void trim_pool(rsr_context* c);
