@@ -66,7 +66,29 @@ template <> struct Files<4> { v32f a, b, c, d; };
 // reference are not promoted to registers by the optimizer).
 template <int T> struct DualFiles;
 template <> struct DualFiles<1> { v32f v, a; };
+template <> struct DualFiles<2> { v32f v, a, b; };
 template <> struct DualFiles<4> { v32f v, a, b, c, d; };
+
+template <int T>
+__device__ __forceinline__ void zero_tangents(DualFiles<T>& G) {
+    G.a = (v32f)(0.0f);
+    if constexpr (T >= 2) G.b = (v32f)(0.0f);
+    if constexpr (T >= 4) {
+        G.c = (v32f)(0.0f);
+        G.d = (v32f)(0.0f);
+    }
+}
+
+// The T tangent files of a DualFiles<T>, in channel order.
+template <int T>
+__device__ __forceinline__ void tangent_files(const DualFiles<T>& G, v32f (&gf)[T]) {
+    gf[0] = G.a;
+    if constexpr (T >= 2) gf[1] = G.b;
+    if constexpr (T >= 4) {
+        gf[2] = G.c;
+        gf[3] = G.d;
+    }
+}
 
 template <int I, int N>
 __device__ __forceinline__ v32f& fget(Files<N>& f) {

@@ -29,7 +29,7 @@
 namespace rsr {
 
 static constexpr int kWaves = 4;  // waves per workgroup
-static constexpr int kTan = 4;    // tangent channels per dual pass
+static constexpr int kTan = 2;    // tangent channels per dual pass
 // Programs per gradient work item (features loaded once for the group): up to 16, fewer
 // when a segment would then give the chip fewer than about four waves per SIMD.  On the
 // Mat3 workload (313 tiles x 16,384 programs) 16 took the launch from 25.3 to 24.1 ms
@@ -409,7 +409,7 @@ __global__ void k_finalize_grad_terms(const SegTable T) {
 
 // ---------------------------------------------------------------------------------
 // Gradients (dual), Kahan summation: work item = (tile, group of grad_group() programs),
-// every pass of four tangent channels of each program inside it
+// every pass of kTan tangent channels of each program inside it
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_slots) {
     extern __shared__ float smem[];
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_s
     load_features<1>(R1, S.feat, S.F, S.npad, base, lane);
     DualFiles<kTan> G;
     G.v = R1.a;
-    G.a = G.b = G.c = G.d = (v32f)(0.0f);
+    zero_tangents<kTan>(G);
     for (int sp = first; sp < last; ++sp) {
         const int prog = (int)uni((uint32_t)kget(S.order, sp));
         const int np = (int)uni(kget(S.nparams, prog));
@@ -452,7 +452,8 @@ __global__ __launch_bounds__(256) void k_eval_grad(const SegTable T, int spill_s
             const v32f vfile = G.v;
             const float diff = vfile[o] - S.target[pt];
             const float two_diff = 2.0f * diff;
-            const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
+            v32f gf[kTan];
+            tangent_files<kTan>(G, gf);
             if (S.terms) {
                 // reference order: per-point terms, row (sp, slot), folded in order later
                 float* rows = S.partials + (int64_t)sp * S.slots * S.n_tiles * 64 + lt * 64 + lane;
@@ -639,7 +640,7 @@ __global__ __launch_bounds__(256) void k_eval_grad_big(const SegTable T, int spi
         load_features<1>(R1, S.feat, S.F, S.npad, base, lane);
         DualFiles<kTan> G;
         G.v = R1.a;
-        G.a = G.b = G.c = G.d = (v32f)(0.0f);
+        zero_tangents<kTan>(G);
         const ProgDesc pd = kget(S.progs, prog);
         const uint32_t o = uni(pd.out_reg) & 31u;
         float* dst = S.partials + (lt * S.n_progs + sp) * S.slots;
@@ -650,7 +651,8 @@ __global__ __launch_bounds__(256) void k_eval_grad_big(const SegTable T, int spi
             const v32f vfile = G.v;
             const float diff = vfile[o] - S.target[pt];
             const float two_diff = 2.0f * diff;
-            const v32f gf[kTan] = {G.a, G.b, G.c, G.d};
+            v32f gf[kTan];
+            tangent_files<kTan>(G, gf);
             if (S.terms) {
                 float* rows = S.partials + (int64_t)sp * S.slots * S.n_tiles * 64 + lt * 64 + lane;
                 const int64_t rstride = S.n_tiles * 64;
