@@ -103,7 +103,7 @@ EvalStep grad_step(const rsr_programs* p, const Lowering* Ld, const rsr_dataset*
                    float* d_mse, float* d_grad) {
     EvalStep st = mse_step(p, Ld, ds, active, d_mse);
     st.grad_out = d_grad;
-    st.nparams = (const uint8_t*)Ld->dnparams.ptr;
+    st.nparams = Ld->dnparams;
     st.max_params = Ld->max_params;
     return st;
 }
@@ -786,7 +786,7 @@ int make_lbfgs_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
             return rc;
         st.K = kRounds;
         st.cand_off = d_coff;
-        j->CP.code = (const UInstr*)Lp->dcode.ptr;
+        j->CP.code = Lp->dcode;
         j->CP.progs = d_desc;
         j->CP.params = st.cand_params;
         j->CP.order = d_corder;
@@ -938,7 +938,7 @@ int make_cmaes_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int
         (rc = sc.get((size_t)n * lambda, &j->d_cmse)) || (rc = sc.get((size_t)n, &j->d_fit)) ||
         (rc = sc.get((size_t)n, &j->d_iters)) || (rc = sc.get(2, &j->d_counters)))
         return rc;
-    j->CP.code = (const UInstr*)Lp->dcode.ptr;
+    j->CP.code = Lp->dcode;
     j->CP.progs = d_desc;
     j->CP.params = j->d_cparams;
     j->CP.order = d_order;
@@ -1054,7 +1054,7 @@ int make_nm_job(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int it
         (rc = sc.get((size_t)n * kNmSlots, &j->d_cmse)) || (rc = sc.get((size_t)n, &j->d_fit)) ||
         (rc = sc.get((size_t)n, &j->d_iters)))
         return rc;
-    j->CP.code = (const UInstr*)Lp->dcode.ptr;
+    j->CP.code = Lp->dcode;
     j->CP.progs = d_desc;
     j->CP.params = j->d_cparams;
     j->CP.order = d_order;

@@ -80,7 +80,7 @@ extern "C" RSR_API int rsr_eval_mse_noalloc(rsr_context* ctx, rsr_programs* p, c
     EvalStep st;
     st.P = dev_view(p, L);
     st.D = dev_view(ds);
-    st.active = (const uint8_t*)L->dactive.ptr;
+    st.active = L->dactive;
     st.mse_out = mse_out_device;
     // the tuning rsr_eval_mse would take for this batch (so the bits agree)
     EvalTuning t;

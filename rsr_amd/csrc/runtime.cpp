@@ -776,13 +776,32 @@ int rsr::get_lowering(rsr_programs* p, int F, bool dual, Lowering** out) {
     });
     L->active_host = active;
     L->desc = descs;
+    // one host block, one device buffer and one copy for the five arrays (256-byte aligned
+    // parts): small batches are dominated by the per-copy cost of the runtime
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t b_code = al(code.size() * sizeof(UInstr)), b_desc = al(std::max<size_t>(1, descs.size()) * sizeof(ProgDesc)),
+                 b_order = al(std::max<size_t>(1, L->order.size()) * sizeof(int32_t)),
+                 b_active = al(std::max<size_t>(1, active.size())), b_np = al(std::max<size_t>(1, L->nparams.size()));
+    std::vector<char> blob(b_code + b_desc + b_order + b_active + b_np, 0);
+    size_t at = 0;
+    auto put = [&](const void* src, size_t bytes, size_t span) {
+        if (bytes) memcpy(blob.data() + at, src, bytes);
+        at += span;
+        return at - span;
+    };
+    const size_t o_code = put(code.data(), code.size() * sizeof(UInstr), b_code);
+    const size_t o_desc = put(descs.data(), descs.size() * sizeof(ProgDesc), b_desc);
+    const size_t o_order = put(L->order.data(), L->order.size() * sizeof(int32_t), b_order);
+    const size_t o_active = put(active.data(), active.size(), b_active);
+    const size_t o_np = put(L->nparams.data(), L->nparams.size(), b_np);
     int rc;
-    if ((rc = L->dcode.upload(ctx, code.data(), code.size() * sizeof(UInstr)))) return rc;
-    if ((rc = L->ddesc.upload(ctx, descs.data(), std::max<size_t>(1, descs.size()) * sizeof(ProgDesc)))) return rc;
-    if ((rc = L->dorder.upload(ctx, L->order.data(), std::max<size_t>(1, L->order.size()) * sizeof(int32_t))))
-        return rc;
-    if ((rc = L->dactive.upload(ctx, active.data(), std::max<size_t>(1, active.size())))) return rc;
-    if ((rc = L->dnparams.upload(ctx, L->nparams.data(), std::max<size_t>(1, L->nparams.size())))) return rc;
+    if ((rc = L->dbuf.upload(ctx, blob.data(), blob.size()))) return rc;
+    const char* base = (const char*)L->dbuf.ptr;
+    L->dcode = (const UInstr*)(base + o_code);
+    L->ddesc = (const ProgDesc*)(base + o_desc);
+    L->dorder = (const int32_t*)(base + o_order);
+    L->dactive = (const uint8_t*)(base + o_active);
+    L->dnparams = (const uint8_t*)(base + o_np);
     HIP_TRY(hipStreamSynchronize(ctx->stream), "lowering upload");
     *out = L.get();
     p->lowerings[key] = std::move(L);
@@ -791,10 +810,10 @@ int rsr::get_lowering(rsr_programs* p, int F, bool dual, Lowering** out) {
 
 DevPrograms rsr::dev_view(const rsr_programs* p, const Lowering* L) {
     DevPrograms v;
-    v.code = (const UInstr*)L->dcode.ptr;
-    v.progs = (const ProgDesc*)L->ddesc.ptr;
+    v.code = L->dcode;
+    v.progs = L->ddesc;
     v.params = (const float*)p->dparams.ptr;
-    v.order = (const int32_t*)L->dorder.ptr;
+    v.order = L->dorder;
     v.n_progs = p->n;
     v.spill_slots = L->spill_slots;
     v.n_big = L->n_big;

@@ -46,7 +46,14 @@ struct Lowering {
     int n_big = 0;                        // big programs; the scheduling order puts them last
     int gslots = 0;                       // device-memory spill slots of the big programs (max)
     uint64_t total_uops = 0;
-    DevBuf dcode, ddesc, dorder, dactive, dnparams;
+    // the device copies of the micro-code, descriptors, scheduling order, active flags and
+    // parameter counts: one buffer, one upload
+    DevBuf dbuf;
+    const UInstr* dcode = nullptr;
+    const ProgDesc* ddesc = nullptr;
+    const int32_t* dorder = nullptr;
+    const uint8_t* dactive = nullptr;
+    const uint8_t* dnparams = nullptr;
 };
 
 int set_error(int code, const char* fmt, ...);

@@ -1,6 +1,6 @@
 // variation.h -- the search loop's variation operators (operators/generator.rs,
 // mutation.rs, crossover.rs) over programs in the compiled wire format, for the device
-// variation kernel (one thread per offspring).
+// variation kernel (one offspring per wave).
 //
 // A program is the postfix code of Program::from_nodes (program.rs:34-55): one opcode and
 // one operand per node, constants in postfix order (LoadConst operand = the constant's

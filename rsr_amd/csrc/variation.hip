@@ -65,8 +65,12 @@ __device__ VProg parent_view(const VarySet& S, int i) {
     return p;
 }
 
+// One offspring per wave (lane 0; the other lanes leave at once).  Each offspring is a
+// serial walk with its own control flow: 64 offspring in one wave's lanes would run the
+// union of their paths one after the other, where one per wave spreads them over the CUs.
 __global__ __launch_bounds__(64) void k_vary(const VarySet S) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x != 0) return;
+    const int t = blockIdx.x;
     if (t >= S.n) return;
     const rsr_vary_task task = S.tasks[t];
     VRng r = vrng_seed(task.seed);
@@ -118,6 +122,23 @@ __global__ __launch_bounds__(64) void k_vary(const VarySet S) {
 }
 
 bool needs_parent(int kind) { return kind != RSR_VARY_GENERATE; }
+
+// The packed offspring's four arrays (code, operands, constant types, values), each at the
+// start of its own capacity-sized region, gathered into one contiguous block for one
+// download: bytes [0, n0) from a, [n0, n0 + n1) from b, and so on.
+struct Gather4 {
+    const uint8_t* src[4];
+    int64_t end[4];  // cumulative byte counts
+    uint8_t* dst;
+};
+__global__ __launch_bounds__(256) void k_gather4(const Gather4 G) {
+    const int64_t total = G.end[3];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int s = i < G.end[0] ? 0 : i < G.end[1] ? 1 : i < G.end[2] ? 2 : 3;
+        const int64_t b = s == 0 ? 0 : G.end[s - 1];
+        G.dst[i] = G.src[s][i - b];
+    }
+}
 
 }  // namespace
 
@@ -179,9 +200,11 @@ int vary(rsr_context* ctx, const std::vector<rsr_vary_task>& tasks, const Wire& 
                  b_bump = al(3 * sizeof(int)), b_oops = al((size_t)n * kVMaxNodes),
                  b_oargs = al((size_t)n * kVMaxNodes * 2), b_oct = al((size_t)n * kVMaxNodes),
                  b_ocv = al((size_t)n * kVMaxVals * 4), b_meta = al((size_t)n * 32);
+    // the uploaded inputs end with the zeroed extents counter, which the kernel's metadata
+    // follows, so one download returns both; then the output slots and their packed copy
     const size_t in_bytes = b_tasks + b_ops + b_args + b_co + b_ct + b_cof + b_cv + b_vo + b_vt + b_vi + b_al + b_bump;
     const size_t slots = b_oops + b_oargs + b_oct + b_ocv;
-    const size_t total = in_bytes + 2 * slots + b_meta;  // the slots, then the packed copy
+    const size_t total = in_bytes + b_meta + 2 * slots;
     DevBuf buf;
     int rc = buf.alloc(ctx, total);
     if (rc) return rc;
@@ -209,7 +232,8 @@ int vary(rsr_context* ctx, const std::vector<rsr_vary_task>& tasks, const Wire& 
     S.allowed = (const uint8_t*)put(env.ops.data(), env.ops.size(), b_al);
     S.n_allowed = (int)env.ops.size();
     S.bump = (int*)put(nullptr, 0, b_bump);  // zero (the staging buffer is zero-filled)
-    char* o = (char*)buf.ptr + in_bytes;
+    S.o_meta = (int32_t*)((char*)buf.ptr + in_bytes);
+    char* o = (char*)buf.ptr + in_bytes + b_meta;
     S.o_ops = (uint8_t*)o;
     S.o_args = (uint16_t*)(o + b_oops);
     S.o_ct = (uint8_t*)(o + b_oops + b_oargs);
@@ -219,37 +243,57 @@ int vary(rsr_context* ctx, const std::vector<rsr_vary_task>& tasks, const Wire& 
     S.p_args = (uint16_t*)(pk + b_oops);
     S.p_ct = (uint8_t*)(pk + b_oops + b_oargs);
     S.p_cv = (float*)(pk + b_oops + b_oargs + b_oct);
-    S.o_meta = (int32_t*)(pk + slots);
     HIP_TRY(hipMemcpyAsync(buf.ptr, host.data(), in_bytes, hipMemcpyHostToDevice, ctx->stream), "variation upload");
-    hipLaunchKernelGGL(k_vary, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, ctx->stream, S);
+    hipLaunchKernelGGL(k_vary, dim3((unsigned)n), dim3(64), 0, ctx->stream, S);
     HIP_TRY(hipGetLastError(), "variation launch");
-    std::vector<int32_t> meta((size_t)n * 8);
-    int used[3] = {0, 0, 0};
-    HIP_TRY(hipMemcpyAsync(meta.data(), S.o_meta, meta.size() * 4, hipMemcpyDeviceToHost, ctx->stream),
+    // the extents counter and the metadata: one download
+    std::vector<int32_t> head(b_bump / 4 + (size_t)n * 8);
+    HIP_TRY(hipMemcpyAsync(head.data(), S.bump, head.size() * 4, hipMemcpyDeviceToHost, ctx->stream),
             "variation meta");
-    HIP_TRY(hipMemcpyAsync(used, S.bump, sizeof(used), hipMemcpyDeviceToHost, ctx->stream), "variation extents");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "variation");
+    const int used[3] = {head[0], head[1], head[2]};
+    const int32_t* meta = head.data() + b_bump / 4;
     for (int t = 0; t < n; ++t)
         if (meta[8 * (size_t)t + 3] & 1)
             return set_error(RSR_E_UNSUPPORTED, "task %d: offspring above %d nodes", t, kVMaxNodes);
-    // the packed extents only
-    std::vector<uint8_t> ops((size_t)used[0] + 1), ct((size_t)used[1] + 1);
-    std::vector<uint16_t> args((size_t)used[0] + 1);
-    std::vector<float> cv((size_t)used[2] + 1);
-    HIP_TRY(hipMemcpyAsync(ops.data(), S.p_ops, used[0], hipMemcpyDeviceToHost, ctx->stream), "variation code");
-    HIP_TRY(hipMemcpyAsync(args.data(), S.p_args, 2 * (size_t)used[0], hipMemcpyDeviceToHost, ctx->stream),
-            "variation operands");
-    HIP_TRY(hipMemcpyAsync(ct.data(), S.p_ct, used[1], hipMemcpyDeviceToHost, ctx->stream), "variation types");
-    HIP_TRY(hipMemcpyAsync(cv.data(), S.p_cv, 4 * (size_t)used[2], hipMemcpyDeviceToHost, ctx->stream),
-            "variation values");
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "variation download");
+    // the packed extents only, gathered into the (no longer needed) output slots and
+    // downloaded in one copy: code, operands (2-byte aligned), types, values (4-byte aligned)
+    const int64_t n_ops = used[0], n_args = 2 * (int64_t)used[0], n_ct = used[1], n_cv = 4 * (int64_t)used[2];
+    const int64_t a_args = (n_ops + 1) / 2 * 2, a_ct = a_args + n_args, a_cv = (a_ct + n_ct + 3) / 4 * 4;
+    const int64_t packed = a_cv + n_cv;
+    if ((size_t)packed > slots) return set_error(RSR_E_HIP, "variation: packed offspring exceed their slots");
+    std::vector<float> blk((size_t)packed / 4 + 1);
+    if (packed > 0) {
+        Gather4 G;
+        G.src[0] = S.p_ops;
+        G.src[1] = (const uint8_t*)S.p_args;
+        G.src[2] = S.p_ct;
+        G.src[3] = (const uint8_t*)S.p_cv;
+        // a part's padding bytes (at most 3) are read past its used extent, inside the
+        // buffer, and never read back below
+        G.end[0] = a_args;
+        G.end[1] = a_ct;
+        G.end[2] = a_cv;
+        G.end[3] = packed;
+        G.dst = (uint8_t*)o;
+        hipLaunchKernelGGL(k_gather4, dim3((unsigned)std::min<int64_t>((packed + 255) / 256, 1024)), dim3(256), 0,
+                           ctx->stream, G);
+        HIP_TRY(hipGetLastError(), "variation gather");
+        HIP_TRY(hipMemcpyAsync(blk.data(), o, (size_t)packed, hipMemcpyDeviceToHost, ctx->stream), "variation download");
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "variation download");
+    }
+    const uint8_t* hb = (const uint8_t*)blk.data();
+    const uint8_t* ops = hb;
+    const uint16_t* args = (const uint16_t*)(hb + a_args);
+    const uint8_t* ct = hb + a_ct;
+    const float* cv = (const float*)(hb + a_cv);
     if (applied) applied->assign((size_t)n, 0);
     for (int t = 0; t < n; ++t) {  // in task order, whatever order the tasks packed in
-        const int32_t* m = meta.data() + 8 * (size_t)t;
-        out.ops.insert(out.ops.end(), ops.begin() + m[4], ops.begin() + m[4] + m[0]);
-        out.args.insert(out.args.end(), args.begin() + m[4], args.begin() + m[4] + m[0]);
-        out.ct.insert(out.ct.end(), ct.begin() + m[5], ct.begin() + m[5] + m[1]);
-        out.cv.insert(out.cv.end(), cv.begin() + m[6], cv.begin() + m[6] + m[2]);
+        const int32_t* m = meta + 8 * (size_t)t;
+        out.ops.insert(out.ops.end(), ops + m[4], ops + m[4] + m[0]);
+        out.args.insert(out.args.end(), args + m[4], args + m[4] + m[0]);
+        out.ct.insert(out.ct.end(), ct + m[5], ct + m[5] + m[1]);
+        out.cv.insert(out.cv.end(), cv + m[6], cv + m[6] + m[2]);
         out.co.push_back((int64_t)out.ops.size());
         out.cof.push_back((int64_t)out.ct.size());
         out.vo.push_back((int64_t)out.cv.size());
