@@ -207,13 +207,29 @@ class GradJob final : public Job {
         return RSR_OK;
     }
     bool needs_counters() const override { return false; }
-    int finish() override {
-        HIP_TRY(hipMemcpyAsync(mse_out, d_mse, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+    size_t stage_bytes() const override { return (size_t)p->n * 33 * sizeof(float); }
+    int stage(char* host) override {  // the MSEs, then the gradients
+        HIP_TRY(hipMemcpyAsync(host, d_mse, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
                 "mse download");
-        HIP_TRY(hipMemcpyAsync(grad_out, d_grad, (size_t)p->n * 32 * sizeof(float), hipMemcpyDeviceToHost,
-                               ctx->stream),
+        HIP_TRY(hipMemcpyAsync(host + (size_t)p->n * sizeof(float), d_grad, (size_t)p->n * 32 * sizeof(float),
+                               hipMemcpyDeviceToHost, ctx->stream),
                 "grad download");
-        HIP_TRY(hipStreamSynchronize(ctx->stream), "grad download");
+        staged = (const float*)host;
+        return RSR_OK;
+    }
+    const float* staged = nullptr;  // results in pinned host memory (two-phase finish)
+    int finish() override {
+        if (staged) {
+            memcpy(mse_out, staged, (size_t)p->n * sizeof(float));
+            memcpy(grad_out, staged + p->n, (size_t)p->n * 32 * sizeof(float));
+        } else {
+            HIP_TRY(hipMemcpyAsync(mse_out, d_mse, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+                    "mse download");
+            HIP_TRY(hipMemcpyAsync(grad_out, d_grad, (size_t)p->n * 32 * sizeof(float), hipMemcpyDeviceToHost,
+                                   ctx->stream),
+                    "grad download");
+            HIP_TRY(hipStreamSynchronize(ctx->stream), "grad download");
+        }
         for (int i = 0; i < p->n; ++i) {
             if (!on[i]) {
                 mse_out[i] = NAN;
@@ -1486,6 +1502,15 @@ int run_job(rsr_context* ctx, Job& job) {
     while (!job.done) {
         int rc = run_tick(ctx, one);
         if (rc) return rc;
+    }
+    // the results through the context's pinned block: a download into pinned memory is one
+    // DMA, where one into pageable memory also costs the runtime a bounce copy (tens of
+    // microseconds, most of a small evaluation's time)
+    const size_t b = job.stage_bytes();
+    if (b > 0 && ensure_host_stage(ctx, b) == RSR_OK) {
+        int rc = job.stage(ctx->host_stage);
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "result download");
     }
     return job.finish();
 }

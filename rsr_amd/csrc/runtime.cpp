@@ -148,6 +148,21 @@ int DevBuf::upload(rsr_context* c, const void* src, size_t n) {
     return RSR_OK;
 }
 
+int rsr::ensure_host_stage(rsr_context* c, size_t need) {
+    if (c->host_stage_cap >= need) return RSR_OK;
+    if (c->host_stage) (void)hipHostFree(c->host_stage);
+    c->host_stage = nullptr;
+    c->host_stage_cap = 0;
+    const size_t cap = std::max<size_t>((need + 255) / 256 * 256 * 2, (size_t)1 << 20);
+    if (hipHostMalloc((void**)&c->host_stage, cap, hipHostMallocDefault) != hipSuccess) {
+        c->host_stage = nullptr;
+        (void)hipGetLastError();
+        return RSR_E_HIP;
+    }
+    c->host_stage_cap = cap;
+    return RSR_OK;
+}
+
 int rsr::ensure_scratch(rsr_context* c, DevBuf& b, size_t n) {
     if (b.bytes >= n) return RSR_OK;
     return b.alloc(c, n + n / 4);

@@ -165,6 +165,10 @@ EvalTuning pick_tuning(const rsr_context* ctx, const EvalStep& st);
 uint64_t tuning_key(int num_features, int64_t count8, int n_progs);
 void trim_pool(rsr_context* c);
 int ensure_scratch(rsr_context* c, DevBuf& b, size_t n);
+// The context's pinned host staging block, grown to at least `need` bytes (the old block
+// freed: callers synchronise before they return, so nothing is in flight into it).
+// RSR_E_HIP when pinned memory cannot be had; the callers then download directly.
+int ensure_host_stage(rsr_context* c, size_t need);
 DevDataset dev_view(const rsr_dataset* d);
 DevPrograms dev_view(const rsr_programs* p, const Lowering* L);
 RefProgram ref_program(const rsr_programs* p, int i);

@@ -289,17 +289,8 @@ struct rsr_server {
             if (a.job->done) need += round_up(a.job->stage_bytes());
         if (need == 0) return RSR_OK;
         rsr_context* c = L.ctx;
-        if (c->host_stage_cap < need) {  // nothing is in flight into the old block: the last tick synchronised
-            if (c->host_stage) (void)hipHostFree(c->host_stage);
-            c->host_stage = nullptr;
-            c->host_stage_cap = 0;
-            const size_t cap = std::max<size_t>(round_up(need) * 2, (size_t)1 << 20);
-            if (hipHostMalloc((void**)&c->host_stage, cap, hipHostMallocDefault) != hipSuccess) {
-                c->host_stage = nullptr;
-                return RSR_OK;  // no staging memory: every job finishes in one phase
-            }
-            c->host_stage_cap = cap;
-        }
+        // nothing is in flight into the old block: the last tick synchronised
+        if (ensure_host_stage(c, need) != RSR_OK) return RSR_OK;  // every job finishes in one phase
         char* at = c->host_stage;
         for (auto& a : L.active) {
             if (!a.job->done) continue;
