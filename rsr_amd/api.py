@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import time
 from typing import List, Optional, Sequence
 
 import numpy as np
@@ -859,11 +860,14 @@ class VaryTask(C.Structure):
 
 
 def vary(ctx: Context, tasks: Sequence[tuple], parents: Sequence[Program], variables: Sequence[tuple],
-         allowed_ops: Sequence[int]):
+         allowed_ops: Sequence[int], packed=None, unpack: bool = True, timing: Optional[dict] = None):
     """rsr_vary: one offspring per task on the device.
 
     tasks: (kind, seed, a=-1, b=-1, type=0, depth=0, max_size=32) tuples; variables: (ValueType,
-    feature index) pairs; allowed_ops: RefOp codes.  Returns (offspring, applied kinds)."""
+    feature index) pairs; allowed_ops: RefOp codes.  Returns (offspring, applied kinds).
+    packed: pack_programs(parents), when the caller has it; unpack=False returns the
+    offspring in the wire format (the seven arrays) instead of Program objects; timing, a
+    dict, receives "call_ms", the time of the rsr_vary call alone."""
     l = lib()
     fn = l.rsr_vary
     fn.restype = C.c_int
@@ -880,22 +884,27 @@ def vary(ctx: Context, tasks: Sequence[tuple], parents: Sequence[Program], varia
         depth = int(t[5]) if len(t) > 5 else 0
         ms = int(t[6]) if len(t) > 6 else 32
         arr[i] = VaryTask(kind, vt, depth, 0, a, b, ms, seed)
-    ops, args, co, ct, cof, cv, vo = pack_programs(list(parents))
+    ops, args, co, ct, cof, cv, vo = packed if packed is not None else pack_programs(list(parents))
     vt_ = np.array([v[0] for v in variables] or [0], np.uint8)
     vi_ = np.array([v[1] for v in variables] or [0], np.uint8)
     al = np.array(list(allowed_ops) or [0], np.uint8)
     cap = 384 * max(n, 1)
-    ops_o, args_o = np.zeros(cap, np.uint8), np.zeros(cap, np.uint16)
-    ct_o, cv_o = np.zeros(cap, np.uint8), np.zeros(9 * cap, np.float32)
+    # output capacities (written only up to what the offspring use)
+    ops_o, args_o = np.empty(cap, np.uint8), np.empty(cap, np.uint16)
+    ct_o, cv_o = np.empty(cap, np.uint8), np.empty(9 * cap, np.float32)
     co_o, cof_o, vo_o = np.zeros(n + 1, np.int64), np.zeros(n + 1, np.int64), np.zeros(n + 1, np.int64)
     kinds = np.zeros(max(n, 1), np.uint8)
     pad = lambda x, t: x if x.size else np.zeros(1, t)
+    t0 = time.perf_counter()
     _check(fn(ctx.h, n, arr, len(parents), _ptr(pad(ops, np.uint8), _u8p), _ptr(pad(args, np.uint16), _u16p),
               _ptr(co, _i64p), _ptr(pad(ct, np.uint8), _u8p), _ptr(cof, _i64p), _ptr(pad(cv, np.float32), _f32p),
               _ptr(vo, _i64p), len(variables), _ptr(vt_, _u8p), _ptr(vi_, _u8p), len(allowed_ops), _ptr(al, _u8p),
               cap, cap, 9 * cap, _ptr(ops_o, _u8p), _ptr(args_o, _u16p), _ptr(co_o, _i64p), _ptr(ct_o, _u8p),
               _ptr(cof_o, _i64p), _ptr(cv_o, _f32p), _ptr(vo_o, _i64p), _ptr(kinds, _u8p)))
-    return unpack_programs(ops_o, args_o, co_o, ct_o, cof_o, cv_o, vo_o), kinds[:n]
+    if timing is not None:
+        timing["call_ms"] = (time.perf_counter() - t0) * 1e3
+    wire = (ops_o[:co_o[n]], args_o[:co_o[n]], co_o, ct_o[:cof_o[n]], cof_o, cv_o[:vo_o[n]], vo_o)
+    return (unpack_programs(*wire) if unpack else wire), kinds[:n]
 
 
 class EngineConfig(C.Structure):
