@@ -63,8 +63,10 @@ def autotune(ctx, P, ds):
     the fastest for the workload's shape, which later evaluations in the per-workload mode
     take."""
     ctx.set_tuning(0, 0)
-    ppl, group = ctx.autotune(P, ds)
-    return {"points_per_lane": ppl, "programs_per_item": group, "by": "rsr_autotune"}
+    ctx.autotune(P, ds)
+    ppl, group, pers = ctx.query_tuning(P, ds)
+    return {"points_per_lane": ppl, "programs_per_item": group, "launch": "persistent" if pers else "oneshot",
+            "by": "rsr_autotune"}
 
 
 def cpu_run(fn, progs, ods, threads):
@@ -326,8 +328,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving job mix")
     ap.add_argument("--no-graphs", action="store_true", help="skip the graph-replay modes")
-    ap.add_argument("--launch", choices=["persistent", "oneshot"], default="oneshot",
-                    help="launch form of the evaluation kernel (the same bits either way)")
+    ap.add_argument("--launch", choices=["auto", "persistent", "oneshot"], default="auto",
+                    help="launch form of the evaluation kernel (auto: the autotuner's choice; the same bits)")
     ap.add_argument("--no-configs", action="store_true", help="skip the matrix / fitting workloads")
     ap.add_argument("--serve-jobs", type=int, default=64)
     ap.add_argument("--serve-gens", type=int, default=3)
@@ -347,7 +349,8 @@ def main():
     X, y, types = W.data_c1(100_000, seed=1001)
     progs = W.gen_c1(8192, seed=2001)
     ctx = api.Context(0)
-    ctx.set_launch_mode(api.LAUNCH_ONESHOT if args.launch == "oneshot" else api.LAUNCH_PERSISTENT)
+    ctx.set_launch_mode({"auto": api.LAUNCH_AUTO, "oneshot": api.LAUNCH_ONESHOT,
+                         "persistent": api.LAUNCH_PERSISTENT}[args.launch])
     ds = api.Dataset.new(ctx, X, y, types)
     P = api.Programs(ctx, progs)
     st = P.status(ds)

@@ -105,15 +105,19 @@ RSR_API int rsr_context_set_summation(rsr_context* ctx, int mode);
 
 /* The launch form of the MSE evaluation kernel (no reference counterpart: the reference's
  * evaluator is a CPU loop, evaluator.rs:153-174).
- *   RSR_LAUNCH_ONESHOT (default): one workgroup per four work items (tile x program group,
- *     over every merged request), dealt to the CUs by the hardware dispatcher;
+ *   RSR_LAUNCH_ONESHOT: one workgroup per four work items (tile x program group, over every
+ *     merged request), dealt to the CUs by the hardware dispatcher;
  *   RSR_LAUNCH_PERSISTENT: a launch with more work items than the device holds at once runs
  *     as a grid of the workgroups that fit, draining the items through a device work queue
- *     with one counter per XCD (a smaller launch stays one-shot).  Measured on one MI355X it
- *     is within 1% of one-shot on the Lorentz workload and slower on the serving mix.
- * Both give the same bits (every work item writes its own partial sums or terms). */
+ *     with one counter per XCD (a smaller launch stays one-shot);
+ *   RSR_LAUNCH_AUTO (default): the form of the workload's tuning -- rsr_autotune times both
+ *     (the work queue at two and four points per lane), the heuristic takes one-shot.  On
+ *     the Lorentz workload in reference order the work queue was 2.6% faster at four points
+ *     per lane; on the serving mix one-shot is faster.
+ * Every form gives the same bits (every work item writes its own partial sums or terms). */
 #define RSR_LAUNCH_ONESHOT 0
 #define RSR_LAUNCH_PERSISTENT 1
+#define RSR_LAUNCH_AUTO 2
 RSR_API int rsr_context_set_launch_mode(rsr_context* ctx, int mode);
 /* Graph mode (default off): rsr_eval_mse captures the evaluation of a program batch over a
  * dataset (its launches and the copy of the results) into a hipGraph on first use and
@@ -137,10 +141,17 @@ RSR_API int rsr_context_set_partials_limit(rsr_context* ctx, uint64_t bytes);
  * counterpart (tuning of the device launches). */
 RSR_API int rsr_context_set_tuning(rsr_context* ctx, int points_per_lane, int programs_per_item);
 RSR_API int rsr_context_get_tuning(const rsr_context* ctx, int* points_per_lane, int* programs_per_item);
+/* The tuning an MSE evaluation of this batch over this dataset takes now: points per lane,
+ * programs per work item and launch form (1: the work queue) -- the fixed tuning, or the
+ * autotuned or heuristic one of the workload's shape.  No reference counterpart. */
+RSR_API int rsr_context_query_tuning(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds,
+                                     int* points_per_lane, int* programs_per_item, int* persistent);
 /* The autotuner: times every tuning (points per lane 1 to 4 x programs per work item 1, 2,
- * 4, 8, 16, 32) on one MSE evaluation of this batch over this dataset and keeps the fastest for
+ * 4, 8, 16, 32, and in RSR_LAUNCH_AUTO mode the work-queue launch form at two and four points
+ * per lane) on one MSE evaluation of this batch over this dataset and keeps the fastest for
  * the workload's shape, where direct calls, the server and the fitters find it; returns
- * the choice.  Costs about 72 evaluations.  No reference counterpart. */
+ * the choice of points per lane and programs per item.  Costs about 108 evaluations (72
+ * with a fixed launch form).  No reference counterpart. */
 RSR_API int rsr_autotune(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds, int* points_per_lane,
                          int* programs_per_item);
 

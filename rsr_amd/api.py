@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_HERE, "librsr_amd.so")
 
 RSR_OK = 0
 RSR_E_INVALID_ARG = -1
-LAUNCH_ONESHOT, LAUNCH_PERSISTENT = 0, 1
+LAUNCH_ONESHOT, LAUNCH_PERSISTENT, LAUNCH_AUTO = 0, 1, 2
 RSR_E_PROGRAM = -5
 
 _lib = None
@@ -46,6 +46,7 @@ SIGNATURES = {
     "rsr_context_memory": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "rsr_context_set_tuning": (C.c_int, [_P, C.c_int, C.c_int]),
     "rsr_context_get_tuning": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "rsr_context_query_tuning": (C.c_int, [_P, _P, _P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "rsr_context_profile": (C.c_int, [_P, C.c_int]),
     "rsr_context_profile_read": (C.c_int, [_P, C.POINTER(C.c_double), _i64p]),
     "rsr_context_synchronize": (C.c_int, [_P]),
@@ -155,6 +156,12 @@ class Context:
         _check(lib().rsr_context_get_tuning(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def query_tuning(self, progs: "Programs", ds: "Dataset"):
+        """(points per lane, programs per item, persistent) an evaluation of progs over ds takes."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().rsr_context_query_tuning(self.h, progs.h, ds.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, bool(c.value)
+
     def profile(self, enable: bool):
         _check(lib().rsr_context_profile(self.h, int(enable)))
 
@@ -168,9 +175,10 @@ class Context:
         _check(lib().rsr_context_set_summation(self.h, 1 if reference_order else 0))
 
     def set_launch_mode(self, mode: int):
-        """LAUNCH_ONESHOT (default): one workgroup per four work items; LAUNCH_PERSISTENT:
-        launches with more work than the device holds at once run as a resident grid draining
-        a per-XCD device work queue (smaller ones one-shot).  The same bits either way."""
+        """LAUNCH_ONESHOT: one workgroup per four work items; LAUNCH_PERSISTENT: launches with
+        more work than the device holds at once run as a resident grid draining a per-XCD
+        device work queue (smaller ones one-shot); LAUNCH_AUTO (default): the form the
+        workload's autotuned tuning chose (one-shot without one).  The same bits either way."""
         _check(lib().rsr_context_set_launch_mode(self.h, int(mode)))
 
     def set_graphs(self, enable: bool):

@@ -1159,7 +1159,7 @@ namespace {
 // most ctx->arena_bytes; when the steps need more, every step is split into the same
 // number of tile ranges, evaluated round by round with the summation state carried
 // between rounds (the sums are the same as unsplit).
-int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int ppl) {
+int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int ppl, int* queue = nullptr) {
     const bool ref = ctx->summation == RSR_SUM_REFERENCE;
     EvalTuning t = ctx->tuning;
     t.ppl = ppl;
@@ -1223,7 +1223,7 @@ int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int
         HIP_TRY(grad ? launch_grad_steps(round.data(), (int)round.size(), terms, ctx->stream,
                                          ctx->next_timer(), big)
                      : launch_mse_steps(round.data(), (int)round.size(), t, terms, ctx->stream, ctx->next_timer(), big,
-                                        ctx->queue_ptr()),
+                                        queue),
                 "evaluation launch");
     }
     return RSR_OK;
@@ -1232,7 +1232,7 @@ int launch_rounds(rsr_context* ctx, std::vector<EvalStep>& steps, bool grad, int
 // MSE steps: each step at the tuning's points per lane, lowered until its own spill area
 // fits the workgroup LDS (per step, so a request's result does not depend on the others).
 int run_mse(rsr_context* ctx, std::vector<EvalStep>& steps) {
-    std::vector<EvalStep> by_ppl[5];  // by points per lane (1 to 4)
+    std::vector<EvalStep> by_ppl[5][2];  // by points per lane (1 to 4) and launch form
     for (const auto& s0 : steps) {
         const EvalTuning t = pick_tuning(ctx, s0);
         EvalStep s = s0;
@@ -1242,11 +1242,14 @@ int run_mse(rsr_context* ctx, std::vector<EvalStep>& steps) {
         if (eval_mse_lds_bytes(s.P.spill_slots, ppl) > ctx->max_lds)
             return set_error(RSR_E_UNSUPPORTED, "spill area of %zu bytes exceeds the workgroup LDS limit",
                              eval_mse_lds_bytes(s.P.spill_slots, ppl));
-        by_ppl[ppl].push_back(s);
+        by_ppl[ppl][ctx->queue_for(t) ? 1 : 0].push_back(s);
     }
     int rc;
     for (int k = 4; k >= 1; --k)
-        if (!by_ppl[k].empty() && (rc = launch_rounds(ctx, by_ppl[k], false, k))) return rc;
+        for (int q = 0; q < 2; ++q)
+            if (!by_ppl[k][q].empty() &&
+                (rc = launch_rounds(ctx, by_ppl[k][q], false, k, q ? (int*)ctx->queue.ptr : nullptr)))
+                return rc;
     return RSR_OK;
 }
 
@@ -1358,7 +1361,7 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
     EvalGraph* g = nullptr;
     for (auto& e : p->graphs)
         if (e->ds_id == ds->id && e->summation == summ && e->ppl == t.ppl && e->group == t.group &&
-            e->launch_mode == ctx->launch_mode)
+            e->persistent == (ctx->queue_for(t) != nullptr))
             g = e.get();
     if (!g) {
         std::vector<int32_t> order;
@@ -1382,7 +1385,7 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
         ng->summation = summ;
         ng->ppl = t.ppl;
         ng->group = t.group;
-        ng->launch_mode = ctx->launch_mode;
+        ng->persistent = ctx->queue_for(t) != nullptr;
         // the graph's own buffers; when they do not fit, the batch's other graphs are
         // released and the allocation tried once more, else the call runs eagerly
         auto setup = [&]() {
@@ -1409,7 +1412,7 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
         st.mse_out = (float*)ng->d_mse.ptr;
         st.partials = (float*)ng->arena.ptr;
         HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed), "graph capture");
-        hipError_t e = launch_mse_steps(&st, 1, t, terms, ctx->stream, nullptr, BigSpill{}, ctx->queue_ptr());
+        hipError_t e = launch_mse_steps(&st, 1, t, terms, ctx->stream, nullptr, BigSpill{}, ctx->queue_for(t));
         if (e == hipSuccess)
             e = hipMemcpyAsync(ng->host, ng->d_mse.ptr, (size_t)p->n * sizeof(float), hipMemcpyDeviceToHost,
                                ctx->stream);
@@ -1441,9 +1444,12 @@ int eval_mse_graph(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, flo
 }
 
 // Points per lane x programs per work item, each candidate timed on the batch itself
-// (best of two evaluations after a warm-up, events on the context's stream).
+// (best of two evaluations after a warm-up, events on the context's stream); in the
+// automatic launch mode also the work-queue form at two and four points per lane (at three
+// its work loop spills, and at one it has not paid).
 int autotune(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int* ppl_out, int* group_out) {
     static const int kPpl[] = {1, 2, 3, 4}, kGroup[] = {1, 2, 4, 8, 16, 32};
+    const bool forms = ctx->launch_mode == RSR_LAUNCH_AUTO;
     const bool saved_auto = ctx->auto_tune;
     const EvalTuning saved = ctx->tuning;
     std::vector<float> out(std::max(p->n, 1));
@@ -1459,9 +1465,11 @@ int autotune(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int* ppl_
     int n_ok = 0;
     ctx->auto_tune = false;
     for (int ppl : kPpl)
-        for (int group : kGroup) {
+        for (int group : kGroup)
+        for (int pers = 0; pers < ((forms && (ppl == 2 || ppl == 4)) ? 2 : 1); ++pers) {
             ctx->tuning.ppl = ppl;
             ctx->tuning.group = group;
+            ctx->tuning.persistent = pers;
             float ms = -1.0f;
             for (int rep = 0; rep < 3 && rc == RSR_OK; ++rep) {
                 std::unique_ptr<Job> job;
@@ -1480,6 +1488,7 @@ int autotune(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int* ppl_
                 best = ms;
                 pick.ppl = ppl;
                 pick.group = group;
+                pick.persistent = pers;
             }
         }
     (void)hipEventDestroy(e0);

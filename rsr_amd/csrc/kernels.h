@@ -50,8 +50,9 @@ inline size_t big_points_floats(int gslots, int channels) {
 
 
 struct EvalTuning {
-    int ppl = 2;        // points per lane: 1 to 4
-    int group = 8;      // programs per work item
+    int ppl = 2;         // points per lane: 1 to 4
+    int group = 8;       // programs per work item
+    int persistent = 0;  // the work-queue launch form (RSR_LAUNCH_AUTO; chosen by the autotuner)
 };
 
 // Optional events recorded around the main evaluation kernel of a launch (profiling).

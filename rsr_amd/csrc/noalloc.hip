@@ -105,7 +105,7 @@ extern "C" RSR_API int rsr_eval_mse_noalloc(rsr_context* ctx, rsr_programs* p, c
                          "the evaluation needs %zu bytes of scratch, above the %zu bytes of rsr_eval_mse_noalloc; "
                          "use rsr_eval_mse",
                          (head + part_aligned + spill) * sizeof(float), kScratchBytes);
-    int* queue = ctx->launch_mode == RSR_LAUNCH_PERSISTENT ? (int*)base : nullptr;
+    int* queue = ctx->queue_for(t) ? (int*)base : nullptr;
     st.partials = base + head;
     BigSpill big;
     if (spill) {

@@ -363,7 +363,7 @@ extern "C" int rsr_context_set_summation(rsr_context* ctx, int mode) {
 
 extern "C" int rsr_context_set_launch_mode(rsr_context* ctx, int mode) {
     if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
-    if (mode != RSR_LAUNCH_ONESHOT && mode != RSR_LAUNCH_PERSISTENT)
+    if (mode != RSR_LAUNCH_ONESHOT && mode != RSR_LAUNCH_PERSISTENT && mode != RSR_LAUNCH_AUTO)
         return set_error(RSR_E_INVALID_ARG, "bad launch mode");
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->launch_mode = mode;
@@ -402,6 +402,26 @@ extern "C" int rsr_context_get_tuning(const rsr_context* ctx, int* ppl, int* gro
     if (!ctx) return set_error(RSR_E_INVALID_ARG, "ctx is null");
     if (ppl) *ppl = ctx->auto_tune ? 0 : ctx->tuning.ppl;
     if (group) *group = ctx->auto_tune ? 0 : ctx->tuning.group;
+    return RSR_OK;
+}
+
+extern "C" int rsr_context_query_tuning(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int* ppl,
+                                        int* group, int* persistent) {
+    if (!ctx || !p || !ds) return set_error(RSR_E_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    Lowering* L;
+    int rc = get_lowering(p, ds->num_features, false, &L);
+    if (rc) return rc;
+    EvalStep probe;
+    probe.D = dev_view(ds);
+    probe.P = dev_view(p, L);
+    probe.P.n_progs = p->n - L->n_invalid;
+    EvalTuning t = pick_tuning(ctx, probe);
+    while (t.ppl > 1 && eval_mse_lds_bytes(L->spill_slots, t.ppl) > ctx->max_lds) t.ppl /= 2;
+    if (ppl) *ppl = t.ppl;
+    if (group) *group = t.group;
+    if (persistent) *persistent = ctx->queue_for(t) != nullptr ? 1 : 0;
     return RSR_OK;
 }
 

@@ -68,7 +68,8 @@ uint64_t next_object_id();  // process-unique, never reused
 struct EvalGraph {
     uint64_t ds_id = 0;
     uint64_t last_use = 0;  // replay order, for the per-batch bound
-    int summation = 0, ppl = 0, group = 0, launch_mode = 0;
+    int summation = 0, ppl = 0, group = 0;
+    bool persistent = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     DevBuf d_mse, d_order, arena;
@@ -104,8 +105,11 @@ struct rsr_context {
     // The device work queue of the persistent evaluation launch (zero between launches) and
     // the launch form the evaluations use.
     rsr::DevBuf queue;
-    int launch_mode = 0;  // RSR_LAUNCH_ONESHOT
-    int* queue_ptr() const { return launch_mode == 1 ? (int*)queue.ptr : nullptr; }
+    int launch_mode = 2;  // RSR_LAUNCH_AUTO
+    // the work queue of a launch at tuning t, or null for the one-shot form
+    int* queue_for(const rsr::EvalTuning& t) const {
+        return launch_mode == 1 || (launch_mode == 2 && t.persistent) ? (int*)queue.ptr : nullptr;
+    }
     // pinned host mirror of the jobs' counters a tick reads back (each job owns its device pair)
     int* host_counters = nullptr;
     size_t host_counters_cap = 0;

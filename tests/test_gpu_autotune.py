@@ -49,3 +49,36 @@ def test_autotune_choice_is_used_by_per_workload_mode():
         ds.close()
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("reference", [True, False])
+def test_autotune_launch_form_automatic_mode(reference):
+    """In the automatic launch mode (the default) the autotuner also times the work-queue
+    form; query_tuning reports the choice, and the evaluation's bits equal the same tuning
+    fixed by hand under either forced form (the forms differ only in how work items are
+    dealt out)."""
+    ctx = api.Context(0)
+    try:
+        X, y, t = W.data_c1(30011, seed=63)
+        progs = W.gen_c1(2048, seed=64)
+        ds = api.Dataset.new(ctx, X, y, t)
+        P = api.Programs(ctx, progs)
+        ctx.set_summation(reference)
+        print("autotune, automatic launch form: 2048 programs x 30011 points", flush=True)
+        ppl, group = ctx.autotune(P, ds)
+        q = ctx.query_tuning(P, ds)
+        assert q[:2] == (ppl, group)
+        assert not q[2] or ppl in (2, 4)
+        auto = api.compute_mse(ctx, P, ds)
+        ctx.set_tuning(ppl, group)
+        for mode in (api.LAUNCH_ONESHOT, api.LAUNCH_PERSISTENT):
+            ctx.set_launch_mode(mode)
+            assert ctx.query_tuning(P, ds) == (ppl, group, mode == api.LAUNCH_PERSISTENT)
+            fixed = api.compute_mse(ctx, P, ds)
+            assert np.array_equal(auto.view(np.uint32), fixed.view(np.uint32)), mode
+        ctx.set_launch_mode(api.LAUNCH_AUTO)
+        assert ctx.query_tuning(P, ds) == (ppl, group, False)  # a fixed tuning is one-shot here
+        P.close()
+        ds.close()
+    finally:
+        ctx.close()

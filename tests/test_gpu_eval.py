@@ -435,7 +435,7 @@ def test_persistent_and_oneshot_launches_agree(gpu_ctx, reference):
                 gpu_ctx.set_launch_mode(mode)
                 outs.append((ppl, api.compute_mse(gpu_ctx, P, ds)))
     finally:
-        gpu_ctx.set_launch_mode(api.LAUNCH_ONESHOT)
+        gpu_ctx.set_launch_mode(api.LAUNCH_AUTO)
         gpu_ctx.set_tuning(2, 8)
         gpu_ctx.set_summation(False)
     for ppl in (1, 2, 3, 4):
