@@ -134,7 +134,7 @@ RSR_API int rsr_context_set_partials_limit(rsr_context* ctx, uint64_t bytes);
  * per work item (the programs that share one tile's features in registers).  By default
  * (or after set_tuning(ctx, 0, 0)) each evaluation takes the tuning that rsr_autotune
  * found for its workload shape -- feature count, point count and program count, each
- * rounded to a power of two -- or, before any, a heuristic (three points per lane and 8
+ * rounded to a power of two, and the summation mode -- or, before any, a heuristic (three points per lane and 8
  * programs per work item for large evaluations; four, or two for small ones, with up to 4
  * programs per item while that keeps 8,000 work items).  A nonzero setting fixes the tuning for
  * every evaluation; get_tuning returns 0, 0 in the per-workload mode.  No reference

@@ -1499,7 +1499,7 @@ int autotune(rsr_context* ctx, rsr_programs* p, const rsr_dataset* ds, int* ppl_
     Lowering* L;
     if ((rc = get_lowering(p, ds->num_features, false, &L))) return rc;
     for (int i = 0; i < p->n; ++i) n_ok += L->status[i] == PROG_OK;
-    ctx->tune_cache[tuning_key(ds->num_features, ds->count8, n_ok)] = pick;
+    ctx->tune_cache[tuning_key(ds->num_features, ds->count8, n_ok, ctx->summation)] = pick;
     ctx->tune_version++;
     if (ppl_out) *ppl_out = pick.ppl;
     if (group_out) *group_out = pick.group;

@@ -312,8 +312,9 @@ static int ceil_log2(int64_t v) {
     return k;
 }
 
-uint64_t rsr::tuning_key(int num_features, int64_t count8, int n_progs) {
-    return ((uint64_t)num_features << 16) | ((uint64_t)ceil_log2(count8) << 8) | (uint64_t)ceil_log2(n_progs);
+uint64_t rsr::tuning_key(int num_features, int64_t count8, int n_progs, int summation) {
+    return ((uint64_t)(summation & 1) << 24) | ((uint64_t)num_features << 16) | ((uint64_t)ceil_log2(count8) << 8) |
+           (uint64_t)ceil_log2(n_progs);
 }
 
 // Heuristic, fitted to timings of every tuning (points per lane 2 to 4 x programs per item
@@ -347,7 +348,7 @@ static EvalTuning heuristic_tuning(int num_features, int64_t count8, int n_progs
 
 EvalTuning rsr::pick_tuning(const rsr_context* ctx, const EvalStep& st) {
     if (!ctx->auto_tune) return ctx->tuning;
-    auto it = ctx->tune_cache.find(tuning_key(st.D.num_features, st.D.count8, st.P.n_progs));
+    auto it = ctx->tune_cache.find(tuning_key(st.D.num_features, st.D.count8, st.P.n_progs, ctx->summation));
     if (it != ctx->tune_cache.end()) return it->second;
     return heuristic_tuning(st.D.num_features, st.D.count8, st.P.n_progs);
 }

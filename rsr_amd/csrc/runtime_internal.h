@@ -163,10 +163,11 @@ struct rsr_programs {
 namespace rsr {
 // The evaluation tuning (points per lane, programs per work item) for an MSE step: the
 // context's fixed tuning, or in auto mode the autotuned choice for the step's workload
-// shape (feature count, point count and program count, each in powers of two), else a
+// shape (feature count, point count and program count, each in powers of two, and the
+// summation mode), else a
 // heuristic fitted to a sweep of every tuning over the four workload types.
 EvalTuning pick_tuning(const rsr_context* ctx, const EvalStep& st);
-uint64_t tuning_key(int num_features, int64_t count8, int n_progs);
+uint64_t tuning_key(int num_features, int64_t count8, int n_progs, int summation);
 void trim_pool(rsr_context* c);
 int ensure_scratch(rsr_context* c, DevBuf& b, size_t n);
 // The context's pinned host staging block, grown to at least `need` bytes (the old block

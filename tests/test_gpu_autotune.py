@@ -1,8 +1,9 @@
 """The library's autotuner (rsr_autotune) and its per-workload tuning mode.
 
 A fresh context is in the per-workload mode (get_tuning 0, 0).  rsr_autotune returns a
-valid choice and caches it for the workload shape: later evaluations in the per-workload
-mode -- direct calls and the no-allocation entry point -- run with that tuning, which is
+valid choice and caches it for the workload shape and summation mode: later evaluations in
+the per-workload mode -- direct calls and the no-allocation entry point -- run with that
+tuning, which is
 visible in Kahan summation (per-tile partials depend on the tile): their bits equal an
 evaluation at that tuning fixed by hand.  Reference-order results equal the oracle under
 any tuning."""
@@ -26,10 +27,11 @@ def test_autotune_choice_is_used_by_per_workload_mode():
         ds = api.Dataset.new(ctx, X, y, t)
         P = api.Programs(ctx, progs)
         print("autotune: 512 programs x 20000 points", flush=True)
+        ctx.set_summation(False)  # the choice is cached per summation mode
         ppl, group = ctx.autotune(P, ds)
         assert ppl in (1, 2, 3, 4) and group in (1, 2, 4, 8, 16, 32)
         assert ctx.tuning() == (0, 0)
-        ctx.set_summation(False)
+        assert ctx.query_tuning(P, ds)[:2] == (ppl, group)
         auto = api.compute_mse(ctx, P, ds)
         ctx.set_tuning(ppl, group)
         fixed = api.compute_mse(ctx, P, ds)
