@@ -3,9 +3,8 @@
 A fresh context is in the per-workload mode (get_tuning 0, 0).  rsr_autotune returns a
 valid choice and caches it for the workload shape and summation mode: later evaluations in
 the per-workload mode -- direct calls and the no-allocation entry point -- run with that
-tuning, which is
-visible in Kahan summation (per-tile partials depend on the tile): their bits equal an
-evaluation at that tuning fixed by hand.  Reference-order results equal the oracle under
+tuning, which is visible in Kahan summation (per-tile partials depend on the tile): their
+bits equal an evaluation at that tuning fixed by hand.  Reference-order results equal the oracle under
 any tuning."""
 import numpy as np
 import pytest
