@@ -183,3 +183,37 @@ def test_context_is_not_destroyed_under_its_datasets_and_programs():
     P.close()
     ctx.close()
     assert ctx.h is None
+
+
+def test_served_work_queue_launches_equal_direct(gpu_ctx):
+    """With the work-queue launch form forced on the context, the server's lanes (which copy
+    its settings) evaluate through their own queues: requests that overlap on the device,
+    each with more work items than the device holds at once, give the bits of a direct
+    one-shot evaluation (reference order)."""
+    gpu_ctx.set_summation(True)
+    made = []
+    try:
+        want = []
+        for k, (n, p) in enumerate([(20_011, 2000), (30_007, 1500), (12_289, 3000)]):
+            X, y, t = W.data_c1(n, seed=91 + k)
+            ds = api.Dataset.new(gpu_ctx, X, y, t)
+            P = api.Programs(gpu_ctx, W.gen_c1(p, seed=95 + k))
+            made.append((ds, P))
+            gpu_ctx.set_launch_mode(api.LAUNCH_ONESHOT)
+            want.append(api.compute_mse(gpu_ctx, P, ds, allow_rejected=True))
+        gpu_ctx.set_launch_mode(api.LAUNCH_PERSISTENT)
+        srv = api.Server(gpu_ctx, max_inflight=8)
+        try:
+            print("served work-queue evaluations: 3 requests", flush=True)
+            tickets = [srv.submit_eval(P, ds) for ds, P in made]
+            got = [np.asarray(tk.wait(60.0)) for tk in tickets]
+        finally:
+            srv.close()
+        for k, (g, w) in enumerate(zip(got, want)):
+            assert np.array_equal(g.view(np.uint32), w.view(np.uint32)), k
+    finally:
+        gpu_ctx.set_launch_mode(api.LAUNCH_AUTO)
+        gpu_ctx.set_summation(False)
+        for ds, P in made:
+            P.close()
+            ds.close()
