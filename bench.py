@@ -338,8 +338,9 @@ def main():
     ap.add_argument("--no-from-host", action="store_true", help="skip the evaluation from host buffers")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs on a single MI355X (see DESIGN.md for how "
-              "the path would shard across GPUs)", file=sys.stderr)
+        print("bench.py measures one GPU only: this build runs on a single MI355X (across GPUs the path would "
+              "shard the programs, or the points with one all-reduce of partial sums per evaluation)",
+              file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)

@@ -244,7 +244,7 @@ def test_set_constants_roundtrip(gpu_ctx):
 
 
 def test_full_size_c1_properties(gpu_ctx):
-    """configs[1] at full size: 8,192 programs x 100,000 points.  Reference order: every
+    """The Lorentz workload at full size: 8,192 programs x 100,000 points.  Reference order: every
     program bit-identical to the oracle's f32 evaluator (oracle on 16 host threads).  Kahan:
     within 1e-5 of the FP64 oracle on 64 programs, and within 5e-5 of the f32 evaluator on
     every program (the f32 evaluator's own rounding at 10^5 points).  Plus size-independent
