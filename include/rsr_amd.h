@@ -143,7 +143,9 @@ RSR_API int rsr_context_set_tuning(rsr_context* ctx, int points_per_lane, int pr
 RSR_API int rsr_context_get_tuning(const rsr_context* ctx, int* points_per_lane, int* programs_per_item);
 /* The tuning an MSE evaluation of this batch over this dataset takes now: points per lane,
  * programs per work item and launch form (1: the work queue) -- the fixed tuning, or the
- * autotuned or heuristic one of the workload's shape.  No reference counterpart. */
+ * autotuned or heuristic one of the workload's shape.  Lowers the batch for the dataset's
+ * feature count first if that has not been done (as an evaluation would).  No reference
+ * counterpart. */
 RSR_API int rsr_context_query_tuning(rsr_context* ctx, rsr_programs* progs, const rsr_dataset* ds,
                                      int* points_per_lane, int* programs_per_item, int* persistent);
 /* The autotuner: times every tuning (points per lane 1 to 4 x programs per work item 1, 2,
