@@ -83,3 +83,28 @@ def test_autotune_launch_form_automatic_mode(reference):
         ds.close()
     finally:
         ctx.close()
+
+
+def test_heuristic_tuning_rule():
+    """Without an autotuned entry the tuning follows the heuristic's rule: three points per
+    lane and 8 programs per item, one-shot, for large evaluations (at least 200,000 pairs of a
+    192-point tile and a program); four points per lane and up to 4 programs per item while
+    8,000 work items remain for mid-size ones; two points per lane below 8,000 pairs of a
+    256-point tile and a program."""
+    ctx = api.Context(0)
+    try:
+        cases = [(W.data_c1, W.gen_c1, 100_000, 2048, (3, 8)),   # 524 tiles x 2048 programs
+                 (W.data_c1, W.gen_c1, 10_000, 2000, (4, 4)),    # 40 tiles: 40 x 500 items
+                 (W.data_c0, W.gen_c0, 4_000, 1200, (4, 2)),     # 16 tiles: 16 x 600 items
+                 (W.data_c0, W.gen_c0, 400, 1000, (2, 1))]       # 2 tiles of 256 x 1000 < 8,000
+        for data, gen, n, p, want in cases:
+            X, y, t = data(n, seed=65)
+            ds = api.Dataset.new(ctx, X, y, t)
+            P = api.Programs(ctx, gen(p, seed=66))
+            try:
+                assert ctx.query_tuning(P, ds) == (want[0], want[1], False), (n, p)
+            finally:
+                P.close()
+                ds.close()
+    finally:
+        ctx.close()
